@@ -1,0 +1,178 @@
+"""ctypes binding of libempm.so (the C-ABI declared in include/empm.h).
+
+This module is the only place that touches the shared library.  It loads the
+in-tree ``empm_amd/libempm.so`` and raises if it is missing: there is no CPU
+or PyTorch fallback for the substep.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libempm.so")
+
+EMPM_ABI_VERSION = 1
+EMPM_MAX_COLLIDERS = 16
+
+EMPM_OK = 0
+EMPM_E_INVALID_ARG = -1
+EMPM_E_HIP = -2
+EMPM_E_OOM = -3
+EMPM_E_UNSUPPORTED_MATERIAL = -4
+EMPM_E_NONFINITE = -5
+EMPM_E_UNSUPPORTED = -6
+
+EMPM_ADJ_EXACT = 0
+EMPM_ADJ_DROP_GRID_MASS = 1
+
+# Every symbol include/empm.h declares (checked by the CPU test-suite).
+EXPORTED_SYMBOLS = (
+    "empm_abi_version",
+    "empm_last_error",
+    "empm_create",
+    "empm_destroy",
+    "empm_mu_lam",
+    "empm_mu_lam_adj",
+    "empm_mass_from_density",
+    "empm_substep_fwd",
+    "empm_substep_adj",
+    "empm_rollout_fwd",
+    "empm_rollout_fwd_replay",
+    "empm_rollout_adj",
+)
+
+
+class EmpmError(RuntimeError):
+    """A libempm call returned a negative status."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"libempm error {code}: {message}")
+        self.code = code
+
+
+class CreateDesc(C.Structure):
+    _fields_ = [
+        ("device", C.c_int),
+        ("n_grid", C.c_int),
+        ("max_particles", C.c_int),
+        ("pad0", C.c_int),
+    ]
+
+
+class Model(C.Structure):
+    _fields_ = [
+        ("n_particles", C.c_int),
+        ("n_grid", C.c_int),
+        ("material", C.c_int),
+        ("pad0", C.c_int),
+        ("grid_lim", C.c_float),
+        ("dx", C.c_float),
+        ("inv_dx", C.c_float),
+        ("rpic_damping", C.c_float),
+        ("grid_v_damping_scale", C.c_float),
+        ("gravity", C.c_float * 3),
+        ("alpha", C.c_float),
+        ("hardening", C.c_float),
+        ("xi", C.c_float),
+        ("plastic_viscosity", C.c_float),
+        ("softening", C.c_float),
+        ("mu", C.c_void_p),
+        ("lam", C.c_void_p),
+        ("yield_stress", C.c_void_p),
+        ("mass", C.c_void_p),
+        ("vol", C.c_void_p),
+        ("selection", C.c_void_p),
+    ]
+
+
+class Particles(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("v", C.c_void_p), ("C", C.c_void_p), ("F", C.c_void_p),
+                ("Fe", C.c_void_p)]
+
+
+class Collider(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("surface_type", C.c_int),
+        ("geo_type", C.c_int),
+        ("reset", C.c_int),
+        ("active", C.c_int),
+        ("pad0", C.c_int),
+        ("start_time", C.c_float),
+        ("end_time", C.c_float),
+        ("point", C.c_float * 3),
+        ("normal", C.c_float * 3),
+        ("size", C.c_float * 3),
+        ("velocity", C.c_float * 3),
+        ("rot_axis", C.c_float * 3),
+        ("angular_velocity", C.c_float),
+        ("radius", C.c_float),
+        ("half_height", C.c_float),
+        ("friction", C.c_float),
+        ("mask", C.c_void_p),
+    ]
+
+
+_lib = None
+
+
+def _declare(lib):
+    P = C.POINTER
+    vp = C.c_void_p
+    lib.empm_abi_version.restype = C.c_int
+    lib.empm_abi_version.argtypes = []
+    lib.empm_last_error.restype = C.c_char_p
+    lib.empm_last_error.argtypes = []
+    lib.empm_create.restype = C.c_int
+    lib.empm_create.argtypes = [P(vp), P(CreateDesc)]
+    lib.empm_destroy.restype = None
+    lib.empm_destroy.argtypes = [vp]
+    lib.empm_mu_lam.restype = C.c_int
+    lib.empm_mu_lam.argtypes = [C.c_int, vp, vp, vp, vp, vp]
+    lib.empm_mu_lam_adj.restype = C.c_int
+    lib.empm_mu_lam_adj.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp]
+    lib.empm_mass_from_density.restype = C.c_int
+    lib.empm_mass_from_density.argtypes = [C.c_int, vp, vp, vp, vp]
+    lib.empm_substep_fwd.restype = C.c_int
+    lib.empm_substep_fwd.argtypes = [vp, P(Model), P(Collider), C.c_int, C.c_float, C.c_float,
+                                     P(Particles), P(Particles), vp]
+    lib.empm_substep_adj.restype = C.c_int
+    lib.empm_substep_adj.argtypes = [vp, P(Model), P(Collider), C.c_int, C.c_float, C.c_float,
+                                     P(Particles), P(Particles), P(Particles),
+                                     vp, vp, vp, vp, vp, C.c_int, vp]
+    lib.empm_rollout_fwd.restype = C.c_int
+    lib.empm_rollout_fwd.argtypes = [vp, P(Model), P(Collider), C.c_int, P(C.c_double), C.c_double,
+                                     C.c_int, P(Particles), P(Collider), P(C.c_double), vp]
+    lib.empm_rollout_fwd_replay.restype = C.c_int
+    lib.empm_rollout_fwd_replay.argtypes = [vp, P(Model), P(Collider), C.c_int, P(C.c_double), C.c_double,
+                                            C.c_int, P(Particles), vp]
+    lib.empm_rollout_adj.restype = C.c_int
+    lib.empm_rollout_adj.argtypes = [vp, P(Model), P(Collider), C.c_int, P(C.c_double), C.c_double,
+                                     C.c_int, P(Particles), P(Particles),
+                                     vp, vp, vp, vp, vp, C.c_int, vp]
+
+
+def load(path: str | None = None):
+    """Load libempm.so (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or os.environ.get("EMPM_LIB", LIB_PATH)
+    if not os.path.exists(p):
+        raise ImportError(
+            f"libempm.so not found at {p}: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(p)
+    _declare(lib)
+    ver = lib.empm_abi_version()
+    if ver != EMPM_ABI_VERSION:
+        raise ImportError(f"libempm ABI version {ver} != expected {EMPM_ABI_VERSION}")
+    _lib = lib
+    return lib
+
+
+def check(rc: int):
+    if rc != EMPM_OK:
+        msg = load().empm_last_error().decode(errors="replace")
+        raise EmpmError(rc, msg)
+    return rc

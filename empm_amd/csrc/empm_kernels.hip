@@ -1,0 +1,1034 @@
+// libempm: the MLS-MPM substep of rai-opensource/empm and its adjoint as
+// hand-written HIP kernels for gfx950 (MI355X), behind the C-ABI of
+// include/empm.h.
+//
+// Per substep the forward runs three kernels:
+//   k_p2g   per particle: return map + stress (one SVD) fused with the APIC
+//           scatter of momentum and mass into the float4 grid (mv.xyz, m);
+//   k_grid  per node: normalise, gravity, damping and every collider in
+//           registration order (a table passed by value), writes v_out and
+//           re-zeroes the accumulation grid for the next substep;
+//   k_g2p   per particle: 27-node gather, x/v/C/F_trial update.
+// The adjoint of one substep recomputes k_p2g/k_grid and then runs
+//   k_g2p_adj (scatter of dL/dv_out), k_grid_adj (per node), k_p2g_adj
+//   (gather of dL/d(mv, m), stress and return-map adjoints).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "empm.h"
+#include "empm_math.h"
+
+using namespace empm;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int set_err(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                         \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return set_err(EMPM_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kBlock = 256;
+
+struct DevModel {
+  int N, n, material, pad0;
+  float grid_lim, dx, inv_dx, rpic, damp_scale;
+  float g[3];
+  MatParams mp;
+  float* mu;
+  float* lam;
+  float* ys;
+  const float* mass;
+  const float* vol;
+  const int* sel;
+};
+
+struct ColTable {
+  empm_collider c[EMPM_MAX_COLLIDERS];
+  int n;
+};
+
+// Quadratic B-spline stencil of one particle (mpm_utils.py:365-383).  The
+// reference's w = wp.mat33(vec, vec, vec) takes the vectors as columns, so
+// w[d][i] = N_i(f_d); base is wp.int(), i.e. truncation toward zero.
+struct Stencil {
+  int base[3];
+  float f[3];
+  float w[3][3];
+  float dw[3][3];
+};
+
+EMPM_DEV void make_stencil(const float* xp, float inv_dx, Stencil& s) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float gp = xp[d] * inv_dx;
+    s.base[d] = (int)(gp - 0.5f);
+    const float f = gp - (float)s.base[d];
+    s.f[d] = f;
+    const float wa = 1.5f - f, wb = f - 1.f, wc = f - 0.5f;
+    s.w[d][0] = wa * wa * 0.5f;
+    s.w[d][1] = 0.75f - wb * wb;
+    s.w[d][2] = wc * wc * 0.5f;
+    s.dw[d][0] = f - 1.5f;
+    s.dw[d][1] = -2.f * (f - 1.f);
+    s.dw[d][2] = f - 0.5f;
+  }
+}
+
+EMPM_DEV bool node_in(int ix, int iy, int iz, int n) {
+  return (unsigned)ix < (unsigned)n && (unsigned)iy < (unsigned)n && (unsigned)iz < (unsigned)n;
+}
+EMPM_DEV size_t node_id(int ix, int iy, int iz, int n) { return ((size_t)ix * n + iy) * n + iz; }
+
+// d(w, grad w)/df chain for node (a,b,c): fb += dL/df given wb = dL/dw and
+// gwb = dL/d(grad w).  grad w = inv_dx (N'a Nb Nc, Na N'b Nc, Na Nb N'c).
+EMPM_DEV void weight_adj(const Stencil& s, int a, int b, int c, float inv_dx, float wb, const float* gwb, float* fb) {
+  const float X = s.w[0][a], Y = s.w[1][b], Z = s.w[2][c];
+  const float dX = s.dw[0][a], dY = s.dw[1][b], dZ = s.dw[2][c];
+  const float ddX = a == 1 ? -2.f : 1.f, ddY = b == 1 ? -2.f : 1.f, ddZ = c == 1 ? -2.f : 1.f;
+  fb[0] += wb * dX * Y * Z + inv_dx * (gwb[0] * ddX * Y * Z + gwb[1] * dX * dY * Z + gwb[2] * dX * Y * dZ);
+  fb[1] += wb * X * dY * Z + inv_dx * (gwb[0] * dX * dY * Z + gwb[1] * X * ddY * Z + gwb[2] * X * dY * dZ);
+  fb[2] += wb * X * Y * dZ + inv_dx * (gwb[0] * dX * Y * dZ + gwb[1] * X * dY * dZ + gwb[2] * X * Y * ddZ);
+}
+
+EMPM_DEV void load9(const float* __restrict__ a, size_t p, float* o) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) o[i] = a[9 * p + i];
+}
+EMPM_DEV void store9(float* __restrict__ a, size_t p, const float* o) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) a[9 * p + i] = o[i];
+}
+
+// APIC affine matrix with the RPIC blend (mpm_utils.py:397-411)
+EMPM_DEV void rpic_blend(const float* Cp, float r, float* Ct) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Ct[i * 3 + j] = (1.f - r) * Cp[i * 3 + j] + r / 2.f * (Cp[i * 3 + j] - Cp[j * 3 + i]);
+  if (r < -0.001f) m_zero(Ct);
+}
+
+// ---------------------------------------------------------------------------
+// Colliders, one node.  v is updated as the reference kernels do; fac tracks
+// dv_out/dv_in per component (every collider op is a per-component overwrite
+// or scale), which is exactly what the adjoint needs.
+// ---------------------------------------------------------------------------
+EMPM_DEV void v_zero(float* v, float* fac) {
+  v[0] = v[1] = v[2] = 0.f;
+  fac[0] = fac[1] = fac[2] = 0.f;
+}
+EMPM_DEV void v_cut(float* v, float* fac, int iz, float dx) {
+  const float z = (float)iz * dx;
+  if (z < 0.4f || z > 0.53f) {
+    v_zero(v, fac);
+  } else {
+    v[0] *= 0.3f;
+    v[1] = 0.f;
+    v[2] *= 0.3f;
+    fac[0] *= 0.3f;
+    fac[1] = 0.f;
+    fac[2] *= 0.3f;
+  }
+}
+EMPM_DEV bool geo_inside(const empm_collider& c, const float* off) {
+  if (c.geo_type == 1)
+    return fabsf(off[0]) < c.size[0] && fabsf(off[1]) < c.size[1] && fabsf(off[2]) < c.size[2];
+  if (c.geo_type == 2) return sqrtf(off[0] * off[0] + off[1] * off[1] + off[2] * off[2]) < c.radius;
+  if (c.geo_type == 3) {
+    const float pr = off[0] * c.normal[0] + off[1] * c.normal[1] + off[2] * c.normal[2];
+    const float r0 = off[0] - pr * c.normal[0], r1 = off[1] - pr * c.normal[1], r2 = off[2] - pr * c.normal[2];
+    return fabsf(pr) < c.half_height && sqrtf(r0 * r0 + r1 * r1 + r2 * r2) < c.radius;
+  }
+  return false;
+}
+
+EMPM_DEV void collide(const empm_collider& c, int ix, int iy, int iz, int n, float time, float dt, float dx, float* v,
+                      float* fac) {
+  const bool win = time >= c.start_time && time < c.end_time;
+  float off[3] = {(float)ix * dx - c.point[0], (float)iy * dx - c.point[1], (float)iz * dx - c.point[2]};
+  switch (c.kind) {
+    case 0: {  // add_surface_collider (mpm_solver_diff.py:473-528): sticky,
+               // slip and separate all end in v = 0; type 11 is "cut"
+      if (!win) return;
+      const float d = off[0] * c.normal[0] + off[1] * c.normal[1] + off[2] * c.normal[2];
+      if (d < 0.f) {
+        if (c.surface_type == 11) v_cut(v, fac, iz, dx);
+        else v_zero(v, fac);
+      }
+      return;
+    }
+    case 1: {  // set_velocity_on_cuboid (mpm_solver_diff.py:835-858)
+      if (win) {
+        if (fabsf(off[0]) < c.size[0] && fabsf(off[1]) < c.size[1] && fabsf(off[2]) < c.size[2]) {
+          v[0] = c.velocity[0];
+          v[1] = c.velocity[1];
+          v[2] = c.velocity[2];
+          fac[0] = fac[1] = fac[2] = 0.f;
+        }
+      } else if (c.reset == 1) {
+        if (time < c.end_time + 15.f * dt) v_zero(v, fac);
+      }
+      return;
+    }
+    case 2: {  // add_bounding_box (mpm_solver_diff.py:878-935), padding 3
+      if (!win) return;
+      const int idx[3] = {ix, iy, iz};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (idx[d] < 3 && v[d] < 0.f) {
+          v[d] = 0.f;
+          fac[d] = 0.f;
+        }
+        if (idx[d] >= n - 3 && v[d] > 0.f) {
+          v[d] = 0.f;
+          fac[d] = 0.f;
+        }
+      }
+      return;
+    }
+    case 3: {  // add_geometry_collider (mpm_solver_diff.py:612-672)
+      if (!(win && c.active)) return;
+      if (c.geo_type == 4) {
+        const float d = off[0] * c.normal[0] + off[1] * c.normal[1] + off[2] * c.normal[2];
+        if (d < 0.f) {
+          if (c.surface_type == 11) v_cut(v, fac, iz, dx);
+          else v_zero(v, fac);
+        }
+        return;
+      }
+      if (geo_inside(c, off)) v_zero(v, fac);
+      return;
+    }
+    case 4: {  // set_grid_velocity_on_geometry (mpm_solver_diff.py:756-793)
+      if (win && c.active) {
+        const float av = c.angular_velocity;
+        const float cr0 = c.rot_axis[1] * off[2] - c.rot_axis[2] * off[1];
+        const float cr1 = c.rot_axis[2] * off[0] - c.rot_axis[0] * off[2];
+        const float cr2 = c.rot_axis[0] * off[1] - c.rot_axis[1] * off[0];
+        bool inside;
+        if (c.geo_type == 4) inside = (off[0] * c.normal[0] + off[1] * c.normal[1] + off[2] * c.normal[2]) < 0.f;
+        else inside = geo_inside(c, off);
+        if (inside) {
+          v[0] = c.velocity[0] + av * cr0;
+          v[1] = c.velocity[1] + av * cr1;
+          v[2] = c.velocity[2] + av * cr2;
+          fac[0] = fac[1] = fac[2] = 0.f;
+        }
+      } else if (c.reset == 1) {
+        if (time < c.end_time + 15.f * dt) v_zero(v, fac);
+      }
+      return;
+    }
+    case 5:  // enforce_grid_velocity_by_mask (mpm_solver_diff.py:1275-1286)
+      if (c.mask[node_id(ix, iy, iz, n)] >= 1) v_zero(v, fac);
+      return;
+    default:
+      return;
+  }
+}
+
+// grid_normalization_and_gravity (mpm_utils.py:427-438) + add_damping_via_grid
+// (mpm_utils.py:722-734) + colliders, for one node.
+EMPM_DEV void grid_node(const DevModel& M, const ColTable& T, float time, float dt, int ix, int iy, int iz,
+                        const float4 g, float* v, float* fac) {
+  v[0] = v[1] = v[2] = 0.f;
+  fac[0] = fac[1] = fac[2] = 1.f;
+  if (g.w > 1e-15f) {
+    const float im = 1.f / g.w;
+    v[0] = g.x * im + dt * M.g[0];
+    v[1] = g.y * im + dt * M.g[1];
+    v[2] = g.z * im + dt * M.g[2];
+  }
+  if (M.damp_scale < 1.f) {
+    const float k = 1.f - M.damp_scale;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      v[d] = v[d] - k * v[d];
+      fac[d] *= M.damp_scale;
+    }
+  }
+  for (int k = 0; k < T.n; ++k) collide(T.c[k], ix, iy, iz, M.n, time, dt, M.dx, v, fac);
+}
+
+// ---------------------------------------------------------------------------
+// Forward kernels
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_p2g(DevModel M, float dt, const float* __restrict__ x,
+                                                const float* __restrict__ v, const float* __restrict__ C,
+                                                const float* __restrict__ F, float* __restrict__ Fe_out,
+                                                float4* __restrict__ grid, int write_params) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M.N) return;
+  if (M.sel && M.sel[p] != 0) return;
+  float Ftr[9];
+  load9(F, p, Ftr);
+  float mu = M.mu[p], lam = M.lam[p], ys = M.ys[p];
+  ConstitState st;
+  float tau[9];
+  constitutive_fwd(st, Ftr, mu, lam, ys, M.mp, dt, tau);
+  if (write_params) {
+    M.mu[p] = mu;
+    M.lam[p] = lam;
+    M.ys[p] = ys;
+  }
+  store9(Fe_out, p, st.Fe);
+
+  const float xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
+  const float vp[3] = {v[3 * p], v[3 * p + 1], v[3 * p + 2]};
+  float Cp[9], Ct[9];
+  load9(C, p, Cp);
+  rpic_blend(Cp, M.rpic, Ct);
+  Stencil s;
+  make_stencil(xp, M.inv_dx, s);
+  const float mp = M.mass[p], Vp = M.vol[p];
+  // elastic term per node: -dt V tau grad_w ; precompute -dt V tau
+  float S[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) S[i] = -Vp * tau[i];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
+        if (!node_in(ix, iy, iz, M.n)) continue;
+        const float dpos[3] = {((float)a - s.f[0]) * M.dx, ((float)b - s.f[1]) * M.dx, ((float)c - s.f[2]) * M.dx};
+        const float wt = s.w[0][a] * s.w[1][b] * s.w[2][c];
+        const float gw[3] = {s.dw[0][a] * s.w[1][b] * s.w[2][c] * M.inv_dx, s.w[0][a] * s.dw[1][b] * s.w[2][c] * M.inv_dx,
+                             s.w[0][a] * s.w[1][b] * s.dw[2][c] * M.inv_dx};
+        float add[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const float cd = Ct[i * 3 + 0] * dpos[0] + Ct[i * 3 + 1] * dpos[1] + Ct[i * 3 + 2] * dpos[2];
+          const float ef = S[i * 3 + 0] * gw[0] + S[i * 3 + 1] * gw[1] + S[i * 3 + 2] * gw[2];
+          add[i] = wt * mp * (vp[i] + cd) + dt * ef;
+        }
+        float4* g = &grid[node_id(ix, iy, iz, M.n)];
+        atomicAdd(&g->x, add[0]);
+        atomicAdd(&g->y, add[1]);
+        atomicAdd(&g->z, add[2]);
+        atomicAdd(&g->w, wt * mp);
+      }
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid(DevModel M, ColTable T, float time, float dt,
+                                                 float4* __restrict__ grid_in, float4* __restrict__ v_out,
+                                                 int zero_in) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t G = (size_t)M.n * M.n * M.n;
+  if (id >= G) return;
+  const int iz = (int)(id % M.n);
+  const int iy = (int)((id / M.n) % M.n);
+  const int ix = (int)(id / ((size_t)M.n * M.n));
+  const float4 g = grid_in[id];
+  float v[3], fac[3];
+  grid_node(M, T, time, dt, ix, iy, iz, g, v, fac);
+  v_out[id] = make_float4(v[0], v[1], v[2], g.w);
+  if (zero_in) grid_in[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// g2p_differentiable (mpm_utils.py:502-579)
+__global__ __launch_bounds__(kBlock) void k_g2p(DevModel M, float dt, const float* __restrict__ x,
+                                                const float* __restrict__ Fe, const float4* __restrict__ v_out,
+                                                float* __restrict__ x1, float* __restrict__ v1,
+                                                float* __restrict__ C1, float* __restrict__ F1) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M.N) return;
+  if (M.sel && M.sel[p] != 0) {  // partial_clone defaults (mpm_data_structure.py:356-387)
+    float I[9], Z[9];
+    m_eye(I);
+    m_zero(Z);
+    x1[3 * p] = x1[3 * p + 1] = x1[3 * p + 2] = 0.f;
+    v1[3 * p] = v1[3 * p + 1] = v1[3 * p + 2] = 0.f;
+    store9(C1, p, Z);
+    store9(F1, p, I);
+    return;
+  }
+  const float xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
+  Stencil s;
+  make_stencil(xp, M.inv_dx, s);
+  float nv[3] = {0.f, 0.f, 0.f}, nC[9], nG[9];
+  m_zero(nC);
+  m_zero(nG);
+  const float c4 = M.inv_dx * 4.f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
+        float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (node_in(ix, iy, iz, M.n)) gv = v_out[node_id(ix, iy, iz, M.n)];
+        const float gvv[3] = {gv.x, gv.y, gv.z};
+        const float dpos[3] = {(float)a - s.f[0], (float)b - s.f[1], (float)c - s.f[2]};
+        const float wt = s.w[0][a] * s.w[1][b] * s.w[2][c];
+        const float gw[3] = {s.dw[0][a] * s.w[1][b] * s.w[2][c] * M.inv_dx, s.w[0][a] * s.dw[1][b] * s.w[2][c] * M.inv_dx,
+                             s.w[0][a] * s.w[1][b] * s.dw[2][c] * M.inv_dx};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          nv[i] += gvv[i] * wt;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            nC[i * 3 + j] += gvv[i] * dpos[j] * (wt * c4);
+            nG[i * 3 + j] += gvv[i] * gw[j];
+          }
+        }
+      }
+  const float amin = M.dx * 2.f;
+  const float amax = M.grid_lim - M.dx * 2.f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    v1[3 * p + i] = nv[i];
+    const float xn = xp[i] + dt * nv[i];
+    x1[3 * p + i] = fminf(fmaxf(xn, amin), amax);
+  }
+  store9(C1, p, nC);
+  float IF[9], Fp[9], Fn[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) IF[i] = ((i % 4 == 0) ? 1.f : 0.f) + nG[i] * dt;
+  load9(Fe, p, Fp);
+  m_mul(IF, Fp, Fn);
+  store9(F1, p, Fn);
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint kernels
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_g2p_adj(DevModel M, float dt, const float* __restrict__ x,
+                                                    const float* __restrict__ Fe, const float4* __restrict__ v_out,
+                                                    const float* __restrict__ dx1, const float* __restrict__ dv1,
+                                                    const float* __restrict__ dC1, const float* __restrict__ dF1,
+                                                    float* __restrict__ xbar, float* __restrict__ Febar,
+                                                    float4* __restrict__ gadj_v) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M.N) return;
+  if (M.sel && M.sel[p] != 0) {
+    xbar[3 * p] = xbar[3 * p + 1] = xbar[3 * p + 2] = 0.f;
+    float Z[9];
+    m_zero(Z);
+    store9(Febar, p, Z);
+    return;
+  }
+  const float xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
+  Stencil s;
+  make_stencil(xp, M.inv_dx, s);
+  // pass 1: v' and grad v for the clamp mask and the F adjoint
+  float nv[3] = {0.f, 0.f, 0.f}, nG[9];
+  m_zero(nG);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
+        float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (node_in(ix, iy, iz, M.n)) gv = v_out[node_id(ix, iy, iz, M.n)];
+        const float gvv[3] = {gv.x, gv.y, gv.z};
+        const float wt = s.w[0][a] * s.w[1][b] * s.w[2][c];
+        const float gw[3] = {s.dw[0][a] * s.w[1][b] * s.w[2][c] * M.inv_dx, s.w[0][a] * s.dw[1][b] * s.w[2][c] * M.inv_dx,
+                             s.w[0][a] * s.w[1][b] * s.dw[2][c] * M.inv_dx};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          nv[i] += gvv[i] * wt;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) nG[i * 3 + j] += gvv[i] * gw[j];
+        }
+      }
+  const float amin = M.dx * 2.f, amax = M.grid_lim - M.dx * 2.f;
+  float xb[3], nvb[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float xn = xp[i] + dt * nv[i];
+    // wp.clamp passes the adjoint to x unless x < lo or x > hi
+    const float msk = (xn < amin || xn > amax) ? 0.f : 1.f;
+    const float g = dx1[3 * p + i] * msk;
+    xb[i] = g;
+    nvb[i] = dv1[3 * p + i] + dt * g;
+  }
+  float dF[9], Fp[9], dC[9];
+  load9(dF1, p, dF);
+  load9(Fe, p, Fp);
+  load9(dC1, p, dC);
+  // F1 = (I + dt G) Fe:  Gbar = dt dF Fe^T ; Febar = (I + dt G)^T dF
+  float Gb[9], IF[9], Feb[9];
+  m_mul_bt(dF, Fp, Gb);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    Gb[i] *= dt;
+    IF[i] = ((i % 4 == 0) ? 1.f : 0.f) + nG[i] * dt;
+  }
+  m_mul_at(IF, dF, Feb);
+  store9(Febar, p, Feb);
+  const float c4 = M.inv_dx * 4.f;
+  float fb[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
+        const bool inb = node_in(ix, iy, iz, M.n);
+        float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (inb) gv = v_out[node_id(ix, iy, iz, M.n)];
+        const float vi[3] = {gv.x, gv.y, gv.z};
+        const float d[3] = {(float)a - s.f[0], (float)b - s.f[1], (float)c - s.f[2]};
+        const float wt = s.w[0][a] * s.w[1][b] * s.w[2][c];
+        const float gw[3] = {s.dw[0][a] * s.w[1][b] * s.w[2][c] * M.inv_dx, s.w[0][a] * s.dw[1][b] * s.w[2][c] * M.inv_dx,
+                             s.w[0][a] * s.w[1][b] * s.dw[2][c] * M.inv_dx};
+        float Cd[3], vb[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          Cd[i] = dC[i * 3 + 0] * d[0] + dC[i * 3 + 1] * d[1] + dC[i * 3 + 2] * d[2];
+          vb[i] = wt * nvb[i] + c4 * wt * Cd[i] + (Gb[i * 3 + 0] * gw[0] + Gb[i * 3 + 1] * gw[1] + Gb[i * 3 + 2] * gw[2]);
+        }
+        if (inb) {
+          float4* g = &gadj_v[node_id(ix, iy, iz, M.n)];
+          atomicAdd(&g->x, vb[0]);
+          atomicAdd(&g->y, vb[1]);
+          atomicAdd(&g->z, vb[2]);
+        }
+        const float wb = nvb[0] * vi[0] + nvb[1] * vi[1] + nvb[2] * vi[2] + c4 * (vi[0] * Cd[0] + vi[1] * Cd[1] + vi[2] * Cd[2]);
+        float gwb[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          // d(d_j): 4 inv_dx w sum_i dC_ij v_i ; d = off - f
+          const float db = c4 * wt * (dC[0 * 3 + j] * vi[0] + dC[1 * 3 + j] * vi[1] + dC[2 * 3 + j] * vi[2]);
+          fb[j] -= db;
+          gwb[j] = Gb[0 * 3 + j] * vi[0] + Gb[1 * 3 + j] * vi[1] + Gb[2 * 3 + j] * vi[2];
+        }
+        weight_adj(s, a, b, c, M.inv_dx, wb, gwb, fb);
+      }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) xbar[3 * p + i] = xb[i] + M.inv_dx * fb[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid_adj(DevModel M, ColTable T, float time, float dt,
+                                                     float4* __restrict__ grid_in, float4* __restrict__ gadj_v,
+                                                     float4* __restrict__ gadj_in, int drop_mass) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t G = (size_t)M.n * M.n * M.n;
+  if (id >= G) return;
+  const int iz = (int)(id % M.n);
+  const int iy = (int)((id / M.n) % M.n);
+  const int ix = (int)(id / ((size_t)M.n * M.n));
+  const float4 g = grid_in[id];
+  const float4 b = gadj_v[id];
+  float v[3], fac[3];
+  grid_node(M, T, time, dt, ix, iy, iz, g, v, fac);
+  float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (g.w > 1e-15f) {
+    const float im = 1.f / g.w;
+    const float b0 = b.x * fac[0], b1 = b.y * fac[1], b2 = b.z * fac[2];
+    o.x = b0 * im;
+    o.y = b1 * im;
+    o.z = b2 * im;
+    o.w = drop_mass ? 0.f : -(b0 * g.x + b1 * g.y + b2 * g.z) * im * im;
+  }
+  gadj_in[id] = o;
+  gadj_v[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+  grid_in[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_p2g_adj(DevModel M, float dt, const float* __restrict__ x,
+                                                    const float* __restrict__ v, const float* __restrict__ C,
+                                                    const float* __restrict__ F, const float4* __restrict__ gadj_in,
+                                                    const float* __restrict__ xbar, const float* __restrict__ Febar_in,
+                                                    float* __restrict__ dx, float* __restrict__ dv,
+                                                    float* __restrict__ dCo, float* __restrict__ dFo,
+                                                    float* __restrict__ dmu, float* __restrict__ dlam,
+                                                    float* __restrict__ dys, float* __restrict__ dmass,
+                                                    float* __restrict__ dvol) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M.N) return;
+  if (M.sel && M.sel[p] != 0) {
+    float Z[9];
+    m_zero(Z);
+    dx[3 * p] = dx[3 * p + 1] = dx[3 * p + 2] = 0.f;
+    dv[3 * p] = dv[3 * p + 1] = dv[3 * p + 2] = 0.f;
+    store9(dCo, p, Z);
+    store9(dFo, p, Z);
+    return;
+  }
+  float Ftr[9];
+  load9(F, p, Ftr);
+  float mu = M.mu[p], lam = M.lam[p], ys = M.ys[p];
+  const float mu0 = mu, lam0 = lam, ys0 = ys;
+  ConstitState st;
+  float tau[9];
+  constitutive_fwd(st, Ftr, mu, lam, ys, M.mp, dt, tau);
+
+  const float xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
+  const float vp[3] = {v[3 * p], v[3 * p + 1], v[3 * p + 2]};
+  float Cp[9], Ct[9];
+  load9(C, p, Cp);
+  rpic_blend(Cp, M.rpic, Ct);
+  Stencil s;
+  make_stencil(xp, M.inv_dx, s);
+  const float mp = M.mass[p], Vp = M.vol[p];
+  float vb[3] = {0.f, 0.f, 0.f}, Ctb[9], taub[9], fb[3] = {0.f, 0.f, 0.f};
+  m_zero(Ctb);
+  m_zero(taub);
+  float mb = 0.f, Vb = 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
+        if (!node_in(ix, iy, iz, M.n)) continue;
+        const float4 ga = gadj_in[node_id(ix, iy, iz, M.n)];
+        const float av[3] = {ga.x, ga.y, ga.z};
+        const float bm = ga.w;
+        const float dpos[3] = {((float)a - s.f[0]) * M.dx, ((float)b - s.f[1]) * M.dx, ((float)c - s.f[2]) * M.dx};
+        const float wt = s.w[0][a] * s.w[1][b] * s.w[2][c];
+        const float gw[3] = {s.dw[0][a] * s.w[1][b] * s.w[2][c] * M.inv_dx, s.w[0][a] * s.dw[1][b] * s.w[2][c] * M.inv_dx,
+                             s.w[0][a] * s.w[1][b] * s.dw[2][c] * M.inv_dx};
+        float u[3], tg[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          u[i] = vp[i] + Ct[i * 3 + 0] * dpos[0] + Ct[i * 3 + 1] * dpos[1] + Ct[i * 3 + 2] * dpos[2];
+          tg[i] = tau[i * 3 + 0] * gw[0] + tau[i * 3 + 1] * gw[1] + tau[i * 3 + 2] * gw[2];
+        }
+        const float au = av[0] * u[0] + av[1] * u[1] + av[2] * u[2];
+        const float atg = av[0] * tg[0] + av[1] * tg[1] + av[2] * tg[2];
+        const float wm = wt * mp;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          vb[i] += wm * av[i];
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            Ctb[i * 3 + j] += wm * av[i] * dpos[j];
+            taub[i * 3 + j] += -dt * Vp * av[i] * gw[j];
+          }
+        }
+        mb += wt * au + wt * bm;
+        Vb += -dt * atg;
+        const float wb = mp * au + mp * bm;
+        float gwb[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const float db = wm * (Ct[0 * 3 + j] * av[0] + Ct[1 * 3 + j] * av[1] + Ct[2 * 3 + j] * av[2]);
+          fb[j] -= M.dx * db;
+          gwb[j] = -dt * Vp * (tau[0 * 3 + j] * av[0] + tau[1 * 3 + j] * av[1] + tau[2 * 3 + j] * av[2]);
+        }
+        weight_adj(s, a, b, c, M.inv_dx, wb, gwb, fb);
+      }
+  float xo[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) xo[i] = xbar[3 * p + i] + M.inv_dx * fb[i];
+  float Cb[9];
+  const float r = M.rpic;
+  if (r < -0.001f) {
+    m_zero(Cb);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        Cb[i * 3 + j] = (1.f - r) * Ctb[i * 3 + j] + r / 2.f * (Ctb[i * 3 + j] - Ctb[j * 3 + i]);
+  }
+  float taur[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) taur[i * 3 + j] = 0.5f * (taub[i * 3 + j] + taub[j * 3 + i]);
+  float Feb[9], Ftrb[9];
+  load9(Febar_in, p, Feb);
+  m_zero(Ftrb);
+  float mub = 0.f, lamb = 0.f, ysb = 0.f;
+  stress_adj(st, M.material, mu0, lam0, taur, Feb, mub, lamb);
+  return_map_adj(st, Ftr, mu0, lam0, ys0, M.mp, dt, Feb, Ftrb, mub, lamb, ysb);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    dx[3 * p + i] = xo[i];
+    dv[3 * p + i] = vb[i];
+  }
+  store9(dCo, p, Cb);
+  store9(dFo, p, Ftrb);
+  if (dmu) dmu[p] += mub;
+  if (dlam) dlam[p] += lamb;
+  if (dys) dys[p] += ysb;
+  if (dmass) dmass[p] += mb;
+  if (dvol) dvol[p] += Vb;
+}
+
+__global__ void k_mu_lam(int n, const float* __restrict__ E, const float* __restrict__ nu, float* __restrict__ mu,
+                         float* __restrict__ lam) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float e = E[p], v = nu[p];
+  mu[p] = e / (2.f * (1.f + v));
+  lam[p] = e * v / ((1.f + v) * (1.f - 2.f * v));
+}
+
+__global__ void k_mu_lam_adj(int n, const float* __restrict__ E, const float* __restrict__ nu,
+                             const float* __restrict__ dmu, const float* __restrict__ dlam, float* __restrict__ dE,
+                             float* __restrict__ dnu) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float e = E[p], v = nu[p];
+  const float q = (1.f + v) * (1.f - 2.f * v);
+  const float gm = dmu ? dmu[p] : 0.f, gl = dlam ? dlam[p] : 0.f;
+  if (dE) dE[p] += gm / (2.f * (1.f + v)) + gl * v / q;
+  if (dnu) dnu[p] += gm * (-e / (2.f * (1.f + v) * (1.f + v))) + gl * e * (1.f + 2.f * v * v) / (q * q);
+}
+
+__global__ void k_mass(int n, const float* __restrict__ rho, const float* __restrict__ vol, float* __restrict__ mass) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  mass[p] = rho[p] * vol[p];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct empm_ctx {
+  int device;
+  int n_grid;
+  int max_particles;
+  size_t G;
+  float4* grid_in;
+  float4* v_out;
+  float4* gadj_v;
+  float4* gadj_in;
+  float* Fe;
+  float* xbar;
+  float* Febar;
+};
+
+namespace {
+
+int check_model(const empm_ctx* ctx, const empm_model* m) {
+  if (!ctx || !m) return set_err(EMPM_E_INVALID_ARG, "null context or model");
+  if (m->n_particles < 0 || m->n_particles > ctx->max_particles)
+    return set_err(EMPM_E_INVALID_ARG, "n_particles=%d outside [0, %d]", m->n_particles, ctx->max_particles);
+  if (m->n_grid != ctx->n_grid)
+    return set_err(EMPM_E_INVALID_ARG, "model n_grid=%d but context n_grid=%d", m->n_grid, ctx->n_grid);
+  if (m->material < 0 || m->material > 7)
+    return set_err(EMPM_E_UNSUPPORTED_MATERIAL, "Undefined material type %d", m->material);
+  if (m->n_particles > 0 && (!m->mu || !m->lam || !m->yield_stress || !m->mass || !m->vol))
+    return set_err(EMPM_E_INVALID_ARG, "null per-particle model array");
+  if (!(m->inv_dx > 0.f) || !(m->dx > 0.f)) return set_err(EMPM_E_INVALID_ARG, "dx/inv_dx must be positive");
+  return EMPM_OK;
+}
+
+int check_particles(const empm_particles* s, int N, const char* what) {
+  if (!s) return set_err(EMPM_E_INVALID_ARG, "null %s", what);
+  if (N > 0 && (!s->x || !s->v || !s->C || !s->F)) return set_err(EMPM_E_INVALID_ARG, "null array in %s", what);
+  return EMPM_OK;
+}
+
+int make_table(const empm_collider* cols, int ncol, ColTable& T, int n) {
+  if (ncol < 0 || ncol > EMPM_MAX_COLLIDERS)
+    return set_err(EMPM_E_INVALID_ARG, "ncol=%d outside [0, %d]", ncol, EMPM_MAX_COLLIDERS);
+  if (ncol > 0 && !cols) return set_err(EMPM_E_INVALID_ARG, "null collider table");
+  memset(&T, 0, sizeof(T));
+  for (int k = 0; k < ncol; ++k) {
+    if (cols[k].kind < 0 || cols[k].kind > 5)
+      return set_err(EMPM_E_INVALID_ARG, "collider %d: unknown kind %d", k, cols[k].kind);
+    if (cols[k].kind == 5 && !cols[k].mask) return set_err(EMPM_E_INVALID_ARG, "collider %d: null mask", k);
+    T.c[k] = cols[k];
+  }
+  T.n = ncol;
+  (void)n;
+  return EMPM_OK;
+}
+
+DevModel dev_model(const empm_model* m) {
+  DevModel d;
+  memset(&d, 0, sizeof(d));
+  d.N = m->n_particles;
+  d.n = m->n_grid;
+  d.material = m->material;
+  d.grid_lim = m->grid_lim;
+  d.dx = m->dx;
+  d.inv_dx = m->inv_dx;
+  d.rpic = m->rpic_damping;
+  d.damp_scale = m->grid_v_damping_scale;
+  d.g[0] = m->gravity[0];
+  d.g[1] = m->gravity[1];
+  d.g[2] = m->gravity[2];
+  d.mp.material = m->material;
+  d.mp.alpha = m->alpha;
+  d.mp.hardening = m->hardening;
+  d.mp.xi = m->xi;
+  d.mp.plastic_viscosity = m->plastic_viscosity;
+  d.mp.softening = m->softening;
+  d.mu = m->mu;
+  d.lam = m->lam;
+  d.ys = m->yield_stress;
+  d.mass = m->mass;
+  d.vol = m->vol;
+  d.sel = m->selection;
+  return d;
+}
+
+unsigned nblocks(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+int launch_fwd(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, float dt, const empm_particles* in,
+               const empm_particles* out, hipStream_t st) {
+  const int write_params = (M.material == 5 || M.mp.hardening == 1.f) ? 1 : 0;
+  float* Fe = in->Fe ? in->Fe : ctx->Fe;
+  if (M.N > 0)
+    hipLaunchKernelGGL(k_p2g, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, in->v, in->C, in->F, Fe,
+                       ctx->grid_in, write_params);
+  hipLaunchKernelGGL(k_grid, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, ctx->grid_in, ctx->v_out, 1);
+  if (M.N > 0)
+    hipLaunchKernelGGL(k_g2p, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, Fe, ctx->v_out, out->x, out->v,
+                       out->C, out->F);
+  HIP_TRY(hipGetLastError());
+  return EMPM_OK;
+}
+
+int launch_adj(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, float dt, const empm_particles* in,
+               const empm_particles* d_out, const empm_particles* d_in, float* d_mu, float* d_lam, float* d_ys,
+               float* d_mass, float* d_vol, int flags, hipStream_t st) {
+  if (M.N > 0)
+    hipLaunchKernelGGL(k_p2g, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, in->v, in->C, in->F, ctx->Fe,
+                       ctx->grid_in, 0);
+  hipLaunchKernelGGL(k_grid, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, ctx->grid_in, ctx->v_out, 0);
+  if (M.N > 0)
+    hipLaunchKernelGGL(k_g2p_adj, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, ctx->Fe, ctx->v_out,
+                       d_out->x, d_out->v, d_out->C, d_out->F, ctx->xbar, ctx->Febar, ctx->gadj_v);
+  hipLaunchKernelGGL(k_grid_adj, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, ctx->grid_in, ctx->gadj_v,
+                     ctx->gadj_in, (flags & EMPM_ADJ_DROP_GRID_MASS) ? 1 : 0);
+  if (M.N > 0)
+    hipLaunchKernelGGL(k_p2g_adj, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, in->v, in->C, in->F,
+                       ctx->gadj_in, ctx->xbar, ctx->Febar, d_in->x, d_in->v, d_in->C, d_in->F, d_mu, d_lam, d_ys,
+                       d_mass, d_vol);
+  HIP_TRY(hipGetLastError());
+  return EMPM_OK;
+}
+
+int check_adj_supported(const empm_model* m) {
+  if (m->material == 5)
+    return set_err(EMPM_E_UNSUPPORTED,
+                   "adjoint of material 5 (plasticine) is not supported: it rewrites yield stress, mu and lam "
+                   "inside the substep");
+  if (m->hardening == 1.f && (m->material == 1 || m->material == 7))
+    return set_err(EMPM_E_UNSUPPORTED, "adjoint with hardening == 1 is not supported (yield stress is state)");
+  return EMPM_OK;
+}
+
+void advance_colliders(empm_collider* cols, int ncol, double t, double dt) {
+  for (int k = 0; k < ncol; ++k) {
+    empm_collider& c = cols[k];
+    if (c.kind != 1 && c.kind != 4) continue;
+    if (t >= (double)c.start_time && t < (double)c.end_time)
+      for (int d = 0; d < 3; ++d) c.point[d] = (float)((double)c.point[d] + dt * (double)c.velocity[d]);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int empm_abi_version(void) { return EMPM_ABI_VERSION; }
+
+const char* empm_last_error(void) { return g_err; }
+
+int empm_create(empm_ctx** out, const empm_create_desc* desc) {
+  if (!out || !desc) return set_err(EMPM_E_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (desc->n_grid < 4 || desc->n_grid > 1024)
+    return set_err(EMPM_E_INVALID_ARG, "n_grid=%d outside [4, 1024]", desc->n_grid);
+  if (desc->max_particles < 0) return set_err(EMPM_E_INVALID_ARG, "max_particles < 0");
+  HIP_TRY(hipSetDevice(desc->device));
+  empm_ctx* c = new empm_ctx();
+  memset(c, 0, sizeof(*c));
+  c->device = desc->device;
+  c->n_grid = desc->n_grid;
+  c->max_particles = desc->max_particles;
+  c->G = (size_t)desc->n_grid * desc->n_grid * desc->n_grid;
+  const size_t np = desc->max_particles > 0 ? (size_t)desc->max_particles : 1;
+  hipError_t e = hipSuccess;
+  e = e == hipSuccess ? hipMalloc((void**)&c->grid_in, c->G * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->v_out, c->G * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->gadj_v, c->G * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->gadj_in, c->G * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->Fe, np * 9 * sizeof(float)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->xbar, np * 3 * sizeof(float)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->Febar, np * 9 * sizeof(float)) : e;
+  e = e == hipSuccess ? hipMemset(c->grid_in, 0, c->G * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMemset(c->gadj_v, 0, c->G * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipDeviceSynchronize() : e;
+  if (e != hipSuccess) {
+    empm_destroy(c);
+    return set_err(e == hipErrorOutOfMemory ? EMPM_E_OOM : EMPM_E_HIP, "empm_create: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return EMPM_OK;
+}
+
+void empm_destroy(empm_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipDeviceSynchronize();
+  hipFree(c->grid_in);
+  hipFree(c->v_out);
+  hipFree(c->gadj_v);
+  hipFree(c->gadj_in);
+  hipFree(c->Fe);
+  hipFree(c->xbar);
+  hipFree(c->Febar);
+  delete c;
+}
+
+int empm_mu_lam(int n, const float* E, const float* nu, float* mu, float* lam, void* stream) {
+  if (n < 0 || (n > 0 && (!E || !nu || !mu || !lam))) return set_err(EMPM_E_INVALID_ARG, "empm_mu_lam: bad arguments");
+  if (n == 0) return EMPM_OK;
+  hipLaunchKernelGGL(k_mu_lam, dim3(nblocks(n)), dim3(kBlock), 0, (hipStream_t)stream, n, E, nu, mu, lam);
+  HIP_TRY(hipGetLastError());
+  return EMPM_OK;
+}
+
+int empm_mu_lam_adj(int n, const float* E, const float* nu, const float* dmu, const float* dlam, float* dE, float* dnu,
+                    void* stream) {
+  if (n < 0 || (n > 0 && (!E || !nu))) return set_err(EMPM_E_INVALID_ARG, "empm_mu_lam_adj: bad arguments");
+  if (n == 0) return EMPM_OK;
+  hipLaunchKernelGGL(k_mu_lam_adj, dim3(nblocks(n)), dim3(kBlock), 0, (hipStream_t)stream, n, E, nu, dmu, dlam, dE, dnu);
+  HIP_TRY(hipGetLastError());
+  return EMPM_OK;
+}
+
+int empm_mass_from_density(int n, const float* density, const float* vol, float* mass, void* stream) {
+  if (n < 0 || (n > 0 && (!density || !vol || !mass)))
+    return set_err(EMPM_E_INVALID_ARG, "empm_mass_from_density: bad arguments");
+  if (n == 0) return EMPM_OK;
+  hipLaunchKernelGGL(k_mass, dim3(nblocks(n)), dim3(kBlock), 0, (hipStream_t)stream, n, density, vol, mass);
+  HIP_TRY(hipGetLastError());
+  return EMPM_OK;
+}
+
+int empm_substep_fwd(empm_ctx* ctx, const empm_model* model, const empm_collider* cols, int ncol, float time, float dt,
+                     const empm_particles* in, const empm_particles* out, void* stream) {
+  int rc = check_model(ctx, model);
+  if (rc) return rc;
+  if ((rc = check_particles(in, model->n_particles, "in")) || (rc = check_particles(out, model->n_particles, "out")))
+    return rc;
+  if (model->n_particles > 0 && (in->x == out->x || in->F == out->F))
+    return set_err(EMPM_E_INVALID_ARG, "in and out states alias");
+  ColTable T;
+  if ((rc = make_table(cols, ncol, T, model->n_grid))) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  return launch_fwd(ctx, dev_model(model), T, time, dt, in, out, (hipStream_t)stream);
+}
+
+int empm_substep_adj(empm_ctx* ctx, const empm_model* model, const empm_collider* cols, int ncol, float time, float dt,
+                     const empm_particles* in, const empm_particles* d_out, const empm_particles* d_in, float* d_mu,
+                     float* d_lam, float* d_ys, float* d_mass, float* d_vol, int flags, void* stream) {
+  int rc = check_model(ctx, model);
+  if (rc) return rc;
+  if ((rc = check_adj_supported(model))) return rc;
+  if ((rc = check_particles(in, model->n_particles, "in")) ||
+      (rc = check_particles(d_out, model->n_particles, "d_out")) ||
+      (rc = check_particles(d_in, model->n_particles, "d_in")))
+    return rc;
+  ColTable T;
+  if ((rc = make_table(cols, ncol, T, model->n_grid))) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  return launch_adj(ctx, dev_model(model), T, time, dt, in, d_out, d_in, d_mu, d_lam, d_ys, d_mass, d_vol, flags,
+                    (hipStream_t)stream);
+}
+
+int empm_rollout_fwd(empm_ctx* ctx, const empm_model* model, empm_collider* cols, int ncol, double* time, double dt,
+                     int n_steps, const empm_particles* states, empm_collider* cols_hist, double* times_hist,
+                     void* stream) {
+  int rc = check_model(ctx, model);
+  if (rc) return rc;
+  if (!time || !states || n_steps < 0) return set_err(EMPM_E_INVALID_ARG, "empm_rollout_fwd: bad arguments");
+  for (int k = 0; k <= n_steps; ++k)
+    if ((rc = check_particles(&states[k], model->n_particles, "states[k]"))) return rc;
+  for (int k = 0; k < n_steps; ++k)
+    if (model->n_particles > 0 && states[k].x == states[k + 1].x)
+      return set_err(EMPM_E_INVALID_ARG, "states[%d] and states[%d] alias", k, k + 1);
+  ColTable T;
+  if ((rc = make_table(cols, ncol, T, model->n_grid))) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const DevModel M = dev_model(model);
+  for (int k = 0; k < n_steps; ++k) {
+    make_table(cols, ncol, T, model->n_grid);
+    if (cols_hist) memcpy(&cols_hist[(size_t)k * ncol], cols, sizeof(empm_collider) * ncol);
+    if (times_hist) times_hist[k] = *time;
+    if ((rc = launch_fwd(ctx, M, T, (float)*time, (float)dt, &states[k], &states[k + 1], (hipStream_t)stream)))
+      return rc;
+    advance_colliders(cols, ncol, *time, dt);
+    *time = *time + dt;
+  }
+  return EMPM_OK;
+}
+
+int empm_rollout_fwd_replay(empm_ctx* ctx, const empm_model* model, const empm_collider* cols_hist, int ncol,
+                            const double* times_hist, double dt, int n_steps, const empm_particles* states,
+                            void* stream) {
+  int rc = check_model(ctx, model);
+  if (rc) return rc;
+  if (!states || !times_hist || n_steps < 0 || (ncol > 0 && !cols_hist))
+    return set_err(EMPM_E_INVALID_ARG, "empm_rollout_fwd_replay: bad arguments");
+  for (int k = 0; k <= n_steps; ++k)
+    if ((rc = check_particles(&states[k], model->n_particles, "states[k]"))) return rc;
+  for (int k = 0; k < n_steps; ++k)
+    if (model->n_particles > 0 && states[k].x == states[k + 1].x)
+      return set_err(EMPM_E_INVALID_ARG, "states[%d] and states[%d] alias", k, k + 1);
+  HIP_TRY(hipSetDevice(ctx->device));
+  const DevModel M = dev_model(model);
+  ColTable T;
+  for (int k = 0; k < n_steps; ++k) {
+    if ((rc = make_table(&cols_hist[(size_t)k * ncol], ncol, T, model->n_grid))) return rc;
+    if ((rc = launch_fwd(ctx, M, T, (float)times_hist[k], (float)dt, &states[k], &states[k + 1], (hipStream_t)stream)))
+      return rc;
+  }
+  return EMPM_OK;
+}
+
+int empm_rollout_adj(empm_ctx* ctx, const empm_model* model, const empm_collider* cols_hist, int ncol,
+                     const double* times_hist, double dt, int n_steps, const empm_particles* states,
+                     const empm_particles* adj, float* d_mu, float* d_lam, float* d_ys, float* d_mass, float* d_vol,
+                     int flags, void* stream) {
+  int rc = check_model(ctx, model);
+  if (rc) return rc;
+  if ((rc = check_adj_supported(model))) return rc;
+  if (!states || !adj || !times_hist || n_steps < 0 || (ncol > 0 && !cols_hist))
+    return set_err(EMPM_E_INVALID_ARG, "empm_rollout_adj: bad arguments");
+  for (int k = 0; k < n_steps; ++k)
+    if ((rc = check_particles(&states[k], model->n_particles, "states[k]"))) return rc;
+  if ((rc = check_particles(&adj[0], model->n_particles, "adj[0]")) ||
+      (rc = check_particles(&adj[1], model->n_particles, "adj[1]")))
+    return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const DevModel M = dev_model(model);
+  ColTable T;
+  for (int j = 0; j < n_steps; ++j) {
+    const int k = n_steps - 1 - j;
+    if ((rc = make_table(&cols_hist[(size_t)k * ncol], ncol, T, model->n_grid))) return rc;
+    const empm_particles* dout = &adj[j % 2];
+    const empm_particles* din = &adj[(j + 1) % 2];
+    if ((rc = launch_adj(ctx, M, T, (float)times_hist[k], (float)dt, &states[k], dout, din, d_mu, d_lam, d_ys, d_mass,
+                         d_vol, flags, (hipStream_t)stream)))
+      return rc;
+  }
+  return EMPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,354 @@
+"""Runtime around libempm: device context, particle states, and the
+checkpointed forward / reverse rollout drivers.
+
+Everything numerical runs in the HIP kernels behind ``empm_amd._lib``; this
+module only owns buffers (torch tensors used as device memory), builds the
+C-ABI descriptors and sequences the calls.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import torch
+
+from . import _lib
+from ._lib import Collider, CreateDesc, Model, Particles, check
+
+F32 = torch.float32
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    if t is None:
+        return None
+    if t.dtype not in (torch.float32, torch.int32):
+        raise RuntimeError(f"expected float32/int32 tensor, got {t.dtype}")
+    if not t.is_contiguous():
+        raise RuntimeError("tensor must be contiguous")
+    if t.device.type != "cuda":
+        raise RuntimeError(f"tensor must live on a GPU, got {t.device}")
+    return t.data_ptr()
+
+
+def stream_handle(device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+class Context:
+    """An empm_ctx: grid buffers + per-particle scratch on one device."""
+
+    def __init__(self, n_grid: int, max_particles: int, device="cuda:0"):
+        self.lib = _lib.load()
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise RuntimeError(f"libempm runs on a GPU; got device {device}")
+        idx = self.device.index if self.device.index is not None else torch.cuda.current_device()
+        self.device = torch.device("cuda", idx)
+        self.n_grid = int(n_grid)
+        self.max_particles = int(max_particles)
+        desc = CreateDesc(device=idx, n_grid=self.n_grid, max_particles=self.max_particles)
+        h = C.c_void_p()
+        check(self.lib.empm_create(C.byref(h), C.byref(desc)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.lib.empm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ParticleState:
+    """x (N,3), v (N,3), C (N,3,3), F (N,3,3) = F_trial, optional Fe (N,3,3)."""
+
+    __slots__ = ("x", "v", "C", "F", "Fe")
+
+    def __init__(self, x, v, C_, F, Fe=None):
+        self.x, self.v, self.C, self.F, self.Fe = x, v, C_, F, Fe
+
+    @staticmethod
+    def empty(n, device, with_fe=False):
+        z3 = lambda: torch.zeros((n, 3), dtype=F32, device=device)
+        z9 = lambda: torch.zeros((n, 3, 3), dtype=F32, device=device)
+        return ParticleState(z3(), z3(), z9(), z9(), z9() if with_fe else None)
+
+    def zero_(self):
+        for t in (self.x, self.v, self.C, self.F):
+            t.zero_()
+        return self
+
+    def copy_(self, o: "ParticleState"):
+        self.x.copy_(o.x)
+        self.v.copy_(o.v)
+        self.C.copy_(o.C)
+        self.F.copy_(o.F)
+        return self
+
+    def clone(self):
+        return ParticleState(self.x.clone(), self.v.clone(), self.C.clone(), self.F.clone())
+
+    def desc(self) -> Particles:
+        return Particles(x=_ptr(self.x), v=_ptr(self.v), C=_ptr(self.C), F=_ptr(self.F),
+                         Fe=_ptr(self.Fe) if self.Fe is not None else None)
+
+
+@dataclass
+class ModelArrays:
+    """Per-particle arrays + scalars that become an empm_model."""
+    material: int
+    n_grid: int
+    grid_lim: float
+    dx: float
+    inv_dx: float
+    mu: torch.Tensor
+    lam: torch.Tensor
+    yield_stress: torch.Tensor
+    mass: torch.Tensor
+    vol: torch.Tensor
+    selection: torch.Tensor | None = None
+    rpic_damping: float = 0.0
+    grid_v_damping_scale: float = 1.1
+    gravity: tuple = (0.0, 0.0, 0.0)
+    alpha: float = 0.0
+    hardening: float = 0.0
+    xi: float = 0.0
+    plastic_viscosity: float = 0.0
+    softening: float = 0.1
+
+    def struct(self) -> Model:
+        m = Model()
+        m.n_particles = int(self.mu.shape[0])
+        m.n_grid = int(self.n_grid)
+        m.material = int(self.material)
+        m.grid_lim = float(self.grid_lim)
+        m.dx = float(self.dx)
+        m.inv_dx = float(self.inv_dx)
+        m.rpic_damping = float(self.rpic_damping)
+        m.grid_v_damping_scale = float(self.grid_v_damping_scale)
+        for i in range(3):
+            m.gravity[i] = float(self.gravity[i])
+        m.alpha = float(self.alpha)
+        m.hardening = float(self.hardening)
+        m.xi = float(self.xi)
+        m.plastic_viscosity = float(self.plastic_viscosity)
+        m.softening = float(self.softening)
+        m.mu = _ptr(self.mu)
+        m.lam = _ptr(self.lam)
+        m.yield_stress = _ptr(self.yield_stress)
+        m.mass = _ptr(self.mass)
+        m.vol = _ptr(self.vol)
+        m.selection = _ptr(self.selection) if self.selection is not None else None
+        return m
+
+
+def collider_array(cols: list[Collider]):
+    n = len(cols)
+    if n > _lib.EMPM_MAX_COLLIDERS:
+        raise ValueError(f"at most {_lib.EMPM_MAX_COLLIDERS} grid post-processes are supported, got {n}")
+    arr = (Collider * max(n, 1))()
+    for i, c in enumerate(cols):
+        arr[i] = c
+    return arr, n
+
+
+def mu_lam(E: torch.Tensor, nu: torch.Tensor, mu: torch.Tensor, lam: torch.Tensor):
+    lib = _lib.load()
+    check(lib.empm_mu_lam(int(E.shape[0]), _ptr(E), _ptr(nu), _ptr(mu), _ptr(lam), stream_handle(E.device)))
+
+
+def mu_lam_adj(E, nu, dmu, dlam, dE, dnu):
+    lib = _lib.load()
+    check(lib.empm_mu_lam_adj(int(E.shape[0]), _ptr(E), _ptr(nu), _ptr(dmu), _ptr(dlam), _ptr(dE), _ptr(dnu),
+                              stream_handle(E.device)))
+
+
+def mass_from_density(rho, vol, mass):
+    lib = _lib.load()
+    check(lib.empm_mass_from_density(int(rho.shape[0]), _ptr(rho), _ptr(vol), _ptr(mass), stream_handle(rho.device)))
+
+
+def substep_fwd(ctx: Context, model: ModelArrays, cols: list[Collider], time: float, dt: float,
+                s_in: ParticleState, s_out: ParticleState):
+    arr, n = collider_array(cols)
+    m = model.struct()
+    check(ctx.lib.empm_substep_fwd(ctx.handle, C.byref(m), arr, n, C.c_float(time), C.c_float(dt),
+                                   C.byref(s_in.desc()), C.byref(s_out.desc()), stream_handle(ctx.device)))
+
+
+def substep_adj(ctx: Context, model: ModelArrays, cols: list[Collider], time: float, dt: float,
+                s_in: ParticleState, d_out: ParticleState, d_in: ParticleState, grads: "ParamGrads", flags=0):
+    arr, n = collider_array(cols)
+    m = model.struct()
+    check(ctx.lib.empm_substep_adj(ctx.handle, C.byref(m), arr, n, C.c_float(time), C.c_float(dt),
+                                   C.byref(s_in.desc()), C.byref(d_out.desc()), C.byref(d_in.desc()),
+                                   *grads.ptrs(), int(flags), stream_handle(ctx.device)))
+
+
+@dataclass
+class ParamGrads:
+    mu: torch.Tensor
+    lam: torch.Tensor
+    ys: torch.Tensor
+    mass: torch.Tensor
+    vol: torch.Tensor
+
+    @staticmethod
+    def zeros(n, device):
+        z = lambda: torch.zeros(n, dtype=F32, device=device)
+        return ParamGrads(z(), z(), z(), z(), z())
+
+    def ptrs(self):
+        return (_ptr(self.mu), _ptr(self.lam), _ptr(self.ys), _ptr(self.mass), _ptr(self.vol))
+
+
+@dataclass
+class RolloutTape:
+    """What a checkpointed reverse sweep needs: the state every `ckpt_every`
+    substeps, the collider table and time of every substep."""
+    n_steps: int
+    dt: float
+    ckpt_every: int
+    ncol: int
+    checkpoints: dict = field(default_factory=dict)  # step -> ParticleState
+    cols_hist: object = None  # (Collider * (n_steps*ncol))
+    times_hist: object = None  # (c_double * n_steps)
+
+
+def _states_array(states: list[ParticleState]):
+    arr = (Particles * len(states))()
+    for i, s in enumerate(states):
+        arr[i] = s.desc()
+    return arr
+
+
+class Rollout:
+    """Forward and reverse rollouts of one scene on one context.
+
+    ``forward`` advances ``state`` (ParticleState, in place via two ping-pong
+    buffers) by ``n_steps`` substeps, returning the copies of x requested by
+    ``record`` (a sorted list of substep counts k >= 1: x after k substeps) and,
+    when ``tape`` is requested, a RolloutTape for ``backward``."""
+
+    def __init__(self, ctx: Context, model: ModelArrays, n_particles: int):
+        self.ctx = ctx
+        self.model = model
+        self.n = int(n_particles)
+        dev = ctx.device
+        self._buf = [ParticleState.empty(self.n, dev), ParticleState.empty(self.n, dev)]
+
+    def forward(self, state: ParticleState, cols: list[Collider], time: float, dt: float, n_steps: int,
+                record=(), tape: bool = False, ckpt_every: int = 50, cols_out: list | None = None):
+        """Returns (final_state, final_time, recorded_x list, tape or None).
+        `cols` (ctypes Collider structs) are advanced in place like the
+        reference's modify_bc; `cols_out` receives the advanced table."""
+        lib = self.ctx.lib
+        arr, ncol = collider_array(cols)
+        m = self.model.struct()
+        t = C.c_double(time)
+        rec = sorted(set(int(k) for k in record if 1 <= int(k) <= n_steps))
+        tp = None
+        if tape:
+            tp = RolloutTape(n_steps=n_steps, dt=dt, ckpt_every=max(1, int(ckpt_every)), ncol=ncol)
+            tp.cols_hist = (Collider * max(1, n_steps * ncol))()
+            tp.times_hist = (C.c_double * max(1, n_steps))()
+        stops = set(rec)
+        if tape:
+            stops.update(range(tp.ckpt_every, n_steps, tp.ckpt_every))
+        stops.add(n_steps)
+        stops = sorted(s for s in stops if s >= 1)
+        cur = state
+        k = 0
+        out_x = []
+        stream = stream_handle(self.ctx.device)
+        if tape:
+            tp.checkpoints[0] = state.clone()
+        for stop in stops:
+            n = stop - k
+            if n <= 0:
+                continue
+            # ping-pong: cur -> buf[0] -> buf[1] -> ...
+            seq = [cur]
+            for _ in range(n):
+                seq.append(self._buf[0] if seq[-1] is not self._buf[0] else self._buf[1])
+            sarr = _states_array(seq)
+            ch = C.cast(C.byref(tp.cols_hist, k * ncol * C.sizeof(Collider)), C.POINTER(Collider)) if tape else None
+            th = C.cast(C.byref(tp.times_hist, k * C.sizeof(C.c_double)), C.POINTER(C.c_double)) if tape else None
+            check(lib.empm_rollout_fwd(self.ctx.handle, C.byref(m), arr, ncol, C.byref(t), C.c_double(dt), n,
+                                       sarr, ch, th, stream))
+            cur = seq[-1]
+            k = stop
+            if k in rec:
+                out_x.append(cur.x.clone())
+            if tape and k < n_steps and k % tp.ckpt_every == 0:
+                tp.checkpoints[k] = cur.clone()
+        if cols_out is not None:
+            cols_out[:] = [arr[i] for i in range(ncol)]
+        return cur, t.value, out_x, tp
+
+    def backward(self, tp: RolloutTape, d_final: ParticleState, seeds: dict | None = None, flags: int = 0,
+                 grads: ParamGrads | None = None):
+        """Reverse sweep.  d_final: dL/d(state n_steps).  seeds: {k: ParticleState
+        cotangent of state k} added when the sweep reaches state k (0 < k < n).
+        Returns (dL/d(state 0) as ParticleState, ParamGrads)."""
+        lib = self.ctx.lib
+        dev = self.ctx.device
+        n = self.n
+        if grads is None:
+            grads = ParamGrads.zeros(n, dev)
+        seeds = seeds or {}
+        m = self.model.struct()
+        stream = stream_handle(dev)
+        ncol = tp.ncol
+        K = tp.ckpt_every
+        adj = [ParticleState.empty(n, dev), ParticleState.empty(n, dev)]
+        adj[0].copy_(d_final)
+        seg_cache = [ParticleState.empty(n, dev) for _ in range(min(K, tp.n_steps) + 1)]
+        seg_starts = sorted(tp.checkpoints.keys())
+        k_hi = tp.n_steps
+        for s0 in reversed(seg_starts):
+            if s0 >= k_hi:
+                continue
+            L = k_hi - s0
+            states = [tp.checkpoints[s0]] + seg_cache[1:L + 1]
+            sarr = _states_array(states)
+            ch = C.cast(C.byref(tp.cols_hist, s0 * ncol * C.sizeof(Collider)), C.POINTER(Collider))
+            th = C.cast(C.byref(tp.times_hist, s0 * C.sizeof(C.c_double)), C.POINTER(C.c_double))
+            if L > 1:
+                check(lib.empm_rollout_fwd_replay(self.ctx.handle, C.byref(m), ch, ncol, th, C.c_double(tp.dt), L - 1,
+                                                  _states_array(states[:L]), stream))
+            # reverse over [s0, k_hi) in chunks split at seed points
+            k = k_hi
+            while k > s0:
+                if k in seeds and k != tp.n_steps:
+                    sd = seeds[k]
+                    adj[0].x.add_(sd.x)
+                    adj[0].v.add_(sd.v)
+                    adj[0].C.add_(sd.C)
+                    adj[0].F.add_(sd.F)
+                j = max([s for s in seeds if s0 < s < k] + [s0])
+                cnt = k - j
+                sub = _states_array(states[j - s0:k - s0])
+                chj = C.cast(C.byref(tp.cols_hist, j * ncol * C.sizeof(Collider)), C.POINTER(Collider))
+                thj = C.cast(C.byref(tp.times_hist, j * C.sizeof(C.c_double)), C.POINTER(C.c_double))
+                check(lib.empm_rollout_adj(self.ctx.handle, C.byref(m), chj, ncol, thj, C.c_double(tp.dt), cnt,
+                                           sub, _states_array(adj), *grads.ptrs(), int(flags), stream))
+                if cnt % 2 == 1:
+                    adj = [adj[1], adj[0]]
+                k = j
+            k_hi = s0
+        if 0 in seeds:
+            sd = seeds[0]
+            adj[0].x.add_(sd.x)
+            adj[0].v.add_(sd.v)
+            adj[0].C.add_(sd.C)
+            adj[0].F.add_(sd.F)
+        return adj[0], grads

@@ -1,0 +1,159 @@
+"""Seeded synthetic scenes for the benchmark workloads and parity tests.
+
+Every generator uses numpy's PCG64 with an explicit seed and returns host
+(numpy) arrays, so the same inputs can be fed to the GPU path and to the CPU
+checker.  Axis convention: z is up, gravity (0, 0, -9.8), as the reference's
+WarpMPMWrapper (warp_mpm_wrapper.py:110).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Scene:
+    name: str
+    n_grid: int
+    grid_lim: float
+    dt: float
+    n_steps: int
+    material: int
+    x: np.ndarray            # (N,3) float32
+    vol: np.ndarray          # (N,) float32
+    rho: np.ndarray          # (N,) float32
+    E: np.ndarray            # (N,) float32
+    nu: np.ndarray           # (N,) float32
+    yield_stress: np.ndarray  # (N,) float32
+    gravity: tuple = (0.0, 0.0, -9.8)
+    friction_angle: float = 25.0
+    softening: float = 0.1
+    grid_v_damping_scale: float = 1.1
+    rpic_damping: float = 0.0
+    # colliders as reference-style dicts: {"kind": "surface"|"cuboid"|"bbox", ...}
+    colliders: list = field(default_factory=list)
+    params: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return int(self.x.shape[0])
+
+    @property
+    def dx(self):
+        return self.grid_lim / self.n_grid
+
+
+def sticky_walls(n_grid, cells=3, grid_lim=1.0):
+    """'k-cell sticky ground/walls': v = 0 on nodes with any index < k or
+    >= n-k, as six sticky surface colliders (strict `offset . n < 0` test,
+    mpm_solver_diff.py:489-491).  The planes sit half a cell away from the
+    nodes ((k-0.5) dx and (n-k-0.5) dx) so the node sets do not depend on
+    float rounding."""
+    dx = grid_lim / n_grid
+    lo = (cells - 0.5) * dx
+    hi = (n_grid - cells - 0.5) * dx
+    cols = []
+    for d in range(3):
+        nrm = [0.0, 0.0, 0.0]
+        nrm[d] = 1.0
+        p = [0.0, 0.0, 0.0]
+        p[d] = lo
+        cols.append({"kind": "surface", "point": tuple(p), "normal": tuple(nrm)})
+        nrm2 = [0.0, 0.0, 0.0]
+        nrm2[d] = -1.0
+        p2 = [0.0, 0.0, 0.0]
+        p2[d] = hi
+        cols.append({"kind": "surface", "point": tuple(p2), "normal": tuple(nrm2)})
+    return cols
+
+
+def jittered_cube(center, side, n_grid, ppc, rng, grid_lim=1.0):
+    """ppc particles per covered cell, uniformly jittered inside the cell."""
+    dx = grid_lim / n_grid
+    lo = np.asarray(center) - side / 2
+    hi = np.asarray(center) + side / 2
+    i0 = np.floor(lo / dx + 1e-9).astype(int)
+    i1 = np.ceil(hi / dx - 1e-9).astype(int)
+    pts = []
+    for i in range(i0[0], i1[0]):
+        for j in range(i0[1], i1[1]):
+            for k in range(i0[2], i1[2]):
+                cell_lo = np.array([i, j, k]) * dx
+                p = cell_lo + rng.random((ppc, 3)) * dx
+                m = np.all((p >= lo) & (p <= hi), axis=1)
+                pts.append(p[m])
+    return np.concatenate(pts, axis=0)
+
+
+def ellipsoid_points(center, radii, n, rng):
+    out = np.zeros((0, 3))
+    radii = np.asarray(radii, np.float64)
+    while out.shape[0] < n:
+        p = (rng.random((2 * n, 3)) * 2 - 1)
+        m = np.sum(p * p, axis=1) <= 1.0
+        out = np.concatenate([out, p[m] * radii + np.asarray(center)], axis=0)
+    return out[:n]
+
+
+def cube_c1(seed=0):
+    """Plain forward, reference CPU case: 32^3 grid on the unit cube, cube of
+    side 0.3 centred at (0.5, 0.4, 0.5) with 8 jittered particles per covered
+    cell, V = dx^3/8, rho = 1000, FCR E=2e5 nu=0.3, g = -9.8 (z), dt=1e-4,
+    200 substeps, 3-cell sticky walls."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = 32
+    dx = 1.0 / n
+    x = jittered_cube((0.5, 0.4, 0.5), 0.3, n, 8, rng)
+    N = x.shape[0]
+    f = lambda v: np.full(N, v, np.float32)
+    return Scene(name="cube_32^3_fcr", n_grid=n, grid_lim=1.0, dt=1e-4, n_steps=200, material=0,
+                 x=x.astype(np.float32), vol=f(dx ** 3 / 8), rho=f(1000.0), E=f(2e5), nu=f(0.3),
+                 yield_stress=f(0.0), colliders=sticky_walls(n, 3))
+
+
+def sample_params_c2(seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    logE = rng.uniform(np.log(1e4), np.log(1e6))
+    nu = rng.uniform(0.2, 0.45)
+    logY = rng.uniform(np.log(1e2), np.log(1e4))
+    return {"E": float(np.exp(logE)), "nu": float(nu), "yield_stress": float(np.exp(logY))}
+
+
+def ellipsoid_c2(n_particles=200_000, n_grid=64, n_steps=400, param_seed=3, material=7, seed=1):
+    """Single rollout forward + adjoint: 200k particles in an ellipsoid with
+    radii (0.2, 0.1, 0.15) centred at (0.5, 0.5, 0.3) on a 64^3 grid,
+    Neo-Hookean + von Mises (material 7), dt = 5e-5, 400 substeps,
+    rho = 1000, 3-cell sticky ground/walls.  Parameters drawn from
+    `param_seed` (log-uniform E in [1e4,1e6], uniform nu in [0.2,0.45],
+    log-uniform yield stress in [1e2,1e4])."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    radii = (0.2, 0.1, 0.15)
+    x = ellipsoid_points((0.5, 0.5, 0.3), radii, n_particles, rng)
+    N = x.shape[0]
+    vol_total = 4.0 / 3.0 * np.pi * radii[0] * radii[1] * radii[2]
+    prm = sample_params_c2(param_seed)
+    f = lambda v: np.full(N, v, np.float32)
+    return Scene(name=f"ellipsoid_{N}_{n_grid}^3_nh_vm", n_grid=n_grid, grid_lim=1.0, dt=5e-5, n_steps=n_steps,
+                 material=material, x=x.astype(np.float32), vol=f(vol_total / N), rho=f(1000.0), E=f(prm["E"]),
+                 nu=f(prm["nu"]), yield_stress=f(prm["yield_stress"]), colliders=sticky_walls(n_grid, 3),
+                 params=prm)
+
+
+def small_scene(n_particles=512, n_grid=16, material=0, seed=0, E=2e4, nu=0.3, ys=300.0, center=(0.5, 0.5, 0.45),
+                radii=(0.15, 0.12, 0.1), vel=(0.3, -0.2, 0.1), dt=1e-4):
+    """A small ellipsoid with an initial velocity and rotation, for parity
+    and finite-difference tests."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = ellipsoid_points(center, radii, n_particles, rng)
+    N = x.shape[0]
+    vol_total = 4.0 / 3.0 * np.pi * radii[0] * radii[1] * radii[2]
+    f = lambda v: np.full(N, v, np.float32)
+    sc = Scene(name=f"small_{N}_{n_grid}", n_grid=n_grid, grid_lim=1.0, dt=dt, n_steps=20, material=material,
+               x=x.astype(np.float32), vol=f(vol_total / N), rho=f(1000.0), E=f(E), nu=f(nu), yield_stress=f(ys),
+               colliders=sticky_walls(n_grid, 3))
+    c = np.asarray(center)
+    w = np.array([0.0, 0.0, 2.0])
+    v0 = np.asarray(vel)[None, :] + np.cross(w[None, :], x - c[None, :])
+    sc.params["v0"] = v0.astype(np.float32)
+    return sc

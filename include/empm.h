@@ -1,0 +1,211 @@
+/*
+ * empm.h — C-ABI of libempm.so, the MI355X (gfx950) implementation of the
+ * differentiable MLS-MPM substep of rai-opensource/empm and its adjoint.
+ *
+ * The reference runs this path as Warp kernels launched from Python; every
+ * entry point below replaces one of those launches (or a host loop of them)
+ * and says which.  Reference paths are relative to
+ * phys_sim/diff_simulator/warp_mpm/.
+ *
+ * Conventions
+ *  - All array arguments are DEVICE pointers (hipMalloc'd or torch CUDA
+ *    storage) unless marked "host".  Particle arrays are AoS float32 exactly
+ *    as the reference's torch/Warp views: x, v are (N,3); C, F are (N,3,3)
+ *    row-major.  Grid arrays are owned by the context.
+ *  - The caller owns every buffer it passes; the library never frees caller
+ *    memory.  All device allocation happens in empm_create.
+ *  - Every call is stream-ordered on `stream` (a hipStream_t, NULL = the
+ *    default stream).  A context is not thread-safe; distinct contexts are.
+ *  - Return 0 on success or a negative EMPM_E_* code; the message is in
+ *    empm_last_error() (thread-local).  No exception crosses the ABI.
+ */
+#ifndef EMPM_H
+#define EMPM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMPM_ABI_VERSION 1
+#define EMPM_MAX_COLLIDERS 16
+
+enum {
+    EMPM_OK = 0,
+    EMPM_E_INVALID_ARG = -1,
+    EMPM_E_HIP = -2,
+    EMPM_E_OOM = -3,
+    EMPM_E_UNSUPPORTED_MATERIAL = -4,
+    EMPM_E_NONFINITE = -5,
+    EMPM_E_UNSUPPORTED = -6
+};
+
+/* Adjoint flags */
+#define EMPM_ADJ_EXACT 0
+/* Drop the grid-mass adjoint, as the reference Tape does (grid_m has no
+ * grad buffer: mpm_data_structure.py:100-105 with warp_utils.py:116-120). */
+#define EMPM_ADJ_DROP_GRID_MASS 1
+
+typedef struct empm_ctx empm_ctx;
+
+typedef struct {
+    int device;        /* HIP device ordinal */
+    int n_grid;        /* grid resolution n (n^3 nodes) */
+    int max_particles; /* capacity for per-particle scratch */
+    int pad0;
+} empm_create_desc;
+
+/* MPMModelStruct (mpm_data_structure.py:390-422) plus the per-particle mass,
+ * volume and selection of MPMStateStruct (mpm_data_structure.py:13-39).
+ * material: 0 jelly (FCR), 1 metal (vM + StVK), 2 sand (DP), 3 foam
+ * (viscoplastic + StVK), 4 snow (no stress), 5 plasticine (vM with damage +
+ * FCR), 6 neo-hookean, 7 neo-hookean + von Mises (composition of 1's return
+ * map with 6's stress; not a reference id). */
+typedef struct {
+    int n_particles;
+    int n_grid;
+    int material;
+    int pad0;
+    float grid_lim;
+    float dx;
+    float inv_dx;
+    float rpic_damping;
+    float grid_v_damping_scale;
+    float gravity[3];
+    float alpha;
+    float hardening;
+    float xi;
+    float plastic_viscosity;
+    float softening;
+    float *mu;           /* N; written in place by material 5 */
+    float *lam;          /* N; written in place by material 5 */
+    float *yield_stress; /* N; written in place by material 5 / hardening */
+    const float *mass;   /* N */
+    const float *vol;    /* N */
+    const int *selection; /* N or NULL; only selection[p]==0 is simulated */
+} empm_model;
+
+/* One particle state: the fields p2g2p_differentiable reads and writes
+ * (particle_x, particle_v, particle_C, particle_F_trial).  Fe is optional:
+ * when non-NULL, a forward substep reading this state writes the
+ * return-mapped F there (the reference's particle_F, written by
+ * compute_stress_from_F_trial at mpm_utils.py:614-631); when NULL a context
+ * scratch buffer is used.  Adjoint entry points ignore it. */
+typedef struct {
+    float *x;  /* N*3 */
+    float *v;  /* N*3 */
+    float *C;  /* N*9 */
+    float *F;  /* N*9, F_trial */
+    float *Fe; /* N*9 or NULL */
+} empm_particles;
+
+/* One grid post-process, in registration order.  kind:
+ *   0 surface collider       add_surface_collider      mpm_solver_diff.py:437-531
+ *   1 velocity on cuboid     set_velocity_on_cuboid    mpm_solver_diff.py:814-869
+ *   2 bounding box           add_bounding_box          mpm_solver_diff.py:871-938
+ *   3 geometry collider      add_geometry_collider     mpm_solver_diff.py:533-683
+ *   4 grid geometry mover    set_grid_velocity_on_geometry mpm_solver_diff.py:685-804
+ *   5 mask                   enforce_grid_velocity_by_mask mpm_solver_diff.py:1264-1289
+ * geo_type: 1 cube, 2 sphere, 3 cylinder, 4 halfspace (kinds 3, 4).
+ * Kinds 1 and 4 move: the rollout entry points advance `point` by
+ * dt*velocity after each substep while start_time <= t < end_time
+ * (mpm_solver_diff.py:795-801,860-866). */
+typedef struct {
+    int kind;
+    int surface_type;
+    int geo_type;
+    int reset;
+    int active;
+    int pad0;
+    float start_time;
+    float end_time;
+    float point[3];
+    float normal[3];
+    float size[3];
+    float velocity[3];
+    float rot_axis[3];
+    float angular_velocity;
+    float radius;
+    float half_height;
+    float friction;
+    const int *mask; /* device, n^3 int32, kind 5 only */
+} empm_collider;
+
+/* ABI version (EMPM_ABI_VERSION). */
+int empm_abi_version(void);
+
+/* Message of the last failed call on this thread. */
+const char *empm_last_error(void);
+
+/* Allocate a context (grid buffers and per-particle scratch) on a device.
+ * Replaces the per-rollout/per-substep allocations of MPMStateStruct.init /
+ * init_grid / partial_clone (mpm_data_structure.py:41-117,356-387). */
+int empm_create(empm_ctx **out, const empm_create_desc *desc);
+
+/* Free a context (synchronises its device first). */
+void empm_destroy(empm_ctx *ctx);
+
+/* mu, lam from E, nu: compute_mu_lam_from_E_nu (mpm_utils.py:277-283). */
+int empm_mu_lam(int n, const float *E, const float *nu, float *mu, float *lam, void *stream);
+
+/* Adjoint of empm_mu_lam: dE += dmu dmu/dE + dlam dlam/dE (same for nu). */
+int empm_mu_lam_adj(int n, const float *E, const float *nu, const float *dmu, const float *dlam, float *dE, float *dnu,
+                    void *stream);
+
+/* mass = density * vol: get_float_array_product as launched at
+ * interface.py:125-134 (mpm_data_structure.py:719-726). */
+int empm_mass_from_density(int n, const float *density, const float *vol, float *mass, void *stream);
+
+/* One substep state `in` -> state `out` (`out` must not alias `in`):
+ * MPMWARPDiff.p2g2p_differentiable (mpm_solver_diff.py:165-291), i.e.
+ * zero_grid, compute_stress_from_F_trial, p2g_apic_with_stress,
+ * grid_normalization_and_gravity, add_damping_via_grid, the colliders `cols`
+ * (host array, evaluated at `time`) and g2p_differentiable.  Collider points
+ * are NOT advanced (the caller owns them, as the reference's modify_bc). */
+int empm_substep_fwd(empm_ctx *ctx, const empm_model *model, const empm_collider *cols, int ncol, float time,
+                     float dt, const empm_particles *in, const empm_particles *out, void *stream);
+
+/* Adjoint of empm_substep_fwd at state `in`: given d_out = dL/d(out), writes
+ * d_in = dL/d(in) (overwritten) and accumulates (+=) the parameter adjoints
+ * d_mu, d_lam, d_ys, d_mass, d_vol (any may be NULL).  The substep's grid is
+ * recomputed internally.  Replaces the Warp Tape replay of one substep's
+ * launches (interface.py:197, warp_utils.py:93-127).  flags: EMPM_ADJ_*.
+ * Materials 5 and hardening==1 mutate model arrays inside the substep and
+ * are rejected with EMPM_E_UNSUPPORTED. */
+int empm_substep_adj(empm_ctx *ctx, const empm_model *model, const empm_collider *cols, int ncol, float time,
+                     float dt, const empm_particles *in, const empm_particles *d_out, const empm_particles *d_in,
+                     float *d_mu, float *d_lam, float *d_ys, float *d_mass, float *d_vol, int flags, void *stream);
+
+/* n_steps substeps in one call: states[k] -> states[k+1] for k < n_steps
+ * (states: host array of n_steps+1 descriptors; consecutive entries must not
+ * alias, entries further apart may, e.g. a two-buffer ping-pong).  *time is
+ * advanced by dt per substep and moving colliders (kinds 1, 4) in `cols`
+ * (host, ncol entries) are advanced in place, as the reference host loop
+ * does (warp_mpm_wrapper.py:278-306, interface.py:137-145).  If cols_hist /
+ * times_hist are non-NULL (host, n_steps*ncol / n_steps entries) the
+ * collider table and time each substep saw are recorded for the adjoint. */
+int empm_rollout_fwd(empm_ctx *ctx, const empm_model *model, empm_collider *cols, int ncol, double *time, double dt,
+                     int n_steps, const empm_particles *states, empm_collider *cols_hist, double *times_hist,
+                     void *stream);
+
+/* Forward replay of n_steps substeps with the recorded per-substep collider
+ * tables and times of empm_rollout_fwd (no collider advance): the recompute
+ * step of checkpointed reverse mode (the reference keeps every state alive
+ * instead, interface.py:137-145). */
+int empm_rollout_fwd_replay(empm_ctx *ctx, const empm_model *model, const empm_collider *cols_hist, int ncol,
+                            const double *times_hist, double dt, int n_steps, const empm_particles *states,
+                            void *stream);
+
+/* Reverse sweep over n_steps substeps whose inputs are states[0..n_steps-1]
+ * (host descriptors), with the per-substep collider tables and times from
+ * empm_rollout_fwd.  On entry adj[0] holds dL/d(state n_steps); on return
+ * adj[n_steps % 2] holds dL/d(state 0).  Parameter adjoints accumulate. */
+int empm_rollout_adj(empm_ctx *ctx, const empm_model *model, const empm_collider *cols_hist, int ncol,
+                     const double *times_hist, double dt, int n_steps, const empm_particles *states,
+                     const empm_particles *adj, float *d_mu, float *d_lam, float *d_ys, float *d_mass, float *d_vol,
+                     int flags, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMPM_H */

@@ -1,0 +1,52 @@
+"""Shared helpers for the parity tests: seeded random substep inputs, the GPU
+path through the C-ABI (empm_amd.runtime) and the CPU oracle."""
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import oracle as orc
+
+MATERIAL_NAMES = {0: "jelly", 1: "metal", 2: "sand", 3: "foam", 4: "snow", 5: "plasticine", 6: "neo-hookean",
+                  7: "nh-vm"}
+
+
+def random_state(N, n_grid, seed, material, deform=0.05, vscale=0.5, cscale=5.0):
+    """Random particles in the interior of a grid of n_grid cells on [0,1]^3
+    (3..n-3 cells away from the walls), with velocities, affine C and
+    F_trial = R (I + deform * A) with a random rotation R."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dx = 1.0 / n_grid
+    x = rng.uniform(3.5 * dx, 1.0 - 3.5 * dx, (N, 3))
+    v = rng.normal(0, vscale, (N, 3))
+    Cm = rng.normal(0, cscale, (N, 3, 3))
+    A = rng.normal(0, 1.0, (N, 3, 3))
+    q = rng.normal(0, 1, (N, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, a, b, c = q.T
+    R = np.stack([
+        np.stack([1 - 2 * (b * b + c * c), 2 * (a * b - c * w), 2 * (a * c + b * w)], -1),
+        np.stack([2 * (a * b + c * w), 1 - 2 * (a * a + c * c), 2 * (b * c - a * w)], -1),
+        np.stack([2 * (a * c - b * w), 2 * (b * c + a * w), 1 - 2 * (a * a + b * b)], -1),
+    ], 1)
+    F = R @ (np.eye(3)[None] + deform * A)
+    E = rng.uniform(1e4, 1e5, N)
+    nu = rng.uniform(0.2, 0.4, N)
+    mu = E / (2 * (1 + nu))
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    ys = rng.uniform(50.0, 2000.0, N)
+    vol = np.full(N, (dx ** 3) / 8)
+    mass = vol * 1000.0
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    return dict(x=f32(x), v=f32(v), C=f32(Cm), F=f32(F), mu=f32(mu), lam=f32(lam), ys=f32(ys), mass=f32(mass),
+                vol=f32(vol), E=f32(E), nu=f32(nu))
+
+
+def oracle_state(d, precision):
+    return orc.OracleState(d["x"], d["v"], d["C"], d["F"], d["mu"], d["lam"], d["ys"], d["mass"], d["vol"],
+                           precision=precision)
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
