@@ -1,0 +1,259 @@
+"""Benchmark: particle-substeps/s of the differentiable MLS-MPM substep on one
+MI355X, forward + adjoint, on the single-rollout workload: 200k particles in
+an ellipsoid (radii 0.2, 0.1, 0.15) on a 64^3 grid, Neo-Hookean + von Mises,
+dt = 5e-5, 400 substeps, loss = mean squared distance of the final
+positions to a target rollout made with other parameters, output the
+gradient w.r.t. (log E, nu, log yield).
+
+One step = one forward rollout (checkpoint every 50 substeps) + loss + one
+reverse sweep (segment recompute + adjoint).  value = N * 400 / step time.
+
+Prints one JSON line (see --help).  --gpus N > 1 exits: this build runs on
+one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+CKPT_EVERY = 50
+
+
+def kernel_bytes(N, G):
+    """Algorithmic HBM bytes per launch of each kernel (FP32): the state,
+    constants and grid fields that kernel must read or write once."""
+    return {
+        # reads x, v, C, F_trial (96 B) + mass, vol, mu, lam, yield (20 B); writes (mv, m) once per node
+        "k_p2g": 116 * N + 16 * G,
+        # reads (mv, m), writes v_out (12 B) per node
+        "k_grid": 28 * G,
+        # reads x, F (48 B) and v_out once per node; writes x, v, C, F_trial (96 B)
+        "k_g2p": 144 * N + 12 * G,
+        # reads x, F (48), adjoint of the next state (96), v_out; writes partial adjoints (48) and dL/dv_out
+        "k_g2p_adj": 192 * N + 24 * G,
+        # reads (mv, m) and dL/dv_out, writes dL/d(mv, m)
+        "k_grid_adj": 44 * G,
+        # reads state + constants (116), partial adjoints (48), dL/d(mv, m) once per node;
+        # writes the adjoint of the state (96) and read-modify-writes 5 parameter adjoints (40)
+        "k_p2g_adj": 300 * N + 16 * G,
+    }
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--particles", type=int, default=200_000)
+    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--substeps", type=int, default=400)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs the MPM hot path on one GPU only; --gpus must be 1 "
+              "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from empm_amd import _lib
+    from empm_amd import runtime as rt
+    from empm_amd import scenes
+
+    dev = torch.device("cuda:0")
+    sc = scenes.ellipsoid_c2(n_particles=args.particles, n_grid=args.grid, n_steps=args.substeps, param_seed=3)
+    tprm = scenes.sample_params_c2(2)
+    N, n, T, dt = sc.n, sc.n_grid, sc.n_steps, sc.dt
+    G = n ** 3
+    log(f"[bench] workload {sc.name}: N={N} grid={n}^3 substeps={T} dt={dt} params={sc.params} target={tprm}")
+
+    def t(a):
+        return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=dev).clone()
+
+    def model_for(prm):
+        E = np.full(N, prm["E"])
+        nu = np.full(N, prm["nu"])
+        Et, nut = t(E), t(nu)
+        mu, lam = torch.empty_like(Et), torch.empty_like(Et)
+        rt.mu_lam(Et, nut, mu, lam)
+        dx = float(np.float32(1.0 / n))
+        return rt.ModelArrays(material=sc.material, n_grid=n, grid_lim=1.0, dx=dx, inv_dx=float(np.float32(n)),
+                              mu=mu, lam=lam, yield_stress=t(np.full(N, prm["yield_stress"])),
+                              mass=t(sc.vol * sc.rho), vol=t(sc.vol), gravity=sc.gravity), Et, nut
+
+    def cols():
+        from empm_amd.diff_simulator.warp_mpm.mpm_solver_diff import KIND_SURFACE
+        out = []
+        for c in sc.colliders:
+            g = _lib.Collider()
+            g.kind = KIND_SURFACE
+            g.start_time, g.end_time = 0.0, 999.0
+            for i in range(3):
+                g.point[i] = c["point"][i]
+                g.normal[i] = c["normal"][i]
+            out.append(g)
+        return out
+
+    x0 = t(sc.x)
+    s0 = rt.ParticleState(x0, torch.zeros_like(x0), torch.zeros((N, 3, 3), device=dev),
+                          torch.eye(3, device=dev).repeat(N, 1, 1).contiguous())
+    ctx = rt.Context(n, N, dev)
+
+    # target rollout (forward only, untimed)
+    tmodel, _, _ = model_for(tprm)
+    tro = rt.Rollout(ctx, tmodel, N)
+    tfinal, _, _, _ = tro.forward(s0, cols(), 0.0, dt, T)
+    target = tfinal.x.clone()
+
+    model, Et, nut = model_for(sc.params)
+    ro = rt.Rollout(ctx, model, N)
+
+    def step():
+        final, _, _, tape = ro.forward(s0, cols(), 0.0, dt, T, tape=True, ckpt_every=CKPT_EVERY)
+        diff = final.x - target
+        loss = (diff * diff).sum(1).mean()
+        d = rt.ParticleState.empty(N, dev)
+        d.x.copy_(diff * (2.0 / N))
+        _, grads = ro.backward(tape, d, flags=_lib.EMPM_ADJ_EXACT)
+        dE = torch.zeros(N, device=dev)
+        dnu = torch.zeros(N, device=dev)
+        rt.mu_lam_adj(Et, nut, grads.mu, grads.lam, dE, dnu)
+        g = torch.stack([(dE * Et).sum(), dnu.sum(), (grads.ys * model.yield_stress).sum()])
+        return loss, g
+
+    def fwd_only():
+        final, _, _, _ = ro.forward(s0, cols(), 0.0, dt, T)
+        return final
+
+    lib = ctx.lib
+    # profiling pass: time every kernel once to find the dominant one
+    lib.empm_profile_enable(ctx.handle, (1 << len(_lib.KERNEL_NAMES)) - 1)
+    step()
+    torch.cuda.synchronize()
+    per_kernel = {}
+    for k, name in enumerate(_lib.KERNEL_NAMES):
+        l, ms = C.c_long(), C.c_double()
+        _lib.check(lib.empm_profile_read(ctx.handle, k, C.byref(l), C.byref(ms)))
+        per_kernel[name] = (l.value, ms.value)
+    lib.empm_profile_enable(ctx.handle, 0)
+    _lib.check(lib.empm_profile_reset(ctx.handle))
+    dom = max(per_kernel, key=lambda k: per_kernel[k][1])
+    log("[bench] per-kernel (launches, ms) in one step: " +
+        ", ".join(f"{k}={v[0]}/{v[1]:.2f}" for k, v in per_kernel.items()) + f" -> dominant {dom}")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    # timed region; the dominant kernel is timed with events on its stream
+    lib.empm_profile_enable(ctx.handle, 1 << _lib.KERNEL_NAMES.index(dom))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss, g = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    l, ms = C.c_long(), C.c_double()
+    _lib.check(lib.empm_profile_read(ctx.handle, _lib.KERNEL_NAMES.index(dom), C.byref(l), C.byref(ms)))
+    lib.empm_profile_enable(ctx.handle, 0)
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    value = N * T / (ms_per_step / 1000.0)
+    avg_ms = ms.value / max(l.value, 1)
+    kb = kernel_bytes(N, G)[dom]
+    achieved = kb / (avg_ms / 1000.0) / 1e9
+    log(f"[bench] fwd+bwd: {ms_per_step:.2f} ms/step, {value:.4e} p-substeps/s; {dom}: {l.value} launches, "
+        f"avg {avg_ms * 1000:.1f} us, {achieved:.0f} GB/s; loss={float(loss):.6e} grad={g.tolist()}")
+
+    # forward-only throughput (reported beside the headline)
+    fwd_only()
+    torch.cuda.synchronize()
+    f0 = time.perf_counter()
+    for _ in range(args.steps):
+        fwd_only()
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - f0) * 1000.0 / args.steps
+    fwd_value = N * T / (fwd_ms / 1000.0)
+    log(f"[bench] fwd only: {fwd_ms:.2f} ms/rollout, {fwd_value:.4e} p-substeps/s")
+
+    B_fb = 776 * N + 196 * G + 192 * N / CKPT_EVERY
+    B_f = 212 * N + 56 * G
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(sc, args.cpu_seconds)
+
+    out = {
+        "metric": "particle-substeps/s (fwd+bwd) @200k pts/64^3",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid, random-init parameters)",
+        "config": {"workload": "single rollout fwd+adjoint: ellipsoid 200k particles, 64^3 grid, "
+                               "neo-hookean + von Mises, dt 5e-5, 400 substeps, checkpoint every 50",
+                   "n_particles": N, "n_grid": n, "substeps": T, "ckpt_every": CKPT_EVERY},
+        "fwd_value": fwd_value,
+        "fwd_ms_per_rollout": fwd_ms,
+        "hbm_roofline_fraction_fwd_bwd": (B_fb * T / (ms_per_step / 1000.0)) / 1e9 / HBM_PEAK_GBS,
+        "hbm_roofline_fraction_fwd": (B_f * T / (fwd_ms / 1000.0)) / 1e9 / HBM_PEAK_GBS,
+        "grad_logE_nu_logY": [float(v) for v in g.tolist()],
+        "loss": float(loss),
+        "per_kernel_ms_one_step": {k: v[1] for k, v in per_kernel.items()},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": kb,
+                     "avg_launch_us": avg_ms * 1000.0, "launches": l.value},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(sc, seconds):
+    """FP32 CPU oracle (a C restatement of the reference substep, OpenMP) on
+    the same workload's forward substeps, on the host cores."""
+    from oracle import oracle as orc
+
+    cores = min(os.cpu_count() or 1, 16)
+    orc.set_threads(cores)
+    N = sc.n
+    E, nu = sc.params["E"], sc.params["nu"]
+    mu = np.full(N, E / (2 * (1 + nu)))
+    lam = np.full(N, E * nu / ((1 + nu) * (1 - 2 * nu)))
+    P = orc.make_params(N, sc.n_grid, material=sc.material, gravity=sc.gravity)
+    cols = [orc.surface_collider(c["point"], c["normal"]) for c in sc.colliders]
+    s = orc.OracleState(sc.x, np.zeros((N, 3)), np.zeros((N, 3, 3)), np.tile(np.eye(3), (N, 1, 1)), mu, lam,
+                        sc.yield_stress, sc.vol * sc.rho, sc.vol, precision="f32")
+    t0 = time.perf_counter()
+    orc.rollout(P, cols, 0.0, sc.dt, 1, s)
+    one = time.perf_counter() - t0
+    k = int(max(1, min(200, seconds / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    orc.rollout(P, cols, sc.dt, sc.dt, k, s)
+    el = time.perf_counter() - t0
+    return {"value": N * k / el, "unit": "particle-substeps/s (forward only)", "cores": cores, "kind": "port",
+            "sample": f"{k} forward substeps of the {N}-particle 64^3 workload (substeps 2..{k + 1}) "
+                      f"on the FP32 CPU oracle, {el:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

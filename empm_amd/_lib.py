@@ -40,7 +40,13 @@ EXPORTED_SYMBOLS = (
     "empm_rollout_fwd",
     "empm_rollout_fwd_replay",
     "empm_rollout_adj",
+    "empm_profile_enable",
+    "empm_profile_read",
+    "empm_profile_reset",
+    "empm_kernel_name",
 )
+
+KERNEL_NAMES = ("k_p2g", "k_grid", "k_g2p", "k_g2p_adj", "k_grid_adj", "k_p2g_adj")
 
 
 class EmpmError(RuntimeError):
@@ -147,6 +153,14 @@ def _declare(lib):
     lib.empm_rollout_fwd_replay.restype = C.c_int
     lib.empm_rollout_fwd_replay.argtypes = [vp, P(Model), P(Collider), C.c_int, P(C.c_double), C.c_double,
                                             C.c_int, P(Particles), vp]
+    lib.empm_profile_enable.restype = C.c_int
+    lib.empm_profile_enable.argtypes = [vp, C.c_uint]
+    lib.empm_profile_read.restype = C.c_int
+    lib.empm_profile_read.argtypes = [vp, C.c_int, P(C.c_long), P(C.c_double)]
+    lib.empm_profile_reset.restype = C.c_int
+    lib.empm_profile_reset.argtypes = [vp]
+    lib.empm_kernel_name.restype = C.c_char_p
+    lib.empm_kernel_name.argtypes = [C.c_int]
     lib.empm_rollout_adj.restype = C.c_int
     lib.empm_rollout_adj.argtypes = [vp, P(Model), P(Collider), C.c_int, P(C.c_double), C.c_double,
                                      C.c_int, P(Particles), P(Particles),

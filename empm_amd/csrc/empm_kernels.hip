@@ -17,6 +17,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <utility>
+#include <vector>
 
 #include "empm.h"
 #include "empm_math.h"
@@ -703,6 +705,17 @@ __global__ void k_mass(int n, const float* __restrict__ rho, const float* __rest
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+// Per-kernel timing with HIP events recorded on the launch stream around the
+// kernels selected by a mask (instrumentation for the benchmark; off unless
+// empm_profile_enable is called).
+struct KernelProfile {
+  unsigned mask = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[EMPM_NUM_KERNELS];
+  std::vector<hipEvent_t> pool;
+  long launches[EMPM_NUM_KERNELS] = {};
+  double ms[EMPM_NUM_KERNELS] = {};
+};
+
 struct empm_ctx {
   int device;
   int n_grid;
@@ -715,9 +728,46 @@ struct empm_ctx {
   float* Fe;
   float* xbar;
   float* Febar;
+  KernelProfile* prof;
 };
 
 namespace {
+
+const char* const kKernelNames[EMPM_NUM_KERNELS] = {"k_p2g", "k_grid", "k_g2p", "k_g2p_adj", "k_grid_adj",
+                                                    "k_p2g_adj"};
+
+hipEvent_t pool_get(KernelProfile* p) {
+  if (!p->pool.empty()) {
+    hipEvent_t e = p->pool.back();
+    p->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// RAII guard: records a start event before and a stop event after the launch
+// it brackets, when the kernel is selected.
+struct ProfScope {
+  KernelProfile* p;
+  int id;
+  hipStream_t st;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(empm_ctx* ctx, int id_, hipStream_t st_) : p(ctx->prof), id(id_), st(st_) {
+    if (p && (p->mask & (1u << id))) {
+      a = pool_get(p);
+      b = pool_get(p);
+      if (a && b) (void)hipEventRecord(a, st);
+    }
+  }
+  ~ProfScope() {
+    if (a && b) {
+      (void)hipEventRecord(b, st);
+      p->pending[id].push_back({a, b});
+    }
+  }
+};
 
 int check_model(const empm_ctx* ctx, const empm_model* m) {
   if (!ctx || !m) return set_err(EMPM_E_INVALID_ARG, "null context or model");
@@ -790,13 +840,21 @@ int launch_fwd(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, 
                const empm_particles* out, hipStream_t st) {
   const int write_params = (M.material == 5 || M.mp.hardening == 1.f) ? 1 : 0;
   float* Fe = in->Fe ? in->Fe : ctx->Fe;
-  if (M.N > 0)
+  if (M.N > 0) {
+    ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
     hipLaunchKernelGGL(k_p2g, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, in->v, in->C, in->F, Fe,
                        ctx->grid_in, write_params);
-  hipLaunchKernelGGL(k_grid, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, ctx->grid_in, ctx->v_out, 1);
-  if (M.N > 0)
+  }
+  {
+    ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
+    hipLaunchKernelGGL(k_grid, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, ctx->grid_in, ctx->v_out,
+                       1);
+  }
+  if (M.N > 0) {
+    ProfScope ps(ctx, EMPM_KERNEL_G2P, st);
     hipLaunchKernelGGL(k_g2p, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, Fe, ctx->v_out, out->x, out->v,
                        out->C, out->F);
+  }
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }
@@ -804,19 +862,32 @@ int launch_fwd(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, 
 int launch_adj(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, float dt, const empm_particles* in,
                const empm_particles* d_out, const empm_particles* d_in, float* d_mu, float* d_lam, float* d_ys,
                float* d_mass, float* d_vol, int flags, hipStream_t st) {
-  if (M.N > 0)
+  if (M.N > 0) {
+    ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
     hipLaunchKernelGGL(k_p2g, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, in->v, in->C, in->F, ctx->Fe,
                        ctx->grid_in, 0);
-  hipLaunchKernelGGL(k_grid, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, ctx->grid_in, ctx->v_out, 0);
-  if (M.N > 0)
+  }
+  {
+    ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
+    hipLaunchKernelGGL(k_grid, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, ctx->grid_in, ctx->v_out,
+                       0);
+  }
+  if (M.N > 0) {
+    ProfScope ps(ctx, EMPM_KERNEL_G2P_ADJ, st);
     hipLaunchKernelGGL(k_g2p_adj, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, ctx->Fe, ctx->v_out,
                        d_out->x, d_out->v, d_out->C, d_out->F, ctx->xbar, ctx->Febar, ctx->gadj_v);
-  hipLaunchKernelGGL(k_grid_adj, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, ctx->grid_in, ctx->gadj_v,
-                     ctx->gadj_in, (flags & EMPM_ADJ_DROP_GRID_MASS) ? 1 : 0);
-  if (M.N > 0)
+  }
+  {
+    ProfScope ps(ctx, EMPM_KERNEL_GRID_ADJ, st);
+    hipLaunchKernelGGL(k_grid_adj, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, ctx->grid_in,
+                       ctx->gadj_v, ctx->gadj_in, (flags & EMPM_ADJ_DROP_GRID_MASS) ? 1 : 0);
+  }
+  if (M.N > 0) {
+    ProfScope ps(ctx, EMPM_KERNEL_P2G_ADJ, st);
     hipLaunchKernelGGL(k_p2g_adj, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, in->v, in->C, in->F,
                        ctx->gadj_in, ctx->xbar, ctx->Febar, d_in->x, d_in->v, d_in->C, d_in->F, d_mu, d_lam, d_ys,
                        d_mass, d_vol);
+  }
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }
@@ -885,6 +956,15 @@ void empm_destroy(empm_ctx* c) {
   if (!c) return;
   hipSetDevice(c->device);
   hipDeviceSynchronize();
+  if (c->prof) {
+    for (int k = 0; k < EMPM_NUM_KERNELS; ++k)
+      for (auto& pr : c->prof->pending[k]) {
+        hipEventDestroy(pr.first);
+        hipEventDestroy(pr.second);
+      }
+    for (hipEvent_t e : c->prof->pool) hipEventDestroy(e);
+    delete c->prof;
+  }
   hipFree(c->grid_in);
   hipFree(c->v_out);
   hipFree(c->gadj_v);
@@ -893,6 +973,54 @@ void empm_destroy(empm_ctx* c) {
   hipFree(c->xbar);
   hipFree(c->Febar);
   delete c;
+}
+
+int empm_profile_enable(empm_ctx* ctx, unsigned kernel_mask) {
+  if (!ctx) return set_err(EMPM_E_INVALID_ARG, "null context");
+  if (!ctx->prof) ctx->prof = new KernelProfile();
+  ctx->prof->mask = kernel_mask;
+  return EMPM_OK;
+}
+
+int empm_profile_read(empm_ctx* ctx, int kernel_id, long* launches, double* total_ms) {
+  if (!ctx || kernel_id < 0 || kernel_id >= EMPM_NUM_KERNELS || !launches || !total_ms)
+    return set_err(EMPM_E_INVALID_ARG, "empm_profile_read: bad arguments");
+  *launches = 0;
+  *total_ms = 0.0;
+  KernelProfile* p = ctx->prof;
+  if (!p) return EMPM_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  for (auto& pr : p->pending[kernel_id]) {
+    HIP_TRY(hipEventSynchronize(pr.second));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, pr.first, pr.second));
+    p->ms[kernel_id] += ms;
+    p->launches[kernel_id] += 1;
+    p->pool.push_back(pr.first);
+    p->pool.push_back(pr.second);
+  }
+  p->pending[kernel_id].clear();
+  *launches = p->launches[kernel_id];
+  *total_ms = p->ms[kernel_id];
+  return EMPM_OK;
+}
+
+int empm_profile_reset(empm_ctx* ctx) {
+  if (!ctx) return set_err(EMPM_E_INVALID_ARG, "null context");
+  if (!ctx->prof) return EMPM_OK;
+  for (int k = 0; k < EMPM_NUM_KERNELS; ++k) {
+    long l;
+    double ms;
+    int rc = empm_profile_read(ctx, k, &l, &ms);
+    if (rc) return rc;
+    ctx->prof->launches[k] = 0;
+    ctx->prof->ms[k] = 0.0;
+  }
+  return EMPM_OK;
+}
+
+const char* empm_kernel_name(int kernel_id) {
+  return (kernel_id >= 0 && kernel_id < EMPM_NUM_KERNELS) ? kKernelNames[kernel_id] : "";
 }
 
 int empm_mu_lam(int n, const float* E, const float* nu, float* mu, float* lam, void* stream) {

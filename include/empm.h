@@ -204,6 +204,33 @@ int empm_rollout_adj(empm_ctx *ctx, const empm_model *model, const empm_collider
                      const empm_particles *adj, float *d_mu, float *d_lam, float *d_ys, float *d_mass, float *d_vol,
                      int flags, void *stream);
 
+/* Kernel ids for per-kernel timing */
+enum {
+    EMPM_KERNEL_P2G = 0,      /* stress + P2G scatter */
+    EMPM_KERNEL_GRID = 1,     /* grid normalise / gravity / colliders */
+    EMPM_KERNEL_G2P = 2,      /* G2P gather + state update */
+    EMPM_KERNEL_G2P_ADJ = 3,  /* adjoint of G2P (scatter) */
+    EMPM_KERNEL_GRID_ADJ = 4, /* adjoint of the grid stage */
+    EMPM_KERNEL_P2G_ADJ = 5,  /* adjoint of P2G + stress (gather) */
+    EMPM_NUM_KERNELS = 6
+};
+
+/* Instrumentation (no counterpart in the reference, whose per-stage
+ * ScopedTimer syncs the device five times per substep,
+ * mpm_solver_diff.py:214-289): time the kernels whose bit is set in
+ * kernel_mask with HIP events recorded on their launch stream.  0 disables. */
+int empm_profile_enable(empm_ctx *ctx, unsigned kernel_mask);
+
+/* Resolve pending events of one kernel (waits for them) and return the
+ * cumulative launch count and milliseconds since the last reset. */
+int empm_profile_read(empm_ctx *ctx, int kernel_id, long *launches, double *total_ms);
+
+/* Zero the cumulative counters. */
+int empm_profile_reset(empm_ctx *ctx);
+
+/* Name of a kernel id ("" if out of range). */
+const char *empm_kernel_name(int kernel_id);
+
 #ifdef __cplusplus
 }
 #endif

@@ -830,6 +830,34 @@ int oracle_substep(const oracle_params *P, const oracle_collider *cols, int ncol
     return 0;
 }
 
+/* The grid stage alone (steps 5-7 of the substep) on a given (mv, m) grid:
+ * exported so the collider node sets can be checked directly. */
+void oracle_grid_update(const oracle_params *P, const oracle_collider *cols, int ncol, double time, double dt,
+                        const real *grid, real *vout)
+{
+    const int n = P->n_grid;
+    const float tf = (float)time, dtf = (float)dt, dxf = (float)P->dx;
+    const real ds = (real)P->grid_v_damping_scale, rdt = (real)dt;
+    for (int ix = 0; ix < n; ++ix)
+        for (int iy = 0; iy < n; ++iy)
+            for (int iz = 0; iz < n; ++iz) {
+                size_t id = ((size_t)ix * n + iy) * n + iz;
+                const real *g = &grid[4 * id];
+                real vv[3] = {0, 0, 0};
+                if (g[3] > (real)1e-15) {
+                    real im = 1 / g[3];
+                    for (int i = 0; i < 3; ++i)
+                        vv[i] = g[i] * im + rdt * (real)P->gravity[i];
+                }
+                if (P->grid_v_damping_scale < 1.0)
+                    for (int i = 0; i < 3; ++i)
+                        vv[i] = vv[i] - (1 - ds) * vv[i];
+                for (int k = 0; k < ncol; ++k)
+                    collide_node(&cols[k], ix, iy, iz, n, tf, dtf, dxf, vv);
+                memcpy(&vout[3 * id], vv, sizeof(vv));
+            }
+}
+
 /* A rollout of n_steps substeps from state (x, v, C, F), in place, with the
  * per-substep host collider advance and time += dt (mpm_solver_diff.py:291).
  * Returns the final time. */

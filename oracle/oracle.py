@@ -84,6 +84,9 @@ def load(precision: str = "f64"):
     lib.oracle_rollout.argtypes = [C.POINTER(OracleParams), C.POINTER(OracleCollider), C.c_int, C.c_double,
                                    C.c_double, C.c_int] + [vp] * 10
     lib.oracle_rollout.restype = C.c_double
+    lib.oracle_grid_update.argtypes = [C.POINTER(OracleParams), C.POINTER(OracleCollider), C.c_int, C.c_double,
+                                       C.c_double, vp, vp]
+    lib.oracle_grid_update.restype = None
     lib.oracle_advance_colliders.argtypes = [C.POINTER(OracleCollider), C.c_int, C.c_double, C.c_double]
     lib.oracle_advance_colliders.restype = None
     lib.oracle_num_threads.restype = C.c_int
@@ -193,6 +196,24 @@ def rollout(P, cols, t0, dt, n_steps, s: OracleState):
     if t < 0:
         raise RuntimeError("oracle_rollout failed")
     return t
+
+
+def grid_update(P, cols, time, dt, grid, precision="f64"):
+    """v_out (n,n,n,3) from a (n,n,n,4) grid of (mv, m)."""
+    lib = load(precision)
+    d = _dt(precision)
+    n = P.n_grid
+    g = np.ascontiguousarray(grid, dtype=d).reshape(n, n, n, 4)
+    vout = np.zeros((n, n, n, 3), d)
+    arr, nc = colliders_array(cols)
+    lib.oracle_grid_update(C.byref(P), arr, nc, time, dt, _p(g), _p(vout))
+    return vout
+
+
+def advance_colliders(cols, time, dt):
+    arr, nc = colliders_array(cols)
+    load("f64").oracle_advance_colliders(arr, nc, time, dt)
+    return [arr[i] for i in range(nc)]
 
 
 def svd3(F, precision="f64"):

@@ -1,0 +1,234 @@
+"""The drop-in Python API (empm_amd.diff_simulator, mirroring
+phys_sim.diff_simulator) on the GPU: the reference's own call sequences
+(warp_mpm_wrapper.py:135-306, interface.py:20-986, trainer_warp_mpm.py:332-400)
+against the FP32 oracle and central differences."""
+import numpy as np
+import pytest
+import torch
+
+from empm_amd import scenes
+from empm_amd.diff_simulator import WarpMPMWrapper
+from empm_amd.diff_simulator.warp_mpm.interface import (MPMDifferentiableSimulation,
+                                                        MPMDifferentiableSimulationClean,
+                                                        MPMDifferentiableSimulationTrail,
+                                                        MPMDifferentiableSimulationWCheckpoint)
+from empm_amd.diff_simulator.warp_mpm.warp_mpm_wrapper import material_gallery
+from oracle import oracle as orc
+from tests.helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _wrapper(sc, material="jelly", requires_grad=False, frame_dt=None, colliders=True):
+    mpm = WarpMPMWrapper(grid_size=sc.n_grid, grid_lim=1.0, frame_dt=frame_dt or sc.dt * 10, sim_dt=sc.dt,
+                         device="cuda:0", requires_grad=requires_grad)
+    mpm.material_params = dict(material_gallery[material])
+    mpm.add_particles(sc.x, sc.params.get("v0"), sc.vol.astype(np.float32), sc.E.astype(np.float32),
+                      sc.nu.astype(np.float32), sc.rho.astype(np.float32))
+    if colliders:
+        for c in sc.colliders:
+            mpm.surface_colliders.append([c["point"], c["normal"], "sticky", 0.0])
+    mpm.initialize()
+    return mpm
+
+
+def _oracle_rollout(sc, steps, material_id, v0=None, prec="f32", cols=None):
+    N = sc.n
+    mu, lam = orc.mu_lam(sc.E.astype(np.float64), sc.nu.astype(np.float64))
+    P = orc.make_params(N, sc.n_grid, material=material_id, gravity=(0, 0, -9.8))
+    ocols = cols if cols is not None else [orc.surface_collider(c["point"], c["normal"]) for c in sc.colliders]
+    o = orc.OracleState(sc.x, np.zeros((N, 3)) if v0 is None else v0, np.zeros((N, 3, 3)),
+                        np.tile(np.eye(3), (N, 1, 1)), mu, lam, sc.yield_stress, sc.vol * sc.rho, sc.vol,
+                        precision=prec)
+    orc.rollout(P, ocols, 0.0, sc.dt, steps, o)
+    return o
+
+
+def test_wrapper_advance_to_matches_oracle():
+    sc = scenes.cube_c1()
+    mpm = _wrapper(sc)
+    frames = mpm.advance_to(n_frames=5)  # 50 substeps
+    assert frames.shape == (6, sc.n, 3)
+    o = _oracle_rollout(sc, 50, 0)
+    err = rel_err(frames[-1].cpu().numpy(), o.x)
+    print(f"[wrapper advance_to] N={sc.n} rel err x {err:.2e}", flush=True)
+    assert err < 1e-4
+    assert abs(mpm.mpm_solver.time - 50 * sc.dt) < 1e-12
+
+
+def test_wrapper_step_callback_path_equals_native_loop():
+    sc = scenes.small_scene(n_particles=300, n_grid=16, material=0)
+    a = _wrapper(sc)
+    fa = a.advance_to(3)
+    b = _wrapper(sc)
+    seen = []
+    b.step_callback = lambda m, k: seen.append(k)
+    fb = b.advance_to(3)
+    assert seen == list(range(30))
+    assert torch.equal(fa, fb) or rel_err(fa.cpu().numpy(), fb.cpu().numpy()) < 1e-6
+
+
+def test_moving_cuboid_frame_callback_and_point_advance():
+    """trainer_warp_mpm.py:365-397: ground plane + a velocity cuboid whose
+    velocity a frame callback sets; the cuboid point advances on the host."""
+    sc = scenes.small_scene(n_particles=400, n_grid=16, material=0)
+    mpm = WarpMPMWrapper(grid_size=16, frame_dt=sc.dt * 5, sim_dt=sc.dt)
+    mpm.material_params = dict(material_gallery["jelly"])
+    mpm.material_params["grid_v_damping_scale"] = 0.98
+    mpm.add_particles(sc.x, None, float(sc.vol[0]), float(sc.E[0]), float(sc.nu[0]), float(sc.rho[0]))
+    mpm.surface_colliders.append([(0.0, 0.0, 0.3), (0.0, 0.0, 1.0), "sticky", 0.0])
+    mpm.moving_boundaries.append([(0.5, 0.5, 0.55), 0.06, (0.0, 0.0, 0.0), 0.0])
+
+    def cb(m, f):
+        m.mpm_solver.collider_params[1].velocity = (0.1 * (f + 1), 0.0, -0.2)
+
+    mpm.frame_callback = cb
+    mpm.initialize()
+    p0 = mpm.mpm_solver.collider_params[1].point
+    frames = mpm.advance_to(4)
+    p1 = mpm.mpm_solver.collider_params[1].point
+    exp_dx = sum(0.1 * (f + 1) for f in range(4)) * 5 * sc.dt
+    assert abs((p1[0] - p0[0]) - exp_dx) < 1e-6
+    # oracle replay of the same schedule
+    N = sc.n
+    cols = [orc.surface_collider((0, 0, 0.3), (0, 0, 1)), orc.cuboid((0.5, 0.5, 0.55), 0.06, (0, 0, 0))]
+    mu, lam = orc.mu_lam(sc.E.astype(np.float64), sc.nu.astype(np.float64))
+    P = orc.make_params(N, 16, material=0, gravity=(0, 0, -9.8), grid_v_damping_scale=0.98)
+    o = orc.OracleState(sc.x, np.zeros((N, 3)), np.zeros((N, 3, 3)), np.tile(np.eye(3), (N, 1, 1)), mu, lam,
+                        sc.yield_stress, sc.vol * sc.rho, sc.vol, precision="f32")
+    arr, nc = orc.colliders_array(cols)
+    t = 0.0
+    for f in range(4):
+        arr[1].velocity[0] = float(np.float32(0.1 * (f + 1)))
+        arr[1].velocity[2] = float(np.float32(-0.2))
+        t = orc.rollout(P, arr, t, sc.dt, 5, o)
+    err = rel_err(frames[-1].cpu().numpy(), o.x)
+    print(f"[cuboid + frame callback] rel err {err:.2e}", flush=True)
+    assert err < 1e-4
+
+
+def _trail_setup(sc, material="neo-hookean"):
+    mpm = _wrapper(sc, material=material, requires_grad=True)
+    return mpm
+
+
+def test_trail_gradients_exact_mode_match_fd():
+    """Scalar E (x1e6 inside the Trail, interface.py:70) and nu; the returned
+    grads are means over particles (interface.py:207-232)."""
+    sc = scenes.small_scene(n_particles=250, n_grid=16, material=6, seed=2, E=2e4, dt=2e-4)
+    mpm = _trail_setup(sc)
+    mpm.mpm_solver.adjoint_mode = "exact"
+    mpm.mpm_solver.checkpoint_every = 4
+    N = sc.n
+    x0 = torch.as_tensor(sc.x, device="cuda:0")
+    v0 = torch.as_tensor(sc.params["v0"], device="cuda:0")
+    target = (x0 + torch.tensor([0.01, -0.02, 0.0], device="cuda:0")).detach()
+    rho = torch.full((N,), 1000.0, device="cuda:0")
+    mask = torch.ones(N, dtype=torch.bool, device="cuda:0")
+
+    def loss_of(E_val, nu_val, need_grad):
+        E = torch.tensor(E_val, device="cuda:0", requires_grad=need_grad)
+        nu = torch.tensor(nu_val, device="cuda:0", requires_grad=need_grad)
+        mpm.mpm_solver.time = 0.0
+        pos = MPMDifferentiableSimulationTrail.apply(mpm.mpm_solver, mpm.mpm_state, mpm.mpm_model, 5, sc.dt, 15, x0,
+                                                    v0, E, nu, rho, mask, None, "cuda:0", True, 0)
+        loss = ((pos - target) ** 2).sum(-1).mean()
+        if need_grad:
+            loss.backward()
+            return float(loss), float(E.grad), float(nu.grad), pos
+        return float(loss)
+
+    loss, gE, gnu, pos = loss_of(0.02, 0.3, True)
+    assert pos.shape == (3, N, 3)
+    # FD on the GPU forward in FP32 is too noisy; use the FP64 oracle with the
+    # same frames (states after substeps 1, 6, 11)
+    def oracle_loss(Ephys, nu):
+        mu = np.full(N, Ephys / (2 * (1 + nu)))
+        lam = np.full(N, Ephys * nu / ((1 + nu) * (1 - 2 * nu)))
+        P = orc.make_params(N, 16, material=6, gravity=(0, 0, -9.8))
+        cols = [orc.surface_collider(c["point"], c["normal"]) for c in sc.colliders]
+        o = orc.OracleState(sc.x, sc.params["v0"], np.zeros((N, 3, 3)), np.tile(np.eye(3), (N, 1, 1)), mu, lam,
+                            sc.yield_stress, sc.vol * sc.rho, sc.vol)
+        arr, _ = orc.colliders_array(cols)
+        tg = target.cpu().numpy().astype(np.float64)
+        total, t, done = 0.0, 0.0, 0
+        for k in (1, 6, 11):
+            t = orc.rollout(P, arr, t, sc.dt, k - done, o)
+            done = k
+            total += np.mean(np.sum((o.x - tg) ** 2, axis=1))
+        return total / 3  # the Trail loss: mean over (frames, particles)
+
+    h = 1e-4
+    Ep = 0.02e6
+    fdE = (oracle_loss(Ep * (1 + h), 0.3) - oracle_loss(Ep * (1 - h), 0.3)) / (2 * h * Ep)
+    fdnu = (oracle_loss(Ep, 0.3 + 1e-5) - oracle_loss(Ep, 0.3 - 1e-5)) / 2e-5
+    # Trail returns the mean over particles of per-particle grads
+    print(f"[trail exact] loss {loss:.6e}  E.grad {gE:.6e} vs FD/N {fdE / N:.6e}  nu.grad {gnu:.6e} vs FD/N "
+          f"{fdnu / N:.6e}", flush=True)
+    assert abs(gE - fdE / N) < 2e-3 * abs(fdE / N)
+    assert abs(gnu - fdnu / N) < 2e-3 * abs(fdnu / N)
+
+
+def test_function_variants_shapes_and_consistency():
+    sc = scenes.small_scene(n_particles=200, n_grid=16, material=0, seed=5)
+    mpm = _trail_setup(sc, material="jelly")
+    N = sc.n
+    dev = "cuda:0"
+    x0 = torch.as_tensor(sc.x, device=dev)
+    v0 = torch.as_tensor(sc.params["v0"], device=dev).requires_grad_()
+    E = torch.full((N,), 2e4, device=dev, requires_grad=True)
+    nu = torch.full((N,), 0.3, device=dev, requires_grad=True)
+    rho = torch.full((N,), 1000.0, device=dev)
+    x_last = MPMDifferentiableSimulation.apply(mpm.mpm_solver, mpm.mpm_state, mpm.mpm_model, 1, sc.dt, 10, x0, v0, E,
+                                               nu, rho, None, None, dev, True, 0)
+    assert x_last.shape == (N, 3)
+    x_last.sum().backward()
+    assert E.grad.shape == (N,) and nu.grad.shape == (N,) and v0.grad.shape == (N, 3)
+    F0 = torch.eye(3, device=dev).repeat(N, 1, 1)
+    C0 = torch.zeros((N, 3, 3), device=dev)
+    xw = x0.clone().requires_grad_()
+    outs = MPMDifferentiableSimulationWCheckpoint.apply(mpm.mpm_solver, mpm.mpm_state, mpm.mpm_model, sc.dt, 10, xw,
+                                                        v0.detach(), F0, C0, E.detach(), nu.detach(), rho, None, dev,
+                                                        True, 0)
+    assert [o.shape for o in outs] == [(N, 3), (N, 3), (N, 3, 3), (N, 3, 3)]
+    assert rel_err(outs[0].detach().cpu().numpy(), x_last.detach().cpu().numpy()) < 1e-6
+    outs[0].sum().backward()
+    assert xw.grad is not None and torch.isfinite(xw.grad).all()
+    outs_c = MPMDifferentiableSimulationClean.apply(mpm.mpm_solver, mpm.mpm_state, mpm.mpm_model, sc.dt, 10, x0,
+                                                    v0.detach(), F0, C0, E.detach(), nu.detach(), rho, None, dev,
+                                                    True, 0)
+    assert outs_c[4].shape == (N, 6)
+
+
+def test_material_5_forward_rewrites_yield_and_backward_is_rejected():
+    sc = scenes.small_scene(n_particles=200, n_grid=16, material=5, seed=6, E=3e4)
+    mpm = _wrapper(sc, material="plasticine", requires_grad=True)
+    ys0 = mpm.mpm_model.yield_stress.clone()
+    mpm.advance_to(2)
+    assert torch.all(mpm.mpm_model.yield_stress <= ys0)
+    from empm_amd._lib import EmpmError
+
+    N = sc.n
+    E = torch.tensor(0.03, device="cuda:0", requires_grad=True)
+    nu = torch.tensor(0.3, device="cuda:0")
+    pos = MPMDifferentiableSimulationTrail.apply(mpm.mpm_solver, mpm.mpm_state, mpm.mpm_model, 1, sc.dt, 3,
+                                                torch.as_tensor(sc.x, device="cuda:0"),
+                                                torch.zeros((N, 3), device="cuda:0"), E, nu,
+                                                torch.full((N,), 1000.0, device="cuda:0"), None, None, "cuda:0", True,
+                                                0)
+    with pytest.raises(EmpmError):
+        pos.sum().backward()
+
+
+def test_errors_mirror_reference():
+    sc = scenes.small_scene(n_particles=100, n_grid=16)
+    with pytest.raises(TypeError):
+        m = _wrapper(sc, material="fluid")
+    m = _wrapper(sc)
+    with pytest.raises(ValueError):
+        m.mpm_solver.add_surface_collider((0, 0, 0), (0, 0, 1), "sticky", friction=0.5)
+    with pytest.raises(ValueError):
+        m.mpm_solver.add_geometry_collider({"center": (0.5, 0.5, 0.5)})
+    w = WarpMPMWrapper(grid_size=16)
+    with pytest.raises(AssertionError):
+        w.add_particles(np.array([[0.5, 0.5, 1.5]]))

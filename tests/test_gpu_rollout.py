@@ -1,0 +1,182 @@
+"""GPU rollout parity and end-to-end gradients through the C-ABI.
+
+- Forward rollouts of the plain cube workload (32^3, FCR, 200 substeps,
+  3-cell sticky walls) at full size, and of small scenes for every material,
+  against the FP32 oracle (bar 1e-4 relative on x and F_trial) with the FP64
+  error reported.
+- Gradients of a position loss w.r.t. (log E, nu, log Y) through the
+  checkpointed reverse sweep, against central differences of FP64 oracle
+  rollouts (bar 1e-3 relative), and checkpoint-interval invariance.
+"""
+import numpy as np
+import pytest
+import torch
+
+from empm_amd import runtime as rt
+from empm_amd import scenes
+from oracle import oracle as orc
+from tests.helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_collider(c):
+    from empm_amd._lib import Collider
+
+    g = Collider()
+    for f, _ in Collider._fields_:
+        if f == "mask":
+            continue
+        val = getattr(c, f)
+        if isinstance(val, (int, float)):
+            setattr(g, f, val)
+        else:
+            for i in range(3):
+                getattr(g, f)[i] = val[i]
+    return g
+
+
+def setup(sc, dev, E=None, nu=None, ys=None, v0=None):
+    N = sc.n
+    E = sc.E.astype(np.float64) if E is None else np.full(N, E)
+    nu = sc.nu.astype(np.float64) if nu is None else np.full(N, nu)
+    ys = sc.yield_stress.astype(np.float64) if ys is None else np.full(N, ys)
+    mu = (E / (2 * (1 + nu))).astype(np.float32)
+    lam = (E * nu / ((1 + nu) * (1 - 2 * nu))).astype(np.float32)
+    mass = (sc.vol * sc.rho).astype(np.float32)
+    v0 = np.zeros((N, 3), np.float32) if v0 is None else v0.astype(np.float32)
+    P = orc.make_params(N, sc.n_grid, material=sc.material, gravity=sc.gravity, friction_angle=sc.friction_angle)
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=dev).clone()
+    model = rt.ModelArrays(material=sc.material, n_grid=sc.n_grid, grid_lim=1.0, dx=P.dx, inv_dx=P.inv_dx,
+                           mu=t(mu), lam=t(lam), yield_stress=t(ys), mass=t(mass), vol=t(sc.vol),
+                           gravity=sc.gravity, alpha=P.alpha)
+    s0 = rt.ParticleState(t(sc.x), t(v0), torch.zeros((N, 3, 3), device=dev),
+                          torch.eye(3, device=dev).repeat(N, 1, 1).contiguous())
+    ocols = [orc.surface_collider(c["point"], c["normal"]) for c in sc.colliders]
+    ostate = lambda prec, **kw: orc.OracleState(sc.x, v0, np.zeros((N, 3, 3)), np.tile(np.eye(3), (N, 1, 1)),
+                                                kw.get("mu", mu), kw.get("lam", lam), kw.get("ys", ys), mass,
+                                                sc.vol, precision=prec)
+    return model, s0, P, ocols, ostate
+
+
+def test_cube_workload_full_size_forward(dev):
+    sc = scenes.cube_c1()
+    print(f"[cube 32^3] N={sc.n} steps={sc.n_steps}", flush=True)
+    model, s0, P, ocols, ostate = setup(sc, dev)
+    ctx = rt.Context(sc.n_grid, sc.n, dev)
+    ro = rt.Rollout(ctx, model, sc.n)
+    final, t1, _, _ = ro.forward(s0, [gpu_collider(c) for c in ocols], 0.0, sc.dt, sc.n_steps)
+    torch.cuda.synchronize()
+    g = {"x": final.x.cpu().numpy(), "F": final.F.cpu().numpy(), "v": final.v.cpu().numpy()}
+    res = {}
+    for prec in ("f32", "f64"):
+        o = ostate(prec)
+        orc.rollout(P, list(ocols), 0.0, sc.dt, sc.n_steps, o)
+        res[prec] = {k: rel_err(g[k], getattr(o, k)) for k in g}
+    print(f"  rel err vs f32 oracle {res['f32']}\n  rel err vs f64 oracle {res['f64']}", flush=True)
+    assert abs(t1 - sc.n_steps * sc.dt) < 1e-12
+    assert res["f32"]["x"] < 1e-4 and res["f32"]["F"] < 1e-4
+    assert res["f64"]["x"] < 1e-4 and res["f64"]["F"] < 1e-4
+
+
+@pytest.mark.parametrize("material", [0, 1, 2, 3, 4, 5, 6, 7])
+def test_small_rollout_every_material(dev, material):
+    sc = scenes.small_scene(n_particles=400, n_grid=16, material=material, seed=material, E=3e4, ys=200.0)
+    print(f"[small rollout] material={material} N={sc.n}", flush=True)
+    steps = 30
+    model, s0, P, ocols, ostate = setup(sc, dev, v0=sc.params["v0"])
+    ctx = rt.Context(sc.n_grid, sc.n, dev)
+    ro = rt.Rollout(ctx, model, sc.n)
+    final, _, _, _ = ro.forward(s0, [gpu_collider(c) for c in ocols], 0.0, sc.dt, steps)
+    torch.cuda.synchronize()
+    o = ostate("f32")
+    orc.rollout(P, list(ocols), 0.0, sc.dt, steps, o)
+    ex = rel_err(final.x.cpu().numpy(), o.x)
+    eF = rel_err(final.F.cpu().numpy(), o.F)
+    print(f"  rel err x {ex:.2e} F {eF:.2e}", flush=True)
+    assert ex < 1e-4 and eF < 1e-4
+
+
+def _loss_grad_gpu(sc, dev, target, prm, steps, ckpt, flags=0):
+    N = sc.n
+    model, s0, P, ocols, _ = setup(sc, dev, E=prm["E"], nu=prm["nu"], ys=prm["ys"], v0=sc.params["v0"])
+    ctx = rt.Context(sc.n_grid, N, dev)
+    ro = rt.Rollout(ctx, model, N)
+    final, _, _, tape = ro.forward(s0, [gpu_collider(c) for c in ocols], 0.0, sc.dt, steps, tape=True,
+                                   ckpt_every=ckpt)
+    x = final.x.clone()
+    tgt = torch.as_tensor(target, device=dev, dtype=torch.float32)
+    loss = float(((x - tgt) ** 2).sum(1).mean())
+    d = rt.ParticleState.empty(N, dev)
+    d.x.copy_(2.0 * (x - tgt) / N)
+    _, grads = ro.backward(tape, d, flags=flags)
+    Et = torch.full((N,), prm["E"], device=dev)
+    nut = torch.full((N,), prm["nu"], device=dev)
+    dE = torch.zeros(N, device=dev)
+    dnu = torch.zeros(N, device=dev)
+    rt.mu_lam_adj(Et, nut, grads.mu, grads.lam, dE, dnu)
+    g = np.array([float((dE * Et).sum()), float(dnu.sum()), float((grads.ys * prm["ys"]).sum())])
+    return loss, g
+
+
+def _loss_oracle(sc, target, prm, steps):
+    N = sc.n
+    E, nu, ys = prm["E"], prm["nu"], prm["ys"]
+    mu = np.full(N, E / (2 * (1 + nu)))
+    lam = np.full(N, E * nu / ((1 + nu) * (1 - 2 * nu)))
+    P = orc.make_params(N, sc.n_grid, material=sc.material, gravity=sc.gravity)
+    ocols = [orc.surface_collider(c["point"], c["normal"]) for c in sc.colliders]
+    o = orc.OracleState(sc.x, sc.params["v0"], np.zeros((N, 3, 3)), np.tile(np.eye(3), (N, 1, 1)), mu, lam,
+                        np.full(N, ys), sc.vol * sc.rho, sc.vol, precision="f64")
+    orc.rollout(P, ocols, 0.0, sc.dt, steps, o)
+    return float(np.mean(np.sum((o.x - target) ** 2, axis=1)))
+
+
+@pytest.mark.parametrize("material", [7, 0, 6, 1])
+def test_parameter_gradients_vs_central_differences(dev, material):
+    steps = 20
+    sc = scenes.small_scene(n_particles=300, n_grid=16, material=material, seed=3, E=2e4, ys=150.0, dt=2e-4)
+    base = {"E": 2e4, "nu": 0.3, "ys": 150.0}
+    tprm = {"E": 5e4, "nu": 0.25, "ys": 400.0}
+    target = sc.x.astype(np.float64).copy()
+    # target rollout with different parameters (FP64 oracle)
+    N = sc.n
+    P = orc.make_params(N, sc.n_grid, material=material, gravity=sc.gravity)
+    ocols = [orc.surface_collider(c["point"], c["normal"]) for c in sc.colliders]
+    mu_t = np.full(N, tprm["E"] / (2 * (1 + tprm["nu"])))
+    lam_t = np.full(N, tprm["E"] * tprm["nu"] / ((1 + tprm["nu"]) * (1 - 2 * tprm["nu"])))
+    o = orc.OracleState(sc.x, sc.params["v0"], np.zeros((N, 3, 3)), np.tile(np.eye(3), (N, 1, 1)), mu_t, lam_t,
+                        np.full(N, tprm["ys"]), sc.vol * sc.rho, sc.vol)
+    orc.rollout(P, ocols, 0.0, sc.dt, steps, o)
+    target = o.x.copy()
+    print(f"[grad vs FD] material={material} N={N} steps={steps}", flush=True)
+    loss, g = _loss_grad_gpu(sc, dev, target, base, steps, ckpt=7)
+    fd = []
+    for i, (key, h) in enumerate((("E", 1e-4), ("nu", 1e-5), ("ys", 1e-4))):
+        pp, pm = dict(base), dict(base)
+        if key == "nu":
+            pp[key] += h
+            pm[key] -= h
+            fd.append((_loss_oracle(sc, target, pp, steps) - _loss_oracle(sc, target, pm, steps)) / (2 * h))
+        else:
+            pp[key] *= np.exp(h)
+            pm[key] *= np.exp(-h)
+            fd.append((_loss_oracle(sc, target, pp, steps) - _loss_oracle(sc, target, pm, steps)) / (2 * h))
+    fd = np.array(fd)
+    print(f"  loss {loss:.6e}\n  adjoint d/d(logE, nu, logY) {g}\n  central FD              {fd}", flush=True)
+    for i in range(3):
+        if i == 2 and material not in (1, 7):
+            assert abs(g[i]) < 1e-12 + 1e-6 * np.abs(g).max()
+            continue
+        assert abs(g[i] - fd[i]) <= 1e-3 * abs(fd[i]) + 1e-9 * np.abs(fd).max(), (i, g[i], fd[i])
+
+
+def test_checkpoint_interval_does_not_change_gradients(dev):
+    sc = scenes.small_scene(n_particles=300, n_grid=16, material=7, seed=4, E=2e4, ys=150.0)
+    target = sc.x + 0.01
+    prm = {"E": 2e4, "nu": 0.3, "ys": 150.0}
+    l1, g1 = _loss_grad_gpu(sc, dev, target, prm, 24, ckpt=24)
+    l2, g2 = _loss_grad_gpu(sc, dev, target, prm, 24, ckpt=5)
+    l3, g3 = _loss_grad_gpu(sc, dev, target, prm, 24, ckpt=1)
+    assert l1 == l2 == l3
+    assert np.allclose(g1, g2, rtol=1e-5) and np.allclose(g1, g3, rtol=1e-5), (g1, g2, g3)
