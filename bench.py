@@ -112,7 +112,7 @@ def main():
         return out
 
     x0 = t(sc.x)
-    s0 = rt.ParticleState(x0, torch.zeros_like(x0), torch.zeros((N, 3, 3), device=dev),
+    s0 = rt.ParticleState(x0, t(sc.params["v0"]), torch.zeros((N, 3, 3), device=dev),
                           torch.eye(3, device=dev).repeat(N, 1, 1).contiguous())
     ctx = rt.Context(n, N, dev)
 
@@ -241,7 +241,7 @@ def cpu_baseline(sc, seconds):
     lam = np.full(N, E * nu / ((1 + nu) * (1 - 2 * nu)))
     P = orc.make_params(N, sc.n_grid, material=sc.material, gravity=sc.gravity)
     cols = [orc.surface_collider(c["point"], c["normal"]) for c in sc.colliders]
-    s = orc.OracleState(sc.x, np.zeros((N, 3)), np.zeros((N, 3, 3)), np.tile(np.eye(3), (N, 1, 1)), mu, lam,
+    s = orc.OracleState(sc.x, sc.params["v0"], np.zeros((N, 3, 3)), np.tile(np.eye(3), (N, 1, 1)), mu, lam,
                         sc.yield_stress, sc.vol * sc.rho, sc.vol, precision="f32")
     t0 = time.perf_counter()
     orc.rollout(P, cols, 0.0, sc.dt, 1, s)

@@ -40,6 +40,8 @@ EXPORTED_SYMBOLS = (
     "empm_rollout_fwd",
     "empm_rollout_fwd_replay",
     "empm_rollout_adj",
+    "empm_sort_by_cell",
+    "empm_set_rebin_interval",
     "empm_profile_enable",
     "empm_profile_read",
     "empm_profile_reset",
@@ -153,6 +155,10 @@ def _declare(lib):
     lib.empm_rollout_fwd_replay.restype = C.c_int
     lib.empm_rollout_fwd_replay.argtypes = [vp, P(Model), P(Collider), C.c_int, P(C.c_double), C.c_double,
                                             C.c_int, P(Particles), vp]
+    lib.empm_sort_by_cell.restype = C.c_int
+    lib.empm_sort_by_cell.argtypes = [vp, C.c_int, vp, C.c_float, vp, vp]
+    lib.empm_set_rebin_interval.restype = C.c_int
+    lib.empm_set_rebin_interval.argtypes = [vp, C.c_int]
     lib.empm_profile_enable.restype = C.c_int
     lib.empm_profile_enable.argtypes = [vp, C.c_uint]
     lib.empm_profile_read.restype = C.c_int

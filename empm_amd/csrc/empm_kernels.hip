@@ -3,19 +3,22 @@
 // include/empm.h.
 //
 // Per substep the forward runs three kernels:
-//   k_p2g   per particle: return map + stress (one SVD) fused with the APIC
-//           scatter of momentum and mass into the float4 grid (mv.xyz, m);
-//   k_grid  per node: normalise, gravity, damping and every collider in
-//           registration order (a table passed by value), writes v_out and
-//           re-zeroes the accumulation grid for the next substep;
+//   k_p2g   per particle block: return map + stress (one SVD) fused with the
+//           APIC scatter of momentum and mass into block-owned LDS tiles;
+//   k_grid  per node: gather of the covering tiles, normalise, gravity,
+//           damping and every collider in registration order (a table passed
+//           by value), writes v_out;
 //   k_g2p   per particle: 27-node gather, x/v/C/F_trial update.
 // The adjoint of one substep recomputes k_p2g/k_grid and then runs
-//   k_g2p_adj (scatter of dL/dv_out), k_grid_adj (per node), k_p2g_adj
-//   (gather of dL/d(mv, m), stress and return-map adjoints).
+//   k_g2p_adj (scatter of dL/dv_out into tiles), k_grid_adj (per node),
+//   k_p2g_adj (gather of dL/d(mv, m), stress and return-map adjoints).
+// Particles are binned by cell (k_bin_*) at the start of every call and
+// every `rebin_every` substeps of a rollout.
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <utility>
 #include <vector>
@@ -90,6 +93,12 @@ EMPM_DEV void make_stencil(const float* xp, float inv_dx, Stencil& s) {
   }
 }
 
+// Weight lookups by a possibly run-time node index through selects, so the
+// stencil stays in registers whether or not the node loops are unrolled.
+EMPM_DEV float pick3(int i, const float* a) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
+EMPM_DEV float sw(const Stencil& s, int d, int i) { return pick3(i, s.w[d]); }
+EMPM_DEV float sdw(const Stencil& s, int d, int i) { return pick3(i, s.dw[d]); }
+
 EMPM_DEV bool node_in(int ix, int iy, int iz, int n) {
   return (unsigned)ix < (unsigned)n && (unsigned)iy < (unsigned)n && (unsigned)iz < (unsigned)n;
 }
@@ -98,8 +107,8 @@ EMPM_DEV size_t node_id(int ix, int iy, int iz, int n) { return ((size_t)ix * n 
 // d(w, grad w)/df chain for node (a,b,c): fb += dL/df given wb = dL/dw and
 // gwb = dL/d(grad w).  grad w = inv_dx (N'a Nb Nc, Na N'b Nc, Na Nb N'c).
 EMPM_DEV void weight_adj(const Stencil& s, int a, int b, int c, float inv_dx, float wb, const float* gwb, float* fb) {
-  const float X = s.w[0][a], Y = s.w[1][b], Z = s.w[2][c];
-  const float dX = s.dw[0][a], dY = s.dw[1][b], dZ = s.dw[2][c];
+  const float X = sw(s, 0, a), Y = sw(s, 1, b), Z = sw(s, 2, c);
+  const float dX = sdw(s, 0, a), dY = sdw(s, 1, b), dZ = sdw(s, 2, c);
   const float ddX = a == 1 ? -2.f : 1.f, ddY = b == 1 ? -2.f : 1.f, ddZ = c == 1 ? -2.f : 1.f;
   fb[0] += wb * dX * Y * Z + inv_dx * (gwb[0] * ddX * Y * Z + gwb[1] * dX * dY * Z + gwb[2] * dX * Y * dZ);
   fb[1] += wb * X * dY * Z + inv_dx * (gwb[0] * dX * dY * Z + gwb[1] * X * ddY * Z + gwb[2] * X * dY * dZ);
@@ -267,81 +276,303 @@ EMPM_DEV void grid_node(const DevModel& M, const ColTable& T, float time, float 
 }
 
 // ---------------------------------------------------------------------------
-// Forward kernels
+// Block-owned scatter.
+//
+// Particles are binned by the cell their 3x3x3 node stencil starts at, in
+// blocks of 2x2x2 cells (block-major, cell-minor order).  One workgroup owns
+// one block at a time: it scatters the stencils of the block's particles
+// into a 6x6x6-node LDS tile (the block's 4 stencil nodes per axis plus a
+// one-cell margin each side for particles that drift between re-binnings),
+// then writes the whole tile to its slot with plain stores.  The grid pass
+// gathers, for each node, the <= 27 tiles that cover it.  No global atomics
+// are issued except for particles that drifted out of their tile, which add
+// into a dense overflow grid.
+//
+// Inside a wavefront, lanes of the same cell add into the same 27 nodes, so
+// their contributions are first summed across lanes by a transposed
+// butterfly (32 values per lane halved at each xor-shuffle step: xor 32, 16,
+// 8, 4, 2), after which lane L holds the group total of value L >> 1 and one
+// LDS atomic per (node, component) per cell group remains.  Waves with many
+// small groups (sparse particles) fall back to per-lane LDS atomics.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_p2g(DevModel M, float dt, const float* __restrict__ x,
-                                                const float* __restrict__ v, const float* __restrict__ C,
-                                                const float* __restrict__ F, float* __restrict__ Fe_out,
-                                                float4* __restrict__ grid, int write_params) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= M.N) return;
-  if (M.sel && M.sel[p] != 0) return;
-  float Ftr[9];
-  load9(F, p, Ftr);
-  float mu = M.mu[p], lam = M.lam[p], ys = M.ys[p];
-  ConstitState st;
-  float tau[9];
-  constitutive_fwd(st, Ftr, mu, lam, ys, M.mp, dt, tau);
-  if (write_params) {
-    M.mu[p] = mu;
-    M.lam[p] = lam;
-    M.ys[p] = ys;
-  }
-  store9(Fe_out, p, st.Fe);
+constexpr int kBlkCells = 2;                                    // cells per block edge
+constexpr int kTileMargin = 1;                                  // cells of drift tolerated
+constexpr int kTileN = kBlkCells + 2 + 2 * kTileMargin;         // 6 tile nodes per edge
+constexpr int kTileNodes = kTileN * kTileN * kTileN;            // 216
+constexpr int kMaxGroupsPerWave = 4;
+constexpr int kMinGroupSize = 6;
+constexpr int kBlkWG = 256;       // workgroup size of the block-owned scatter kernels
+constexpr int kBlkGridCap = 1024; // max workgroups of a block-owned scatter launch (4 per CU)
 
-  const float xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
-  const float vp[3] = {v[3 * p], v[3 * p + 1], v[3 * p + 2]};
-  float Cp[9], Ct[9];
-  load9(C, p, Cp);
-  rpic_blend(Cp, M.rpic, Ct);
-  Stencil s;
-  make_stencil(xp, M.inv_dx, s);
-  const float mp = M.mass[p], Vp = M.vol[p];
-  // elastic term per node: -dt V tau grad_w ; precompute -dt V tau
-  float S[9];
+// Particle binning of the current segment: perm[k] = k-th particle in
+// block-major order, offs[b * 8 + c] = first position of cell c of block b
+// (offs has nb^3 * 8 + 1 entries).
+struct Binning {
+  const int* perm;
+  const int* offs;
+  const int* active;   // ids of the non-empty blocks, in no particular order
+  const int* nactive;  // their count (device scalar)
+  int nb;
+};
+
+EMPM_DEV float wave_transpose_sum32(float* v, int lane) {
+  {
+    const bool hi = lane & 32;
 #pragma unroll
-  for (int i = 0; i < 9; ++i) S[i] = -Vp * tau[i];
+    for (int i = 0; i < 16; ++i) {
+      const float s = hi ? v[i] : v[i + 16];
+      const float k = hi ? v[i + 16] : v[i];
+      v[i] = k + __shfl_xor(s, 32);
+    }
+  }
+  {
+    const bool hi = lane & 16;
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
+    for (int i = 0; i < 8; ++i) {
+      const float s = hi ? v[i] : v[i + 8];
+      const float k = hi ? v[i + 8] : v[i];
+      v[i] = k + __shfl_xor(s, 16);
+    }
+  }
+  {
+    const bool hi = lane & 8;
 #pragma unroll
-    for (int b = 0; b < 3; ++b)
+    for (int i = 0; i < 4; ++i) {
+      const float s = hi ? v[i] : v[i + 4];
+      const float k = hi ? v[i + 4] : v[i];
+      v[i] = k + __shfl_xor(s, 8);
+    }
+  }
+  {
+    const bool hi = lane & 4;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
-        if (!node_in(ix, iy, iz, M.n)) continue;
-        const float dpos[3] = {((float)a - s.f[0]) * M.dx, ((float)b - s.f[1]) * M.dx, ((float)c - s.f[2]) * M.dx};
-        const float wt = s.w[0][a] * s.w[1][b] * s.w[2][c];
-        const float gw[3] = {s.dw[0][a] * s.w[1][b] * s.w[2][c] * M.inv_dx, s.w[0][a] * s.dw[1][b] * s.w[2][c] * M.inv_dx,
-                             s.w[0][a] * s.w[1][b] * s.dw[2][c] * M.inv_dx};
-        float add[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          const float cd = Ct[i * 3 + 0] * dpos[0] + Ct[i * 3 + 1] * dpos[1] + Ct[i * 3 + 2] * dpos[2];
-          const float ef = S[i * 3 + 0] * gw[0] + S[i * 3 + 1] * gw[1] + S[i * 3 + 2] * gw[2];
-          add[i] = wt * mp * (vp[i] + cd) + dt * ef;
-        }
-        float4* g = &grid[node_id(ix, iy, iz, M.n)];
-        atomicAdd(&g->x, add[0]);
-        atomicAdd(&g->y, add[1]);
-        atomicAdd(&g->z, add[2]);
-        atomicAdd(&g->w, wt * mp);
-      }
+    for (int i = 0; i < 2; ++i) {
+      const float s = hi ? v[i] : v[i + 2];
+      const float k = hi ? v[i + 2] : v[i];
+      v[i] = k + __shfl_xor(s, 4);
+    }
+  }
+  {
+    const bool hi = lane & 2;
+    const float s = hi ? v[0] : v[1];
+    const float k = hi ? v[1] : v[0];
+    v[0] = k + __shfl_xor(s, 2);
+  }
+  return v[0] + __shfl_xor(v[0], 1);
 }
 
-__global__ __launch_bounds__(kBlock) void k_grid(DevModel M, ColTable T, float time, float dt,
-                                                 float4* __restrict__ grid_in, float4* __restrict__ v_out,
-                                                 int zero_in) {
+// contrib(j, out4): the float4 this lane adds to stencil node j = 9a + 3b + c.
+// add(ix, iy, iz, comp, value): the destination.  Every lane of the wave must
+// call this (it shuffles); `active` marks lanes that own a particle.
+template <int NCOMP, typename Contrib, typename Add>
+EMPM_DEV void scatter_stencil(bool active, const int* base, Contrib contrib, Add add) {
+  const int lane = __lane_id();
+  const int key = active ? (((base[0] + 4) << 20) | ((base[1] + 4) << 10) | (base[2] + 4)) : -1;
+  unsigned long long pending = __ballot(active);
+  for (int iter = 0; iter < kMaxGroupsPerWave && pending; ++iter) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int lkey = __shfl(key, leader);
+    const bool in = ((pending >> lane) & 1ull) && key == lkey;
+    const unsigned long long grp = __ballot(in);
+    if (__popcll(grp) < kMinGroupSize) break;
+    pending &= ~grp;
+    const int b0 = __shfl(base[0], leader), b1 = __shfl(base[1], leader), b2 = __shfl(base[2], leader);
+#pragma unroll 1
+    for (int r = 0; r < 4; ++r) {
+      float v[32];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        float c4[4] = {0.f, 0.f, 0.f, 0.f};
+        if (r * 8 + j < 27 && in) contrib(r * 8 + j, c4);
+        v[4 * j + 0] = c4[0];
+        v[4 * j + 1] = c4[1];
+        v[4 * j + 2] = c4[2];
+        v[4 * j + 3] = c4[3];
+      }
+      const float s = wave_transpose_sum32(v, lane);
+      const int idx = lane >> 1;
+      const int node = r * 8 + (idx >> 2);
+      const int comp = idx & 3;
+      if (!(lane & 1) && node < 27 && comp < NCOMP && s != 0.f)
+        add(b0 + node / 9, b1 + (node / 3) % 3, b2 + node % 3, comp, s);
+    }
+  }
+  if ((pending >> lane) & 1ull) {
+#pragma unroll 1
+    for (int j = 0; j < 27; ++j) {
+      float c4[4] = {0.f, 0.f, 0.f, 0.f};
+      contrib(j, c4);
+#pragma unroll
+      for (int k = 0; k < NCOMP; ++k) add(base[0] + j / 9, base[1] + (j / 3) % 3, base[2] + j % 3, k, c4[k]);
+    }
+  }
+}
+
+// Per-lane global atomics into the overflow grid, for a particle whose stencil
+// left its block's tile (rare: drift of more than one cell between binnings).
+template <int NCOMP, typename Contrib>
+EMPM_DEV void scatter_overflow(float4* __restrict__ ovf, int n, const int* base, Contrib contrib) {
+  for (int j = 0; j < 27; ++j) {
+    const int ix = base[0] + j / 9, iy = base[1] + (j / 3) % 3, iz = base[2] + j % 3;
+    if (!node_in(ix, iy, iz, n)) continue;
+    float c4[4] = {0.f, 0.f, 0.f, 0.f};
+    contrib(j, c4);
+    float* g = reinterpret_cast<float*>(&ovf[node_id(ix, iy, iz, n)]);
+    for (int k = 0; k < NCOMP; ++k) atomicAdd(g + k, c4[k]);
+  }
+}
+
+// Block id -> tile origin (node coordinates of tile node 0)
+EMPM_DEV void tile_origin(int blk, int nb, int* o) {
+  o[2] = (blk % nb) * kBlkCells - kTileMargin;
+  o[1] = ((blk / nb) % nb) * kBlkCells - kTileMargin;
+  o[0] = (blk / (nb * nb)) * kBlkCells - kTileMargin;
+}
+
+EMPM_DEV bool stencil_in_tile(const int* base, const int* o) {
+  return (unsigned)(base[0] - o[0]) <= (unsigned)(kTileN - 3) && (unsigned)(base[1] - o[1]) <= (unsigned)(kTileN - 3) &&
+         (unsigned)(base[2] - o[2]) <= (unsigned)(kTileN - 3);
+}
+
+// Sum of the tiles (and the overflow grid) at node (ix, iy, iz).
+EMPM_DEV float4 gather_node(const Binning& B, const float4* __restrict__ tiles, int ix, int iy, int iz) {
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  const int nb = B.nb;
+  // block b covers nodes [2b - 1, 2b + 4]: b in [ceil((i - 4) / 2), floor((i + 1) / 2)]
+  const int lo0 = max(0, (ix - 3) >> 1), hi0 = min(nb - 1, (ix + 1) >> 1);
+  const int lo1 = max(0, (iy - 3) >> 1), hi1 = min(nb - 1, (iy + 1) >> 1);
+  const int lo2 = max(0, (iz - 3) >> 1), hi2 = min(nb - 1, (iz + 1) >> 1);
+  for (int bx = lo0; bx <= hi0; ++bx)
+    for (int by = lo1; by <= hi1; ++by)
+      for (int bz = lo2; bz <= hi2; ++bz) {
+        const int blk = (bx * nb + by) * nb + bz;
+        if (B.offs[blk * 8 + 8] == B.offs[blk * 8]) continue;
+        const int lx = ix - (bx * kBlkCells - kTileMargin), ly = iy - (by * kBlkCells - kTileMargin),
+                  lz = iz - (bz * kBlkCells - kTileMargin);
+        const float4 t = tiles[(size_t)blk * kTileNodes + (lx * kTileN + ly) * kTileN + lz];
+        acc.x += t.x;
+        acc.y += t.y;
+        acc.z += t.z;
+        acc.w += t.w;
+      }
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// Forward kernels
+// ---------------------------------------------------------------------------
+// compute_stress_from_F_trial + p2g_apic_with_stress (mpm_utils.py:358-423,
+// 603-665), one block of particles per workgroup iteration.
+template <int MAT>
+__global__ __launch_bounds__(kBlkWG) void k_p2g(DevModel M, float dt, const float* __restrict__ x,
+                                                const float* __restrict__ v, const float* __restrict__ C,
+                                                const float* __restrict__ F, float* __restrict__ Fe_out, Binning B,
+                                                float4* __restrict__ tiles, float4* __restrict__ ovf,
+                                                int write_params) {
+  __shared__ float tile[4][kTileNodes];
+  const int nblk = B.nb * B.nb * B.nb;
+  const float dx = M.dx, inv_dx = M.inv_dx;
+  MatParams matp = M.mp;
+  matp.material = MAT;  // compile-time material: one code path, constitutive state stays in registers
+  const int nact = *B.nactive;
+  (void)nblk;
+  for (int ai = blockIdx.x; ai < nact; ai += gridDim.x) {
+    const int blk = B.active[ai];
+    const int start = B.offs[blk * 8], end = B.offs[blk * 8 + 8];
+    int o[3];
+    tile_origin(blk, B.nb, o);
+    for (int i = threadIdx.x; i < 4 * kTileNodes; i += blockDim.x) (&tile[0][0])[i] = 0.f;
+    __syncthreads();
+    for (int i0 = start; i0 < end; i0 += blockDim.x) {
+      const int i = i0 + (int)threadIdx.x;
+      const int p = i < end ? B.perm[i] : -1;
+      const bool active = p >= 0 && !(M.sel && M.sel[p] != 0);
+      float Ct[9], S[9], xp[3] = {(o[0] + 2.f) * dx, (o[1] + 2.f) * dx, (o[2] + 2.f) * dx}, vp[3] = {0.f, 0.f, 0.f},
+                                mp = 0.f;
+      m_zero(Ct);
+      m_zero(S);
+      if (active) {
+        float Ftr[9];
+        load9(F, p, Ftr);
+        float mu = M.mu[p], lam = M.lam[p], ys = M.ys[p];
+        ConstitState st;
+        float tau[9];
+        constitutive_fwd(st, Ftr, mu, lam, ys, matp, dt, tau);
+        if (write_params) {
+          M.mu[p] = mu;
+          M.lam[p] = lam;
+          M.ys[p] = ys;
+        }
+        store9(Fe_out, p, st.Fe);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          xp[d] = x[3 * p + d];
+          vp[d] = v[3 * p + d];
+        }
+        float Cp[9];
+        load9(C, p, Cp);
+        rpic_blend(Cp, M.rpic, Ct);
+        mp = M.mass[p];
+        const float Vp = M.vol[p];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) S[k] = -Vp * tau[k];
+      }
+      Stencil s;
+      make_stencil(xp, inv_dx, s);
+      auto contrib = [&](int j, float* out) {
+        const int a = j / 9, b = (j / 3) % 3, c = j % 3;
+        const float dpos[3] = {((float)a - s.f[0]) * dx, ((float)b - s.f[1]) * dx, ((float)c - s.f[2]) * dx};
+        const float wt = sw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c);
+        const float gw[3] = {sdw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c) * inv_dx, sw(s, 0, a) * sdw(s, 1, b) * sw(s, 2, c) * inv_dx,
+                             sw(s, 0, a) * sw(s, 1, b) * sdw(s, 2, c) * inv_dx};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const float cd = Ct[k * 3 + 0] * dpos[0] + Ct[k * 3 + 1] * dpos[1] + Ct[k * 3 + 2] * dpos[2];
+          const float ef = S[k * 3 + 0] * gw[0] + S[k * 3 + 1] * gw[1] + S[k * 3 + 2] * gw[2];
+          out[k] = wt * mp * (vp[k] + cd) + dt * ef;
+        }
+        out[3] = wt * mp;
+      };
+      const bool in_tile = active && stencil_in_tile(s.base, o);
+      scatter_stencil<4>(in_tile, s.base, contrib, [&](int ix, int iy, int iz, int comp, float val) {
+        atomicAdd(&tile[comp][((ix - o[0]) * kTileN + (iy - o[1])) * kTileN + (iz - o[2])], val);
+      });
+      if (active && !in_tile) scatter_overflow<4>(ovf, M.n, s.base, contrib);
+    }
+    __syncthreads();
+    float4* dst = tiles + (size_t)blk * kTileNodes;
+    for (int k = threadIdx.x; k < kTileNodes; k += blockDim.x)
+      dst[k] = make_float4(tile[0][k], tile[1][k], tile[2][k], tile[3][k]);
+    __syncthreads();
+  }
+}
+
+// Per node: gather (mv, m) from the tiles and the overflow grid, then
+// grid_normalization_and_gravity, damping and the colliders.  The overflow
+// grid is re-zeroed for the next scatter; (mv, m) is kept when `keep` is set
+// (the adjoint needs it).
+__global__ __launch_bounds__(kBlock) void k_grid(DevModel M, ColTable T, float time, float dt, Binning B,
+                                                 const float4* __restrict__ tiles, float4* __restrict__ ovf,
+                                                 float4* __restrict__ keep, float4* __restrict__ v_out) {
   const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t G = (size_t)M.n * M.n * M.n;
   if (id >= G) return;
   const int iz = (int)(id % M.n);
   const int iy = (int)((id / M.n) % M.n);
   const int ix = (int)(id / ((size_t)M.n * M.n));
-  const float4 g = grid_in[id];
+  float4 g = gather_node(B, tiles, ix, iy, iz);
+  const float4 o = ovf[id];
+  if (o.x != 0.f || o.y != 0.f || o.z != 0.f || o.w != 0.f) {
+    g.x += o.x;
+    g.y += o.y;
+    g.z += o.z;
+    g.w += o.w;
+    ovf[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  if (keep) keep[id] = g;
   float v[3], fac[3];
   grid_node(M, T, time, dt, ix, iy, iz, g, v, fac);
   v_out[id] = make_float4(v[0], v[1], v[2], g.w);
-  if (zero_in) grid_in[id] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // g2p_differentiable (mpm_utils.py:502-579)
@@ -379,9 +610,9 @@ __global__ __launch_bounds__(kBlock) void k_g2p(DevModel M, float dt, const floa
         if (node_in(ix, iy, iz, M.n)) gv = v_out[node_id(ix, iy, iz, M.n)];
         const float gvv[3] = {gv.x, gv.y, gv.z};
         const float dpos[3] = {(float)a - s.f[0], (float)b - s.f[1], (float)c - s.f[2]};
-        const float wt = s.w[0][a] * s.w[1][b] * s.w[2][c];
-        const float gw[3] = {s.dw[0][a] * s.w[1][b] * s.w[2][c] * M.inv_dx, s.w[0][a] * s.dw[1][b] * s.w[2][c] * M.inv_dx,
-                             s.w[0][a] * s.w[1][b] * s.dw[2][c] * M.inv_dx};
+        const float wt = sw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c);
+        const float gw[3] = {sdw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c) * M.inv_dx, sw(s, 0, a) * sdw(s, 1, b) * sw(s, 2, c) * M.inv_dx,
+                             sw(s, 0, a) * sw(s, 1, b) * sdw(s, 2, c) * M.inv_dx};
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
           nv[i] += gvv[i] * wt;
@@ -412,127 +643,170 @@ __global__ __launch_bounds__(kBlock) void k_g2p(DevModel M, float dt, const floa
 // ---------------------------------------------------------------------------
 // Adjoint kernels
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_g2p_adj(DevModel M, float dt, const float* __restrict__ x,
+// Adjoint of g2p_differentiable: per particle dL/dx (partial) and dL/dFe,
+// and the scatter of dL/dv_out into block tiles (same ownership as k_p2g).
+__global__ __launch_bounds__(kBlkWG) void k_g2p_adj(DevModel M, float dt, const float* __restrict__ x,
                                                     const float* __restrict__ Fe, const float4* __restrict__ v_out,
                                                     const float* __restrict__ dx1, const float* __restrict__ dv1,
                                                     const float* __restrict__ dC1, const float* __restrict__ dF1,
-                                                    float* __restrict__ xbar, float* __restrict__ Febar,
-                                                    float4* __restrict__ gadj_v) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= M.N) return;
-  if (M.sel && M.sel[p] != 0) {
-    xbar[3 * p] = xbar[3 * p + 1] = xbar[3 * p + 2] = 0.f;
-    float Z[9];
-    m_zero(Z);
-    store9(Febar, p, Z);
-    return;
-  }
-  const float xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
-  Stencil s;
-  make_stencil(xp, M.inv_dx, s);
-  // pass 1: v' and grad v for the clamp mask and the F adjoint
-  float nv[3] = {0.f, 0.f, 0.f}, nG[9];
-  m_zero(nG);
+                                                    float* __restrict__ xbar, float* __restrict__ Febar, Binning B,
+                                                    float4* __restrict__ tiles, float4* __restrict__ ovf) {
+  __shared__ float tile[4][kTileNodes];
+  const int nblk = B.nb * B.nb * B.nb;
+  const float inv_dx = M.inv_dx;
+  const float c4 = inv_dx * 4.f;
+  const int nact = *B.nactive;
+  (void)nblk;
+  for (int ai = blockIdx.x; ai < nact; ai += gridDim.x) {
+    const int blk = B.active[ai];
+    const int start = B.offs[blk * 8], end = B.offs[blk * 8 + 8];
+    int o[3];
+    tile_origin(blk, B.nb, o);
+    for (int i = threadIdx.x; i < 4 * kTileNodes; i += blockDim.x) (&tile[0][0])[i] = 0.f;
+    __syncthreads();
+    for (int i0 = start; i0 < end; i0 += blockDim.x) {
+      const int i = i0 + (int)threadIdx.x;
+      const int p = i < end ? B.perm[i] : -1;
+      const bool active = p >= 0 && !(M.sel && M.sel[p] != 0);
+      float xp[3] = {(o[0] + 2.f) * M.dx, (o[1] + 2.f) * M.dx, (o[2] + 2.f) * M.dx};
+      if (active) {
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
-        float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (node_in(ix, iy, iz, M.n)) gv = v_out[node_id(ix, iy, iz, M.n)];
-        const float gvv[3] = {gv.x, gv.y, gv.z};
-        const float wt = s.w[0][a] * s.w[1][b] * s.w[2][c];
-        const float gw[3] = {s.dw[0][a] * s.w[1][b] * s.w[2][c] * M.inv_dx, s.w[0][a] * s.dw[1][b] * s.w[2][c] * M.inv_dx,
-                             s.w[0][a] * s.w[1][b] * s.dw[2][c] * M.inv_dx};
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          nv[i] += gvv[i] * wt;
-#pragma unroll
-          for (int j = 0; j < 3; ++j) nG[i * 3 + j] += gvv[i] * gw[j];
-        }
+        for (int d = 0; d < 3; ++d) xp[d] = x[3 * p + d];
       }
-  const float amin = M.dx * 2.f, amax = M.grid_lim - M.dx * 2.f;
-  float xb[3], nvb[3];
+      Stencil s;
+      make_stencil(xp, inv_dx, s);
+      float nvb[3] = {0.f, 0.f, 0.f}, Gb[9], dC[9];
+      m_zero(Gb);
+      m_zero(dC);
+      if (active) {
+        // pass 1: v' and grad v for the clamp mask and the F adjoint
+        float nv[3] = {0.f, 0.f, 0.f}, nG[9];
+        m_zero(nG);
+#pragma unroll 1
+        for (int j = 0; j < 27; ++j) {
+          const int a = j / 9, b = (j / 3) % 3, c = j % 3;
+          const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
+          float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (node_in(ix, iy, iz, M.n)) gv = v_out[node_id(ix, iy, iz, M.n)];
+          const float gvv[3] = {gv.x, gv.y, gv.z};
+          const float wt = sw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c);
+          const float gw[3] = {sdw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c) * inv_dx, sw(s, 0, a) * sdw(s, 1, b) * sw(s, 2, c) * inv_dx,
+                               sw(s, 0, a) * sw(s, 1, b) * sdw(s, 2, c) * inv_dx};
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    const float xn = xp[i] + dt * nv[i];
-    // wp.clamp passes the adjoint to x unless x < lo or x > hi
-    const float msk = (xn < amin || xn > amax) ? 0.f : 1.f;
-    const float g = dx1[3 * p + i] * msk;
-    xb[i] = g;
-    nvb[i] = dv1[3 * p + i] + dt * g;
-  }
-  float dF[9], Fp[9], dC[9];
-  load9(dF1, p, dF);
-  load9(Fe, p, Fp);
-  load9(dC1, p, dC);
-  // F1 = (I + dt G) Fe:  Gbar = dt dF Fe^T ; Febar = (I + dt G)^T dF
-  float Gb[9], IF[9], Feb[9];
-  m_mul_bt(dF, Fp, Gb);
+          for (int k = 0; k < 3; ++k) {
+            nv[k] += gvv[k] * wt;
 #pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    Gb[i] *= dt;
-    IF[i] = ((i % 4 == 0) ? 1.f : 0.f) + nG[i] * dt;
-  }
-  m_mul_at(IF, dF, Feb);
-  store9(Febar, p, Feb);
-  const float c4 = M.inv_dx * 4.f;
-  float fb[3] = {0.f, 0.f, 0.f};
+            for (int l = 0; l < 3; ++l) nG[k * 3 + l] += gvv[k] * gw[l];
+          }
+        }
+        const float amin = M.dx * 2.f, amax = M.grid_lim - M.dx * 2.f;
+        float xb[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
+        for (int k = 0; k < 3; ++k) {
+          const float xn = xp[k] + dt * nv[k];
+          // wp.clamp passes the adjoint to x unless x < lo or x > hi
+          const float msk = (xn < amin || xn > amax) ? 0.f : 1.f;
+          const float g = dx1[3 * p + k] * msk;
+          xb[k] = g;
+          nvb[k] = dv1[3 * p + k] + dt * g;
+        }
+        float dF[9], Fp[9];
+        load9(dF1, p, dF);
+        load9(Fe, p, Fp);
+        load9(dC1, p, dC);
+        // F1 = (I + dt G) Fe:  Gbar = dt dF Fe^T ; Febar = (I + dt G)^T dF
+        float IF[9], Feb[9];
+        m_mul_bt(dF, Fp, Gb);
 #pragma unroll
-    for (int b = 0; b < 3; ++b)
+        for (int k = 0; k < 9; ++k) {
+          Gb[k] *= dt;
+          IF[k] = ((k % 4 == 0) ? 1.f : 0.f) + nG[k] * dt;
+        }
+        m_mul_at(IF, dF, Feb);
+        store9(Febar, p, Feb);
+        // pass 2: d(w, grad w, d)/dx
+        float fb[3] = {0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int j = 0; j < 27; ++j) {
+          const int a = j / 9, b = (j / 3) % 3, c = j % 3;
+          const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
+          float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (node_in(ix, iy, iz, M.n)) gv = v_out[node_id(ix, iy, iz, M.n)];
+          const float vi[3] = {gv.x, gv.y, gv.z};
+          const float d[3] = {(float)a - s.f[0], (float)b - s.f[1], (float)c - s.f[2]};
+          const float wt = sw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c);
+          float Cd[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
-        const bool inb = node_in(ix, iy, iz, M.n);
-        float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (inb) gv = v_out[node_id(ix, iy, iz, M.n)];
-        const float vi[3] = {gv.x, gv.y, gv.z};
+          for (int k = 0; k < 3; ++k) Cd[k] = dC[k * 3 + 0] * d[0] + dC[k * 3 + 1] * d[1] + dC[k * 3 + 2] * d[2];
+          const float wb =
+              nvb[0] * vi[0] + nvb[1] * vi[1] + nvb[2] * vi[2] + c4 * (vi[0] * Cd[0] + vi[1] * Cd[1] + vi[2] * Cd[2]);
+          float gwb[3];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            // d = off - f: dd_k = 4 inv_dx w sum_i dC_ik v_i
+            fb[k] -= c4 * wt * (dC[0 * 3 + k] * vi[0] + dC[1 * 3 + k] * vi[1] + dC[2 * 3 + k] * vi[2]);
+            gwb[k] = Gb[0 * 3 + k] * vi[0] + Gb[1 * 3 + k] * vi[1] + Gb[2 * 3 + k] * vi[2];
+          }
+          weight_adj(s, a, b, c, inv_dx, wb, gwb, fb);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) xbar[3 * p + k] = xb[k] + inv_dx * fb[k];
+      } else if (p >= 0) {
+        xbar[3 * p] = xbar[3 * p + 1] = xbar[3 * p + 2] = 0.f;
+        float Z[9];
+        m_zero(Z);
+        store9(Febar, p, Z);
+      }
+      auto contrib = [&](int j, float* out) {
+        const int a = j / 9, b = (j / 3) % 3, c = j % 3;
         const float d[3] = {(float)a - s.f[0], (float)b - s.f[1], (float)c - s.f[2]};
-        const float wt = s.w[0][a] * s.w[1][b] * s.w[2][c];
-        const float gw[3] = {s.dw[0][a] * s.w[1][b] * s.w[2][c] * M.inv_dx, s.w[0][a] * s.dw[1][b] * s.w[2][c] * M.inv_dx,
-                             s.w[0][a] * s.w[1][b] * s.dw[2][c] * M.inv_dx};
-        float Cd[3], vb[3];
+        const float wt = sw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c);
+        const float gw[3] = {sdw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c) * inv_dx, sw(s, 0, a) * sdw(s, 1, b) * sw(s, 2, c) * inv_dx,
+                             sw(s, 0, a) * sw(s, 1, b) * sdw(s, 2, c) * inv_dx};
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          Cd[i] = dC[i * 3 + 0] * d[0] + dC[i * 3 + 1] * d[1] + dC[i * 3 + 2] * d[2];
-          vb[i] = wt * nvb[i] + c4 * wt * Cd[i] + (Gb[i * 3 + 0] * gw[0] + Gb[i * 3 + 1] * gw[1] + Gb[i * 3 + 2] * gw[2]);
+        for (int k = 0; k < 3; ++k) {
+          const float Cd = dC[k * 3 + 0] * d[0] + dC[k * 3 + 1] * d[1] + dC[k * 3 + 2] * d[2];
+          out[k] = wt * nvb[k] + c4 * wt * Cd + (Gb[k * 3 + 0] * gw[0] + Gb[k * 3 + 1] * gw[1] + Gb[k * 3 + 2] * gw[2]);
         }
-        if (inb) {
-          float4* g = &gadj_v[node_id(ix, iy, iz, M.n)];
-          atomicAdd(&g->x, vb[0]);
-          atomicAdd(&g->y, vb[1]);
-          atomicAdd(&g->z, vb[2]);
-        }
-        const float wb = nvb[0] * vi[0] + nvb[1] * vi[1] + nvb[2] * vi[2] + c4 * (vi[0] * Cd[0] + vi[1] * Cd[1] + vi[2] * Cd[2]);
-        float gwb[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          // d(d_j): 4 inv_dx w sum_i dC_ij v_i ; d = off - f
-          const float db = c4 * wt * (dC[0 * 3 + j] * vi[0] + dC[1 * 3 + j] * vi[1] + dC[2 * 3 + j] * vi[2]);
-          fb[j] -= db;
-          gwb[j] = Gb[0 * 3 + j] * vi[0] + Gb[1 * 3 + j] * vi[1] + Gb[2 * 3 + j] * vi[2];
-        }
-        weight_adj(s, a, b, c, M.inv_dx, wb, gwb, fb);
-      }
-#pragma unroll
-  for (int i = 0; i < 3; ++i) xbar[3 * p + i] = xb[i] + M.inv_dx * fb[i];
+        out[3] = 0.f;
+      };
+      const bool in_tile = active && stencil_in_tile(s.base, o);
+      scatter_stencil<3>(in_tile, s.base, contrib, [&](int ix, int iy, int iz, int comp, float val) {
+        atomicAdd(&tile[comp][((ix - o[0]) * kTileN + (iy - o[1])) * kTileN + (iz - o[2])], val);
+      });
+      if (active && !in_tile) scatter_overflow<3>(ovf, M.n, s.base, contrib);
+    }
+    __syncthreads();
+    float4* dst = tiles + (size_t)blk * kTileNodes;
+    for (int k = threadIdx.x; k < kTileNodes; k += blockDim.x)
+      dst[k] = make_float4(tile[0][k], tile[1][k], tile[2][k], 0.f);
+    __syncthreads();
+  }
 }
 
-__global__ __launch_bounds__(kBlock) void k_grid_adj(DevModel M, ColTable T, float time, float dt,
-                                                     float4* __restrict__ grid_in, float4* __restrict__ gadj_v,
-                                                     float4* __restrict__ gadj_in, int drop_mass) {
+// Adjoint of the grid stage: gather dL/dv_out (tiles + overflow), replay the
+// node's forward (normalisation, damping, colliders) to get dv_out/dv per
+// component, and write dL/d(mv, m).  The grid-mass term is dropped when
+// `drop_mass` is set (reference Tape behaviour).
+__global__ __launch_bounds__(kBlock) void k_grid_adj(DevModel M, ColTable T, float time, float dt, Binning B,
+                                                     const float4* __restrict__ tiles, float4* __restrict__ ovf,
+                                                     const float4* __restrict__ keep, float4* __restrict__ gadj_in,
+                                                     int drop_mass) {
   const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t G = (size_t)M.n * M.n * M.n;
   if (id >= G) return;
   const int iz = (int)(id % M.n);
   const int iy = (int)((id / M.n) % M.n);
   const int ix = (int)(id / ((size_t)M.n * M.n));
-  const float4 g = grid_in[id];
-  const float4 b = gadj_v[id];
+  float4 b = gather_node(B, tiles, ix, iy, iz);
+  const float4 ob = ovf[id];
+  if (ob.x != 0.f || ob.y != 0.f || ob.z != 0.f) {
+    b.x += ob.x;
+    b.y += ob.y;
+    b.z += ob.z;
+    ovf[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  const float4 g = keep[id];
   float v[3], fac[3];
   grid_node(M, T, time, dt, ix, iy, iz, g, v, fac);
   float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -545,10 +819,9 @@ __global__ __launch_bounds__(kBlock) void k_grid_adj(DevModel M, ColTable T, flo
     o.w = drop_mass ? 0.f : -(b0 * g.x + b1 * g.y + b2 * g.z) * im * im;
   }
   gadj_in[id] = o;
-  gadj_v[id] = make_float4(0.f, 0.f, 0.f, 0.f);
-  grid_in[id] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+template <int MAT>
 __global__ __launch_bounds__(kBlock) void k_p2g_adj(DevModel M, float dt, const float* __restrict__ x,
                                                     const float* __restrict__ v, const float* __restrict__ C,
                                                     const float* __restrict__ F, const float4* __restrict__ gadj_in,
@@ -573,9 +846,11 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(DevModel M, float dt, const 
   load9(F, p, Ftr);
   float mu = M.mu[p], lam = M.lam[p], ys = M.ys[p];
   const float mu0 = mu, lam0 = lam, ys0 = ys;
+  MatParams matp = M.mp;
+  matp.material = MAT;
   ConstitState st;
   float tau[9];
-  constitutive_fwd(st, Ftr, mu, lam, ys, M.mp, dt, tau);
+  constitutive_fwd(st, Ftr, mu, lam, ys, matp, dt, tau);
 
   const float xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
   const float vp[3] = {v[3 * p], v[3 * p + 1], v[3 * p + 2]};
@@ -601,9 +876,9 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(DevModel M, float dt, const 
         const float av[3] = {ga.x, ga.y, ga.z};
         const float bm = ga.w;
         const float dpos[3] = {((float)a - s.f[0]) * M.dx, ((float)b - s.f[1]) * M.dx, ((float)c - s.f[2]) * M.dx};
-        const float wt = s.w[0][a] * s.w[1][b] * s.w[2][c];
-        const float gw[3] = {s.dw[0][a] * s.w[1][b] * s.w[2][c] * M.inv_dx, s.w[0][a] * s.dw[1][b] * s.w[2][c] * M.inv_dx,
-                             s.w[0][a] * s.w[1][b] * s.dw[2][c] * M.inv_dx};
+        const float wt = sw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c);
+        const float gw[3] = {sdw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c) * M.inv_dx, sw(s, 0, a) * sdw(s, 1, b) * sw(s, 2, c) * M.inv_dx,
+                             sw(s, 0, a) * sw(s, 1, b) * sdw(s, 2, c) * M.inv_dx};
         float u[3], tg[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -657,8 +932,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(DevModel M, float dt, const 
   load9(Febar_in, p, Feb);
   m_zero(Ftrb);
   float mub = 0.f, lamb = 0.f, ysb = 0.f;
-  stress_adj(st, M.material, mu0, lam0, taur, Feb, mub, lamb);
-  return_map_adj(st, Ftr, mu0, lam0, ys0, M.mp, dt, Feb, Ftrb, mub, lamb, ysb);
+  stress_adj(st, MAT, mu0, lam0, taur, Feb, mub, lamb);
+  return_map_adj(st, Ftr, mu0, lam0, ys0, matp, dt, Feb, Ftrb, mub, lamb, ysb);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     dx[3 * p + i] = xo[i];
@@ -673,7 +948,134 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(DevModel M, float dt, const 
   if (dvol) dvol[p] += Vb;
 }
 
-__global__ void k_mu_lam(int n, const float* __restrict__ E, const float* __restrict__ nu, float* __restrict__ mu,
+// ---------------------------------------------------------------------------
+// Binning: counting sort of particles by (block, cell) of their stencil base,
+// block-major (blocks of 2x2x2 cells, x-major block index), cell-minor.
+// Histogram with per-particle rank (same-bin lanes of a wave are counted with
+// one atomic), exclusive scan of the nb^3 * 8 + 1 counts, scatter.
+// ---------------------------------------------------------------------------
+EMPM_DEV int bin_of(const float* __restrict__ x, int p, float inv_dx, int n, int nb) {
+  int c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int v = (int)(x[3 * p + d] * inv_dx - 0.5f);
+    c[d] = v < 0 ? 0 : (v >= n ? n - 1 : v);
+  }
+  const int blk = ((c[0] >> 1) * nb + (c[1] >> 1)) * nb + (c[2] >> 1);
+  return blk * 8 + (((c[0] & 1) << 2) | ((c[1] & 1) << 1) | (c[2] & 1));
+}
+
+__global__ void k_fill_i32(int* __restrict__ a, size_t n, int v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin_hist(int N, const float* __restrict__ x, float inv_dx, int n, int nb,
+                                                     int* __restrict__ count, int* __restrict__ rank) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = __lane_id();
+  const bool active = p < N;
+  const int key = active ? bin_of(x, p, inv_dx, n, nb) : -1;
+  unsigned long long pending = __ballot(active);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int lkey = __shfl(key, leader);
+    const bool in = ((pending >> lane) & 1ull) && key == lkey;
+    const unsigned long long grp = __ballot(in);
+    pending &= ~grp;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&count[lkey], __popcll(grp));
+    base = __shfl(base, leader);
+    if (in) rank[p] = base + __popcll(grp & ((1ull << lane) - 1ull));
+  }
+}
+
+constexpr int kScanBlock = 256;
+constexpr int kScanPerThread = 4;
+constexpr int kScanTile = kScanBlock * kScanPerThread;
+
+// block-wide exclusive scan of one value per thread; returns the block total
+EMPM_DEV int block_exclusive_scan(int v, int* smem, int& total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) smem[wid] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int w = 0; w < kScanBlock / 64; ++w) {
+      const int t = smem[w];
+      smem[w] = acc;
+      acc += t;
+    }
+    smem[kScanBlock / 64] = acc;
+  }
+  __syncthreads();
+  total = smem[kScanBlock / 64];
+  const int r = smem[wid] + incl - v;
+  __syncthreads();
+  return r;
+}
+
+// in-place exclusive scan of each tile; tile totals to sums[]
+__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(int* __restrict__ a, size_t n, int* __restrict__ sums) {
+  __shared__ int smem[kScanBlock / 64 + 1];
+  const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanPerThread;
+  int v[kScanPerThread], s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPerThread; ++k) {
+    v[k] = base + k < n ? a[base + k] : 0;
+    s += v[k];
+  }
+  int total;
+  int off = block_exclusive_scan(s, smem, total);
+#pragma unroll
+  for (int k = 0; k < kScanPerThread; ++k) {
+    if (base + k < n) a[base + k] = off;
+    off += v[k];
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// exclusive scan of the tile sums by one workgroup (loops over chunks)
+__global__ __launch_bounds__(kScanBlock) void k_scan_sums(int* __restrict__ sums, int m) {
+  __shared__ int smem[kScanBlock / 64 + 1];
+  int carry = 0;
+  for (int start = 0; start < m; start += kScanBlock) {
+    const int i = start + threadIdx.x;
+    const int v = i < m ? sums[i] : 0;
+    int total;
+    const int e = block_exclusive_scan(v, smem, total);
+    if (i < m) sums[i] = carry + e;
+    carry += total;
+  }
+}
+
+__global__ void k_scan_add(int* __restrict__ a, size_t n, const int* __restrict__ sums) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] += sums[i / kScanTile];
+}
+
+// Compact list of the non-empty blocks (after the scan: offs are offsets).
+__global__ void k_bin_active(int nblk, const int* __restrict__ offs, int* __restrict__ active,
+                             int* __restrict__ nactive) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblk) return;
+  if (offs[b * 8 + 8] != offs[b * 8]) active[atomicAdd(nactive, 1)] = b;
+}
+
+__global__ void k_bin_scatter(int N, const float* __restrict__ x, float inv_dx, int n, int nb,
+                              const int* __restrict__ offs, const int* __restrict__ rank, int* __restrict__ perm) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  perm[offs[bin_of(x, p, inv_dx, n, nb)] + rank[p]] = p;
+}
+
+__global__ void k_mu_lam(int n,const float* __restrict__ E, const float* __restrict__ nu, float* __restrict__ mu,
                          float* __restrict__ lam) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -720,14 +1122,24 @@ struct empm_ctx {
   int device;
   int n_grid;
   int max_particles;
+  int nb;           // blocks per axis
+  int rebin_every;  // substeps between re-binnings inside a rollout call
   size_t G;
-  float4* grid_in;
-  float4* v_out;
-  float4* gadj_v;
-  float4* gadj_in;
-  float* Fe;
-  float* xbar;
-  float* Febar;
+  size_t nbins;     // nb^3 * 8
+  float4* tiles;    // nb^3 tiles of kTileNodes float4 (block-owned scatter)
+  float4* ovf;      // G: overflow grid for particles outside their tile (kept zero between uses)
+  float4* keep;     // G: (mv, m) of the substep being differentiated
+  float4* v_out;    // G
+  float4* gadj_in;  // G: dL/d(mv, m)
+  float* Fe;        // N*9 scratch
+  float* xbar;      // N*3 scratch
+  float* Febar;     // N*9 scratch
+  int* perm;        // N: particles in bin order
+  int* bin_count;   // nbins + 1: counts, then offsets
+  int* bin_rank;    // N
+  int* tile_sums;   // scan scratch
+  int* active;      // nb^3: compact list of non-empty blocks
+  int* nactive;     // 1: its length
   KernelProfile* prof;
 };
 
@@ -836,19 +1248,64 @@ DevModel dev_model(const empm_model* m) {
 
 unsigned nblocks(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+// Launch KERNEL<material>(args...): the constitutive kernels are compiled
+// once per material so each runs a single branch-free code path.
+#define EMPM_LAUNCH_MAT(material, KERNEL, grid, block, stream, ...)                              \
+  switch (material) {                                                                           \
+    case 0: hipLaunchKernelGGL(KERNEL<0>, grid, block, 0, stream, __VA_ARGS__); break;         \
+    case 1: hipLaunchKernelGGL(KERNEL<1>, grid, block, 0, stream, __VA_ARGS__); break;         \
+    case 2: hipLaunchKernelGGL(KERNEL<2>, grid, block, 0, stream, __VA_ARGS__); break;         \
+    case 3: hipLaunchKernelGGL(KERNEL<3>, grid, block, 0, stream, __VA_ARGS__); break;         \
+    case 4: hipLaunchKernelGGL(KERNEL<4>, grid, block, 0, stream, __VA_ARGS__); break;         \
+    case 5: hipLaunchKernelGGL(KERNEL<5>, grid, block, 0, stream, __VA_ARGS__); break;         \
+    case 6: hipLaunchKernelGGL(KERNEL<6>, grid, block, 0, stream, __VA_ARGS__); break;         \
+    default: hipLaunchKernelGGL(KERNEL<7>, grid, block, 0, stream, __VA_ARGS__); break;        \
+  }
+
+Binning binning(const empm_ctx* ctx) { return Binning{ctx->perm, ctx->bin_count, ctx->active, ctx->nactive, ctx->nb}; }
+
+unsigned block_grid(const empm_ctx* ctx) {
+  const size_t nblk = (size_t)ctx->nb * ctx->nb * ctx->nb;
+  const size_t cap = (size_t)kBlkGridCap;
+  return (unsigned)(nblk < cap ? nblk : cap);
+}
+
+// Bin the particles of a state (positions x) for the block-owned scatters.
+int do_bin(empm_ctx* ctx, int N, const float* x, float inv_dx, hipStream_t st) {
+  if (N <= 0) return EMPM_OK;
+  const int n = ctx->n_grid, nb = ctx->nb;
+  const size_t nbins = ctx->nbins + 1;
+  const unsigned tiles = (unsigned)((nbins + kScanTile - 1) / kScanTile);
+  hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(nbins)), dim3(kBlock), 0, st, ctx->bin_count, nbins, 0);
+  hipLaunchKernelGGL(k_bin_hist, dim3(nblocks(N)), dim3(kBlock), 0, st, N, x, inv_dx, n, nb, ctx->bin_count,
+                     ctx->bin_rank);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanBlock), 0, st, ctx->bin_count, nbins, ctx->tile_sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, st, ctx->tile_sums, (int)tiles);
+  hipLaunchKernelGGL(k_scan_add, dim3(nblocks(nbins)), dim3(kBlock), 0, st, ctx->bin_count, nbins, ctx->tile_sums);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(N)), dim3(kBlock), 0, st, N, x, inv_dx, n, nb, ctx->bin_count,
+                     ctx->bin_rank, ctx->perm);
+  const int nblk = nb * nb * nb;
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, ctx->nactive, (size_t)1, 0);
+  hipLaunchKernelGGL(k_bin_active, dim3(nblocks(nblk)), dim3(kBlock), 0, st, nblk, ctx->bin_count, ctx->active,
+                     ctx->nactive);
+  HIP_TRY(hipGetLastError());
+  return EMPM_OK;
+}
+
 int launch_fwd(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, float dt, const empm_particles* in,
                const empm_particles* out, hipStream_t st) {
   const int write_params = (M.material == 5 || M.mp.hardening == 1.f) ? 1 : 0;
   float* Fe = in->Fe ? in->Fe : ctx->Fe;
+  const Binning B = binning(ctx);
   if (M.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
-    hipLaunchKernelGGL(k_p2g, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, in->v, in->C, in->F, Fe,
-                       ctx->grid_in, write_params);
+    EMPM_LAUNCH_MAT(M.material, k_p2g, dim3(block_grid(ctx)), dim3(kBlkWG), st, M, dt, in->x, in->v, in->C, in->F, Fe,
+                    B, ctx->tiles, ctx->ovf, write_params);
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
-    hipLaunchKernelGGL(k_grid, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, ctx->grid_in, ctx->v_out,
-                       1);
+    hipLaunchKernelGGL(k_grid, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, B, ctx->tiles, ctx->ovf,
+                       (float4*)nullptr, ctx->v_out);
   }
   if (M.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_G2P, st);
@@ -862,31 +1319,32 @@ int launch_fwd(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, 
 int launch_adj(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, float dt, const empm_particles* in,
                const empm_particles* d_out, const empm_particles* d_in, float* d_mu, float* d_lam, float* d_ys,
                float* d_mass, float* d_vol, int flags, hipStream_t st) {
+  const Binning B = binning(ctx);
   if (M.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
-    hipLaunchKernelGGL(k_p2g, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, in->v, in->C, in->F, ctx->Fe,
-                       ctx->grid_in, 0);
+    EMPM_LAUNCH_MAT(M.material, k_p2g, dim3(block_grid(ctx)), dim3(kBlkWG), st, M, dt, in->x, in->v, in->C, in->F,
+                    ctx->Fe, B, ctx->tiles, ctx->ovf, 0);
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
-    hipLaunchKernelGGL(k_grid, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, ctx->grid_in, ctx->v_out,
-                       0);
+    hipLaunchKernelGGL(k_grid, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, B, ctx->tiles, ctx->ovf,
+                       ctx->keep, ctx->v_out);
   }
   if (M.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_G2P_ADJ, st);
-    hipLaunchKernelGGL(k_g2p_adj, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, ctx->Fe, ctx->v_out,
-                       d_out->x, d_out->v, d_out->C, d_out->F, ctx->xbar, ctx->Febar, ctx->gadj_v);
+    hipLaunchKernelGGL(k_g2p_adj, dim3(block_grid(ctx)), dim3(kBlkWG), 0, st, M, dt, in->x, ctx->Fe, ctx->v_out,
+                       d_out->x, d_out->v, d_out->C, d_out->F, ctx->xbar, ctx->Febar, B, ctx->tiles, ctx->ovf);
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID_ADJ, st);
-    hipLaunchKernelGGL(k_grid_adj, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, ctx->grid_in,
-                       ctx->gadj_v, ctx->gadj_in, (flags & EMPM_ADJ_DROP_GRID_MASS) ? 1 : 0);
+    hipLaunchKernelGGL(k_grid_adj, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, B, ctx->tiles,
+                       ctx->ovf, ctx->keep, ctx->gadj_in, (flags & EMPM_ADJ_DROP_GRID_MASS) ? 1 : 0);
   }
   if (M.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G_ADJ, st);
-    hipLaunchKernelGGL(k_p2g_adj, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, in->v, in->C, in->F,
-                       ctx->gadj_in, ctx->xbar, ctx->Febar, d_in->x, d_in->v, d_in->C, d_in->F, d_mu, d_lam, d_ys,
-                       d_mass, d_vol);
+    EMPM_LAUNCH_MAT(M.material, k_p2g_adj, dim3(nblocks(M.N)), dim3(kBlock), st, M, dt, in->x, in->v, in->C, in->F,
+                    ctx->gadj_in, ctx->xbar, ctx->Febar, d_in->x, d_in->v, d_in->C, d_in->F, d_mu, d_lam, d_ys, d_mass,
+                    d_vol);
   }
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
@@ -931,18 +1389,29 @@ int empm_create(empm_ctx** out, const empm_create_desc* desc) {
   c->device = desc->device;
   c->n_grid = desc->n_grid;
   c->max_particles = desc->max_particles;
+  c->nb = (desc->n_grid + kBlkCells - 1) / kBlkCells;
+  c->rebin_every = 25;
   c->G = (size_t)desc->n_grid * desc->n_grid * desc->n_grid;
+  c->nbins = (size_t)c->nb * c->nb * c->nb * 8;
   const size_t np = desc->max_particles > 0 ? (size_t)desc->max_particles : 1;
+  const size_t ntiles = (size_t)c->nb * c->nb * c->nb;
   hipError_t e = hipSuccess;
-  e = e == hipSuccess ? hipMalloc((void**)&c->grid_in, c->G * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->tiles, ntiles * kTileNodes * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->ovf, c->G * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->keep, c->G * sizeof(float4)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->v_out, c->G * sizeof(float4)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->gadj_v, c->G * sizeof(float4)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->gadj_in, c->G * sizeof(float4)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->Fe, np * 9 * sizeof(float)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->xbar, np * 3 * sizeof(float)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->Febar, np * 9 * sizeof(float)) : e;
-  e = e == hipSuccess ? hipMemset(c->grid_in, 0, c->G * sizeof(float4)) : e;
-  e = e == hipSuccess ? hipMemset(c->gadj_v, 0, c->G * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->perm, np * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->bin_count, (c->nbins + 1) * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->bin_rank, np * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->tile_sums, ((c->nbins + 1) / kScanTile + 2) * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->active, ntiles * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->nactive, sizeof(int)) : e;
+  e = e == hipSuccess ? hipMemset(c->nactive, 0, sizeof(int)) : e;
+  e = e == hipSuccess ? hipMemset(c->ovf, 0, c->G * sizeof(float4)) : e;
   e = e == hipSuccess ? hipDeviceSynchronize() : e;
   if (e != hipSuccess) {
     empm_destroy(c);
@@ -965,14 +1434,40 @@ void empm_destroy(empm_ctx* c) {
     for (hipEvent_t e : c->prof->pool) hipEventDestroy(e);
     delete c->prof;
   }
-  hipFree(c->grid_in);
+  hipFree(c->tiles);
+  hipFree(c->ovf);
+  hipFree(c->keep);
   hipFree(c->v_out);
-  hipFree(c->gadj_v);
   hipFree(c->gadj_in);
   hipFree(c->Fe);
   hipFree(c->xbar);
   hipFree(c->Febar);
+  hipFree(c->perm);
+  hipFree(c->bin_count);
+  hipFree(c->bin_rank);
+  hipFree(c->tile_sums);
+  hipFree(c->active);
+  hipFree(c->nactive);
   delete c;
+}
+
+int empm_sort_by_cell(empm_ctx* ctx, int n_particles, const float* x, float inv_dx, int* perm, void* stream) {
+  if (!ctx || n_particles < 0 || n_particles > ctx->max_particles || (n_particles > 0 && (!x || !perm)) ||
+      !(inv_dx > 0.f))
+    return set_err(EMPM_E_INVALID_ARG, "empm_sort_by_cell: bad arguments");
+  if (n_particles == 0) return EMPM_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  int rc = do_bin(ctx, n_particles, x, inv_dx, (hipStream_t)stream);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(perm, ctx->perm, (size_t)n_particles * sizeof(int), hipMemcpyDeviceToDevice,
+                         (hipStream_t)stream));
+  return EMPM_OK;
+}
+
+int empm_set_rebin_interval(empm_ctx* ctx, int substeps) {
+  if (!ctx || substeps < 1) return set_err(EMPM_E_INVALID_ARG, "empm_set_rebin_interval: substeps must be >= 1");
+  ctx->rebin_every = substeps;
+  return EMPM_OK;
 }
 
 int empm_profile_enable(empm_ctx* ctx, unsigned kernel_mask) {
@@ -1060,6 +1555,7 @@ int empm_substep_fwd(empm_ctx* ctx, const empm_model* model, const empm_collider
   ColTable T;
   if ((rc = make_table(cols, ncol, T, model->n_grid))) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
+  if ((rc = do_bin(ctx, model->n_particles, in->x, model->inv_dx, (hipStream_t)stream))) return rc;
   return launch_fwd(ctx, dev_model(model), T, time, dt, in, out, (hipStream_t)stream);
 }
 
@@ -1076,6 +1572,7 @@ int empm_substep_adj(empm_ctx* ctx, const empm_model* model, const empm_collider
   ColTable T;
   if ((rc = make_table(cols, ncol, T, model->n_grid))) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
+  if ((rc = do_bin(ctx, model->n_particles, in->x, model->inv_dx, (hipStream_t)stream))) return rc;
   return launch_adj(ctx, dev_model(model), T, time, dt, in, d_out, d_in, d_mu, d_lam, d_ys, d_mass, d_vol, flags,
                     (hipStream_t)stream);
 }
@@ -1099,6 +1596,9 @@ int empm_rollout_fwd(empm_ctx* ctx, const empm_model* model, empm_collider* cols
     make_table(cols, ncol, T, model->n_grid);
     if (cols_hist) memcpy(&cols_hist[(size_t)k * ncol], cols, sizeof(empm_collider) * ncol);
     if (times_hist) times_hist[k] = *time;
+    if (k % ctx->rebin_every == 0 &&
+        (rc = do_bin(ctx, model->n_particles, states[k].x, model->inv_dx, (hipStream_t)stream)))
+      return rc;
     if ((rc = launch_fwd(ctx, M, T, (float)*time, (float)dt, &states[k], &states[k + 1], (hipStream_t)stream)))
       return rc;
     advance_colliders(cols, ncol, *time, dt);
@@ -1124,6 +1624,9 @@ int empm_rollout_fwd_replay(empm_ctx* ctx, const empm_model* model, const empm_c
   ColTable T;
   for (int k = 0; k < n_steps; ++k) {
     if ((rc = make_table(&cols_hist[(size_t)k * ncol], ncol, T, model->n_grid))) return rc;
+    if (k % ctx->rebin_every == 0 &&
+        (rc = do_bin(ctx, model->n_particles, states[k].x, model->inv_dx, (hipStream_t)stream)))
+      return rc;
     if ((rc = launch_fwd(ctx, M, T, (float)times_hist[k], (float)dt, &states[k], &states[k + 1], (hipStream_t)stream)))
       return rc;
   }
@@ -1152,6 +1655,9 @@ int empm_rollout_adj(empm_ctx* ctx, const empm_model* model, const empm_collider
     if ((rc = make_table(&cols_hist[(size_t)k * ncol], ncol, T, model->n_grid))) return rc;
     const empm_particles* dout = &adj[j % 2];
     const empm_particles* din = &adj[(j + 1) % 2];
+    if (j % ctx->rebin_every == 0 &&
+        (rc = do_bin(ctx, model->n_particles, states[k].x, model->inv_dx, (hipStream_t)stream)))
+      return rc;
     if ((rc = launch_adj(ctx, M, T, (float)times_hist[k], (float)dt, &states[k], dout, din, d_mu, d_lam, d_ys, d_mass,
                          d_vol, flags, (hipStream_t)stream)))
       return rc;

@@ -55,6 +55,9 @@ class Context:
     def handle(self):
         return self._h
 
+    def set_rebin_interval(self, substeps: int):
+        check(self.lib.empm_set_rebin_interval(self._h, int(substeps)))
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             self.lib.empm_destroy(self._h)
@@ -221,6 +224,8 @@ class RolloutTape:
     checkpoints: dict = field(default_factory=dict)  # step -> ParticleState
     cols_hist: object = None  # (Collider * (n_steps*ncol))
     times_hist: object = None  # (c_double * n_steps)
+    perm: object = None  # cell order used internally (None: caller order)
+    model: object = None  # ModelArrays in the internal order
 
 
 def _states_array(states: list[ParticleState]):
@@ -238,21 +243,72 @@ class Rollout:
     ``record`` (a sorted list of substep counts k >= 1: x after k substeps) and,
     when ``tape`` is requested, a RolloutTape for ``backward``."""
 
-    def __init__(self, ctx: Context, model: ModelArrays, n_particles: int):
+    def __init__(self, ctx: Context, model: ModelArrays, n_particles: int, sort: bool = True):
         self.ctx = ctx
         self.model = model
         self.n = int(n_particles)
+        self.sort = sort
         dev = ctx.device
         self._buf = [ParticleState.empty(self.n, dev), ParticleState.empty(self.n, dev)]
+
+    # ------------------------------------------------------------ ordering
+    def cell_order(self, x: torch.Tensor) -> torch.Tensor:
+        perm = torch.empty(self.n, dtype=torch.int32, device=x.device)
+        check(self.ctx.lib.empm_sort_by_cell(self.ctx.handle, self.n, _ptr(x.contiguous()),
+                                             C.c_float(self.model.inv_dx), _ptr(perm), stream_handle(x.device)))
+        return perm.long()
+
+    @staticmethod
+    def _gather(perm, t):
+        return t.index_select(0, perm).contiguous() if t is not None else None
+
+    @staticmethod
+    def _scatter(perm, t):
+        out = torch.empty_like(t)
+        out.index_copy_(0, perm, t)
+        return out
+
+    def _permuted_model(self, perm) -> ModelArrays:
+        m = self.model
+        g = lambda t: self._gather(perm, t)
+        return ModelArrays(material=m.material, n_grid=m.n_grid, grid_lim=m.grid_lim, dx=m.dx, inv_dx=m.inv_dx,
+                           mu=g(m.mu), lam=g(m.lam), yield_stress=g(m.yield_stress), mass=g(m.mass), vol=g(m.vol),
+                           selection=g(m.selection), rpic_damping=m.rpic_damping,
+                           grid_v_damping_scale=m.grid_v_damping_scale, gravity=m.gravity, alpha=m.alpha,
+                           hardening=m.hardening, xi=m.xi, plastic_viscosity=m.plastic_viscosity,
+                           softening=m.softening)
+
+    def _perm_state(self, perm, s: ParticleState, inverse=False):
+        f = self._scatter if inverse else self._gather
+        return ParticleState(f(perm, s.x), f(perm, s.v), f(perm, s.C), f(perm, s.F))
 
     def forward(self, state: ParticleState, cols: list[Collider], time: float, dt: float, n_steps: int,
                 record=(), tape: bool = False, ckpt_every: int = 50, cols_out: list | None = None):
         """Returns (final_state, final_time, recorded_x list, tape or None).
         `cols` (ctypes Collider structs) are advanced in place like the
-        reference's modify_bc; `cols_out` receives the advanced table."""
+        reference's modify_bc; `cols_out` receives the advanced table.  With
+        sort=True the rollout runs on a cell-ordered copy of the particles
+        and every output is returned in the caller's order."""
+        if not self.sort or self.n == 0:
+            return self._forward_core(self.model, state, cols, time, dt, n_steps, record, tape, ckpt_every,
+                                      cols_out)
+        perm = self.cell_order(state.x)
+        pm = self._permuted_model(perm)
+        s_sorted = self._perm_state(perm, state)
+        final, t1, xs, tp = self._forward_core(pm, s_sorted, cols, time, dt, n_steps, record, tape, ckpt_every,
+                                               cols_out)
+        if pm.material == 5 or pm.hardening == 1.0:  # in-place parameter updates go back to the caller's arrays
+            for a, b in ((self.model.mu, pm.mu), (self.model.lam, pm.lam), (self.model.yield_stress, pm.yield_stress)):
+                a.index_copy_(0, perm, b)
+        if tp is not None:
+            tp.perm = perm
+            tp.model = pm
+        return self._perm_state(perm, final, inverse=True), t1, [self._scatter(perm, x) for x in xs], tp
+
+    def _forward_core(self, model, state, cols, time, dt, n_steps, record, tape, ckpt_every, cols_out):
         lib = self.ctx.lib
         arr, ncol = collider_array(cols)
-        m = self.model.struct()
+        m = model.struct()
         t = C.c_double(time)
         rec = sorted(set(int(k) for k in record if 1 <= int(k) <= n_steps))
         tp = None
@@ -298,14 +354,30 @@ class Rollout:
                  grads: ParamGrads | None = None):
         """Reverse sweep.  d_final: dL/d(state n_steps).  seeds: {k: ParticleState
         cotangent of state k} added when the sweep reaches state k (0 < k < n).
-        Returns (dL/d(state 0) as ParticleState, ParamGrads)."""
+        Returns (dL/d(state 0) as ParticleState, ParamGrads); all in the
+        caller's particle order.  `grads`, if given, is accumulated into."""
+        seeds = seeds or {}
+        if tp.perm is None:
+            return self._backward_core(self.model, tp, d_final, seeds, flags, grads)
+        perm = tp.perm
+        d_sorted = self._perm_state(perm, d_final)
+        seeds_sorted = {k: self._perm_state(perm, v) for k, v in seeds.items()}
+        d0, g = self._backward_core(tp.model, tp, d_sorted, seeds_sorted, flags, None)
+        gu = ParamGrads(*(self._scatter(perm, t) for t in (g.mu, g.lam, g.ys, g.mass, g.vol)))
+        if grads is not None:
+            for a, b in ((grads.mu, gu.mu), (grads.lam, gu.lam), (grads.ys, gu.ys), (grads.mass, gu.mass),
+                         (grads.vol, gu.vol)):
+                a.add_(b)
+            gu = grads
+        return self._perm_state(perm, d0, inverse=True), gu
+
+    def _backward_core(self, model, tp, d_final, seeds, flags, grads):
         lib = self.ctx.lib
         dev = self.ctx.device
         n = self.n
         if grads is None:
             grads = ParamGrads.zeros(n, dev)
-        seeds = seeds or {}
-        m = self.model.struct()
+        m = model.struct()
         stream = stream_handle(dev)
         ncol = tp.ncol
         K = tp.ckpt_every

@@ -120,30 +120,38 @@ def sample_params_c2(seed):
     return {"E": float(np.exp(logE)), "nu": float(nu), "yield_stress": float(np.exp(logY))}
 
 
-def ellipsoid_c2(n_particles=200_000, n_grid=64, n_steps=400, param_seed=3, material=7, seed=1):
+def ellipsoid_c2(n_particles=200_000, n_grid=64, n_steps=400, param_seed=3, material=7, seed=1, drop_height=0.3):
     """Single rollout forward + adjoint: 200k particles in an ellipsoid with
-    radii (0.2, 0.1, 0.15) centred at (0.5, 0.5, 0.3) on a 64^3 grid,
-    Neo-Hookean + von Mises (material 7), dt = 5e-5, 400 substeps,
-    rho = 1000, 3-cell sticky ground/walls.  Parameters drawn from
-    `param_seed` (log-uniform E in [1e4,1e6], uniform nu in [0.2,0.45],
-    log-uniform yield stress in [1e2,1e4])."""
+    radii (0.2, 0.1, 0.15) on a 64^3 grid, Neo-Hookean + von Mises
+    (material 7), dt = 5e-5, 400 substeps, rho = 1000, 3-cell sticky
+    ground/walls.  "Dropped from height 0.3": a 400-substep rollout lasts
+    0.02 s, too short for a body released at rest 0.3 m up to reach the
+    ground, so the rollout starts at the moment of impact, bottom just above
+    the sticky band with the impact velocity sqrt(2 g h) downwards.
+    Parameters drawn from `param_seed` (log-uniform E in [1e4,1e6], uniform
+    nu in [0.2,0.45], log-uniform yield stress in [1e2,1e4])."""
     rng = np.random.Generator(np.random.PCG64(seed))
     radii = (0.2, 0.1, 0.15)
-    x = ellipsoid_points((0.5, 0.5, 0.3), radii, n_particles, rng)
+    dx = 1.0 / n_grid
+    cz = 3 * dx + radii[2] + 0.25 * dx
+    x = ellipsoid_points((0.5, 0.5, cz), radii, n_particles, rng)
     N = x.shape[0]
     vol_total = 4.0 / 3.0 * np.pi * radii[0] * radii[1] * radii[2]
     prm = sample_params_c2(param_seed)
     f = lambda v: np.full(N, v, np.float32)
-    return Scene(name=f"ellipsoid_{N}_{n_grid}^3_nh_vm", n_grid=n_grid, grid_lim=1.0, dt=5e-5, n_steps=n_steps,
-                 material=material, x=x.astype(np.float32), vol=f(vol_total / N), rho=f(1000.0), E=f(prm["E"]),
-                 nu=f(prm["nu"]), yield_stress=f(prm["yield_stress"]), colliders=sticky_walls(n_grid, 3),
-                 params=prm)
+    sc = Scene(name=f"ellipsoid_{N}_{n_grid}^3_nh_vm", n_grid=n_grid, grid_lim=1.0, dt=5e-5, n_steps=n_steps,
+               material=material, x=x.astype(np.float32), vol=f(vol_total / N), rho=f(1000.0), E=f(prm["E"]),
+               nu=f(prm["nu"]), yield_stress=f(prm["yield_stress"]), colliders=sticky_walls(n_grid, 3), params=prm)
+    v0 = np.zeros((N, 3), np.float32)
+    v0[:, 2] = -np.sqrt(2 * 9.8 * drop_height)
+    sc.params["v0"] = v0
+    return sc
 
 
 def small_scene(n_particles=512, n_grid=16, material=0, seed=0, E=2e4, nu=0.3, ys=300.0, center=(0.5, 0.5, 0.45),
-                radii=(0.15, 0.12, 0.1), vel=(0.3, -0.2, 0.1), dt=1e-4):
-    """A small ellipsoid with an initial velocity and rotation, for parity
-    and finite-difference tests."""
+                radii=(0.15, 0.12, 0.1), vel=(0.3, -0.2, 0.1), dt=1e-4, spin=2.0):
+    """A small ellipsoid with an initial velocity and spin about z, for
+    parity and finite-difference tests."""
     rng = np.random.Generator(np.random.PCG64(seed))
     x = ellipsoid_points(center, radii, n_particles, rng)
     N = x.shape[0]
@@ -153,7 +161,7 @@ def small_scene(n_particles=512, n_grid=16, material=0, seed=0, E=2e4, nu=0.3, y
                x=x.astype(np.float32), vol=f(vol_total / N), rho=f(1000.0), E=f(E), nu=f(nu), yield_stress=f(ys),
                colliders=sticky_walls(n_grid, 3))
     c = np.asarray(center)
-    w = np.array([0.0, 0.0, 2.0])
+    w = np.array([0.0, 0.0, spin])
     v0 = np.asarray(vel)[None, :] + np.cross(w[None, :], x - c[None, :])
     sc.params["v0"] = v0.astype(np.float32)
     return sc

@@ -204,6 +204,22 @@ int empm_rollout_adj(empm_ctx *ctx, const empm_model *model, const empm_collider
                      const empm_particles *adj, float *d_mu, float *d_lam, float *d_ys, float *d_mass, float *d_vol,
                      int flags, void *stream);
 
+/* Order particles by the grid cell their B-spline stencil starts at:
+ * perm[k] = index of the k-th particle in cell order (device, n_particles
+ * ints).  Positions are those of any state.  The scatter kernels reduce
+ * contributions of same-cell neighbours within a wavefront before their
+ * global atomics, so running a rollout on particles in this order cuts the
+ * atomic count by the particles-per-cell factor; results do not depend on
+ * the order.  No counterpart in the reference (it scatters in caller order,
+ * mpm_utils.py:358-423). */
+int empm_sort_by_cell(empm_ctx *ctx, int n_particles, const float *x, float inv_dx, int *perm, void *stream);
+
+/* Substeps between re-binnings of the particles inside one rollout call
+ * (default 25).  Every substep/rollout call bins at entry; a particle that
+ * drifts more than one cell from its bin between re-binnings is still
+ * scattered correctly (through the overflow grid), only more slowly. */
+int empm_set_rebin_interval(empm_ctx *ctx, int substeps);
+
 /* Kernel ids for per-kernel timing */
 enum {
     EMPM_KERNEL_P2G = 0,      /* stress + P2G scatter */

@@ -134,10 +134,13 @@ def _loss_oracle(sc, target, prm, steps):
 
 @pytest.mark.parametrize("material", [7, 0, 6, 1])
 def test_parameter_gradients_vs_central_differences(dev, material):
-    steps = 20
-    sc = scenes.small_scene(n_particles=300, n_grid=16, material=material, seed=3, E=2e4, ys=150.0, dt=2e-4)
-    base = {"E": 2e4, "nu": 0.3, "ys": 150.0}
-    tprm = {"E": 5e4, "nu": 0.25, "ys": 400.0}
+    steps = 40
+    # a soft body hitting the sticky ground: deforms (and yields) within the
+    # horizon, so the loss is far above FP32 round-off
+    sc = scenes.small_scene(n_particles=300, n_grid=16, material=material, seed=3, E=3e3, ys=60.0, dt=2e-4,
+                            center=(0.5, 0.5, 0.27), vel=(0.5, 0.0, -2.0), spin=1.0)
+    base = {"E": 3e3, "nu": 0.3, "ys": 60.0}
+    tprm = {"E": 6e3, "nu": 0.25, "ys": 150.0}
     target = sc.x.astype(np.float64).copy()
     # target rollout with different parameters (FP64 oracle)
     N = sc.n
@@ -148,7 +151,10 @@ def test_parameter_gradients_vs_central_differences(dev, material):
     o = orc.OracleState(sc.x, sc.params["v0"], np.zeros((N, 3, 3)), np.tile(np.eye(3), (N, 1, 1)), mu_t, lam_t,
                         np.full(N, tprm["ys"]), sc.vol * sc.rho, sc.vol)
     orc.rollout(P, ocols, 0.0, sc.dt, steps, o)
-    target = o.x.copy()
+    # the parameter change alone moves particles by ~1e-5, a few hundred FP32
+    # ulps of a position; a seeded offset keeps the residual x - target far
+    # above FP32 round-off so the comparison tests the adjoint, not rounding
+    target = o.x + 0.01 * np.random.default_rng(11).standard_normal((N, 3))
     print(f"[grad vs FD] material={material} N={N} steps={steps}", flush=True)
     loss, g = _loss_grad_gpu(sc, dev, target, base, steps, ckpt=7)
     fd = []
@@ -180,3 +186,28 @@ def test_checkpoint_interval_does_not_change_gradients(dev):
     l3, g3 = _loss_grad_gpu(sc, dev, target, prm, 24, ckpt=1)
     assert l1 == l2 == l3
     assert np.allclose(g1, g2, rtol=1e-5) and np.allclose(g1, g3, rtol=1e-5), (g1, g2, g3)
+
+
+@pytest.mark.parametrize("rebin", [1, 1000])
+def test_drift_beyond_tile_matches_oracle(dev, rebin):
+    """Fast particles binned once (rebin=1000) drift out of their block tiles
+    and go through the overflow grid; binned every substep they never do.
+    Both must match the oracle."""
+    sc = scenes.small_scene(n_particles=500, n_grid=16, material=6, seed=9, E=3e4, vel=(3.0, -2.0, 1.0), dt=1e-3,
+                            center=(0.45, 0.55, 0.5), spin=0.0)
+    steps = 40
+    model, s0, P, ocols, ostate = setup(sc, dev, v0=sc.params["v0"])
+    ctx = rt.Context(sc.n_grid, sc.n, dev)
+    ctx.set_rebin_interval(rebin)
+    ro = rt.Rollout(ctx, model, sc.n)
+    final, _, _, tape = ro.forward(s0, [gpu_collider(c) for c in ocols], 0.0, sc.dt, steps, tape=True, ckpt_every=40)
+    torch.cuda.synchronize()
+    o = ostate("f32")
+    orc.rollout(P, list(ocols), 0.0, sc.dt, steps, o)
+    ex = rel_err(final.x.cpu().numpy(), o.x)
+    print(f"[drift rebin={rebin}] rel err x {ex:.2e}", flush=True)
+    assert ex < 1e-4
+    d = rt.ParticleState.empty(sc.n, dev)
+    d.x.fill_(1.0)
+    d0, g = ro.backward(tape, d)
+    assert torch.isfinite(d0.v).all() and torch.isfinite(g.mu).all()

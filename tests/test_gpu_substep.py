@@ -122,6 +122,8 @@ def test_substep_adjoint_directional(dev, material, mode):
             continue
         if key == "ys" and material not in (1, 3, 7):
             continue
+        if key in ("mu", "lam") and material == 4:  # snow has no stress: no dependence on mu, lam
+            continue
         du = u[key] * (scales[key] if np.ndim(scales[key]) == 0 else scales[key].reshape((-1,) + (1,) * (u[key].ndim - 1)))
         an = float(np.sum(g[key] * du))
         dd_p = {k: (np.asarray(v, np.float64).copy()) for k, v in d.items()}
