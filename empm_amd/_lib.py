@@ -12,7 +12,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libempm.so")
 
-EMPM_ABI_VERSION = 1
+EMPM_ABI_VERSION = 2
 EMPM_MAX_COLLIDERS = 16
 
 EMPM_OK = 0
@@ -99,6 +99,10 @@ class Particles(C.Structure):
                 ("Fe", C.c_void_p)]
 
 
+class GridRec(C.Structure):
+    _fields_ = [("mv_m", C.c_void_p), ("v_out", C.c_void_p)]
+
+
 class Collider(C.Structure):
     _fields_ = [
         ("kind", C.c_int),
@@ -151,10 +155,10 @@ def _declare(lib):
                                      vp, vp, vp, vp, vp, C.c_int, vp]
     lib.empm_rollout_fwd.restype = C.c_int
     lib.empm_rollout_fwd.argtypes = [vp, P(Model), P(Collider), C.c_int, P(C.c_double), C.c_double,
-                                     C.c_int, P(Particles), P(Collider), P(C.c_double), vp]
+                                     C.c_int, P(Particles), P(GridRec), P(Collider), P(C.c_double), vp]
     lib.empm_rollout_fwd_replay.restype = C.c_int
     lib.empm_rollout_fwd_replay.argtypes = [vp, P(Model), P(Collider), C.c_int, P(C.c_double), C.c_double,
-                                            C.c_int, P(Particles), vp]
+                                            C.c_int, P(Particles), P(GridRec), vp]
     lib.empm_sort_by_cell.restype = C.c_int
     lib.empm_sort_by_cell.argtypes = [vp, C.c_int, vp, C.c_float, vp, vp]
     lib.empm_set_rebin_interval.restype = C.c_int
@@ -169,7 +173,7 @@ def _declare(lib):
     lib.empm_kernel_name.argtypes = [C.c_int]
     lib.empm_rollout_adj.restype = C.c_int
     lib.empm_rollout_adj.argtypes = [vp, P(Model), P(Collider), C.c_int, P(C.c_double), C.c_double,
-                                     C.c_int, P(Particles), P(Particles),
+                                     C.c_int, P(Particles), P(GridRec), P(Particles),
                                      vp, vp, vp, vp, vp, C.c_int, vp]
 
 

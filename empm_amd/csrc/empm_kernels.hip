@@ -1293,9 +1293,11 @@ int do_bin(empm_ctx* ctx, int N, const float* x, float inv_dx, hipStream_t st) {
 }
 
 int launch_fwd(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, float dt, const empm_particles* in,
-               const empm_particles* out, hipStream_t st) {
+               const empm_particles* out, const empm_grid_rec* rec, hipStream_t st) {
   const int write_params = (M.material == 5 || M.mp.hardening == 1.f) ? 1 : 0;
   float* Fe = in->Fe ? in->Fe : ctx->Fe;
+  float4* keep = rec ? reinterpret_cast<float4*>(rec->mv_m) : nullptr;
+  float4* v_out = rec ? reinterpret_cast<float4*>(rec->v_out) : ctx->v_out;
   const Binning B = binning(ctx);
   if (M.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
@@ -1305,40 +1307,48 @@ int launch_fwd(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, 
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
     hipLaunchKernelGGL(k_grid, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, B, ctx->tiles, ctx->ovf,
-                       (float4*)nullptr, ctx->v_out);
+                       keep, v_out);
   }
   if (M.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_G2P, st);
-    hipLaunchKernelGGL(k_g2p, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, Fe, ctx->v_out, out->x, out->v,
+    hipLaunchKernelGGL(k_g2p, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, Fe, v_out, out->x, out->v,
                        out->C, out->F);
   }
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }
 
+// Adjoint of one substep.  With a recorded grid (rec) and Fe (in->Fe) the
+// forward P2G and boundary pass are not recomputed.
 int launch_adj(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, float dt, const empm_particles* in,
-               const empm_particles* d_out, const empm_particles* d_in, float* d_mu, float* d_lam, float* d_ys,
-               float* d_mass, float* d_vol, int flags, hipStream_t st) {
+               const empm_grid_rec* rec, const empm_particles* d_out, const empm_particles* d_in, float* d_mu,
+               float* d_lam, float* d_ys, float* d_mass, float* d_vol, int flags, hipStream_t st) {
   const Binning B = binning(ctx);
-  if (M.N > 0) {
-    ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
-    EMPM_LAUNCH_MAT(M.material, k_p2g, dim3(block_grid(ctx)), dim3(kBlkWG), st, M, dt, in->x, in->v, in->C, in->F,
-                    ctx->Fe, B, ctx->tiles, ctx->ovf, 0);
-  }
-  {
-    ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
-    hipLaunchKernelGGL(k_grid, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, B, ctx->tiles, ctx->ovf,
-                       ctx->keep, ctx->v_out);
+  const bool recorded = rec && rec->mv_m && rec->v_out && in->Fe;
+  float* Fe = recorded ? in->Fe : ctx->Fe;
+  float4* keep = recorded ? reinterpret_cast<float4*>(rec->mv_m) : ctx->keep;
+  float4* v_out = recorded ? reinterpret_cast<float4*>(rec->v_out) : ctx->v_out;
+  if (!recorded) {
+    if (M.N > 0) {
+      ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
+      EMPM_LAUNCH_MAT(M.material, k_p2g, dim3(block_grid(ctx)), dim3(kBlkWG), st, M, dt, in->x, in->v, in->C, in->F,
+                      Fe, B, ctx->tiles, ctx->ovf, 0);
+    }
+    {
+      ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
+      hipLaunchKernelGGL(k_grid, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, B, ctx->tiles, ctx->ovf,
+                         keep, v_out);
+    }
   }
   if (M.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_G2P_ADJ, st);
-    hipLaunchKernelGGL(k_g2p_adj, dim3(block_grid(ctx)), dim3(kBlkWG), 0, st, M, dt, in->x, ctx->Fe, ctx->v_out,
+    hipLaunchKernelGGL(k_g2p_adj, dim3(block_grid(ctx)), dim3(kBlkWG), 0, st, M, dt, in->x, Fe, v_out,
                        d_out->x, d_out->v, d_out->C, d_out->F, ctx->xbar, ctx->Febar, B, ctx->tiles, ctx->ovf);
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID_ADJ, st);
     hipLaunchKernelGGL(k_grid_adj, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, B, ctx->tiles,
-                       ctx->ovf, ctx->keep, ctx->gadj_in, (flags & EMPM_ADJ_DROP_GRID_MASS) ? 1 : 0);
+                       ctx->ovf, keep, ctx->gadj_in, (flags & EMPM_ADJ_DROP_GRID_MASS) ? 1 : 0);
   }
   if (M.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G_ADJ, st);
@@ -1556,7 +1566,7 @@ int empm_substep_fwd(empm_ctx* ctx, const empm_model* model, const empm_collider
   if ((rc = make_table(cols, ncol, T, model->n_grid))) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
   if ((rc = do_bin(ctx, model->n_particles, in->x, model->inv_dx, (hipStream_t)stream))) return rc;
-  return launch_fwd(ctx, dev_model(model), T, time, dt, in, out, (hipStream_t)stream);
+  return launch_fwd(ctx, dev_model(model), T, time, dt, in, out, nullptr, (hipStream_t)stream);
 }
 
 int empm_substep_adj(empm_ctx* ctx, const empm_model* model, const empm_collider* cols, int ncol, float time, float dt,
@@ -1573,13 +1583,13 @@ int empm_substep_adj(empm_ctx* ctx, const empm_model* model, const empm_collider
   if ((rc = make_table(cols, ncol, T, model->n_grid))) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
   if ((rc = do_bin(ctx, model->n_particles, in->x, model->inv_dx, (hipStream_t)stream))) return rc;
-  return launch_adj(ctx, dev_model(model), T, time, dt, in, d_out, d_in, d_mu, d_lam, d_ys, d_mass, d_vol, flags,
-                    (hipStream_t)stream);
+  return launch_adj(ctx, dev_model(model), T, time, dt, in, nullptr, d_out, d_in, d_mu, d_lam, d_ys, d_mass, d_vol,
+                    flags, (hipStream_t)stream);
 }
 
 int empm_rollout_fwd(empm_ctx* ctx, const empm_model* model, empm_collider* cols, int ncol, double* time, double dt,
-                     int n_steps, const empm_particles* states, empm_collider* cols_hist, double* times_hist,
-                     void* stream) {
+                     int n_steps, const empm_particles* states, const empm_grid_rec* grids, empm_collider* cols_hist,
+                     double* times_hist, void* stream) {
   int rc = check_model(ctx, model);
   if (rc) return rc;
   if (!time || !states || n_steps < 0) return set_err(EMPM_E_INVALID_ARG, "empm_rollout_fwd: bad arguments");
@@ -1599,7 +1609,8 @@ int empm_rollout_fwd(empm_ctx* ctx, const empm_model* model, empm_collider* cols
     if (k % ctx->rebin_every == 0 &&
         (rc = do_bin(ctx, model->n_particles, states[k].x, model->inv_dx, (hipStream_t)stream)))
       return rc;
-    if ((rc = launch_fwd(ctx, M, T, (float)*time, (float)dt, &states[k], &states[k + 1], (hipStream_t)stream)))
+    if ((rc = launch_fwd(ctx, M, T, (float)*time, (float)dt, &states[k], &states[k + 1], grids ? &grids[k] : nullptr,
+                         (hipStream_t)stream)))
       return rc;
     advance_colliders(cols, ncol, *time, dt);
     *time = *time + dt;
@@ -1609,7 +1620,7 @@ int empm_rollout_fwd(empm_ctx* ctx, const empm_model* model, empm_collider* cols
 
 int empm_rollout_fwd_replay(empm_ctx* ctx, const empm_model* model, const empm_collider* cols_hist, int ncol,
                             const double* times_hist, double dt, int n_steps, const empm_particles* states,
-                            void* stream) {
+                            const empm_grid_rec* grids, void* stream) {
   int rc = check_model(ctx, model);
   if (rc) return rc;
   if (!states || !times_hist || n_steps < 0 || (ncol > 0 && !cols_hist))
@@ -1627,7 +1638,8 @@ int empm_rollout_fwd_replay(empm_ctx* ctx, const empm_model* model, const empm_c
     if (k % ctx->rebin_every == 0 &&
         (rc = do_bin(ctx, model->n_particles, states[k].x, model->inv_dx, (hipStream_t)stream)))
       return rc;
-    if ((rc = launch_fwd(ctx, M, T, (float)times_hist[k], (float)dt, &states[k], &states[k + 1], (hipStream_t)stream)))
+    if ((rc = launch_fwd(ctx, M, T, (float)times_hist[k], (float)dt, &states[k], &states[k + 1],
+                         grids ? &grids[k] : nullptr, (hipStream_t)stream)))
       return rc;
   }
   return EMPM_OK;
@@ -1635,8 +1647,8 @@ int empm_rollout_fwd_replay(empm_ctx* ctx, const empm_model* model, const empm_c
 
 int empm_rollout_adj(empm_ctx* ctx, const empm_model* model, const empm_collider* cols_hist, int ncol,
                      const double* times_hist, double dt, int n_steps, const empm_particles* states,
-                     const empm_particles* adj, float* d_mu, float* d_lam, float* d_ys, float* d_mass, float* d_vol,
-                     int flags, void* stream) {
+                     const empm_grid_rec* grids, const empm_particles* adj, float* d_mu, float* d_lam, float* d_ys,
+                     float* d_mass, float* d_vol, int flags, void* stream) {
   int rc = check_model(ctx, model);
   if (rc) return rc;
   if ((rc = check_adj_supported(model))) return rc;
@@ -1658,8 +1670,8 @@ int empm_rollout_adj(empm_ctx* ctx, const empm_model* model, const empm_collider
     if (j % ctx->rebin_every == 0 &&
         (rc = do_bin(ctx, model->n_particles, states[k].x, model->inv_dx, (hipStream_t)stream)))
       return rc;
-    if ((rc = launch_adj(ctx, M, T, (float)times_hist[k], (float)dt, &states[k], dout, din, d_mu, d_lam, d_ys, d_mass,
-                         d_vol, flags, (hipStream_t)stream)))
+    if ((rc = launch_adj(ctx, M, T, (float)times_hist[k], (float)dt, &states[k], grids ? &grids[k] : nullptr, dout,
+                         din, d_mu, d_lam, d_ys, d_mass, d_vol, flags, (hipStream_t)stream)))
       return rc;
   }
   return EMPM_OK;

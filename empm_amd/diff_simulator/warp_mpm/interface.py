@@ -4,9 +4,11 @@ positional signatures and backward return tuples, backed by libempm's
 hand-written adjoint instead of a Warp Tape.
 
 Memory: the reference keeps every substep's full state alive for its Tape
-(interface.py:137-145).  Here the forward keeps a checkpoint every
-``mpm_solver.checkpoint_every`` substeps (default 50) and the backward
-recomputes each segment before reversing it.
+(interface.py:137-145).  Here, with ``mpm_solver.checkpoint_every`` = 1
+(the default) the forward records every substep's input state, Fe and grid
+(132 B per particle + 32 B per node per substep) and the backward recomputes
+nothing; with checkpoint_every = K > 1 it keeps a state every K substeps and
+the backward replays each segment before reversing it.
 
 Adjoint mode (``mpm_solver.adjoint_mode``):
   "reference_compat" (default) reproduces the reference Tape's gradient
@@ -56,7 +58,7 @@ def _run_forward(solver, state, model, dt, n_steps, record, want_tape, step_call
     s0 = rt.ParticleState(state.particle_x.contiguous(), state.particle_v.contiguous(),
                           state.particle_C.contiguous(), state.particle_F_trial.contiguous())
     cb = solver.step_callback if step_callback else None
-    K = int(getattr(solver, "checkpoint_every", 50))
+    K = int(getattr(solver, "checkpoint_every", 1))
     if cb is None:
         cols = solver.collider_table()
         cols_out = []
@@ -65,7 +67,9 @@ def _run_forward(solver, state, model, dt, n_steps, record, want_tape, step_call
         solver.sync_colliders_from(cols_out)
         solver.time = t1
         return final, xs, tape, ro
-    # a per-substep Python callback may mutate colliders: one call per substep
+    # a per-substep Python callback may mutate colliders: one call per substep,
+    # in the caller's particle order so that the per-substep tapes merge
+    ro = rt.Rollout(ctx, arrays, n, sort=False)
     xs = []
     tapes = []
     cur = s0
@@ -96,8 +100,11 @@ def _merge_tapes(tapes, dt):
     tp = rt.RolloutTape(n_steps=n, dt=dt, ckpt_every=1, ncol=ncol)
     tp.cols_hist = (Collider * max(1, n * ncol))()
     tp.times_hist = (C.c_double * max(1, n))()
+    tp.rec_states, tp.rec_grids = [], []
     for k, t in enumerate(tapes):
         tp.checkpoints[k] = t.checkpoints[0]
+        tp.rec_states.append(t.rec_states[0])
+        tp.rec_grids.append(t.rec_grids[0])
         for j in range(ncol):
             tp.cols_hist[k * ncol + j] = t.cols_hist[j]
         tp.times_hist[k] = t.times_hist[0]

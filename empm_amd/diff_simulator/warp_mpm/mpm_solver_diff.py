@@ -85,6 +85,10 @@ class MPMWARPDiff:
     def __init__(self):
         self.time_profile = {}
         self.step_callback = None
+        # reverse-mode memory policy of the autograd Functions: 1 records every
+        # substep (no recompute); K > 1 keeps a state every K substeps and
+        # replays each segment in the backward
+        self.checkpoint_every = 1
         self._ctx = None
         self._ctx_key = None
 

@@ -13,7 +13,7 @@ from dataclasses import dataclass, field
 import torch
 
 from . import _lib
-from ._lib import Collider, CreateDesc, Model, Particles, check
+from ._lib import Collider, CreateDesc, GridRec, Model, Particles, check
 
 F32 = torch.float32
 
@@ -215,8 +215,13 @@ class ParamGrads:
 
 @dataclass
 class RolloutTape:
-    """What a checkpointed reverse sweep needs: the state every `ckpt_every`
-    substeps, the collider table and time of every substep."""
+    """What a reverse sweep needs.
+
+    Recorded mode (ckpt_every == 1): every substep's input state (with its
+    return-mapped F, Fe) and grid, so the reverse sweep recomputes nothing.
+    Checkpointed mode (ckpt_every = K > 1): the state every K substeps; each
+    segment is replayed (and recorded) before its reverse sweep.  Both keep
+    the collider table and time of every substep."""
     n_steps: int
     dt: float
     ckpt_every: int
@@ -226,12 +231,38 @@ class RolloutTape:
     times_hist: object = None  # (c_double * n_steps)
     perm: object = None  # cell order used internally (None: caller order)
     model: object = None  # ModelArrays in the internal order
+    rec_states: list | None = None  # recorded mode: input state of substep k (with Fe)
+    rec_grids: list | None = None  # recorded mode: (2, n^3, 4) grid of substep k
+
+
+class Records:
+    """Device storage for `n` fully recorded substeps: states 0..n (x, v, C,
+    F_trial and Fe; Fe of state n unused) and the grid of substeps 0..n-1
+    ((m v, m) and v_out, float4 per node), one tensor per field."""
+
+    def __init__(self, n: int, n_particles: int, n_grid: int, device):
+        N, G = int(n_particles), int(n_grid) ** 3
+        e = lambda *shape: torch.empty(shape, dtype=F32, device=device)
+        self.n = int(n)
+        self.x, self.v = e(n + 1, N, 3), e(n + 1, N, 3)
+        self.C, self.F, self.Fe = e(n + 1, N, 3, 3), e(n + 1, N, 3, 3), e(n + 1, N, 3, 3)
+        self.grid = e(max(n, 1), 2, G, 4)
+
+    def state(self, k: int) -> ParticleState:
+        return ParticleState(self.x[k], self.v[k], self.C[k], self.F[k], self.Fe[k])
 
 
 def _states_array(states: list[ParticleState]):
     arr = (Particles * len(states))()
-    for i, s in enumerate(states):
-        arr[i] = s.desc()
+    for i, st in enumerate(states):
+        arr[i] = st.desc()
+    return arr
+
+
+def _grid_array(grids: list):
+    arr = (GridRec * max(1, len(grids)))()
+    for i, g in enumerate(grids):
+        arr[i] = GridRec(mv_m=_ptr(g[0]), v_out=_ptr(g[1]))
     return arr
 
 
@@ -283,7 +314,7 @@ class Rollout:
         return ParticleState(f(perm, s.x), f(perm, s.v), f(perm, s.C), f(perm, s.F))
 
     def forward(self, state: ParticleState, cols: list[Collider], time: float, dt: float, n_steps: int,
-                record=(), tape: bool = False, ckpt_every: int = 50, cols_out: list | None = None):
+                record=(), tape: bool = False, ckpt_every: int = 1, cols_out: list | None = None):
         """Returns (final_state, final_time, recorded_x list, tape or None).
         `cols` (ctypes Collider structs) are advanced in place like the
         reference's modify_bc; `cols_out` receives the advanced table.  With
@@ -316,6 +347,22 @@ class Rollout:
             tp = RolloutTape(n_steps=n_steps, dt=dt, ckpt_every=max(1, int(ckpt_every)), ncol=ncol)
             tp.cols_hist = (Collider * max(1, n_steps * ncol))()
             tp.times_hist = (C.c_double * max(1, n_steps))()
+        stream = stream_handle(self.ctx.device)
+        if tape and tp.ckpt_every == 1:
+            # recorded mode: every substep writes its own output slot, Fe and grid
+            R = Records(n_steps, self.n, model.n_grid, self.ctx.device)
+            R.state(0).copy_(state)
+            states = [R.state(k) for k in range(n_steps + 1)]
+            grids = [R.grid[k] for k in range(n_steps)]
+            check(lib.empm_rollout_fwd(self.ctx.handle, C.byref(m), arr, ncol, C.byref(t), C.c_double(dt), n_steps,
+                                       _states_array(states), _grid_array(grids), tp.cols_hist, tp.times_hist,
+                                       stream))
+            tp.rec_states = states[:n_steps]
+            tp.rec_grids = grids
+            tp.checkpoints[0] = states[0]
+            if cols_out is not None:
+                cols_out[:] = [arr[i] for i in range(ncol)]
+            return states[n_steps], t.value, [states[k].x.clone() for k in rec], tp
         stops = set(rec)
         if tape:
             stops.update(range(tp.ckpt_every, n_steps, tp.ckpt_every))
@@ -324,7 +371,6 @@ class Rollout:
         cur = state
         k = 0
         out_x = []
-        stream = stream_handle(self.ctx.device)
         if tape:
             tp.checkpoints[0] = state.clone()
         for stop in stops:
@@ -339,7 +385,7 @@ class Rollout:
             ch = C.cast(C.byref(tp.cols_hist, k * ncol * C.sizeof(Collider)), C.POINTER(Collider)) if tape else None
             th = C.cast(C.byref(tp.times_hist, k * C.sizeof(C.c_double)), C.POINTER(C.c_double)) if tape else None
             check(lib.empm_rollout_fwd(self.ctx.handle, C.byref(m), arr, ncol, C.byref(t), C.c_double(dt), n,
-                                       sarr, ch, th, stream))
+                                       sarr, None, ch, th, stream))
             cur = seq[-1]
             k = stop
             if k in rec:
@@ -380,47 +426,57 @@ class Rollout:
         m = model.struct()
         stream = stream_handle(dev)
         ncol = tp.ncol
-        K = tp.ckpt_every
         adj = [ParticleState.empty(n, dev), ParticleState.empty(n, dev)]
         adj[0].copy_(d_final)
-        seg_cache = [ParticleState.empty(n, dev) for _ in range(min(K, tp.n_steps) + 1)]
-        seg_starts = sorted(tp.checkpoints.keys())
-        k_hi = tp.n_steps
-        for s0 in reversed(seg_starts):
-            if s0 >= k_hi:
-                continue
-            L = k_hi - s0
-            states = [tp.checkpoints[s0]] + seg_cache[1:L + 1]
-            sarr = _states_array(states)
-            ch = C.cast(C.byref(tp.cols_hist, s0 * ncol * C.sizeof(Collider)), C.POINTER(Collider))
-            th = C.cast(C.byref(tp.times_hist, s0 * C.sizeof(C.c_double)), C.POINTER(C.c_double))
-            if L > 1:
-                check(lib.empm_rollout_fwd_replay(self.ctx.handle, C.byref(m), ch, ncol, th, C.c_double(tp.dt), L - 1,
-                                                  _states_array(states[:L]), stream))
-            # reverse over [s0, k_hi) in chunks split at seed points
-            k = k_hi
-            while k > s0:
-                if k in seeds and k != tp.n_steps:
-                    sd = seeds[k]
-                    adj[0].x.add_(sd.x)
-                    adj[0].v.add_(sd.v)
-                    adj[0].C.add_(sd.C)
-                    adj[0].F.add_(sd.F)
-                j = max([s for s in seeds if s0 < s < k] + [s0])
-                cnt = k - j
-                sub = _states_array(states[j - s0:k - s0])
-                chj = C.cast(C.byref(tp.cols_hist, j * ncol * C.sizeof(Collider)), C.POINTER(Collider))
-                thj = C.cast(C.byref(tp.times_hist, j * C.sizeof(C.c_double)), C.POINTER(C.c_double))
-                check(lib.empm_rollout_adj(self.ctx.handle, C.byref(m), chj, ncol, thj, C.c_double(tp.dt), cnt,
-                                           sub, _states_array(adj), *grads.ptrs(), int(flags), stream))
-                if cnt % 2 == 1:
-                    adj = [adj[1], adj[0]]
-                k = j
-            k_hi = s0
-        if 0 in seeds:
-            sd = seeds[0]
+
+        def add_seed(k):
+            sd = seeds[k]
             adj[0].x.add_(sd.x)
             adj[0].v.add_(sd.v)
             adj[0].C.add_(sd.C)
             adj[0].F.add_(sd.F)
+
+        def reverse(s0, k_hi, states, grids):
+            """Reverse over substeps [s0, k_hi) whose input states / grids are
+            states[k - s0] / grids[k - s0], in chunks split at seed points."""
+            nonlocal adj
+            k = k_hi
+            while k > s0:
+                if k in seeds and k != tp.n_steps:
+                    add_seed(k)
+                j = max([s for s in seeds if s0 < s < k] + [s0])
+                cnt = k - j
+                sub = _states_array(states[j - s0:k - s0])
+                garr = _grid_array(grids[j - s0:k - s0])
+                chj = C.cast(C.byref(tp.cols_hist, j * ncol * C.sizeof(Collider)), C.POINTER(Collider))
+                thj = C.cast(C.byref(tp.times_hist, j * C.sizeof(C.c_double)), C.POINTER(C.c_double))
+                check(lib.empm_rollout_adj(self.ctx.handle, C.byref(m), chj, ncol, thj, C.c_double(tp.dt), cnt,
+                                           sub, garr, _states_array(adj), *grads.ptrs(), int(flags), stream))
+                if cnt % 2 == 1:
+                    adj = [adj[1], adj[0]]
+                k = j
+
+        if tp.rec_states is not None:
+            reverse(0, tp.n_steps, tp.rec_states, tp.rec_grids)
+        else:
+            K = tp.ckpt_every
+            seg = Records(min(K, tp.n_steps), n, model.n_grid, dev)
+            seg_starts = sorted(tp.checkpoints.keys())
+            k_hi = tp.n_steps
+            for s0 in reversed(seg_starts):
+                if s0 >= k_hi:
+                    continue
+                L = k_hi - s0
+                seg.state(0).copy_(tp.checkpoints[s0])
+                states = [seg.state(i) for i in range(L + 1)]
+                grids = [seg.grid[i] for i in range(L)]
+                ch = C.cast(C.byref(tp.cols_hist, s0 * ncol * C.sizeof(Collider)), C.POINTER(Collider))
+                th = C.cast(C.byref(tp.times_hist, s0 * C.sizeof(C.c_double)), C.POINTER(C.c_double))
+                # replay the segment, recording every substep's Fe and grid
+                check(lib.empm_rollout_fwd_replay(self.ctx.handle, C.byref(m), ch, ncol, th, C.c_double(tp.dt), L,
+                                                  _states_array(states), _grid_array(grids), stream))
+                reverse(s0, k_hi, states[:L], grids)
+                k_hi = s0
+        if 0 in seeds:
+            add_seed(0)
         return adj[0], grads

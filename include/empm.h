@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define EMPM_ABI_VERSION 1
+#define EMPM_ABI_VERSION 2
 #define EMPM_MAX_COLLIDERS 16
 
 enum {
@@ -89,7 +89,8 @@ typedef struct {
  * when non-NULL, a forward substep reading this state writes the
  * return-mapped F there (the reference's particle_F, written by
  * compute_stress_from_F_trial at mpm_utils.py:614-631); when NULL a context
- * scratch buffer is used.  Adjoint entry points ignore it. */
+ * scratch buffer is used.  empm_rollout_adj reads it back for substeps whose
+ * grid was recorded (empm_grid_rec); other adjoint uses ignore it. */
 typedef struct {
     float *x;  /* N*3 */
     float *v;  /* N*3 */
@@ -97,6 +98,15 @@ typedef struct {
     float *F;  /* N*9, F_trial */
     float *Fe; /* N*9 or NULL */
 } empm_particles;
+
+/* The grid of one forward substep, recorded for reverse mode so that the
+ * adjoint does not recompute P2G and the boundary pass: the reference keeps
+ * grid_m / grid_v_in / grid_v_out of every substep alive for its Tape
+ * (interface.py:137-145 with mpm_data_structure.py:356-387). */
+typedef struct {
+    float *mv_m;  /* n^3*4: (m v, m) after P2G, node-major float4 */
+    float *v_out; /* n^3*4: grid velocity after the boundary pass (w unused) */
+} empm_grid_rec;
 
 /* One grid post-process, in registration order.  kind:
  *   0 surface collider       add_surface_collider      mpm_solver_diff.py:437-531
@@ -182,27 +192,33 @@ int empm_substep_adj(empm_ctx *ctx, const empm_model *model, const empm_collider
  * (host, ncol entries) are advanced in place, as the reference host loop
  * does (warp_mpm_wrapper.py:278-306, interface.py:137-145).  If cols_hist /
  * times_hist are non-NULL (host, n_steps*ncol / n_steps entries) the
- * collider table and time each substep saw are recorded for the adjoint. */
+ * collider table and time each substep saw are recorded for the adjoint.
+ * If grids is non-NULL (host, n_steps entries) substep k writes its grid
+ * into grids[k]; with states[k].Fe non-NULL as well, the substep is fully
+ * recorded and empm_rollout_adj differentiates it without recompute. */
 int empm_rollout_fwd(empm_ctx *ctx, const empm_model *model, empm_collider *cols, int ncol, double *time, double dt,
-                     int n_steps, const empm_particles *states, empm_collider *cols_hist, double *times_hist,
-                     void *stream);
+                     int n_steps, const empm_particles *states, const empm_grid_rec *grids, empm_collider *cols_hist,
+                     double *times_hist, void *stream);
 
 /* Forward replay of n_steps substeps with the recorded per-substep collider
  * tables and times of empm_rollout_fwd (no collider advance): the recompute
  * step of checkpointed reverse mode (the reference keeps every state alive
- * instead, interface.py:137-145). */
+ * instead, interface.py:137-145).  grids: as for empm_rollout_fwd. */
 int empm_rollout_fwd_replay(empm_ctx *ctx, const empm_model *model, const empm_collider *cols_hist, int ncol,
                             const double *times_hist, double dt, int n_steps, const empm_particles *states,
-                            void *stream);
+                            const empm_grid_rec *grids, void *stream);
 
 /* Reverse sweep over n_steps substeps whose inputs are states[0..n_steps-1]
  * (host descriptors), with the per-substep collider tables and times from
  * empm_rollout_fwd.  On entry adj[0] holds dL/d(state n_steps); on return
- * adj[n_steps % 2] holds dL/d(state 0).  Parameter adjoints accumulate. */
+ * adj[n_steps % 2] holds dL/d(state 0).  Parameter adjoints accumulate.
+ * If grids is non-NULL (host, n_steps entries, as written by the forward)
+ * and states[k].Fe is non-NULL, substep k uses the recorded grid and Fe;
+ * otherwise its P2G and boundary pass are recomputed. */
 int empm_rollout_adj(empm_ctx *ctx, const empm_model *model, const empm_collider *cols_hist, int ncol,
                      const double *times_hist, double dt, int n_steps, const empm_particles *states,
-                     const empm_particles *adj, float *d_mu, float *d_lam, float *d_ys, float *d_mass, float *d_vol,
-                     int flags, void *stream);
+                     const empm_grid_rec *grids, const empm_particles *adj, float *d_mu, float *d_lam, float *d_ys,
+                     float *d_mass, float *d_vol, int flags, void *stream);
 
 /* Order particles by the grid cell their B-spline stencil starts at:
  * perm[k] = index of the k-th particle in cell order (device, n_particles
