@@ -48,7 +48,7 @@ EXPORTED_SYMBOLS = (
     "empm_kernel_name",
 )
 
-KERNEL_NAMES = ("k_p2g", "k_grid", "k_g2p", "k_g2p_adj", "k_grid_adj", "k_p2g_adj")
+KERNEL_NAMES = ("k_p2g", "k_grid", "k_g2p", "k_g2p_adj", "k_grid_adj", "k_p2g_adj", "k_stress", "k_g2p_adj_scatter")
 
 
 class EmpmError(RuntimeError):

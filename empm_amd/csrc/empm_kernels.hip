@@ -288,21 +288,22 @@ EMPM_DEV void grid_node(const DevModel& M, const ColTable& T, float time, float 
 // are issued except for particles that drifted out of their tile, which add
 // into a dense overflow grid.
 //
-// Inside a wavefront, lanes of the same cell add into the same 27 nodes, so
-// their contributions are first summed across lanes by a transposed
-// butterfly (32 values per lane halved at each xor-shuffle step: xor 32, 16,
-// 8, 4, 2), after which lane L holds the group total of value L >> 1 and one
-// LDS atomic per (node, component) per cell group remains.  Waves with many
-// small groups (sparse particles) fall back to per-lane LDS atomics.
+// The waves of a workgroup take the block's particles in chunks of <= 64
+// that never straddle a cell, so all lanes of a chunk whose particle is still
+// in its binned cell share one stencil base.  Their per-node contributions
+// are summed across the wave by a transposed butterfly (v_permlane32_swap,
+// v_permlane16_swap, then DPP row/quad exchanges), leaving one LDS atomic
+// per (node, component) per chunk.  Particles that left their cell since the
+// binning add their own contributions lane by lane.
 // ---------------------------------------------------------------------------
 constexpr int kBlkCells = 2;                                    // cells per block edge
 constexpr int kTileMargin = 1;                                  // cells of drift tolerated
 constexpr int kTileN = kBlkCells + 2 + 2 * kTileMargin;         // 6 tile nodes per edge
 constexpr int kTileNodes = kTileN * kTileN * kTileN;            // 216
-constexpr int kMaxGroupsPerWave = 4;
-constexpr int kMinGroupSize = 6;
 constexpr int kBlkWG = 256;       // workgroup size of the block-owned scatter kernels
 constexpr int kBlkGridCap = 1024; // max workgroups of a block-owned scatter launch (4 per CU)
+constexpr int kWave = 64;
+constexpr int kAdjPayload = 24;   // floats per particle of the G2P-adjoint scatter payload
 
 // Particle binning of the current segment: perm[k] = k-th particle in
 // block-major order, offs[b * 8 + c] = first position of cell c of block b
@@ -315,110 +316,163 @@ struct Binning {
   int nb;
 };
 
-EMPM_DEV float wave_transpose_sum32(float* v, int lane) {
-  {
-    const bool hi = lane & 32;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const float s = hi ? v[i] : v[i + 16];
-      const float k = hi ? v[i + 16] : v[i];
-      v[i] = k + __shfl_xor(s, 32);
-    }
-  }
-  {
-    const bool hi = lane & 16;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const float s = hi ? v[i] : v[i + 8];
-      const float k = hi ? v[i + 8] : v[i];
-      v[i] = k + __shfl_xor(s, 16);
-    }
-  }
-  {
-    const bool hi = lane & 8;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const float s = hi ? v[i] : v[i + 4];
-      const float k = hi ? v[i + 4] : v[i];
-      v[i] = k + __shfl_xor(s, 8);
-    }
-  }
-  {
-    const bool hi = lane & 4;
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const float s = hi ? v[i] : v[i + 2];
-      const float k = hi ? v[i + 2] : v[i];
-      v[i] = k + __shfl_xor(s, 4);
-    }
-  }
-  {
-    const bool hi = lane & 2;
-    const float s = hi ? v[0] : v[1];
-    const float k = hi ? v[1] : v[0];
-    v[0] = k + __shfl_xor(s, 2);
-  }
-  return v[0] + __shfl_xor(v[0], 1);
+// ---- cross-lane exchanges (wave64) ----------------------------------------
+constexpr int kDppRowRor8 = 0x128;        // lane i <- lane (i + 8) mod 16 within each row of 16
+constexpr int kDppRowHalfMirror = 0x141;  // lane i <- lane 7 - i within each group of 8
+constexpr int kDppQuadXor2 = 0x4E;        // quad_perm [2, 3, 0, 1]
+constexpr int kDppQuadXor1 = 0xB1;        // quad_perm [1, 0, 3, 2]
+
+template <int CTRL>
+EMPM_DEV float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
-// contrib(j, out4): the float4 this lane adds to stencil node j = 9a + 3b + c.
-// add(ix, iy, iz, comp, value): the destination.  Every lane of the wave must
-// call this (it shuffles); `active` marks lanes that own a particle.
-template <int NCOMP, typename Contrib, typename Add>
-EMPM_DEV void scatter_stencil(bool active, const int* base, Contrib contrib, Add add) {
-  const int lane = __lane_id();
-  const int key = active ? (((base[0] + 4) << 20) | ((base[1] + 4) << 10) | (base[2] + 4)) : -1;
-  unsigned long long pending = __ballot(active);
-  for (int iter = 0; iter < kMaxGroupsPerWave && pending; ++iter) {
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const int lkey = __shfl(key, leader);
-    const bool in = ((pending >> lane) & 1ull) && key == lkey;
-    const unsigned long long grp = __ballot(in);
-    if (__popcll(grp) < kMinGroupSize) break;
-    pending &= ~grp;
-    const int b0 = __shfl(base[0], leader), b1 = __shfl(base[1], leader), b2 = __shfl(base[2], leader);
-#pragma unroll 1
-    for (int r = 0; r < 4; ++r) {
-      float v[32];
+// One transposed step over pairs (v[i], v[i + H]), i < H: afterwards v[i] of
+// the lanes in one half of each exchange pair holds the pair sum of the old
+// v[i], and of the other half the pair sum of the old v[i + H].
+template <int H>
+EMPM_DEV void tstep_lanes32(float* v) {  // halves: lanes 0-31 keep v[i], 32-63 v[i + H]
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        float c4[4] = {0.f, 0.f, 0.f, 0.f};
-        if (r * 8 + j < 27 && in) contrib(r * 8 + j, c4);
-        v[4 * j + 0] = c4[0];
-        v[4 * j + 1] = c4[1];
-        v[4 * j + 2] = c4[2];
-        v[4 * j + 3] = c4[3];
+  for (int i = 0; i < H; ++i) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[i]), __float_as_uint(v[i + H]), false, false);
+    v[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  }
+}
+template <int H>
+EMPM_DEV void tstep_lanes16(float* v) {  // rows of 16: even rows keep v[i], odd rows v[i + H]
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[i]), __float_as_uint(v[i + H]), false, false);
+    v[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  }
+}
+template <int H, int CTRL, int HIBIT>
+EMPM_DEV void tstep_dpp(float* v, int lane) {  // lanes with HIBIT set keep v[i + H]
+  const bool hi = (lane & HIBIT) != 0;
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const float send = hi ? v[i] : v[i + H];
+    const float keep = hi ? v[i + H] : v[i];
+    v[i] = keep + dpp_f<CTRL>(send);
+  }
+}
+
+// Sum over the 64 lanes of NV (32 or 16) values per lane; on return lane L
+// holds the wave total of value L >> 1 (NV = 32) or L >> 2 (NV = 16).
+template <int NV>
+EMPM_DEV float wave_transpose_sum(float* v, int lane) {
+  static_assert(NV == 32 || NV == 16, "NV");
+  tstep_lanes32<NV / 2>(v);
+  tstep_lanes16<NV / 4>(v);
+  tstep_dpp<NV / 8, kDppRowRor8, 8>(v, lane);
+  tstep_dpp<NV / 16, kDppRowHalfMirror, 4>(v, lane);
+  if constexpr (NV == 32) {
+    tstep_dpp<1, kDppQuadXor2, 2>(v, lane);
+  } else {
+    v[0] += dpp_f<kDppQuadXor2>(v[0]);
+  }
+  return v[0] + dpp_f<kDppQuadXor1>(v[0]);
+}
+
+// What one particle adds to the nodes of its stencil: for node offset
+// d = (a, b, c) - f, out[k] = w (A_k + B_k . d) + S_k . gw, with w the
+// B-spline weight and gw its gradient without the inv_dx factor (folded into
+// S by the producer); with NC = 4, out[3] = w * m.
+struct ScatterPayload {
+  float A[3], B[9], S[9], m;
+};
+
+template <int NC>
+EMPM_DEV void node_contrib(const ScatterPayload& P, const Stencil& s, int a, int b, int c, float wa, float wb, float wc,
+                           float dwa, float dwb, float dwc, float* out) {
+  const float d0 = (float)a - s.f[0], d1 = (float)b - s.f[1], d2 = (float)c - s.f[2];
+  const float wab = wa * wb;
+  const float wt = wab * wc;
+  const float g0 = dwa * wb * wc, g1 = wa * dwb * wc, g2 = wab * dwc;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    out[k] = wt * (P.A[k] + P.B[3 * k] * d0 + P.B[3 * k + 1] * d1 + P.B[3 * k + 2] * d2) + P.S[3 * k] * g0 +
+             P.S[3 * k + 1] * g1 + P.S[3 * k + 2] * g2;
+  if (NC == 4) out[3] = wt * P.m;
+}
+
+// Wave-level scatter of the lanes flagged `in_group` (all with stencil base =
+// the chunk's cell base) into the LDS tile at tile-local node offset t0.
+// Nodes are taken NPR per round (NPR * NC <= 32 values); every lane of the
+// wave must call this.
+template <int NC>
+EMPM_DEV void group_scatter(float (*tile)[kTileNodes], const int* t0, bool in_group, const ScatterPayload& P,
+                            const Stencil& s, int lane) {
+  constexpr int NPR = NC == 4 ? 8 : 10;
+  float w0[3], w1[3], w2[3], g0[3], g1[3], g2[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    w0[i] = in_group ? s.w[0][i] : 0.f;
+    w1[i] = s.w[1][i];
+    w2[i] = s.w[2][i];
+    g0[i] = in_group ? s.dw[0][i] : 0.f;
+    g1[i] = s.dw[1][i];
+    g2[i] = s.dw[2][i];
+  }
+#pragma unroll
+  for (int n0 = 0; n0 < 27; n0 += NPR) {
+    constexpr int kNV = 32;
+    const int nn = (27 - n0) < NPR ? (27 - n0) : NPR;
+    float v[kNV];
+#pragma unroll
+    for (int q = 0; q < kNV; ++q) v[q] = 0.f;
+#pragma unroll
+    for (int j = 0; j < NPR; ++j) {
+      const int node = n0 + j;
+      if (node < 27) {
+        const int a = node / 9, b = (node / 3) % 3, c = node % 3;
+        float o[4];
+        node_contrib<NC>(P, s, a, b, c, w0[a], w1[b], w2[c], g0[a], g1[b], g2[c], o);
+#pragma unroll
+        for (int k = 0; k < NC; ++k) v[j * NC + k] = o[k];
       }
-      const float s = wave_transpose_sum32(v, lane);
-      const int idx = lane >> 1;
-      const int node = r * 8 + (idx >> 2);
-      const int comp = idx & 3;
-      if (!(lane & 1) && node < 27 && comp < NCOMP && s != 0.f)
-        add(b0 + node / 9, b1 + (node / 3) % 3, b2 + node % 3, comp, s);
     }
-  }
-  if ((pending >> lane) & 1ull) {
-#pragma unroll 1
-    for (int j = 0; j < 27; ++j) {
-      float c4[4] = {0.f, 0.f, 0.f, 0.f};
-      contrib(j, c4);
-#pragma unroll
-      for (int k = 0; k < NCOMP; ++k) add(base[0] + j / 9, base[1] + (j / 3) % 3, base[2] + j % 3, k, c4[k]);
+    float tot;
+    int idx;
+    if (NC == 4 && nn * NC <= 16) {
+      tot = wave_transpose_sum<16>(v, lane);
+      idx = (lane & 3) == 0 ? (lane >> 2) : 99;
+    } else {
+      tot = wave_transpose_sum<32>(v, lane);
+      idx = (lane & 1) == 0 ? (lane >> 1) : 99;
+    }
+    if (idx < nn * NC) {
+      const int node = n0 + idx / NC, comp = idx % NC;
+      const int a = node / 9, b = (node / 3) % 3, c = node % 3;
+      atomicAdd(&tile[comp][((t0[0] + a) * kTileN + (t0[1] + b)) * kTileN + (t0[2] + c)], tot);
     }
   }
 }
 
-// Per-lane global atomics into the overflow grid, for a particle whose stencil
-// left its block's tile (rare: drift of more than one cell between binnings).
-template <int NCOMP, typename Contrib>
-EMPM_DEV void scatter_overflow(float4* __restrict__ ovf, int n, const int* base, Contrib contrib) {
+// One lane's own 27-node scatter: into the tile if the stencil lies inside
+// it, else into the overflow grid with global atomics.
+template <int NC>
+EMPM_DEV void lane_scatter(float (*tile)[kTileNodes], const int* o, float4* __restrict__ ovf, int n,
+                           const ScatterPayload& P, const Stencil& s) {
+  const bool in_tile = (unsigned)(s.base[0] - o[0]) <= (unsigned)(kTileN - 3) &&
+                       (unsigned)(s.base[1] - o[1]) <= (unsigned)(kTileN - 3) &&
+                       (unsigned)(s.base[2] - o[2]) <= (unsigned)(kTileN - 3);
+#pragma unroll 1
   for (int j = 0; j < 27; ++j) {
-    const int ix = base[0] + j / 9, iy = base[1] + (j / 3) % 3, iz = base[2] + j % 3;
-    if (!node_in(ix, iy, iz, n)) continue;
-    float c4[4] = {0.f, 0.f, 0.f, 0.f};
-    contrib(j, c4);
-    float* g = reinterpret_cast<float*>(&ovf[node_id(ix, iy, iz, n)]);
-    for (int k = 0; k < NCOMP; ++k) atomicAdd(g + k, c4[k]);
+    const int a = j / 9, b = (j / 3) % 3, c = j % 3;
+    float out[4];
+    node_contrib<NC>(P, s, a, b, c, sw(s, 0, a), sw(s, 1, b), sw(s, 2, c), sdw(s, 0, a), sdw(s, 1, b), sdw(s, 2, c),
+                     out);
+    const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
+    if (in_tile) {
+      const int t = ((ix - o[0]) * kTileN + (iy - o[1])) * kTileN + (iz - o[2]);
+#pragma unroll
+      for (int k = 0; k < NC; ++k) atomicAdd(&tile[k][t], out[k]);
+    } else if (node_in(ix, iy, iz, n)) {
+      float* g = reinterpret_cast<float*>(&ovf[node_id(ix, iy, iz, n)]);
+#pragma unroll
+      for (int k = 0; k < NC; ++k) atomicAdd(g + k, out[k]);
+    }
   }
 }
 
@@ -429,9 +483,58 @@ EMPM_DEV void tile_origin(int blk, int nb, int* o) {
   o[0] = (blk / (nb * nb)) * kBlkCells - kTileMargin;
 }
 
-EMPM_DEV bool stencil_in_tile(const int* base, const int* o) {
-  return (unsigned)(base[0] - o[0]) <= (unsigned)(kTileN - 3) && (unsigned)(base[1] - o[1]) <= (unsigned)(kTileN - 3) &&
-         (unsigned)(base[2] - o[2]) <= (unsigned)(kTileN - 3);
+// The block-owned scatter driver.  load(p, xp, P) fills a particle's position
+// and payload.  Every active block is taken by one workgroup; its waves take
+// the block's cell chunks round robin.
+template <int NC, typename Load>
+EMPM_DEV void block_scatter(const Binning& B, const int* __restrict__ sel, float inv_dx, int n,
+                            float4* __restrict__ tiles, float4* __restrict__ ovf, Load load) {
+  __shared__ float tile[NC][kTileNodes];
+  const int lane = __lane_id();
+  const int wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
+  const int nact = *B.nactive;
+  for (int ai = blockIdx.x; ai < nact; ai += gridDim.x) {
+    const int blk = B.active[ai];
+    int o[3];
+    tile_origin(blk, B.nb, o);
+    for (int i = threadIdx.x; i < NC * kTileNodes; i += blockDim.x) (&tile[0][0])[i] = 0.f;
+    __syncthreads();
+    int cs[9];
+#pragma unroll
+    for (int c = 0; c <= 8; ++c) cs[c] = B.offs[blk * 8 + c];
+    int nch = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) nch += (cs[c + 1] - cs[c] + kWave - 1) / kWave;
+    for (int ch = wave; ch < nch; ch += nwaves) {
+      // chunk ch -> (cell c, chunk j within the cell)
+      int c = 0, j = ch;
+      while (true) {
+        const int k = (cs[c + 1] - cs[c] + kWave - 1) / kWave;
+        if (j < k) break;
+        j -= k;
+        ++c;
+      }
+      const int i = cs[c] + j * kWave + lane;
+      const int p = i < cs[c + 1] ? B.perm[i] : -1;
+      const bool active = p >= 0 && !(sel && sel[p] != 0);
+      // the cell's stencil base and its tile-local offset
+      const int t0[3] = {kTileMargin + ((c >> 2) & 1), kTileMargin + ((c >> 1) & 1), kTileMargin + (c & 1)};
+      float xp[3] = {(o[0] + t0[0] + 1.f) * (1.f / inv_dx), (o[1] + t0[1] + 1.f) * (1.f / inv_dx),
+                     (o[2] + t0[2] + 1.f) * (1.f / inv_dx)};
+      ScatterPayload P = {};
+      if (active) load(p, xp, P);
+      Stencil s;
+      make_stencil(xp, inv_dx, s);
+      const bool home = active && s.base[0] == o[0] + t0[0] && s.base[1] == o[1] + t0[1] && s.base[2] == o[2] + t0[2];
+      if (__ballot(home)) group_scatter<NC>(tile, t0, home, P, s, lane);
+      if (active && !home) lane_scatter<NC>(tile, o, ovf, n, P, s);
+    }
+    __syncthreads();
+    float4* dst = tiles + (size_t)blk * kTileNodes;
+    for (int k = threadIdx.x; k < kTileNodes; k += blockDim.x)
+      dst[k] = make_float4(tile[0][k], tile[1][k], tile[2][k], NC == 4 ? tile[NC - 1][k] : 0.f);
+    __syncthreads();
+  }
 }
 
 // Sum of the tiles (and the overflow grid) at node (ix, iy, iz).
@@ -461,90 +564,65 @@ EMPM_DEV float4 gather_node(const Binning& B, const float4* __restrict__ tiles, 
 // ---------------------------------------------------------------------------
 // Forward kernels
 // ---------------------------------------------------------------------------
-// compute_stress_from_F_trial + p2g_apic_with_stress (mpm_utils.py:358-423,
-// 603-665), one block of particles per workgroup iteration.
+// compute_stress_from_F_trial (mpm_utils.py:603-665) for one particle: the
+// return map writes Fe (and, for material 5 / hardening, mu, lam, yield
+// stress in place); S = -dt V tau inv_dx is the stress part of the P2G
+// payload.
 template <int MAT>
-__global__ __launch_bounds__(kBlkWG) void k_p2g(DevModel M, float dt, const float* __restrict__ x,
-                                                const float* __restrict__ v, const float* __restrict__ C,
-                                                const float* __restrict__ F, float* __restrict__ Fe_out, Binning B,
-                                                float4* __restrict__ tiles, float4* __restrict__ ovf,
-                                                int write_params) {
-  __shared__ float tile[4][kTileNodes];
-  const int nblk = B.nb * B.nb * B.nb;
-  const float dx = M.dx, inv_dx = M.inv_dx;
+EMPM_DEV void particle_stress(const DevModel& M, float dt, int p, const float* Ftr, float* Fe_out,
+                              float* __restrict__ S_out, int write_params) {
+  float mu = M.mu[p], lam = M.lam[p], ys = M.ys[p];
   MatParams matp = M.mp;
   matp.material = MAT;  // compile-time material: one code path, constitutive state stays in registers
-  const int nact = *B.nactive;
-  (void)nblk;
-  for (int ai = blockIdx.x; ai < nact; ai += gridDim.x) {
-    const int blk = B.active[ai];
-    const int start = B.offs[blk * 8], end = B.offs[blk * 8 + 8];
-    int o[3];
-    tile_origin(blk, B.nb, o);
-    for (int i = threadIdx.x; i < 4 * kTileNodes; i += blockDim.x) (&tile[0][0])[i] = 0.f;
-    __syncthreads();
-    for (int i0 = start; i0 < end; i0 += blockDim.x) {
-      const int i = i0 + (int)threadIdx.x;
-      const int p = i < end ? B.perm[i] : -1;
-      const bool active = p >= 0 && !(M.sel && M.sel[p] != 0);
-      float Ct[9], S[9], xp[3] = {(o[0] + 2.f) * dx, (o[1] + 2.f) * dx, (o[2] + 2.f) * dx}, vp[3] = {0.f, 0.f, 0.f},
-                                mp = 0.f;
-      m_zero(Ct);
-      m_zero(S);
-      if (active) {
-        float Ftr[9];
-        load9(F, p, Ftr);
-        float mu = M.mu[p], lam = M.lam[p], ys = M.ys[p];
-        ConstitState st;
-        float tau[9];
-        constitutive_fwd(st, Ftr, mu, lam, ys, matp, dt, tau);
-        if (write_params) {
-          M.mu[p] = mu;
-          M.lam[p] = lam;
-          M.ys[p] = ys;
-        }
-        store9(Fe_out, p, st.Fe);
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          xp[d] = x[3 * p + d];
-          vp[d] = v[3 * p + d];
-        }
-        float Cp[9];
-        load9(C, p, Cp);
-        rpic_blend(Cp, M.rpic, Ct);
-        mp = M.mass[p];
-        const float Vp = M.vol[p];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) S[k] = -Vp * tau[k];
-      }
-      Stencil s;
-      make_stencil(xp, inv_dx, s);
-      auto contrib = [&](int j, float* out) {
-        const int a = j / 9, b = (j / 3) % 3, c = j % 3;
-        const float dpos[3] = {((float)a - s.f[0]) * dx, ((float)b - s.f[1]) * dx, ((float)c - s.f[2]) * dx};
-        const float wt = sw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c);
-        const float gw[3] = {sdw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c) * inv_dx, sw(s, 0, a) * sdw(s, 1, b) * sw(s, 2, c) * inv_dx,
-                             sw(s, 0, a) * sw(s, 1, b) * sdw(s, 2, c) * inv_dx};
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const float cd = Ct[k * 3 + 0] * dpos[0] + Ct[k * 3 + 1] * dpos[1] + Ct[k * 3 + 2] * dpos[2];
-          const float ef = S[k * 3 + 0] * gw[0] + S[k * 3 + 1] * gw[1] + S[k * 3 + 2] * gw[2];
-          out[k] = wt * mp * (vp[k] + cd) + dt * ef;
-        }
-        out[3] = wt * mp;
-      };
-      const bool in_tile = active && stencil_in_tile(s.base, o);
-      scatter_stencil<4>(in_tile, s.base, contrib, [&](int ix, int iy, int iz, int comp, float val) {
-        atomicAdd(&tile[comp][((ix - o[0]) * kTileN + (iy - o[1])) * kTileN + (iz - o[2])], val);
-      });
-      if (active && !in_tile) scatter_overflow<4>(ovf, M.n, s.base, contrib);
-    }
-    __syncthreads();
-    float4* dst = tiles + (size_t)blk * kTileNodes;
-    for (int k = threadIdx.x; k < kTileNodes; k += blockDim.x)
-      dst[k] = make_float4(tile[0][k], tile[1][k], tile[2][k], tile[3][k]);
-    __syncthreads();
+  ConstitState st;
+  float tau[9];
+  constitutive_fwd(st, Ftr, mu, lam, ys, matp, dt, tau);
+  if (write_params) {
+    M.mu[p] = mu;
+    M.lam[p] = lam;
+    M.ys[p] = ys;
   }
+  store9(Fe_out, p, st.Fe);
+  const float k = -dt * M.vol[p] * M.inv_dx;
+  float S[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) S[i] = k * tau[i];
+  store9(S_out, p, S);
+}
+
+// The stress stage of the first substep of a forward call (later substeps
+// get it from the previous substep's G2P).
+template <int MAT>
+__global__ __launch_bounds__(kBlock) void k_stress(DevModel M, float dt, const float* __restrict__ F,
+                                                   float* __restrict__ Fe_out, float* __restrict__ S_out,
+                                                   int write_params) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M.N || (M.sel && M.sel[p] != 0)) return;
+  float Ftr[9];
+  load9(F, p, Ftr);
+  particle_stress<MAT>(M, dt, p, Ftr, Fe_out, S_out, write_params);
+}
+
+// p2g_apic_with_stress (mpm_utils.py:358-423): APIC momentum, mass and the
+// stress impulse into the block tiles.
+__global__ __launch_bounds__(kBlkWG) void k_p2g(DevModel M, const float* __restrict__ x, const float* __restrict__ v,
+                                                const float* __restrict__ C, const float* __restrict__ S, Binning B,
+                                                float4* __restrict__ tiles, float4* __restrict__ ovf) {
+  const float dx = M.dx, r = M.rpic;
+  block_scatter<4>(B, M.sel, M.inv_dx, M.n, tiles, ovf, [&](int p, float* xp, ScatterPayload& P) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xp[d] = x[3 * p + d];
+    const float mp = M.mass[p];
+    float Cp[9], Ct[9];
+    load9(C, p, Cp);
+    rpic_blend(Cp, r, Ct);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) P.A[d] = mp * v[3 * p + d];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) P.B[k] = mp * dx * Ct[k];
+    load9(S, p, P.S);
+    P.m = mp;
+  });
 }
 
 // Per node: gather (mv, m) from the tiles and the overflow grid, then
@@ -575,11 +653,17 @@ __global__ __launch_bounds__(kBlock) void k_grid(DevModel M, ColTable T, float t
   v_out[id] = make_float4(v[0], v[1], v[2], g.w);
 }
 
-// g2p_differentiable (mpm_utils.py:502-579)
-__global__ __launch_bounds__(kBlock) void k_g2p(DevModel M, float dt, const float* __restrict__ x,
-                                                const float* __restrict__ Fe, const float4* __restrict__ v_out,
-                                                float* __restrict__ x1, float* __restrict__ v1,
-                                                float* __restrict__ C1, float* __restrict__ F1) {
+// g2p_differentiable (mpm_utils.py:502-579).  With `next_stress` set the
+// stress stage of the next substep runs on the new F_trial here (writing
+// Fe_next and S), so the scatter kernel needs no SVD.
+// Fe and Fe_next may be the same buffer (each particle reads its Fe before
+// writing Fe_next).
+template <int MAT>
+__global__ __launch_bounds__(kBlock) void k_g2p(DevModel M, float dt, const float* __restrict__ x, const float* Fe,
+                                                const float4* __restrict__ v_out, float* __restrict__ x1,
+                                                float* __restrict__ v1, float* __restrict__ C1,
+                                                float* __restrict__ F1, int next_stress, float* Fe_next,
+                                                float* __restrict__ S_next, int write_params) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= M.N) return;
   if (M.sel && M.sel[p] != 0) {  // partial_clone defaults (mpm_data_structure.py:356-387)
@@ -610,9 +694,9 @@ __global__ __launch_bounds__(kBlock) void k_g2p(DevModel M, float dt, const floa
         if (node_in(ix, iy, iz, M.n)) gv = v_out[node_id(ix, iy, iz, M.n)];
         const float gvv[3] = {gv.x, gv.y, gv.z};
         const float dpos[3] = {(float)a - s.f[0], (float)b - s.f[1], (float)c - s.f[2]};
-        const float wt = sw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c);
-        const float gw[3] = {sdw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c) * M.inv_dx, sw(s, 0, a) * sdw(s, 1, b) * sw(s, 2, c) * M.inv_dx,
-                             sw(s, 0, a) * sw(s, 1, b) * sdw(s, 2, c) * M.inv_dx};
+        const float wt = s.w[0][a] * s.w[1][b] * s.w[2][c];
+        const float gw[3] = {s.dw[0][a] * s.w[1][b] * s.w[2][c] * M.inv_dx, s.w[0][a] * s.dw[1][b] * s.w[2][c] * M.inv_dx,
+                             s.w[0][a] * s.w[1][b] * s.dw[2][c] * M.inv_dx};
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
           nv[i] += gvv[i] * wt;
@@ -638,150 +722,134 @@ __global__ __launch_bounds__(kBlock) void k_g2p(DevModel M, float dt, const floa
   load9(Fe, p, Fp);
   m_mul(IF, Fp, Fn);
   store9(F1, p, Fn);
+  if (next_stress) particle_stress<MAT>(M, dt, p, Fn, Fe_next, S_next, write_params);
 }
 
 // ---------------------------------------------------------------------------
 // Adjoint kernels
 // ---------------------------------------------------------------------------
-// Adjoint of g2p_differentiable: per particle dL/dx (partial) and dL/dFe,
-// and the scatter of dL/dv_out into block tiles (same ownership as k_p2g).
-__global__ __launch_bounds__(kBlkWG) void k_g2p_adj(DevModel M, float dt, const float* __restrict__ x,
+// Adjoint of g2p_differentiable, per particle: dL/dx (partial), dL/dFe, and
+// the payload of the dL/dv_out scatter (A = dL/dv', B = 4 inv_dx dL/dC',
+// S = dt inv_dx dL/dF' Fe^T), stored kAdjPayload floats per particle.
+__global__ __launch_bounds__(kBlock) void k_g2p_adj(DevModel M, float dt, const float* __restrict__ x,
                                                     const float* __restrict__ Fe, const float4* __restrict__ v_out,
                                                     const float* __restrict__ dx1, const float* __restrict__ dv1,
                                                     const float* __restrict__ dC1, const float* __restrict__ dF1,
-                                                    float* __restrict__ xbar, float* __restrict__ Febar, Binning B,
-                                                    float4* __restrict__ tiles, float4* __restrict__ ovf) {
-  __shared__ float tile[4][kTileNodes];
-  const int nblk = B.nb * B.nb * B.nb;
+                                                    float* __restrict__ xbar, float* __restrict__ Febar,
+                                                    float* __restrict__ payload) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= M.N) return;
+  if (M.sel && M.sel[p] != 0) {
+    xbar[3 * p] = xbar[3 * p + 1] = xbar[3 * p + 2] = 0.f;
+    float Z[9];
+    m_zero(Z);
+    store9(Febar, p, Z);
+    return;
+  }
   const float inv_dx = M.inv_dx;
   const float c4 = inv_dx * 4.f;
-  const int nact = *B.nactive;
-  (void)nblk;
-  for (int ai = blockIdx.x; ai < nact; ai += gridDim.x) {
-    const int blk = B.active[ai];
-    const int start = B.offs[blk * 8], end = B.offs[blk * 8 + 8];
-    int o[3];
-    tile_origin(blk, B.nb, o);
-    for (int i = threadIdx.x; i < 4 * kTileNodes; i += blockDim.x) (&tile[0][0])[i] = 0.f;
-    __syncthreads();
-    for (int i0 = start; i0 < end; i0 += blockDim.x) {
-      const int i = i0 + (int)threadIdx.x;
-      const int p = i < end ? B.perm[i] : -1;
-      const bool active = p >= 0 && !(M.sel && M.sel[p] != 0);
-      float xp[3] = {(o[0] + 2.f) * M.dx, (o[1] + 2.f) * M.dx, (o[2] + 2.f) * M.dx};
-      if (active) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) xp[d] = x[3 * p + d];
-      }
-      Stencil s;
-      make_stencil(xp, inv_dx, s);
-      float nvb[3] = {0.f, 0.f, 0.f}, Gb[9], dC[9];
-      m_zero(Gb);
-      m_zero(dC);
-      if (active) {
-        // pass 1: v' and grad v for the clamp mask and the F adjoint
-        float nv[3] = {0.f, 0.f, 0.f}, nG[9];
-        m_zero(nG);
+  const float xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
+  Stencil s;
+  make_stencil(xp, inv_dx, s);
+  // pass 1: v' and grad v for the clamp mask and the F adjoint
+  float nv[3] = {0.f, 0.f, 0.f}, nG[9];
+  m_zero(nG);
 #pragma unroll 1
-        for (int j = 0; j < 27; ++j) {
-          const int a = j / 9, b = (j / 3) % 3, c = j % 3;
-          const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
-          float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
-          if (node_in(ix, iy, iz, M.n)) gv = v_out[node_id(ix, iy, iz, M.n)];
-          const float gvv[3] = {gv.x, gv.y, gv.z};
-          const float wt = sw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c);
-          const float gw[3] = {sdw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c) * inv_dx, sw(s, 0, a) * sdw(s, 1, b) * sw(s, 2, c) * inv_dx,
-                               sw(s, 0, a) * sw(s, 1, b) * sdw(s, 2, c) * inv_dx};
+  for (int j = 0; j < 27; ++j) {
+    const int a = j / 9, b = (j / 3) % 3, c = j % 3;
+    const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
+    float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (node_in(ix, iy, iz, M.n)) gv = v_out[node_id(ix, iy, iz, M.n)];
+    const float vi[3] = {gv.x, gv.y, gv.z};
+    const float wt = sw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c);
+    const float gw[3] = {sdw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c) * inv_dx, sw(s, 0, a) * sdw(s, 1, b) * sw(s, 2, c) * inv_dx,
+                         sw(s, 0, a) * sw(s, 1, b) * sdw(s, 2, c) * inv_dx};
 #pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            nv[k] += gvv[k] * wt;
+    for (int k = 0; k < 3; ++k) {
+      nv[k] += vi[k] * wt;
 #pragma unroll
-            for (int l = 0; l < 3; ++l) nG[k * 3 + l] += gvv[k] * gw[l];
-          }
-        }
-        const float amin = M.dx * 2.f, amax = M.grid_lim - M.dx * 2.f;
-        float xb[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const float xn = xp[k] + dt * nv[k];
-          // wp.clamp passes the adjoint to x unless x < lo or x > hi
-          const float msk = (xn < amin || xn > amax) ? 0.f : 1.f;
-          const float g = dx1[3 * p + k] * msk;
-          xb[k] = g;
-          nvb[k] = dv1[3 * p + k] + dt * g;
-        }
-        float dF[9], Fp[9];
-        load9(dF1, p, dF);
-        load9(Fe, p, Fp);
-        load9(dC1, p, dC);
-        // F1 = (I + dt G) Fe:  Gbar = dt dF Fe^T ; Febar = (I + dt G)^T dF
-        float IF[9], Feb[9];
-        m_mul_bt(dF, Fp, Gb);
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-          Gb[k] *= dt;
-          IF[k] = ((k % 4 == 0) ? 1.f : 0.f) + nG[k] * dt;
-        }
-        m_mul_at(IF, dF, Feb);
-        store9(Febar, p, Feb);
-        // pass 2: d(w, grad w, d)/dx
-        float fb[3] = {0.f, 0.f, 0.f};
-#pragma unroll 1
-        for (int j = 0; j < 27; ++j) {
-          const int a = j / 9, b = (j / 3) % 3, c = j % 3;
-          const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
-          float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
-          if (node_in(ix, iy, iz, M.n)) gv = v_out[node_id(ix, iy, iz, M.n)];
-          const float vi[3] = {gv.x, gv.y, gv.z};
-          const float d[3] = {(float)a - s.f[0], (float)b - s.f[1], (float)c - s.f[2]};
-          const float wt = sw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c);
-          float Cd[3];
-#pragma unroll
-          for (int k = 0; k < 3; ++k) Cd[k] = dC[k * 3 + 0] * d[0] + dC[k * 3 + 1] * d[1] + dC[k * 3 + 2] * d[2];
-          const float wb =
-              nvb[0] * vi[0] + nvb[1] * vi[1] + nvb[2] * vi[2] + c4 * (vi[0] * Cd[0] + vi[1] * Cd[1] + vi[2] * Cd[2]);
-          float gwb[3];
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            // d = off - f: dd_k = 4 inv_dx w sum_i dC_ik v_i
-            fb[k] -= c4 * wt * (dC[0 * 3 + k] * vi[0] + dC[1 * 3 + k] * vi[1] + dC[2 * 3 + k] * vi[2]);
-            gwb[k] = Gb[0 * 3 + k] * vi[0] + Gb[1 * 3 + k] * vi[1] + Gb[2 * 3 + k] * vi[2];
-          }
-          weight_adj(s, a, b, c, inv_dx, wb, gwb, fb);
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) xbar[3 * p + k] = xb[k] + inv_dx * fb[k];
-      } else if (p >= 0) {
-        xbar[3 * p] = xbar[3 * p + 1] = xbar[3 * p + 2] = 0.f;
-        float Z[9];
-        m_zero(Z);
-        store9(Febar, p, Z);
-      }
-      auto contrib = [&](int j, float* out) {
-        const int a = j / 9, b = (j / 3) % 3, c = j % 3;
-        const float d[3] = {(float)a - s.f[0], (float)b - s.f[1], (float)c - s.f[2]};
-        const float wt = sw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c);
-        const float gw[3] = {sdw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c) * inv_dx, sw(s, 0, a) * sdw(s, 1, b) * sw(s, 2, c) * inv_dx,
-                             sw(s, 0, a) * sw(s, 1, b) * sdw(s, 2, c) * inv_dx};
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const float Cd = dC[k * 3 + 0] * d[0] + dC[k * 3 + 1] * d[1] + dC[k * 3 + 2] * d[2];
-          out[k] = wt * nvb[k] + c4 * wt * Cd + (Gb[k * 3 + 0] * gw[0] + Gb[k * 3 + 1] * gw[1] + Gb[k * 3 + 2] * gw[2]);
-        }
-        out[3] = 0.f;
-      };
-      const bool in_tile = active && stencil_in_tile(s.base, o);
-      scatter_stencil<3>(in_tile, s.base, contrib, [&](int ix, int iy, int iz, int comp, float val) {
-        atomicAdd(&tile[comp][((ix - o[0]) * kTileN + (iy - o[1])) * kTileN + (iz - o[2])], val);
-      });
-      if (active && !in_tile) scatter_overflow<3>(ovf, M.n, s.base, contrib);
+      for (int l = 0; l < 3; ++l) nG[k * 3 + l] += vi[k] * gw[l];
     }
-    __syncthreads();
-    float4* dst = tiles + (size_t)blk * kTileNodes;
-    for (int k = threadIdx.x; k < kTileNodes; k += blockDim.x)
-      dst[k] = make_float4(tile[0][k], tile[1][k], tile[2][k], 0.f);
-    __syncthreads();
   }
+  const float amin = M.dx * 2.f, amax = M.grid_lim - M.dx * 2.f;
+  float xb[3], nvb[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float xn = xp[k] + dt * nv[k];
+    // wp.clamp passes the adjoint to x unless x < lo or x > hi
+    const float msk = (xn < amin || xn > amax) ? 0.f : 1.f;
+    const float g = dx1[3 * p + k] * msk;
+    xb[k] = g;
+    nvb[k] = dv1[3 * p + k] + dt * g;
+  }
+  float dF[9], Fp[9], dC[9], Gb[9];
+  load9(dF1, p, dF);
+  load9(Fe, p, Fp);
+  load9(dC1, p, dC);
+  // F1 = (I + dt G) Fe:  Gbar = dt dF Fe^T ; Febar = (I + dt G)^T dF
+  float IF[9], Feb[9];
+  m_mul_bt(dF, Fp, Gb);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    Gb[k] *= dt;
+    IF[k] = ((k % 4 == 0) ? 1.f : 0.f) + nG[k] * dt;
+  }
+  m_mul_at(IF, dF, Feb);
+  store9(Febar, p, Feb);
+  // pass 2: d(w, grad w, d)/dx
+  float fb[3] = {0.f, 0.f, 0.f};
+#pragma unroll 1
+  for (int j = 0; j < 27; ++j) {
+    const int a = j / 9, b = (j / 3) % 3, c = j % 3;
+    const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
+    float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (node_in(ix, iy, iz, M.n)) gv = v_out[node_id(ix, iy, iz, M.n)];
+    const float vi[3] = {gv.x, gv.y, gv.z};
+    const float d[3] = {(float)a - s.f[0], (float)b - s.f[1], (float)c - s.f[2]};
+    const float wt = sw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c);
+    float Cd[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) Cd[k] = dC[k * 3 + 0] * d[0] + dC[k * 3 + 1] * d[1] + dC[k * 3 + 2] * d[2];
+    const float wb =
+        nvb[0] * vi[0] + nvb[1] * vi[1] + nvb[2] * vi[2] + c4 * (vi[0] * Cd[0] + vi[1] * Cd[1] + vi[2] * Cd[2]);
+    float gwb[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      // d = off - f: dd_k = 4 inv_dx w sum_i dC_ik v_i
+      fb[k] -= c4 * wt * (dC[0 * 3 + k] * vi[0] + dC[1 * 3 + k] * vi[1] + dC[2 * 3 + k] * vi[2]);
+      gwb[k] = Gb[0 * 3 + k] * vi[0] + Gb[1 * 3 + k] * vi[1] + Gb[2 * 3 + k] * vi[2];
+    }
+    weight_adj(s, a, b, c, inv_dx, wb, gwb, fb);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) xbar[3 * p + k] = xb[k] + inv_dx * fb[k];
+  float* q = payload + (size_t)kAdjPayload * p;
+  float4* q4 = reinterpret_cast<float4*>(q);
+  q4[0] = make_float4(nvb[0], nvb[1], nvb[2], 0.f);
+  q4[1] = make_float4(c4 * dC[0], c4 * dC[1], c4 * dC[2], c4 * dC[3]);
+  q4[2] = make_float4(c4 * dC[4], c4 * dC[5], c4 * dC[6], c4 * dC[7]);
+  q4[3] = make_float4(c4 * dC[8], inv_dx * Gb[0], inv_dx * Gb[1], inv_dx * Gb[2]);
+  q4[4] = make_float4(inv_dx * Gb[3], inv_dx * Gb[4], inv_dx * Gb[5], inv_dx * Gb[6]);
+  q4[5] = make_float4(inv_dx * Gb[7], inv_dx * Gb[8], 0.f, 0.f);
+}
+
+// Scatter of dL/dv_out (the transpose of the G2P gather) into the block tiles.
+__global__ __launch_bounds__(kBlkWG) void k_g2p_adj_scatter(DevModel M, const float* __restrict__ x,
+                                                            const float* __restrict__ payload, Binning B,
+                                                            float4* __restrict__ tiles, float4* __restrict__ ovf) {
+  block_scatter<3>(B, M.sel, M.inv_dx, M.n, tiles, ovf, [&](int p, float* xp, ScatterPayload& P) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xp[d] = x[3 * p + d];
+    const float4* q4 = reinterpret_cast<const float4*>(payload + (size_t)kAdjPayload * p);
+    const float4 a = q4[0], b = q4[1], c = q4[2], d = q4[3], e = q4[4], f = q4[5];
+    P.A[0] = a.x, P.A[1] = a.y, P.A[2] = a.z;
+    P.B[0] = b.x, P.B[1] = b.y, P.B[2] = b.z, P.B[3] = b.w;
+    P.B[4] = c.x, P.B[5] = c.y, P.B[6] = c.z, P.B[7] = c.w;
+    P.B[8] = d.x, P.S[0] = d.y, P.S[1] = d.z, P.S[2] = d.w;
+    P.S[3] = e.x, P.S[4] = e.y, P.S[5] = e.z, P.S[6] = e.w;
+    P.S[7] = f.x, P.S[8] = f.y;
+    P.m = 0.f;
+  });
 }
 
 // Adjoint of the grid stage: gather dL/dv_out (tiles + overflow), replay the
@@ -1132,6 +1200,8 @@ struct empm_ctx {
   float4* v_out;    // G
   float4* gadj_in;  // G: dL/d(mv, m)
   float* Fe;        // N*9 scratch
+  float* S;         // N*9: stress part of the P2G payload (written by k_stress / k_g2p)
+  float* payload;   // N*kAdjPayload: G2P-adjoint scatter payload
   float* xbar;      // N*3 scratch
   float* Febar;     // N*9 scratch
   int* perm;        // N: particles in bin order
@@ -1145,8 +1215,8 @@ struct empm_ctx {
 
 namespace {
 
-const char* const kKernelNames[EMPM_NUM_KERNELS] = {"k_p2g", "k_grid", "k_g2p", "k_g2p_adj", "k_grid_adj",
-                                                    "k_p2g_adj"};
+const char* const kKernelNames[EMPM_NUM_KERNELS] = {"k_p2g",     "k_grid",     "k_g2p",    "k_g2p_adj",
+                                                    "k_grid_adj", "k_p2g_adj", "k_stress", "k_g2p_adj_scatter"};
 
 hipEvent_t pool_get(KernelProfile* p) {
   if (!p->pool.empty()) {
@@ -1292,17 +1362,25 @@ int do_bin(empm_ctx* ctx, int N, const float* x, float inv_dx, hipStream_t st) {
   return EMPM_OK;
 }
 
+// One forward substep in -> out.  `first`: run the stress stage here (else
+// the previous substep's G2P did); `last`: no stress stage for a next substep.
 int launch_fwd(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, float dt, const empm_particles* in,
-               const empm_particles* out, const empm_grid_rec* rec, hipStream_t st) {
+               const empm_particles* out, const empm_grid_rec* rec, bool first, bool last, hipStream_t st) {
   const int write_params = (M.material == 5 || M.mp.hardening == 1.f) ? 1 : 0;
   float* Fe = in->Fe ? in->Fe : ctx->Fe;
+  float* Fe_next = out->Fe ? out->Fe : ctx->Fe;
   float4* keep = rec ? reinterpret_cast<float4*>(rec->mv_m) : nullptr;
   float4* v_out = rec ? reinterpret_cast<float4*>(rec->v_out) : ctx->v_out;
   const Binning B = binning(ctx);
+  if (M.N > 0 && first) {
+    ProfScope ps(ctx, EMPM_KERNEL_STRESS, st);
+    EMPM_LAUNCH_MAT(M.material, k_stress, dim3(nblocks(M.N)), dim3(kBlock), st, M, dt, in->F, Fe, ctx->S,
+                    write_params);
+  }
   if (M.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
-    EMPM_LAUNCH_MAT(M.material, k_p2g, dim3(block_grid(ctx)), dim3(kBlkWG), st, M, dt, in->x, in->v, in->C, in->F, Fe,
-                    B, ctx->tiles, ctx->ovf, write_params);
+    hipLaunchKernelGGL(k_p2g, dim3(block_grid(ctx)), dim3(kBlkWG), 0, st, M, in->x, in->v, in->C, ctx->S, B,
+                       ctx->tiles, ctx->ovf);
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
@@ -1311,15 +1389,15 @@ int launch_fwd(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, 
   }
   if (M.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_G2P, st);
-    hipLaunchKernelGGL(k_g2p, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, Fe, v_out, out->x, out->v,
-                       out->C, out->F);
+    EMPM_LAUNCH_MAT(M.material, k_g2p, dim3(nblocks(M.N)), dim3(kBlock), st, M, dt, in->x, Fe, v_out, out->x, out->v,
+                    out->C, out->F, last ? 0 : 1, Fe_next, ctx->S, write_params);
   }
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }
 
 // Adjoint of one substep.  With a recorded grid (rec) and Fe (in->Fe) the
-// forward P2G and boundary pass are not recomputed.
+// forward stress, P2G and boundary pass are not recomputed.
 int launch_adj(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, float dt, const empm_particles* in,
                const empm_grid_rec* rec, const empm_particles* d_out, const empm_particles* d_in, float* d_mu,
                float* d_lam, float* d_ys, float* d_mass, float* d_vol, int flags, hipStream_t st) {
@@ -1330,9 +1408,13 @@ int launch_adj(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, 
   float4* v_out = recorded ? reinterpret_cast<float4*>(rec->v_out) : ctx->v_out;
   if (!recorded) {
     if (M.N > 0) {
+      ProfScope ps(ctx, EMPM_KERNEL_STRESS, st);
+      EMPM_LAUNCH_MAT(M.material, k_stress, dim3(nblocks(M.N)), dim3(kBlock), st, M, dt, in->F, Fe, ctx->S, 0);
+    }
+    if (M.N > 0) {
       ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
-      EMPM_LAUNCH_MAT(M.material, k_p2g, dim3(block_grid(ctx)), dim3(kBlkWG), st, M, dt, in->x, in->v, in->C, in->F,
-                      Fe, B, ctx->tiles, ctx->ovf, 0);
+      hipLaunchKernelGGL(k_p2g, dim3(block_grid(ctx)), dim3(kBlkWG), 0, st, M, in->x, in->v, in->C, ctx->S, B,
+                         ctx->tiles, ctx->ovf);
     }
     {
       ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
@@ -1341,9 +1423,16 @@ int launch_adj(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, 
     }
   }
   if (M.N > 0) {
-    ProfScope ps(ctx, EMPM_KERNEL_G2P_ADJ, st);
-    hipLaunchKernelGGL(k_g2p_adj, dim3(block_grid(ctx)), dim3(kBlkWG), 0, st, M, dt, in->x, Fe, v_out,
-                       d_out->x, d_out->v, d_out->C, d_out->F, ctx->xbar, ctx->Febar, B, ctx->tiles, ctx->ovf);
+    {
+      ProfScope ps(ctx, EMPM_KERNEL_G2P_ADJ, st);
+      hipLaunchKernelGGL(k_g2p_adj, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, Fe, v_out, d_out->x,
+                         d_out->v, d_out->C, d_out->F, ctx->xbar, ctx->Febar, ctx->payload);
+    }
+    {
+      ProfScope ps(ctx, EMPM_KERNEL_G2P_ADJ_SCATTER, st);
+      hipLaunchKernelGGL(k_g2p_adj_scatter, dim3(block_grid(ctx)), dim3(kBlkWG), 0, st, M, in->x, ctx->payload, B,
+                         ctx->tiles, ctx->ovf);
+    }
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID_ADJ, st);
@@ -1412,6 +1501,8 @@ int empm_create(empm_ctx** out, const empm_create_desc* desc) {
   e = e == hipSuccess ? hipMalloc((void**)&c->v_out, c->G * sizeof(float4)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->gadj_in, c->G * sizeof(float4)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->Fe, np * 9 * sizeof(float)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->S, np * 9 * sizeof(float)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->payload, np * kAdjPayload * sizeof(float)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->xbar, np * 3 * sizeof(float)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->Febar, np * 9 * sizeof(float)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->perm, np * sizeof(int)) : e;
@@ -1450,6 +1541,8 @@ void empm_destroy(empm_ctx* c) {
   hipFree(c->v_out);
   hipFree(c->gadj_in);
   hipFree(c->Fe);
+  hipFree(c->S);
+  hipFree(c->payload);
   hipFree(c->xbar);
   hipFree(c->Febar);
   hipFree(c->perm);
@@ -1566,7 +1659,7 @@ int empm_substep_fwd(empm_ctx* ctx, const empm_model* model, const empm_collider
   if ((rc = make_table(cols, ncol, T, model->n_grid))) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
   if ((rc = do_bin(ctx, model->n_particles, in->x, model->inv_dx, (hipStream_t)stream))) return rc;
-  return launch_fwd(ctx, dev_model(model), T, time, dt, in, out, nullptr, (hipStream_t)stream);
+  return launch_fwd(ctx, dev_model(model), T, time, dt, in, out, nullptr, true, true, (hipStream_t)stream);
 }
 
 int empm_substep_adj(empm_ctx* ctx, const empm_model* model, const empm_collider* cols, int ncol, float time, float dt,
@@ -1610,7 +1703,7 @@ int empm_rollout_fwd(empm_ctx* ctx, const empm_model* model, empm_collider* cols
         (rc = do_bin(ctx, model->n_particles, states[k].x, model->inv_dx, (hipStream_t)stream)))
       return rc;
     if ((rc = launch_fwd(ctx, M, T, (float)*time, (float)dt, &states[k], &states[k + 1], grids ? &grids[k] : nullptr,
-                         (hipStream_t)stream)))
+                         k == 0, k == n_steps - 1, (hipStream_t)stream)))
       return rc;
     advance_colliders(cols, ncol, *time, dt);
     *time = *time + dt;
@@ -1639,7 +1732,7 @@ int empm_rollout_fwd_replay(empm_ctx* ctx, const empm_model* model, const empm_c
         (rc = do_bin(ctx, model->n_particles, states[k].x, model->inv_dx, (hipStream_t)stream)))
       return rc;
     if ((rc = launch_fwd(ctx, M, T, (float)times_hist[k], (float)dt, &states[k], &states[k + 1],
-                         grids ? &grids[k] : nullptr, (hipStream_t)stream)))
+                         grids ? &grids[k] : nullptr, k == 0, k == n_steps - 1, (hipStream_t)stream)))
       return rc;
   }
   return EMPM_OK;

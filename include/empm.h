@@ -238,13 +238,15 @@ int empm_set_rebin_interval(empm_ctx *ctx, int substeps);
 
 /* Kernel ids for per-kernel timing */
 enum {
-    EMPM_KERNEL_P2G = 0,      /* stress + P2G scatter */
+    EMPM_KERNEL_P2G = 0,      /* P2G scatter (momentum, mass, stress impulse) */
     EMPM_KERNEL_GRID = 1,     /* grid normalise / gravity / colliders */
-    EMPM_KERNEL_G2P = 2,      /* G2P gather + state update */
-    EMPM_KERNEL_G2P_ADJ = 3,  /* adjoint of G2P (scatter) */
+    EMPM_KERNEL_G2P = 2,      /* G2P gather + state update + next substep's stress */
+    EMPM_KERNEL_G2P_ADJ = 3,  /* adjoint of G2P, per particle */
     EMPM_KERNEL_GRID_ADJ = 4, /* adjoint of the grid stage */
     EMPM_KERNEL_P2G_ADJ = 5,  /* adjoint of P2G + stress (gather) */
-    EMPM_NUM_KERNELS = 6
+    EMPM_KERNEL_STRESS = 6,   /* stress of the first substep of a call */
+    EMPM_KERNEL_G2P_ADJ_SCATTER = 7, /* adjoint of G2P: scatter of dL/dv_out */
+    EMPM_NUM_KERNELS = 8
 };
 
 /* Instrumentation (no counterpart in the reference, whose per-stage

@@ -184,7 +184,11 @@ def test_checkpoint_interval_does_not_change_gradients(dev):
     l1, g1 = _loss_grad_gpu(sc, dev, target, prm, 24, ckpt=24)
     l2, g2 = _loss_grad_gpu(sc, dev, target, prm, 24, ckpt=5)
     l3, g3 = _loss_grad_gpu(sc, dev, target, prm, 24, ckpt=1)
-    assert l1 == l2 == l3
+    # the forward splits into one call per checkpoint segment; a call's first
+    # substep evaluates the stress from the stored F_trial and later ones in
+    # the G2P that produced it, which may contract the F update into the SVD
+    # differently (FP32 round-off only)
+    assert np.allclose([l2, l3], l1, rtol=1e-6, atol=0), (l1, l2, l3)
     assert np.allclose(g1, g2, rtol=1e-5) and np.allclose(g1, g3, rtol=1e-5), (g1, g2, g3)
 
 
