@@ -37,23 +37,47 @@ B_FB_K = 50
 
 
 def kernel_bytes(N, G):
-    """Algorithmic HBM bytes per launch of each kernel (FP32): the state,
-    constants and grid fields that kernel must read or write once."""
+    """Algorithmic HBM bytes per launch of each kernel (FP32): the particle
+    and node fields that kernel must read or write, each once."""
     return {
-        # reads x, v, C, F_trial (96 B) + mass, vol, mu, lam, yield (20 B); writes (mv, m) once per node
-        "k_p2g": 116 * N + 16 * G,
+        # reads x, v, C (60 B), the stress payload S (36 B) and mass (4 B); writes (mv, m) once per node
+        "k_p2g": 100 * N + 16 * G,
         # reads (mv, m), writes v_out (12 B) per node
         "k_grid": 28 * G,
-        # reads x, F (48 B) and v_out once per node; writes x, v, C, F_trial (96 B)
-        "k_g2p": 144 * N + 12 * G,
-        # reads x, F (48), adjoint of the next state (96), v_out; writes partial adjoints (48) and dL/dv_out
-        "k_g2p_adj": 192 * N + 24 * G,
+        # reads x, Fe (48 B), mu, lam, yield, vol (16 B) and v_out once per node; writes x, v, C, F_trial
+        # (96 B) and the next substep's Fe and S (72 B)
+        "k_g2p": 232 * N + 12 * G,
+        # reads F_trial (36 B) and mu, lam, yield, vol (16 B); writes Fe and S (72 B)
+        "k_stress": 124 * N,
+        # reads x, Fe (48), the adjoint of the next state (96) and v_out; writes dL/dx partial (12), dL/dFe (36)
+        # and the scatter payload (84)
+        "k_g2p_adj": 276 * N + 12 * G,
+        # reads x and the payload (96 B); writes dL/dv_out once per node (12 B)
+        "k_g2p_adj_scatter": 96 * N + 12 * G,
         # reads (mv, m) and dL/dv_out, writes dL/d(mv, m)
         "k_grid_adj": 44 * G,
         # reads state + constants (116), partial adjoints (48), dL/d(mv, m) once per node;
         # writes the adjoint of the state (96) and read-modify-writes 5 parameter adjoints (40)
         "k_p2g_adj": 300 * N + 16 * G,
     }
+
+
+# HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE counter runs of
+# this workload (calibrated on a plain copy); the file gives the commands
+TRAFFIC_PROFILE = "profiles/c2_hbm_traffic_pmc.md"
+
+
+def measured_traffic(kernel):
+    """Counter-measured HBM bytes per launch of `kernel`, or None."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return int(parts[3])
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -238,7 +262,9 @@ def main():
         "loss": float(loss),
         "per_kernel_ms_one_step": {k: v[1] for k, v in per_kernel.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": kb,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom) if (N, n) == (200_000, 64) else None,
+                     "traffic_source": TRAFFIC_PROFILE, "algorithmic_bytes": kb,
                      "avg_launch_us": avg_ms * 1000.0, "launches": l.value},
         "cpu_baseline": cpu,
     }

@@ -212,6 +212,33 @@ def test_drucker_prager_three_cases():
     assert abs(np.linalg.norm(ee - ee.mean()) + c * ee.sum()) < 1e-12
 
 
+def test_viscoplastic_return_limits():
+    """Material 3 (mpm_utils.py:189-234): with zero viscosity the return is
+    the radial return onto |dev s| = sqrt(2/3) Y with the log-volume kept;
+    with viscosity the deviatoric stress lies strictly between that surface
+    and the trial state and tends to the trial state as viscosity grows."""
+    mu, lam, dt = 3e4, 5e4, 1e-4
+    for _ in range(20):
+        F = rand_rot(RNG) @ np.diag(RNG.uniform(0.8, 1.25, 3)) @ rand_rot(RNG).T
+        ys = RNG.uniform(100, 2000)
+        s_tr = np.linalg.svd(F, compute_uv=False)
+        e_tr = np.log(np.maximum(s_tr, 0.01))
+        dev_tr = 2 * mu * np.linalg.norm(e_tr - e_tr.mean())
+        yv = np.sqrt(2.0 / 3.0) * ys
+        devs = []
+        for eta in (0.0, 10.0, 1e9):
+            Fe, *_ = orc.constitutive(3, F, mu, lam, ys, P_(3, plastic_viscosity=eta), dt=dt)
+            e_e = np.log(np.linalg.svd(Fe, compute_uv=False))
+            assert abs(e_e.sum() - e_tr.sum()) < 1e-10
+            devs.append(2 * mu * np.linalg.norm(e_e - e_e.mean()))
+        if dev_tr > yv:
+            assert abs(devs[0] - yv) < 1e-8 * yv
+            assert yv < devs[1] < dev_tr
+            assert abs(devs[2] - dev_tr) < 1e-6 * dev_tr
+        else:
+            assert max(abs(d - dev_tr) for d in devs) < 1e-9 * max(dev_tr, 1)
+
+
 def test_mu_lam():
     E = RNG.uniform(1e4, 1e6, 100)
     nu = RNG.uniform(0.1, 0.45, 100)
@@ -348,6 +375,80 @@ def test_bounding_box_components():
     assert np.all(v[:3, :, :, 0] == 0) and np.all(v[3:, :, :, 0] == -1)
     assert np.all(v[:, n - 3:, :, 1] == 0) and np.all(v[:, :n - 3, :, 1] == 1)
     assert np.all(v[:, :, :3, 2] == 0) and np.all(v[:, :, 3:, 2] == -1)
+
+
+def _geo(kind, geo_type, point, size=(0, 0, 0), radius=0.0, half_height=0.0, normal=(0, 0, 1), velocity=(0, 0, 0),
+         rot_axis=(0, 0, 1), angular_velocity=0.0, active=1, start_time=0.0, end_time=999.0):
+    c = orc.OracleCollider()
+    c.kind, c.geo_type, c.active = kind, geo_type, active
+    for i in range(3):
+        c.point[i], c.size[i], c.normal[i] = point[i], size[i], normal[i]
+        c.velocity[i], c.rot_axis[i] = velocity[i], rot_axis[i]
+    c.radius, c.half_height, c.angular_velocity = radius, half_height, angular_velocity
+    c.start_time, c.end_time = start_time, end_time
+    return c
+
+
+def test_geometry_collider_index_sets():
+    """add_geometry_collider (kind 3): nodes strictly inside a cube, sphere or
+    cylinder (axis = normal) are zeroed, nothing else changes; inactive or
+    out-of-window colliders do nothing."""
+    n = 16
+    P = orc.make_params(1, n, gravity=(0, 0, 0))
+    pos = np.indices((n, n, n)).transpose(1, 2, 3, 0) / n
+    c0 = np.array([0.5, 0.5, 0.5])
+    off = pos - c0
+    cases = [
+        (_geo(3, 1, c0, size=(0.2, 0.1, 0.15)), np.all(np.abs(off) < [0.2, 0.1, 0.15], axis=-1)),
+        (_geo(3, 2, c0, radius=0.23), np.linalg.norm(off, axis=-1) < 0.23),
+        (_geo(3, 3, c0, radius=0.2, half_height=0.13, normal=(1, 0, 0)),
+         (np.abs(off[..., 0]) < 0.13) & (np.linalg.norm(off[..., 1:], axis=-1) < 0.2)),
+    ]
+    for c, inside in cases:
+        v = orc.grid_update(P, [c], 0.0, 1e-4, _uniform_grid(n))
+        assert inside.any() and (~inside).any()
+        assert np.all(v[inside] == 0) and np.all(v[~inside] == np.array([1.0, -2.0, 3.0]))
+    for c in (_geo(3, 2, c0, radius=0.3, active=0), _geo(3, 2, c0, radius=0.3, start_time=1.0)):
+        assert np.all(orc.grid_update(P, [c], 0.0, 1e-4, _uniform_grid(n)) == np.array([1.0, -2.0, 3.0]))
+
+
+def test_grid_geometry_mover_is_a_rigid_motion():
+    """set_grid_velocity_on_geometry (kind 4): inside the sphere the node
+    velocities form a rigid motion (relative velocity orthogonal to relative
+    position, angular speed as given), outside nothing changes."""
+    n = 16
+    P = orc.make_params(1, n, gravity=(0, 0, 0))
+    axis = np.array([1.0, 2.0, 2.0]) / 3.0
+    c = _geo(4, 2, (0.5, 0.5, 0.5), radius=0.25, velocity=(0.1, -0.2, 0.3), rot_axis=axis, angular_velocity=2.0)
+    v = orc.grid_update(P, [c], 0.0, 1e-4, _uniform_grid(n))
+    pos = np.indices((n, n, n)).transpose(1, 2, 3, 0) / n
+    inside = np.linalg.norm(pos - 0.5, axis=-1) < 0.25
+    assert np.all(v[~inside] == np.array([1.0, -2.0, 3.0]))
+    xi, vi = pos[inside], v[inside]
+    rng = np.random.Generator(np.random.PCG64(7))
+    for _ in range(50):
+        a, b = rng.integers(0, len(xi), 2)
+        dxv, dvv = xi[a] - xi[b], vi[a] - vi[b]
+        assert abs(np.dot(dxv, dvv)) < 1e-6
+        # |omega x r| = omega |r| sin(angle(axis, r))
+        r = dxv
+        expect = 2.0 * np.linalg.norm(np.cross(axis, r))
+        assert abs(np.linalg.norm(dvv) - expect) < 1e-5
+    centre = np.all(np.isclose(pos, 0.5), axis=-1)
+    assert np.allclose(v[centre], [0.1, -0.2, 0.3], atol=1e-7)
+
+
+def test_mask_collider_zeroes_marked_nodes():
+    """enforce_grid_velocity_by_mask (kind 5): nodes with mask >= 1 are
+    zeroed, all others keep their velocity."""
+    n = 16
+    P = orc.make_params(1, n, gravity=(0, 0, 0))
+    mask = np.random.Generator(np.random.PCG64(3)).integers(-1, 3, (n, n, n)).astype(np.int32)
+    c = orc.OracleCollider()
+    c.kind = 5
+    c.mask = mask.ctypes.data
+    v = orc.grid_update(P, [c], 0.0, 1e-4, _uniform_grid(n))
+    assert np.all(v[mask >= 1] == 0) and np.all(v[mask < 1] == np.array([1.0, -2.0, 3.0]))
 
 
 def test_float32_and_float64_oracles_agree_on_a_rollout():
