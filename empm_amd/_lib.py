@@ -12,7 +12,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libempm.so")
 
-EMPM_ABI_VERSION = 2
+EMPM_ABI_VERSION = 3
 EMPM_MAX_COLLIDERS = 16
 
 EMPM_OK = 0
@@ -41,14 +41,15 @@ EXPORTED_SYMBOLS = (
     "empm_rollout_fwd_replay",
     "empm_rollout_adj",
     "empm_sort_by_cell",
-    "empm_set_rebin_interval",
+    "empm_bin_record_ints",
     "empm_profile_enable",
     "empm_profile_read",
     "empm_profile_reset",
     "empm_kernel_name",
 )
 
-KERNEL_NAMES = ("k_p2g", "k_grid", "k_g2p", "k_g2p_adj", "k_grid_adj", "k_p2g_adj", "k_stress", "k_g2p_adj_scatter")
+KERNEL_NAMES = ("k_p2g", "k_grid", "k_g2p", "k_g2p_adj", "k_grid_adj", "k_p2g_adj", "k_stress", "k_g2p_adj_scatter",
+                "k_bin")
 
 
 class EmpmError(RuntimeError):
@@ -100,7 +101,7 @@ class Particles(C.Structure):
 
 
 class GridRec(C.Structure):
-    _fields_ = [("mv_m", C.c_void_p), ("v_out", C.c_void_p)]
+    _fields_ = [("mv_m", C.c_void_p), ("v_out", C.c_void_p), ("bins", C.c_void_p)]
 
 
 class Collider(C.Structure):
@@ -161,8 +162,8 @@ def _declare(lib):
                                             C.c_int, P(Particles), P(GridRec), vp]
     lib.empm_sort_by_cell.restype = C.c_int
     lib.empm_sort_by_cell.argtypes = [vp, C.c_int, vp, C.c_float, vp, vp]
-    lib.empm_set_rebin_interval.restype = C.c_int
-    lib.empm_set_rebin_interval.argtypes = [vp, C.c_int]
+    lib.empm_bin_record_ints.restype = C.c_longlong
+    lib.empm_bin_record_ints.argtypes = [C.c_int, C.c_int]
     lib.empm_profile_enable.restype = C.c_int
     lib.empm_profile_enable.argtypes = [vp, C.c_uint]
     lib.empm_profile_read.restype = C.c_int

@@ -2,18 +2,21 @@
 // hand-written HIP kernels for gfx950 (MI355X), behind the C-ABI of
 // include/empm.h.
 //
-// Per substep the forward runs three kernels:
-//   k_p2g   per particle block: return map + stress (one SVD) fused with the
-//           APIC scatter of momentum and mass into block-owned LDS tiles;
-//   k_grid  per node: gather of the covering tiles, normalise, gravity,
-//           damping and every collider in registration order (a table passed
-//           by value), writes v_out;
-//   k_g2p   per particle: 27-node gather, x/v/C/F_trial update.
-// The adjoint of one substep recomputes k_p2g/k_grid and then runs
-//   k_g2p_adj (scatter of dL/dv_out into tiles), k_grid_adj (per node),
-//   k_p2g_adj (gather of dL/d(mv, m), stress and return-map adjoints).
-// Particles are binned by cell (k_bin_*) at the start of every call and
-// every `rebin_every` substeps of a rollout.
+// Per substep the forward runs:
+//   k_bin_*  counting sort of the particles by stencil cell (4 launches);
+//   k_p2g    per particle block: APIC scatter of momentum, mass and the
+//            stress impulse into block-owned LDS tiles;
+//   k_grid   per node: gather of the covering tiles, normalise, gravity,
+//            damping and every collider in registration order (a table
+//            passed by value), writes v_out;
+//   k_g2p    per particle: 27-node gather, x/v/C/F_trial update, then the
+//            next substep's return map + stress (one SVD), so the scatter
+//            carries no SVD (k_stress does it for a call's first substep).
+// The adjoint of one substep uses the recorded grid, Fe and binning when
+// the forward kept them (else recomputes them), then runs
+//   k_g2p_adj (per particle) and k_g2p_adj_scatter (dL/dv_out into tiles),
+//   k_grid_adj (per node), k_p2g_adj (gather of dL/d(mv, m), stress and
+//   return-map adjoints).
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
@@ -278,28 +281,27 @@ EMPM_DEV void grid_node(const DevModel& M, const ColTable& T, float time, float 
 // ---------------------------------------------------------------------------
 // Block-owned scatter.
 //
-// Particles are binned by the cell their 3x3x3 node stencil starts at, in
-// blocks of 2x2x2 cells (block-major, cell-minor order).  One workgroup owns
-// one block at a time: it scatters the stencils of the block's particles
-// into a 6x6x6-node LDS tile (the block's 4 stencil nodes per axis plus a
-// one-cell margin each side for particles that drift between re-binnings),
-// then writes the whole tile to its slot with plain stores.  The grid pass
-// gathers, for each node, the <= 27 tiles that cover it.  No global atomics
-// are issued except for particles that drifted out of their tile, which add
-// into a dense overflow grid.
+// Particles are binned every substep by the cell their 3x3x3 node stencil
+// starts at, in blocks of 2x2x2 cells (block-major, cell-minor order).  One
+// workgroup owns one block at a time: it scatters the stencils of the block's
+// particles into a 4x4x4-node LDS tile (the block's stencil nodes), then
+// writes the whole tile to its slot with plain stores.  The grid pass
+// gathers, for each node, the <= 8 tiles that cover it.  No global atomics
+// are issued except for particles outside the grid (binned to an edge cell),
+// which add into a dense overflow grid and raise a flag the grid pass reads.
 //
 // The waves of a workgroup take the block's particles in chunks of <= 64
-// that never straddle a cell, so all lanes of a chunk whose particle is still
-// in its binned cell share one stencil base.  Their per-node contributions
-// are summed across the wave by a transposed butterfly (v_permlane32_swap,
-// v_permlane16_swap, then DPP row/quad exchanges), leaving one LDS atomic
-// per (node, component) per chunk.  Particles that left their cell since the
-// binning add their own contributions lane by lane.
+// that never straddle a cell, so all lanes of a chunk share one stencil
+// base.  Their per-node contributions are summed across the wave by a
+// transposed butterfly (v_permlane32_swap, v_permlane16_swap, then DPP
+// row/quad exchanges), leaving one LDS atomic per (node, component) per
+// chunk.  A particle whose base differs from its bin's (outside the grid)
+// adds its own contributions lane by lane.
 // ---------------------------------------------------------------------------
 constexpr int kBlkCells = 2;                                    // cells per block edge
-constexpr int kTileMargin = 1;                                  // cells of drift tolerated
-constexpr int kTileN = kBlkCells + 2 + 2 * kTileMargin;         // 6 tile nodes per edge
-constexpr int kTileNodes = kTileN * kTileN * kTileN;            // 216
+constexpr int kTileMargin = 0;                                  // binning is exact: no drift margin
+constexpr int kTileN = kBlkCells + 2 + 2 * kTileMargin;         // 4 tile nodes per edge
+constexpr int kTileNodes = kTileN * kTileN * kTileN;            // 64
 constexpr int kBlkWG = 256;       // workgroup size of the block-owned scatter kernels
 constexpr int kBlkGridCap = 1024; // max workgroups of a block-owned scatter launch (4 per CU)
 constexpr int kWave = 64;
@@ -308,11 +310,20 @@ constexpr int kAdjPayload = 24;   // floats per particle of the G2P-adjoint scat
 // Particle binning of the current segment: perm[k] = k-th particle in
 // block-major order, offs[b * 8 + c] = first position of cell c of block b
 // (offs has nb^3 * 8 + 1 entries).
+// Dense grid for contributions of particles outside their block's tile (only
+// particles outside the grid), and a flag raised when it holds anything.
+struct Overflow {
+  float4* grid;  // n^3, zero between uses
+  int* flag;     // cleared by the binning
+};
+
 struct Binning {
   const int* perm;
   const int* offs;
   const int* active;   // ids of the non-empty blocks, in no particular order
   const int* nactive;  // their count (device scalar)
+  const int* occ;      // nb^3: block holds particles
+  const int* near;     // nb^3: some tile covering a node of this block (blocks o-2..o+1 per axis) is non-empty
   int nb;
 };
 
@@ -452,7 +463,7 @@ EMPM_DEV void group_scatter(float (*tile)[kTileNodes], const int* t0, bool in_gr
 // One lane's own 27-node scatter: into the tile if the stencil lies inside
 // it, else into the overflow grid with global atomics.
 template <int NC>
-EMPM_DEV void lane_scatter(float (*tile)[kTileNodes], const int* o, float4* __restrict__ ovf, int n,
+EMPM_DEV void lane_scatter(float (*tile)[kTileNodes], const int* o, float4* __restrict__ ovf, int* ovf_flag, int n,
                            const ScatterPayload& P, const Stencil& s) {
   const bool in_tile = (unsigned)(s.base[0] - o[0]) <= (unsigned)(kTileN - 3) &&
                        (unsigned)(s.base[1] - o[1]) <= (unsigned)(kTileN - 3) &&
@@ -472,6 +483,7 @@ EMPM_DEV void lane_scatter(float (*tile)[kTileNodes], const int* o, float4* __re
       float* g = reinterpret_cast<float*>(&ovf[node_id(ix, iy, iz, n)]);
 #pragma unroll
       for (int k = 0; k < NC; ++k) atomicAdd(g + k, out[k]);
+      *ovf_flag = 1;
     }
   }
 }
@@ -488,7 +500,7 @@ EMPM_DEV void tile_origin(int blk, int nb, int* o) {
 // the block's cell chunks round robin.
 template <int NC, typename Load>
 EMPM_DEV void block_scatter(const Binning& B, const int* __restrict__ sel, float inv_dx, int n,
-                            float4* __restrict__ tiles, float4* __restrict__ ovf, Load load) {
+                            float4* __restrict__ tiles, float4* __restrict__ ovf, int* ovf_flag, Load load) {
   __shared__ float tile[NC][kTileNodes];
   const int lane = __lane_id();
   const int wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
@@ -527,7 +539,7 @@ EMPM_DEV void block_scatter(const Binning& B, const int* __restrict__ sel, float
       make_stencil(xp, inv_dx, s);
       const bool home = active && s.base[0] == o[0] + t0[0] && s.base[1] == o[1] + t0[1] && s.base[2] == o[2] + t0[2];
       if (__ballot(home)) group_scatter<NC>(tile, t0, home, P, s, lane);
-      if (active && !home) lane_scatter<NC>(tile, o, ovf, n, P, s);
+      if (active && !home) lane_scatter<NC>(tile, o, ovf, ovf_flag, n, P, s);
     }
     __syncthreads();
     float4* dst = tiles + (size_t)blk * kTileNodes;
@@ -537,27 +549,42 @@ EMPM_DEV void block_scatter(const Binning& B, const int* __restrict__ sel, float
   }
 }
 
-// Sum of the tiles (and the overflow grid) at node (ix, iy, iz).
+// Sum of the tiles at node (ix, iy, iz).  Nodes whose owning block (i >> 1)
+// has no non-empty block among the candidates return at once.  The node
+// lies in the tiles of blocks b = ((i - 2) >> 1) + {0, 1} per axis (a tile
+// spans nodes 2b .. 2b + 3); all 8 occupancy flags and tile entries are
+// loaded before any is used, so the gather costs one memory latency.
 EMPM_DEV float4 gather_node(const Binning& B, const float4* __restrict__ tiles, int ix, int iy, int iz) {
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   const int nb = B.nb;
-  // block b covers nodes [2b - 1, 2b + 4]: b in [ceil((i - 4) / 2), floor((i + 1) / 2)]
-  const int lo0 = max(0, (ix - 3) >> 1), hi0 = min(nb - 1, (ix + 1) >> 1);
-  const int lo1 = max(0, (iy - 3) >> 1), hi1 = min(nb - 1, (iy + 1) >> 1);
-  const int lo2 = max(0, (iz - 3) >> 1), hi2 = min(nb - 1, (iz + 1) >> 1);
-  for (int bx = lo0; bx <= hi0; ++bx)
-    for (int by = lo1; by <= hi1; ++by)
-      for (int bz = lo2; bz <= hi2; ++bz) {
-        const int blk = (bx * nb + by) * nb + bz;
-        if (B.offs[blk * 8 + 8] == B.offs[blk * 8]) continue;
-        const int lx = ix - (bx * kBlkCells - kTileMargin), ly = iy - (by * kBlkCells - kTileMargin),
-                  lz = iz - (bz * kBlkCells - kTileMargin);
-        const float4 t = tiles[(size_t)blk * kTileNodes + (lx * kTileN + ly) * kTileN + lz];
-        acc.x += t.x;
-        acc.y += t.y;
-        acc.z += t.z;
-        acc.w += t.w;
-      }
+  if (!B.near[((ix >> 1) * nb + (iy >> 1)) * nb + (iz >> 1)]) return acc;
+  const int l0 = (ix - 2) >> 1, l1 = (iy - 2) >> 1, l2 = (iz - 2) >> 1;
+  bool ok[8];
+  int blk[8];
+  size_t k[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int b0 = l0 + (j >> 2), b1 = l1 + ((j >> 1) & 1), b2 = l2 + (j & 1);
+    ok[j] = (unsigned)b0 < (unsigned)nb && (unsigned)b1 < (unsigned)nb && (unsigned)b2 < (unsigned)nb;
+    blk[j] = ok[j] ? (b0 * nb + b1) * nb + b2 : 0;
+    const int lx = ix - 2 * b0, ly = iy - 2 * b1, lz = iz - 2 * b2;
+    k[j] = ok[j] ? (size_t)blk[j] * kTileNodes + (lx * kTileN + ly) * kTileN + lz : 0;
+  }
+  int occ[8];
+  float4 t[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    occ[j] = B.occ[blk[j]];
+    t[j] = tiles[k[j]];
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const bool use = ok[j] && occ[j];
+    acc.x += use ? t[j].x : 0.f;
+    acc.y += use ? t[j].y : 0.f;
+    acc.z += use ? t[j].z : 0.f;
+    acc.w += use ? t[j].w : 0.f;
+  }
   return acc;
 }
 
@@ -607,9 +634,9 @@ __global__ __launch_bounds__(kBlock) void k_stress(DevModel M, float dt, const f
 // stress impulse into the block tiles.
 __global__ __launch_bounds__(kBlkWG) void k_p2g(DevModel M, const float* __restrict__ x, const float* __restrict__ v,
                                                 const float* __restrict__ C, const float* __restrict__ S, Binning B,
-                                                float4* __restrict__ tiles, float4* __restrict__ ovf) {
+                                                float4* __restrict__ tiles, Overflow ov) {
   const float dx = M.dx, r = M.rpic;
-  block_scatter<4>(B, M.sel, M.inv_dx, M.n, tiles, ovf, [&](int p, float* xp, ScatterPayload& P) {
+  block_scatter<4>(B, M.sel, M.inv_dx, M.n, tiles, ov.grid, ov.flag, [&](int p, float* xp, ScatterPayload& P) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) xp[d] = x[3 * p + d];
     const float mp = M.mass[p];
@@ -629,23 +656,34 @@ __global__ __launch_bounds__(kBlkWG) void k_p2g(DevModel M, const float* __restr
 // grid_normalization_and_gravity, damping and the colliders.  The overflow
 // grid is re-zeroed for the next scatter; (mv, m) is kept when `keep` is set
 // (the adjoint needs it).
-__global__ __launch_bounds__(kBlock) void k_grid(DevModel M, ColTable T, float time, float dt, Binning B,
-                                                 const float4* __restrict__ tiles, float4* __restrict__ ovf,
-                                                 float4* __restrict__ keep, float4* __restrict__ v_out) {
-  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t G = (size_t)M.n * M.n * M.n;
-  if (id >= G) return;
-  const int iz = (int)(id % M.n);
-  const int iy = (int)((id / M.n) % M.n);
-  const int ix = (int)(id / ((size_t)M.n * M.n));
+// Node-pass launch shape: 64 consecutive z per wave, kGridRows y-rows per
+// workgroup, one x-plane per grid z index.
+constexpr int kGridRows = 4;
+EMPM_DEV bool grid_thread(const DevModel& M, int& ix, int& iy, int& iz, size_t& id) {
+  iz = blockIdx.x * 64 + threadIdx.x;
+  iy = blockIdx.y * kGridRows + threadIdx.y;
+  ix = blockIdx.z;
+  if (iz >= M.n || iy >= M.n) return false;
+  id = node_id(ix, iy, iz, M.n);
+  return true;
+}
+
+__global__ __launch_bounds__(64 * kGridRows) void k_grid(DevModel M, ColTable T, float time, float dt, Binning B,
+                                                         const float4* __restrict__ tiles, Overflow ov,
+                                                         float4* __restrict__ keep, float4* __restrict__ v_out) {
+  int ix, iy, iz;
+  size_t id;
+  if (!grid_thread(M, ix, iy, iz, id)) return;
   float4 g = gather_node(B, tiles, ix, iy, iz);
-  const float4 o = ovf[id];
-  if (o.x != 0.f || o.y != 0.f || o.z != 0.f || o.w != 0.f) {
-    g.x += o.x;
-    g.y += o.y;
-    g.z += o.z;
-    g.w += o.w;
-    ovf[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (*ov.flag) {
+    const float4 o = ov.grid[id];
+    if (o.x != 0.f || o.y != 0.f || o.z != 0.f || o.w != 0.f) {
+      g.x += o.x;
+      g.y += o.y;
+      g.z += o.z;
+      g.w += o.w;
+      ov.grid[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
   }
   if (keep) keep[id] = g;
   float v[3], fac[3];
@@ -836,8 +874,8 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(DevModel M, float dt, const 
 // Scatter of dL/dv_out (the transpose of the G2P gather) into the block tiles.
 __global__ __launch_bounds__(kBlkWG) void k_g2p_adj_scatter(DevModel M, const float* __restrict__ x,
                                                             const float* __restrict__ payload, Binning B,
-                                                            float4* __restrict__ tiles, float4* __restrict__ ovf) {
-  block_scatter<3>(B, M.sel, M.inv_dx, M.n, tiles, ovf, [&](int p, float* xp, ScatterPayload& P) {
+                                                            float4* __restrict__ tiles, Overflow ov) {
+  block_scatter<3>(B, M.sel, M.inv_dx, M.n, tiles, ov.grid, ov.flag, [&](int p, float* xp, ScatterPayload& P) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) xp[d] = x[3 * p + d];
     const float4* q4 = reinterpret_cast<const float4*>(payload + (size_t)kAdjPayload * p);
@@ -856,23 +894,22 @@ __global__ __launch_bounds__(kBlkWG) void k_g2p_adj_scatter(DevModel M, const fl
 // node's forward (normalisation, damping, colliders) to get dv_out/dv per
 // component, and write dL/d(mv, m).  The grid-mass term is dropped when
 // `drop_mass` is set (reference Tape behaviour).
-__global__ __launch_bounds__(kBlock) void k_grid_adj(DevModel M, ColTable T, float time, float dt, Binning B,
-                                                     const float4* __restrict__ tiles, float4* __restrict__ ovf,
-                                                     const float4* __restrict__ keep, float4* __restrict__ gadj_in,
-                                                     int drop_mass) {
-  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t G = (size_t)M.n * M.n * M.n;
-  if (id >= G) return;
-  const int iz = (int)(id % M.n);
-  const int iy = (int)((id / M.n) % M.n);
-  const int ix = (int)(id / ((size_t)M.n * M.n));
+__global__ __launch_bounds__(64 * kGridRows) void k_grid_adj(DevModel M, ColTable T, float time, float dt, Binning B,
+                                                             const float4* __restrict__ tiles, Overflow ov,
+                                                             const float4* __restrict__ keep,
+                                                             float4* __restrict__ gadj_in, int drop_mass) {
+  int ix, iy, iz;
+  size_t id;
+  if (!grid_thread(M, ix, iy, iz, id)) return;
   float4 b = gather_node(B, tiles, ix, iy, iz);
-  const float4 ob = ovf[id];
-  if (ob.x != 0.f || ob.y != 0.f || ob.z != 0.f) {
-    b.x += ob.x;
-    b.y += ob.y;
-    b.z += ob.z;
-    ovf[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (*ov.flag) {
+    const float4 ob = ov.grid[id];
+    if (ob.x != 0.f || ob.y != 0.f || ob.z != 0.f) {
+      b.x += ob.x;
+      b.y += ob.y;
+      b.z += ob.z;
+      ov.grid[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
   }
   const float4 g = keep[id];
   float v[3], fac[3];
@@ -1017,11 +1054,30 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(DevModel M, float dt, const 
 }
 
 // ---------------------------------------------------------------------------
-// Binning: counting sort of particles by (block, cell) of their stencil base,
-// block-major (blocks of 2x2x2 cells, x-major block index), cell-minor.
-// Histogram with per-particle rank (same-bin lanes of a wave are counted with
-// one atomic), exclusive scan of the nb^3 * 8 + 1 counts, scatter.
+// Binning, every substep: counting sort of the particles by the cell their
+// stencil starts at.  Bins are the 8 cells of a 2x2x2-cell block, in
+// block-major, cell-minor order.
+//   k_bin_count   per particle: bin key and rank within the bin (same-bin
+//                 lanes of a wave share one atomic); counts are zero on entry
+//   k_bin_scan    one thread per block, kBinScanWG blocks per workgroup:
+//                 block totals and their workgroup-local exclusive scan;
+//                 clears `near` and the active count
+//   k_bin_finish  one thread per block: global block offset, per-bin
+//                 offsets, occupancy, active list, near marks; re-zeroes the
+//                 counts for the next binning
+//   k_bin_place   per particle: perm[offs[key] + rank] = p
 // ---------------------------------------------------------------------------
+constexpr int kBinScanWG = 1024;
+
+// What the scatter and grid kernels read of one binning (device arrays).
+struct BinBuf {
+  int* perm;  // N
+  int* offs;  // nb^3 * 8 + 1
+  int* act;   // 1 + nb^3: count, then the ids of the non-empty blocks
+  int* occ;   // nb^3
+  int* near;  // nb^3
+};
+
 EMPM_DEV int bin_of(const float* __restrict__ x, int p, float inv_dx, int n, int nb) {
   int c[3];
 #pragma unroll
@@ -1038,8 +1094,8 @@ __global__ void k_fill_i32(int* __restrict__ a, size_t n, int v) {
   if (i < n) a[i] = v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_bin_hist(int N, const float* __restrict__ x, float inv_dx, int n, int nb,
-                                                     int* __restrict__ count, int* __restrict__ rank) {
+__global__ __launch_bounds__(kBlock) void k_bin_count(int N, const float* __restrict__ x, float inv_dx, int n, int nb,
+                                                      int* __restrict__ count, int2* __restrict__ key_rank) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = __lane_id();
   const bool active = p < N;
@@ -1054,17 +1110,15 @@ __global__ __launch_bounds__(kBlock) void k_bin_hist(int N, const float* __restr
     int base = 0;
     if (lane == leader) base = atomicAdd(&count[lkey], __popcll(grp));
     base = __shfl(base, leader);
-    if (in) rank[p] = base + __popcll(grp & ((1ull << lane) - 1ull));
+    if (in) key_rank[p] = make_int2(key, base + __popcll(grp & ((1ull << lane) - 1ull)));
   }
 }
 
-constexpr int kScanBlock = 256;
-constexpr int kScanPerThread = 4;
-constexpr int kScanTile = kScanBlock * kScanPerThread;
-
-// block-wide exclusive scan of one value per thread; returns the block total
-EMPM_DEV int block_exclusive_scan(int v, int* smem, int& total) {
+// Exclusive scan of one value per thread over the workgroup (kBinScanWG
+// threads); returns the exclusive prefix and sets `total`.
+EMPM_DEV int wg_exclusive_scan(int v, int* smem, int& total) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  constexpr int kWaves = kBinScanWG / 64;
   int incl = v;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -1073,74 +1127,95 @@ EMPM_DEV int block_exclusive_scan(int v, int* smem, int& total) {
   }
   if (lane == 63) smem[wid] = incl;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int acc = 0;
-    for (int w = 0; w < kScanBlock / 64; ++w) {
-      const int t = smem[w];
-      smem[w] = acc;
-      acc += t;
+  if (threadIdx.x < 64) {
+    int w = threadIdx.x < kWaves ? smem[threadIdx.x] : 0;
+    int wi = w;
+#pragma unroll
+    for (int o = 1; o < kWaves; o <<= 1) {
+      const int t = __shfl_up(wi, o);
+      if (lane >= o) wi += t;
     }
-    smem[kScanBlock / 64] = acc;
+    if (threadIdx.x < kWaves) smem[threadIdx.x] = wi - w;
+    if (threadIdx.x == kWaves - 1) smem[kWaves] = wi;
   }
   __syncthreads();
-  total = smem[kScanBlock / 64];
+  total = smem[kWaves];
   const int r = smem[wid] + incl - v;
   __syncthreads();
   return r;
 }
 
-// in-place exclusive scan of each tile; tile totals to sums[]
-__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(int* __restrict__ a, size_t n, int* __restrict__ sums) {
-  __shared__ int smem[kScanBlock / 64 + 1];
-  const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanPerThread;
-  int v[kScanPerThread], s = 0;
-#pragma unroll
-  for (int k = 0; k < kScanPerThread; ++k) {
-    v[k] = base + k < n ? a[base + k] : 0;
-    s += v[k];
-  }
-  int total;
-  int off = block_exclusive_scan(s, smem, total);
-#pragma unroll
-  for (int k = 0; k < kScanPerThread; ++k) {
-    if (base + k < n) a[base + k] = off;
-    off += v[k];
-  }
-  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+EMPM_DEV int block_count(const int* __restrict__ count, int b) {
+  const int4* c4 = reinterpret_cast<const int4*>(count + (size_t)b * 8);
+  const int4 a = c4[0], c = c4[1];
+  return a.x + a.y + a.z + a.w + c.x + c.y + c.z + c.w;
 }
 
-// exclusive scan of the tile sums by one workgroup (loops over chunks)
-__global__ __launch_bounds__(kScanBlock) void k_scan_sums(int* __restrict__ sums, int m) {
-  __shared__ int smem[kScanBlock / 64 + 1];
-  int carry = 0;
-  for (int start = 0; start < m; start += kScanBlock) {
-    const int i = start + threadIdx.x;
-    const int v = i < m ? sums[i] : 0;
-    int total;
-    const int e = block_exclusive_scan(v, smem, total);
-    if (i < m) sums[i] = carry + e;
-    carry += total;
+__global__ __launch_bounds__(kBinScanWG) void k_bin_scan(int nblk, const int* __restrict__ count,
+                                                         int* __restrict__ blkoff, int* __restrict__ wsum,
+                                                         int* __restrict__ near, int* __restrict__ act,
+                                                         int* __restrict__ ovf_flag) {
+  __shared__ int smem[kBinScanWG / 64 + 1];
+  const int b = blockIdx.x * kBinScanWG + threadIdx.x;
+  const int tot = b < nblk ? block_count(count, b) : 0;
+  int wtot;
+  const int ex = wg_exclusive_scan(tot, smem, wtot);
+  if (b < nblk) {
+    blkoff[b] = ex;
+    near[b] = 0;
+  }
+  if (threadIdx.x == 0) wsum[blockIdx.x] = wtot;
+  if (b == 0) {
+    act[0] = 0;
+    *ovf_flag = 0;
   }
 }
 
-__global__ void k_scan_add(int* __restrict__ a, size_t n, const int* __restrict__ sums) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] += sums[i / kScanTile];
-}
-
-// Compact list of the non-empty blocks (after the scan: offs are offsets).
-__global__ void k_bin_active(int nblk, const int* __restrict__ offs, int* __restrict__ active,
-                             int* __restrict__ nactive) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kBinScanWG) void k_bin_finish(int nb, int* __restrict__ count,
+                                                           const int* __restrict__ blkoff,
+                                                           const int* __restrict__ wsum, int* __restrict__ offs,
+                                                           int* __restrict__ act, int* __restrict__ occ,
+                                                           int* __restrict__ near) {
+  const int nblk = nb * nb * nb;
+  const int b = blockIdx.x * kBinScanWG + threadIdx.x;
+  int prefix = 0;
+  for (int w = 0; w < (int)blockIdx.x; ++w) prefix += wsum[w];
   if (b >= nblk) return;
-  if (offs[b * 8 + 8] != offs[b * 8]) active[atomicAdd(nactive, 1)] = b;
+  int4* c4 = reinterpret_cast<int4*>(count + (size_t)b * 8);
+  const int4 lo = c4[0], hi = c4[1];
+  const int c[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+  int o = prefix + blkoff[b];
+  int run[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    run[k] = o;
+    o += c[k];
+  }
+  int4* o4 = reinterpret_cast<int4*>(offs + (size_t)b * 8);
+  o4[0] = make_int4(run[0], run[1], run[2], run[3]);
+  o4[1] = make_int4(run[4], run[5], run[6], run[7]);
+  c4[0] = make_int4(0, 0, 0, 0);
+  c4[1] = make_int4(0, 0, 0, 0);
+  if (b == nblk - 1) offs[(size_t)nblk * 8] = o;
+  const bool full = o != run[0];
+  occ[b] = full ? 1 : 0;
+  if (!full) return;
+  act[1 + atomicAdd(&act[0], 1)] = b;
+  // the blocks owning a node of this block's tile (nodes 2b .. 2b + 3)
+  const int b0 = b / (nb * nb), b1 = (b / nb) % nb, b2 = b % nb;
+#pragma unroll
+  for (int d = 0; d < 8; ++d) {
+    const int o0 = b0 + (d >> 2), o1 = b1 + ((d >> 1) & 1), o2 = b2 + (d & 1);
+    if (o0 < nb && o1 < nb && o2 < nb) near[(o0 * nb + o1) * nb + o2] = 1;
+  }
 }
 
-__global__ void k_bin_scatter(int N, const float* __restrict__ x, float inv_dx, int n, int nb,
-                              const int* __restrict__ offs, const int* __restrict__ rank, int* __restrict__ perm) {
+__global__ void k_bin_place(int N, const int2* __restrict__ key_rank, const int* __restrict__ offs,
+                            int* __restrict__ perm) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
-  perm[offs[bin_of(x, p, inv_dx, n, nb)] + rank[p]] = p;
+  const int2 kr = key_rank[p];
+  perm[offs[kr.x] + kr.y] = p;
 }
 
 __global__ void k_mu_lam(int n,const float* __restrict__ E, const float* __restrict__ nu, float* __restrict__ mu,
@@ -1191,11 +1266,11 @@ struct empm_ctx {
   int n_grid;
   int max_particles;
   int nb;           // blocks per axis
-  int rebin_every;  // substeps between re-binnings inside a rollout call
   size_t G;
   size_t nbins;     // nb^3 * 8
   float4* tiles;    // nb^3 tiles of kTileNodes float4 (block-owned scatter)
   float4* ovf;      // G: overflow grid for particles outside their tile (kept zero between uses)
+  int* ovf_flag;    // 1: overflow grid may be non-zero
   float4* keep;     // G: (mv, m) of the substep being differentiated
   float4* v_out;    // G
   float4* gadj_in;  // G: dL/d(mv, m)
@@ -1204,19 +1279,19 @@ struct empm_ctx {
   float* payload;   // N*kAdjPayload: G2P-adjoint scatter payload
   float* xbar;      // N*3 scratch
   float* Febar;     // N*9 scratch
-  int* perm;        // N: particles in bin order
-  int* bin_count;   // nbins + 1: counts, then offsets
-  int* bin_rank;    // N
-  int* tile_sums;   // scan scratch
-  int* active;      // nb^3: compact list of non-empty blocks
-  int* nactive;     // 1: its length
+  int* bins;        // the context's own binning record (bin_record_ints)
+  int* count;       // nbins: per-bin counts, zero between binnings
+  int2* key_rank;   // N: bin key and rank of each particle
+  int* blkoff;      // nb^3: workgroup-local exclusive block offsets
+  int* wsum;        // block-scan workgroup totals
   KernelProfile* prof;
 };
 
 namespace {
 
-const char* const kKernelNames[EMPM_NUM_KERNELS] = {"k_p2g",     "k_grid",     "k_g2p",    "k_g2p_adj",
-                                                    "k_grid_adj", "k_p2g_adj", "k_stress", "k_g2p_adj_scatter"};
+const char* const kKernelNames[EMPM_NUM_KERNELS] = {"k_p2g",     "k_grid",    "k_g2p",    "k_g2p_adj",
+                                                    "k_grid_adj", "k_p2g_adj", "k_stress", "k_g2p_adj_scatter",
+                                                    "k_bin"};
 
 hipEvent_t pool_get(KernelProfile* p) {
   if (!p->pool.empty()) {
@@ -1316,8 +1391,6 @@ DevModel dev_model(const empm_model* m) {
   return d;
 }
 
-unsigned nblocks(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
-
 // Launch KERNEL<material>(args...): the constitutive kernels are compiled
 // once per material so each runs a single branch-free code path.
 #define EMPM_LAUNCH_MAT(material, KERNEL, grid, block, stream, ...)                              \
@@ -1332,7 +1405,34 @@ unsigned nblocks(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
     default: hipLaunchKernelGGL(KERNEL<7>, grid, block, 0, stream, __VA_ARGS__); break;        \
   }
 
-Binning binning(const empm_ctx* ctx) { return Binning{ctx->perm, ctx->bin_count, ctx->active, ctx->nactive, ctx->nb}; }
+// Layout of one binning record: [perm N | offs nbins + 1 | act 1 + nb^3 |
+// occ nb^3 | near nb^3], each section padded to a multiple of 4 ints so that
+// the int4 accesses of the binning kernels stay 16-byte aligned.
+size_t pad4(size_t k) { return (k + 3) & ~(size_t)3; }
+
+size_t bin_record_ints(int n_grid, int n_particles) {
+  const size_t nb = (size_t)(n_grid + kBlkCells - 1) / kBlkCells;
+  const size_t nblk = nb * nb * nb;
+  return pad4((size_t)n_particles) + pad4(nblk * 8 + 1) + pad4(1 + nblk) + 2 * pad4(nblk);
+}
+
+BinBuf bin_buf(int* rec, int n_grid, int n_particles) {
+  const size_t nb = (size_t)(n_grid + kBlkCells - 1) / kBlkCells;
+  const size_t nblk = nb * nb * nb;
+  BinBuf b;
+  b.perm = rec;
+  b.offs = b.perm + pad4((size_t)n_particles);
+  b.act = b.offs + pad4(nblk * 8 + 1);
+  b.occ = b.act + pad4(1 + nblk);
+  b.near = b.occ + pad4(nblk);
+  return b;
+}
+
+Binning binning(const empm_ctx* ctx, const BinBuf& b) {
+  return Binning{b.perm, b.offs, b.act + 1, b.act, b.occ, b.near, ctx->nb};
+}
+
+Overflow overflow(const empm_ctx* ctx) { return Overflow{ctx->ovf, ctx->ovf_flag}; }
 
 unsigned block_grid(const empm_ctx* ctx) {
   const size_t nblk = (size_t)ctx->nb * ctx->nb * ctx->nb;
@@ -1340,38 +1440,45 @@ unsigned block_grid(const empm_ctx* ctx) {
   return (unsigned)(nblk < cap ? nblk : cap);
 }
 
-// Bin the particles of a state (positions x) for the block-owned scatters.
-int do_bin(empm_ctx* ctx, int N, const float* x, float inv_dx, hipStream_t st) {
-  if (N <= 0) return EMPM_OK;
+unsigned nblocks(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+dim3 grid_launch(const empm_ctx* ctx) {
+  const unsigned n = (unsigned)ctx->n_grid;
+  return dim3((n + 63) / 64, (n + kGridRows - 1) / kGridRows, n);
+}
+
+// Bin the particles of a state (positions x) into `out`.
+int do_bin(empm_ctx* ctx, int N, const float* x, float inv_dx, const BinBuf& out, hipStream_t st) {
+  ProfScope ps(ctx, EMPM_KERNEL_BIN, st);
   const int n = ctx->n_grid, nb = ctx->nb;
-  const size_t nbins = ctx->nbins + 1;
-  const unsigned tiles = (unsigned)((nbins + kScanTile - 1) / kScanTile);
-  hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(nbins)), dim3(kBlock), 0, st, ctx->bin_count, nbins, 0);
-  hipLaunchKernelGGL(k_bin_hist, dim3(nblocks(N)), dim3(kBlock), 0, st, N, x, inv_dx, n, nb, ctx->bin_count,
-                     ctx->bin_rank);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanBlock), 0, st, ctx->bin_count, nbins, ctx->tile_sums);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, st, ctx->tile_sums, (int)tiles);
-  hipLaunchKernelGGL(k_scan_add, dim3(nblocks(nbins)), dim3(kBlock), 0, st, ctx->bin_count, nbins, ctx->tile_sums);
-  hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(N)), dim3(kBlock), 0, st, N, x, inv_dx, n, nb, ctx->bin_count,
-                     ctx->bin_rank, ctx->perm);
   const int nblk = nb * nb * nb;
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, ctx->nactive, (size_t)1, 0);
-  hipLaunchKernelGGL(k_bin_active, dim3(nblocks(nblk)), dim3(kBlock), 0, st, nblk, ctx->bin_count, ctx->active,
-                     ctx->nactive);
+  const unsigned wgs = (unsigned)((nblk + kBinScanWG - 1) / kBinScanWG);
+  if (N > 0)
+    hipLaunchKernelGGL(k_bin_count, dim3(nblocks(N)), dim3(kBlock), 0, st, N, x, inv_dx, n, nb, ctx->count,
+                       ctx->key_rank);
+  hipLaunchKernelGGL(k_bin_scan, dim3(wgs), dim3(kBinScanWG), 0, st, nblk, ctx->count, ctx->blkoff, ctx->wsum,
+                     out.near, out.act, ctx->ovf_flag);
+  hipLaunchKernelGGL(k_bin_finish, dim3(wgs), dim3(kBinScanWG), 0, st, nb, ctx->count, ctx->blkoff, ctx->wsum,
+                     out.offs, out.act, out.occ, out.near);
+  if (N > 0) hipLaunchKernelGGL(k_bin_place, dim3(nblocks(N)), dim3(kBlock), 0, st, N, ctx->key_rank, out.offs, out.perm);
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }
 
-// One forward substep in -> out.  `first`: run the stress stage here (else
-// the previous substep's G2P did); `last`: no stress stage for a next substep.
+// One forward substep in -> out, binning `in` first (into the record's bins
+// when given).  `first`: run the stress stage here (else the previous
+// substep's G2P did); `last`: no stress stage for a next substep.
 int launch_fwd(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, float dt, const empm_particles* in,
                const empm_particles* out, const empm_grid_rec* rec, bool first, bool last, hipStream_t st) {
   const int write_params = (M.material == 5 || M.mp.hardening == 1.f) ? 1 : 0;
   float* Fe = in->Fe ? in->Fe : ctx->Fe;
   float* Fe_next = out->Fe ? out->Fe : ctx->Fe;
   float4* keep = rec ? reinterpret_cast<float4*>(rec->mv_m) : nullptr;
-  float4* v_out = rec ? reinterpret_cast<float4*>(rec->v_out) : ctx->v_out;
-  const Binning B = binning(ctx);
+  float4* v_out = rec && rec->v_out ? reinterpret_cast<float4*>(rec->v_out) : ctx->v_out;
+  const BinBuf bb = bin_buf(rec && rec->bins ? rec->bins : ctx->bins, ctx->n_grid, M.N);
+  int rc = do_bin(ctx, M.N, in->x, M.inv_dx, bb, st);
+  if (rc) return rc;
+  const Binning B = binning(ctx, bb);
   if (M.N > 0 && first) {
     ProfScope ps(ctx, EMPM_KERNEL_STRESS, st);
     EMPM_LAUNCH_MAT(M.material, k_stress, dim3(nblocks(M.N)), dim3(kBlock), st, M, dt, in->F, Fe, ctx->S,
@@ -1380,12 +1487,12 @@ int launch_fwd(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, 
   if (M.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
     hipLaunchKernelGGL(k_p2g, dim3(block_grid(ctx)), dim3(kBlkWG), 0, st, M, in->x, in->v, in->C, ctx->S, B,
-                       ctx->tiles, ctx->ovf);
+                       ctx->tiles, overflow(ctx));
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
-    hipLaunchKernelGGL(k_grid, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, B, ctx->tiles, ctx->ovf,
-                       keep, v_out);
+    hipLaunchKernelGGL(k_grid, grid_launch(ctx), dim3(64, kGridRows), 0, st, M, T, time, dt, B, ctx->tiles,
+                       overflow(ctx), keep, v_out);
   }
   if (M.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_G2P, st);
@@ -1396,16 +1503,23 @@ int launch_fwd(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, 
   return EMPM_OK;
 }
 
-// Adjoint of one substep.  With a recorded grid (rec) and Fe (in->Fe) the
-// forward stress, P2G and boundary pass are not recomputed.
+// Adjoint of one substep.  With a recorded grid and Fe (rec->mv_m, v_out and
+// in->Fe) the forward stress, P2G and boundary pass are not recomputed; with
+// recorded bins (rec->bins) the particles are not re-binned.
 int launch_adj(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, float dt, const empm_particles* in,
                const empm_grid_rec* rec, const empm_particles* d_out, const empm_particles* d_in, float* d_mu,
                float* d_lam, float* d_ys, float* d_mass, float* d_vol, int flags, hipStream_t st) {
-  const Binning B = binning(ctx);
   const bool recorded = rec && rec->mv_m && rec->v_out && in->Fe;
+  const bool have_bins = recorded && rec->bins;
   float* Fe = recorded ? in->Fe : ctx->Fe;
   float4* keep = recorded ? reinterpret_cast<float4*>(rec->mv_m) : ctx->keep;
   float4* v_out = recorded ? reinterpret_cast<float4*>(rec->v_out) : ctx->v_out;
+  const BinBuf bb = bin_buf(have_bins ? rec->bins : ctx->bins, ctx->n_grid, M.N);
+  if (!have_bins) {
+    int rc = do_bin(ctx, M.N, in->x, M.inv_dx, bb, st);
+    if (rc) return rc;
+  }
+  const Binning B = binning(ctx, bb);
   if (!recorded) {
     if (M.N > 0) {
       ProfScope ps(ctx, EMPM_KERNEL_STRESS, st);
@@ -1414,12 +1528,12 @@ int launch_adj(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, 
     if (M.N > 0) {
       ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
       hipLaunchKernelGGL(k_p2g, dim3(block_grid(ctx)), dim3(kBlkWG), 0, st, M, in->x, in->v, in->C, ctx->S, B,
-                         ctx->tiles, ctx->ovf);
+                         ctx->tiles, overflow(ctx));
     }
     {
       ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
-      hipLaunchKernelGGL(k_grid, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, B, ctx->tiles, ctx->ovf,
-                         keep, v_out);
+      hipLaunchKernelGGL(k_grid, grid_launch(ctx), dim3(64, kGridRows), 0, st, M, T, time, dt, B, ctx->tiles,
+                         overflow(ctx), keep, v_out);
     }
   }
   if (M.N > 0) {
@@ -1431,13 +1545,13 @@ int launch_adj(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, 
     {
       ProfScope ps(ctx, EMPM_KERNEL_G2P_ADJ_SCATTER, st);
       hipLaunchKernelGGL(k_g2p_adj_scatter, dim3(block_grid(ctx)), dim3(kBlkWG), 0, st, M, in->x, ctx->payload, B,
-                         ctx->tiles, ctx->ovf);
+                         ctx->tiles, overflow(ctx));
     }
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID_ADJ, st);
-    hipLaunchKernelGGL(k_grid_adj, dim3(nblocks(ctx->G)), dim3(kBlock), 0, st, M, T, time, dt, B, ctx->tiles,
-                       ctx->ovf, keep, ctx->gadj_in, (flags & EMPM_ADJ_DROP_GRID_MASS) ? 1 : 0);
+    hipLaunchKernelGGL(k_grid_adj, grid_launch(ctx), dim3(64, kGridRows), 0, st, M, T, time, dt, B, ctx->tiles,
+                       overflow(ctx), keep, ctx->gadj_in, (flags & EMPM_ADJ_DROP_GRID_MASS) ? 1 : 0);
   }
   if (M.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G_ADJ, st);
@@ -1489,14 +1603,15 @@ int empm_create(empm_ctx** out, const empm_create_desc* desc) {
   c->n_grid = desc->n_grid;
   c->max_particles = desc->max_particles;
   c->nb = (desc->n_grid + kBlkCells - 1) / kBlkCells;
-  c->rebin_every = 25;
   c->G = (size_t)desc->n_grid * desc->n_grid * desc->n_grid;
   c->nbins = (size_t)c->nb * c->nb * c->nb * 8;
   const size_t np = desc->max_particles > 0 ? (size_t)desc->max_particles : 1;
   const size_t ntiles = (size_t)c->nb * c->nb * c->nb;
+  const size_t nwsum = (ntiles + kBinScanWG - 1) / kBinScanWG;
   hipError_t e = hipSuccess;
   e = e == hipSuccess ? hipMalloc((void**)&c->tiles, ntiles * kTileNodes * sizeof(float4)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->ovf, c->G * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->ovf_flag, sizeof(int)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->keep, c->G * sizeof(float4)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->v_out, c->G * sizeof(float4)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->gadj_in, c->G * sizeof(float4)) : e;
@@ -1505,14 +1620,14 @@ int empm_create(empm_ctx** out, const empm_create_desc* desc) {
   e = e == hipSuccess ? hipMalloc((void**)&c->payload, np * kAdjPayload * sizeof(float)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->xbar, np * 3 * sizeof(float)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->Febar, np * 9 * sizeof(float)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->perm, np * sizeof(int)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->bin_count, (c->nbins + 1) * sizeof(int)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->bin_rank, np * sizeof(int)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->tile_sums, ((c->nbins + 1) / kScanTile + 2) * sizeof(int)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->active, ntiles * sizeof(int)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->nactive, sizeof(int)) : e;
-  e = e == hipSuccess ? hipMemset(c->nactive, 0, sizeof(int)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->bins, bin_record_ints(c->n_grid, (int)np) * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->count, c->nbins * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->key_rank, np * sizeof(int2)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->blkoff, ntiles * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->wsum, nwsum * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMemset(c->count, 0, c->nbins * sizeof(int)) : e;
   e = e == hipSuccess ? hipMemset(c->ovf, 0, c->G * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMemset(c->ovf_flag, 0, sizeof(int)) : e;
   e = e == hipSuccess ? hipDeviceSynchronize() : e;
   if (e != hipSuccess) {
     empm_destroy(c);
@@ -1545,12 +1660,12 @@ void empm_destroy(empm_ctx* c) {
   hipFree(c->payload);
   hipFree(c->xbar);
   hipFree(c->Febar);
-  hipFree(c->perm);
-  hipFree(c->bin_count);
-  hipFree(c->bin_rank);
-  hipFree(c->tile_sums);
-  hipFree(c->active);
-  hipFree(c->nactive);
+  hipFree(c->ovf_flag);
+  hipFree(c->bins);
+  hipFree(c->count);
+  hipFree(c->key_rank);
+  hipFree(c->blkoff);
+  hipFree(c->wsum);
   delete c;
 }
 
@@ -1560,17 +1675,17 @@ int empm_sort_by_cell(empm_ctx* ctx, int n_particles, const float* x, float inv_
     return set_err(EMPM_E_INVALID_ARG, "empm_sort_by_cell: bad arguments");
   if (n_particles == 0) return EMPM_OK;
   HIP_TRY(hipSetDevice(ctx->device));
-  int rc = do_bin(ctx, n_particles, x, inv_dx, (hipStream_t)stream);
+  const BinBuf bb = bin_buf(ctx->bins, ctx->n_grid, n_particles);
+  int rc = do_bin(ctx, n_particles, x, inv_dx, bb, (hipStream_t)stream);
   if (rc) return rc;
-  HIP_TRY(hipMemcpyAsync(perm, ctx->perm, (size_t)n_particles * sizeof(int), hipMemcpyDeviceToDevice,
+  HIP_TRY(hipMemcpyAsync(perm, bb.perm, (size_t)n_particles * sizeof(int), hipMemcpyDeviceToDevice,
                          (hipStream_t)stream));
   return EMPM_OK;
 }
 
-int empm_set_rebin_interval(empm_ctx* ctx, int substeps) {
-  if (!ctx || substeps < 1) return set_err(EMPM_E_INVALID_ARG, "empm_set_rebin_interval: substeps must be >= 1");
-  ctx->rebin_every = substeps;
-  return EMPM_OK;
+long long empm_bin_record_ints(int n_grid, int n_particles) {
+  if (n_grid < 4 || n_grid > 1024 || n_particles < 0) return -1;
+  return (long long)bin_record_ints(n_grid, n_particles);
 }
 
 int empm_profile_enable(empm_ctx* ctx, unsigned kernel_mask) {
@@ -1658,7 +1773,6 @@ int empm_substep_fwd(empm_ctx* ctx, const empm_model* model, const empm_collider
   ColTable T;
   if ((rc = make_table(cols, ncol, T, model->n_grid))) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
-  if ((rc = do_bin(ctx, model->n_particles, in->x, model->inv_dx, (hipStream_t)stream))) return rc;
   return launch_fwd(ctx, dev_model(model), T, time, dt, in, out, nullptr, true, true, (hipStream_t)stream);
 }
 
@@ -1675,7 +1789,6 @@ int empm_substep_adj(empm_ctx* ctx, const empm_model* model, const empm_collider
   ColTable T;
   if ((rc = make_table(cols, ncol, T, model->n_grid))) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
-  if ((rc = do_bin(ctx, model->n_particles, in->x, model->inv_dx, (hipStream_t)stream))) return rc;
   return launch_adj(ctx, dev_model(model), T, time, dt, in, nullptr, d_out, d_in, d_mu, d_lam, d_ys, d_mass, d_vol,
                     flags, (hipStream_t)stream);
 }
@@ -1699,9 +1812,6 @@ int empm_rollout_fwd(empm_ctx* ctx, const empm_model* model, empm_collider* cols
     make_table(cols, ncol, T, model->n_grid);
     if (cols_hist) memcpy(&cols_hist[(size_t)k * ncol], cols, sizeof(empm_collider) * ncol);
     if (times_hist) times_hist[k] = *time;
-    if (k % ctx->rebin_every == 0 &&
-        (rc = do_bin(ctx, model->n_particles, states[k].x, model->inv_dx, (hipStream_t)stream)))
-      return rc;
     if ((rc = launch_fwd(ctx, M, T, (float)*time, (float)dt, &states[k], &states[k + 1], grids ? &grids[k] : nullptr,
                          k == 0, k == n_steps - 1, (hipStream_t)stream)))
       return rc;
@@ -1728,9 +1838,6 @@ int empm_rollout_fwd_replay(empm_ctx* ctx, const empm_model* model, const empm_c
   ColTable T;
   for (int k = 0; k < n_steps; ++k) {
     if ((rc = make_table(&cols_hist[(size_t)k * ncol], ncol, T, model->n_grid))) return rc;
-    if (k % ctx->rebin_every == 0 &&
-        (rc = do_bin(ctx, model->n_particles, states[k].x, model->inv_dx, (hipStream_t)stream)))
-      return rc;
     if ((rc = launch_fwd(ctx, M, T, (float)times_hist[k], (float)dt, &states[k], &states[k + 1],
                          grids ? &grids[k] : nullptr, k == 0, k == n_steps - 1, (hipStream_t)stream)))
       return rc;
@@ -1760,9 +1867,6 @@ int empm_rollout_adj(empm_ctx* ctx, const empm_model* model, const empm_collider
     if ((rc = make_table(&cols_hist[(size_t)k * ncol], ncol, T, model->n_grid))) return rc;
     const empm_particles* dout = &adj[j % 2];
     const empm_particles* din = &adj[(j + 1) % 2];
-    if (j % ctx->rebin_every == 0 &&
-        (rc = do_bin(ctx, model->n_particles, states[k].x, model->inv_dx, (hipStream_t)stream)))
-      return rc;
     if ((rc = launch_adj(ctx, M, T, (float)times_hist[k], (float)dt, &states[k], grids ? &grids[k] : nullptr, dout,
                          din, d_mu, d_lam, d_ys, d_mass, d_vol, flags, (hipStream_t)stream)))
       return rc;

@@ -55,9 +55,6 @@ class Context:
     def handle(self):
         return self._h
 
-    def set_rebin_interval(self, substeps: int):
-        check(self.lib.empm_set_rebin_interval(self._h, int(substeps)))
-
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             self.lib.empm_destroy(self._h)
@@ -232,13 +229,14 @@ class RolloutTape:
     perm: object = None  # cell order used internally (None: caller order)
     model: object = None  # ModelArrays in the internal order
     rec_states: list | None = None  # recorded mode: input state of substep k (with Fe)
-    rec_grids: list | None = None  # recorded mode: (2, n^3, 4) grid of substep k
+    rec_grids: list | None = None  # recorded mode: (grid (2, n^3, 4), bins) of substep k
 
 
 class Records:
     """Device storage for `n` fully recorded substeps: states 0..n (x, v, C,
-    F_trial and Fe; Fe of state n unused) and the grid of substeps 0..n-1
-    ((m v, m) and v_out, float4 per node), one tensor per field."""
+    F_trial and Fe; Fe of state n unused), the grid of substeps 0..n-1
+    ((m v, m) and v_out, float4 per node) and their particle binnings, one
+    tensor per field."""
 
     def __init__(self, n: int, n_particles: int, n_grid: int, device):
         N, G = int(n_particles), int(n_grid) ** 3
@@ -247,9 +245,15 @@ class Records:
         self.x, self.v = e(n + 1, N, 3), e(n + 1, N, 3)
         self.C, self.F, self.Fe = e(n + 1, N, 3, 3), e(n + 1, N, 3, 3), e(n + 1, N, 3, 3)
         self.grid = e(max(n, 1), 2, G, 4)
+        nbi = int(_lib.load().empm_bin_record_ints(int(n_grid), N))
+        self.bins = torch.empty((max(n, 1), nbi), dtype=torch.int32, device=device)
 
     def state(self, k: int) -> ParticleState:
         return ParticleState(self.x[k], self.v[k], self.C[k], self.F[k], self.Fe[k])
+
+    def record(self, k: int):
+        """(grid, bins) of substep k, the entry of a grid-record list."""
+        return (self.grid[k], self.bins[k])
 
 
 def _states_array(states: list[ParticleState]):
@@ -259,10 +263,11 @@ def _states_array(states: list[ParticleState]):
     return arr
 
 
-def _grid_array(grids: list):
-    arr = (GridRec * max(1, len(grids)))()
-    for i, g in enumerate(grids):
-        arr[i] = GridRec(mv_m=_ptr(g[0]), v_out=_ptr(g[1]))
+def _grid_array(recs: list):
+    """empm_grid_rec array from (grid (2, n^3, 4), bins) pairs."""
+    arr = (GridRec * max(1, len(recs)))()
+    for i, (g, b) in enumerate(recs):
+        arr[i] = GridRec(mv_m=_ptr(g[0]), v_out=_ptr(g[1]), bins=_ptr(b))
     return arr
 
 
@@ -353,7 +358,7 @@ class Rollout:
             R = Records(n_steps, self.n, model.n_grid, self.ctx.device)
             R.state(0).copy_(state)
             states = [R.state(k) for k in range(n_steps + 1)]
-            grids = [R.grid[k] for k in range(n_steps)]
+            grids = [R.record(k) for k in range(n_steps)]
             check(lib.empm_rollout_fwd(self.ctx.handle, C.byref(m), arr, ncol, C.byref(t), C.c_double(dt), n_steps,
                                        _states_array(states), _grid_array(grids), tp.cols_hist, tp.times_hist,
                                        stream))
@@ -469,7 +474,7 @@ class Rollout:
                 L = k_hi - s0
                 seg.state(0).copy_(tp.checkpoints[s0])
                 states = [seg.state(i) for i in range(L + 1)]
-                grids = [seg.grid[i] for i in range(L)]
+                grids = [seg.record(i) for i in range(L)]
                 ch = C.cast(C.byref(tp.cols_hist, s0 * ncol * C.sizeof(Collider)), C.POINTER(Collider))
                 th = C.cast(C.byref(tp.times_hist, s0 * C.sizeof(C.c_double)), C.POINTER(C.c_double))
                 # replay the segment, recording every substep's Fe and grid

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define EMPM_ABI_VERSION 2
+#define EMPM_ABI_VERSION 3
 #define EMPM_MAX_COLLIDERS 16
 
 enum {
@@ -102,10 +102,13 @@ typedef struct {
 /* The grid of one forward substep, recorded for reverse mode so that the
  * adjoint does not recompute P2G and the boundary pass: the reference keeps
  * grid_m / grid_v_in / grid_v_out of every substep alive for its Tape
- * (interface.py:137-145 with mpm_data_structure.py:356-387). */
+ * (interface.py:137-145 with mpm_data_structure.py:356-387).  `bins`
+ * (optional) also keeps the substep's particle binning, so the reverse
+ * sweep does not re-bin. */
 typedef struct {
     float *mv_m;  /* n^3*4: (m v, m) after P2G, node-major float4 */
     float *v_out; /* n^3*4: grid velocity after the boundary pass (w unused) */
+    int *bins;    /* empm_bin_record_ints(n_grid, N) ints, or NULL */
 } empm_grid_rec;
 
 /* One grid post-process, in registration order.  kind:
@@ -230,11 +233,13 @@ int empm_rollout_adj(empm_ctx *ctx, const empm_model *model, const empm_collider
  * mpm_utils.py:358-423). */
 int empm_sort_by_cell(empm_ctx *ctx, int n_particles, const float *x, float inv_dx, int *perm, void *stream);
 
-/* Substeps between re-binnings of the particles inside one rollout call
- * (default 25).  Every substep/rollout call bins at entry; a particle that
- * drifts more than one cell from its bin between re-binnings is still
- * scattered correctly (through the overflow grid), only more slowly. */
-int empm_set_rebin_interval(empm_ctx *ctx, int substeps);
+/* Size in ints of one substep's binning record (empm_grid_rec.bins) for a
+ * grid of resolution n_grid and n_particles particles; -1 on bad arguments.
+ * Every substep bins its input particles by stencil cell before P2G, so the
+ * scatter never meets a particle outside its block's tile.  No counterpart
+ * in the reference (it scatters unsorted particles with global atomics,
+ * mpm_utils.py:358-423). */
+long long empm_bin_record_ints(int n_grid, int n_particles);
 
 /* Kernel ids for per-kernel timing */
 enum {
@@ -246,7 +251,8 @@ enum {
     EMPM_KERNEL_P2G_ADJ = 5,  /* adjoint of P2G + stress (gather) */
     EMPM_KERNEL_STRESS = 6,   /* stress of the first substep of a call */
     EMPM_KERNEL_G2P_ADJ_SCATTER = 7, /* adjoint of G2P: scatter of dL/dv_out */
-    EMPM_NUM_KERNELS = 8
+    EMPM_KERNEL_BIN = 8,      /* per-substep binning (4 launches, timed as one) */
+    EMPM_NUM_KERNELS = 9
 };
 
 /* Instrumentation (no counterpart in the reference, whose per-stage

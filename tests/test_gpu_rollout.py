@@ -192,26 +192,51 @@ def test_checkpoint_interval_does_not_change_gradients(dev):
     assert np.allclose(g1, g2, rtol=1e-5) and np.allclose(g1, g3, rtol=1e-5), (g1, g2, g3)
 
 
-@pytest.mark.parametrize("rebin", [1, 1000])
-def test_drift_beyond_tile_matches_oracle(dev, rebin):
-    """Fast particles binned once (rebin=1000) drift out of their block tiles
-    and go through the overflow grid; binned every substep they never do.
-    Both must match the oracle."""
+def test_fast_particles_match_oracle(dev):
+    """Fast particles (3 m/s at dt 1e-3: 0.05 cells per substep) are re-binned
+    every substep; positions must match the oracle."""
     sc = scenes.small_scene(n_particles=500, n_grid=16, material=6, seed=9, E=3e4, vel=(3.0, -2.0, 1.0), dt=1e-3,
                             center=(0.45, 0.55, 0.5), spin=0.0)
     steps = 40
     model, s0, P, ocols, ostate = setup(sc, dev, v0=sc.params["v0"])
     ctx = rt.Context(sc.n_grid, sc.n, dev)
-    ctx.set_rebin_interval(rebin)
     ro = rt.Rollout(ctx, model, sc.n)
     final, _, _, tape = ro.forward(s0, [gpu_collider(c) for c in ocols], 0.0, sc.dt, steps, tape=True, ckpt_every=40)
     torch.cuda.synchronize()
     o = ostate("f32")
     orc.rollout(P, list(ocols), 0.0, sc.dt, steps, o)
     ex = rel_err(final.x.cpu().numpy(), o.x)
-    print(f"[drift rebin={rebin}] rel err x {ex:.2e}", flush=True)
+    print(f"[fast particles] rel err x {ex:.2e}", flush=True)
     assert ex < 1e-4
     d = rt.ParticleState.empty(sc.n, dev)
     d.x.fill_(1.0)
     d0, g = ro.backward(tape, d)
     assert torch.isfinite(d0.v).all() and torch.isfinite(g.mu).all()
+
+
+def test_particles_outside_grid_use_the_overflow_path(dev):
+    """A substep whose input has particles outside [0, 1)^3 (binned to an edge
+    cell, so their stencil leaves the block tile) scatters them through the
+    overflow grid; the substep must match the oracle's, which clips nothing
+    on input either."""
+    from tests.helpers import oracle_state, random_state
+
+    N, n = 256, 16
+    d = random_state(N, n, seed=21, material=0)
+    d["x"][:8, 0] = np.float32(-0.5 / n)          # stencil base -1 in x
+    d["x"][8:16, 2] = np.float32(1.0 + 0.4 / n)   # base n in z
+    t = lambda a: torch.as_tensor(a, device=dev).contiguous()
+    P = orc.make_params(N, n, material=0)
+    model = rt.ModelArrays(material=0, n_grid=n, grid_lim=1.0, dx=P.dx, inv_dx=P.inv_dx, mu=t(d["mu"]),
+                           lam=t(d["lam"]), yield_stress=t(d["ys"]), mass=t(d["mass"]), vol=t(d["vol"]),
+                           gravity=(0.0, 0.0, -9.8))
+    ctx = rt.Context(n, N, dev)
+    s_in = rt.ParticleState(t(d["x"]), t(d["v"]), t(d["C"]), t(d["F"]))
+    s_out = rt.ParticleState.empty(N, dev)
+    rt.substep_fwd(ctx, model, [], 0.0, 1e-4, s_in, s_out)
+    torch.cuda.synchronize()
+    o = orc.substep(P, [], 0.0, 1e-4, oracle_state(d, "f32"))
+    ex = rel_err(s_out.x.cpu().numpy(), o.x)
+    ev = rel_err(s_out.v.cpu().numpy(), o.v)
+    print(f"[outside grid] rel err x {ex:.2e} v {ev:.2e}", flush=True)
+    assert ex < 1e-5 and ev < 1e-4
