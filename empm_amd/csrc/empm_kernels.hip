@@ -728,9 +728,9 @@ __global__ __launch_bounds__(kBlock) void k_g2p(DevModel M, float dt, const floa
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
-        float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (node_in(ix, iy, iz, M.n)) gv = v_out[node_id(ix, iy, iz, M.n)];
-        const float gvv[3] = {gv.x, gv.y, gv.z};
+        const bool in = node_in(ix, iy, iz, M.n);
+        const float4 gv = v_out[in ? node_id(ix, iy, iz, M.n) : 0];
+        const float gvv[3] = {in ? gv.x : 0.f, in ? gv.y : 0.f, in ? gv.z : 0.f};
         const float dpos[3] = {(float)a - s.f[0], (float)b - s.f[1], (float)c - s.f[2]};
         const float wt = s.w[0][a] * s.w[1][b] * s.w[2][c];
         const float gw[3] = {s.dw[0][a] * s.w[1][b] * s.w[2][c] * M.inv_dx, s.w[0][a] * s.dw[1][b] * s.w[2][c] * M.inv_dx,
@@ -787,27 +787,55 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(DevModel M, float dt, const 
   const float inv_dx = M.inv_dx;
   const float c4 = inv_dx * 4.f;
   const float xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
+  float dF[9], Fp[9], dC[9], Gb[9];
+  load9(dF1, p, dF);
+  load9(Fe, p, Fp);
+  load9(dC1, p, dC);
+  // F1 = (I + dt G) Fe:  Gbar = dt dF Fe^T ; Febar = (I + dt G)^T dF
+  m_mul_bt(dF, Fp, Gb);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Gb[k] *= dt;
   Stencil s;
   make_stencil(xp, inv_dx, s);
-  // pass 1: v' and grad v for the clamp mask and the F adjoint
-  float nv[3] = {0.f, 0.f, 0.f}, nG[9];
+  // One pass over the stencil: v' and grad v (for the clamp mask and the F
+  // adjoint) and every d(w, grad w, d)/dx term.  The term of dL/dv' through
+  // the weights, sum_j (dL/dv' . v_j) dw_j/df, is (dL/dv')^T grad v / inv_dx,
+  // so it is added after the pass, once dL/dv' (which needs v') is known.
+  float nv[3] = {0.f, 0.f, 0.f}, nG[9], fb[3] = {0.f, 0.f, 0.f};
   m_zero(nG);
 #pragma unroll 1
-  for (int j = 0; j < 27; ++j) {
-    const int a = j / 9, b = (j / 3) % 3, c = j % 3;
+  for (int a = 0; a < 3; ++a) {
+    const float wa = sw(s, 0, a), dwa = sdw(s, 0, a);
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    const int b = j / 3, c = j % 3;
     const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
-    float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (node_in(ix, iy, iz, M.n)) gv = v_out[node_id(ix, iy, iz, M.n)];
-    const float vi[3] = {gv.x, gv.y, gv.z};
-    const float wt = sw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c);
-    const float gw[3] = {sdw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c) * inv_dx, sw(s, 0, a) * sdw(s, 1, b) * sw(s, 2, c) * inv_dx,
-                         sw(s, 0, a) * sw(s, 1, b) * sdw(s, 2, c) * inv_dx};
+    const bool in = node_in(ix, iy, iz, M.n);
+    const float4 g4 = v_out[in ? node_id(ix, iy, iz, M.n) : 0];
+    const float vi[3] = {in ? g4.x : 0.f, in ? g4.y : 0.f, in ? g4.z : 0.f};
+    const float wt = wa * s.w[1][b] * s.w[2][c];
+    const float gw[3] = {dwa * s.w[1][b] * s.w[2][c] * inv_dx, wa * s.dw[1][b] * s.w[2][c] * inv_dx,
+                         wa * s.w[1][b] * s.dw[2][c] * inv_dx};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       nv[k] += vi[k] * wt;
 #pragma unroll
       for (int l = 0; l < 3; ++l) nG[k * 3 + l] += vi[k] * gw[l];
     }
+    const float d[3] = {(float)a - s.f[0], (float)b - s.f[1], (float)c - s.f[2]};
+    float Cd[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) Cd[k] = dC[k * 3 + 0] * d[0] + dC[k * 3 + 1] * d[1] + dC[k * 3 + 2] * d[2];
+    const float wb = c4 * (vi[0] * Cd[0] + vi[1] * Cd[1] + vi[2] * Cd[2]);
+    float gwb[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      // d = off - f: dd_k = 4 inv_dx w sum_i dC_ik v_i
+      fb[k] -= c4 * wt * (dC[0 * 3 + k] * vi[0] + dC[1 * 3 + k] * vi[1] + dC[2 * 3 + k] * vi[2]);
+      gwb[k] = Gb[0 * 3 + k] * vi[0] + Gb[1 * 3 + k] * vi[1] + Gb[2 * 3 + k] * vi[2];
+    }
+    weight_adj(s, a, b, c, inv_dx, wb, gwb, fb);
+  }
   }
   const float amin = M.dx * 2.f, amax = M.grid_lim - M.dx * 2.f;
   float xb[3], nvb[3];
@@ -820,47 +848,14 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(DevModel M, float dt, const 
     xb[k] = g;
     nvb[k] = dv1[3 * p + k] + dt * g;
   }
-  float dF[9], Fp[9], dC[9], Gb[9];
-  load9(dF1, p, dF);
-  load9(Fe, p, Fp);
-  load9(dC1, p, dC);
-  // F1 = (I + dt G) Fe:  Gbar = dt dF Fe^T ; Febar = (I + dt G)^T dF
   float IF[9], Feb[9];
-  m_mul_bt(dF, Fp, Gb);
 #pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    Gb[k] *= dt;
-    IF[k] = ((k % 4 == 0) ? 1.f : 0.f) + nG[k] * dt;
-  }
+  for (int k = 0; k < 9; ++k) IF[k] = ((k % 4 == 0) ? 1.f : 0.f) + nG[k] * dt;
   m_mul_at(IF, dF, Feb);
   store9(Febar, p, Feb);
-  // pass 2: d(w, grad w, d)/dx
-  float fb[3] = {0.f, 0.f, 0.f};
-#pragma unroll 1
-  for (int j = 0; j < 27; ++j) {
-    const int a = j / 9, b = (j / 3) % 3, c = j % 3;
-    const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
-    float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (node_in(ix, iy, iz, M.n)) gv = v_out[node_id(ix, iy, iz, M.n)];
-    const float vi[3] = {gv.x, gv.y, gv.z};
-    const float d[3] = {(float)a - s.f[0], (float)b - s.f[1], (float)c - s.f[2]};
-    const float wt = sw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c);
-    float Cd[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) Cd[k] = dC[k * 3 + 0] * d[0] + dC[k * 3 + 1] * d[1] + dC[k * 3 + 2] * d[2];
-    const float wb =
-        nvb[0] * vi[0] + nvb[1] * vi[1] + nvb[2] * vi[2] + c4 * (vi[0] * Cd[0] + vi[1] * Cd[1] + vi[2] * Cd[2]);
-    float gwb[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      // d = off - f: dd_k = 4 inv_dx w sum_i dC_ik v_i
-      fb[k] -= c4 * wt * (dC[0 * 3 + k] * vi[0] + dC[1 * 3 + k] * vi[1] + dC[2 * 3 + k] * vi[2]);
-      gwb[k] = Gb[0 * 3 + k] * vi[0] + Gb[1 * 3 + k] * vi[1] + Gb[2 * 3 + k] * vi[2];
-    }
-    weight_adj(s, a, b, c, inv_dx, wb, gwb, fb);
-  }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) xbar[3 * p + k] = xb[k] + inv_dx * fb[k];
+  for (int l = 0; l < 3; ++l)
+    xbar[3 * p + l] = xb[l] + inv_dx * fb[l] + (nvb[0] * nG[l] + nvb[1] * nG[3 + l] + nvb[2] * nG[6 + l]);
   float* q = payload + (size_t)kAdjPayload * p;
   float4* q4 = reinterpret_cast<float4*>(q);
   q4[0] = make_float4(nvb[0], nvb[1], nvb[2], 0.f);
