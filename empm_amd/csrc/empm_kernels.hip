@@ -656,24 +656,31 @@ __global__ __launch_bounds__(kBlkWG) void k_p2g(DevModel M, const float* __restr
 // grid_normalization_and_gravity, damping and the colliders.  The overflow
 // grid is re-zeroed for the next scatter; (mv, m) is kept when `keep` is set
 // (the adjoint needs it).
-// Node-pass launch shape: 64 consecutive z per wave, kGridRows y-rows per
-// workgroup, one x-plane per grid z index.
-constexpr int kGridRows = 4;
-EMPM_DEV bool grid_thread(const DevModel& M, int& ix, int& iy, int& iz, size_t& id) {
-  iz = blockIdx.x * 64 + threadIdx.x;
-  iy = blockIdx.y * kGridRows + threadIdx.y;
-  ix = blockIdx.z;
-  if (iz >= M.n || iy >= M.n) return false;
-  id = node_id(ix, iy, iz, M.n);
-  return true;
+// The node passes visit only the nodes a scatter can have touched: the 2x2x2
+// nodes owned by each block (nodes 2o .. 2o + 1 per axis) whose `near` flag
+// is set.  Other nodes keep stale values, and no later kernel reads them:
+// G2P and the P2G adjoint read only the stencil nodes of particles, which
+// lie in their block's tile.  A capped grid strides over (block, owned node)
+// pairs, so non-near blocks cost one flag load each.
+constexpr int kNodePassGridCap = 1024;
+template <typename F>
+EMPM_DEV void for_near_nodes(const DevModel& M, const Binning& B, F f) {
+  const int nb = B.nb;
+  const size_t npairs = (size_t)nb * nb * nb * 8;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < npairs; q += (size_t)gridDim.x * blockDim.x) {
+    const int blk = (int)(q >> 3), r = (int)(q & 7);
+    if (!B.near[blk]) continue;
+    const int ix = 2 * (blk / (nb * nb)) + (r >> 2), iy = 2 * ((blk / nb) % nb) + ((r >> 1) & 1),
+              iz = 2 * (blk % nb) + (r & 1);
+    if (ix >= M.n || iy >= M.n || iz >= M.n) continue;
+    f(ix, iy, iz, node_id(ix, iy, iz, M.n));
+  }
 }
 
-__global__ __launch_bounds__(64 * kGridRows) void k_grid(DevModel M, ColTable T, float time, float dt, Binning B,
-                                                         const float4* __restrict__ tiles, Overflow ov,
-                                                         float4* __restrict__ keep, float4* __restrict__ v_out) {
-  int ix, iy, iz;
-  size_t id;
-  if (!grid_thread(M, ix, iy, iz, id)) return;
+__global__ __launch_bounds__(kBlock) void k_grid(DevModel M, ColTable T, float time, float dt, Binning B,
+                                                 const float4* __restrict__ tiles, Overflow ov,
+                                                 float4* __restrict__ keep, float4* __restrict__ v_out) {
+  for_near_nodes(M, B, [&](int ix, int iy, int iz, size_t id) {
   float4 g = gather_node(B, tiles, ix, iy, iz);
   if (*ov.flag) {
     const float4 o = ov.grid[id];
@@ -689,6 +696,7 @@ __global__ __launch_bounds__(64 * kGridRows) void k_grid(DevModel M, ColTable T,
   float v[3], fac[3];
   grid_node(M, T, time, dt, ix, iy, iz, g, v, fac);
   v_out[id] = make_float4(v[0], v[1], v[2], g.w);
+  });
 }
 
 // g2p_differentiable (mpm_utils.py:502-579).  With `next_stress` set the
@@ -889,13 +897,11 @@ __global__ __launch_bounds__(kBlkWG) void k_g2p_adj_scatter(DevModel M, const fl
 // node's forward (normalisation, damping, colliders) to get dv_out/dv per
 // component, and write dL/d(mv, m).  The grid-mass term is dropped when
 // `drop_mass` is set (reference Tape behaviour).
-__global__ __launch_bounds__(64 * kGridRows) void k_grid_adj(DevModel M, ColTable T, float time, float dt, Binning B,
-                                                             const float4* __restrict__ tiles, Overflow ov,
-                                                             const float4* __restrict__ keep,
-                                                             float4* __restrict__ gadj_in, int drop_mass) {
-  int ix, iy, iz;
-  size_t id;
-  if (!grid_thread(M, ix, iy, iz, id)) return;
+__global__ __launch_bounds__(kBlock) void k_grid_adj(DevModel M, ColTable T, float time, float dt, Binning B,
+                                                     const float4* __restrict__ tiles, Overflow ov,
+                                                     const float4* __restrict__ keep, float4* __restrict__ gadj_in,
+                                                     int drop_mass) {
+  for_near_nodes(M, B, [&](int ix, int iy, int iz, size_t id) {
   float4 b = gather_node(B, tiles, ix, iy, iz);
   if (*ov.flag) {
     const float4 ob = ov.grid[id];
@@ -919,6 +925,7 @@ __global__ __launch_bounds__(64 * kGridRows) void k_grid_adj(DevModel M, ColTabl
     o.w = drop_mass ? 0.f : -(b0 * g.x + b1 * g.y + b2 * g.z) * im * im;
   }
   gadj_in[id] = o;
+  });
 }
 
 template <int MAT>
@@ -1438,8 +1445,9 @@ unsigned block_grid(const empm_ctx* ctx) {
 unsigned nblocks(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 dim3 grid_launch(const empm_ctx* ctx) {
-  const unsigned n = (unsigned)ctx->n_grid;
-  return dim3((n + 63) / 64, (n + kGridRows - 1) / kGridRows, n);
+  const size_t pairs = (size_t)ctx->nb * ctx->nb * ctx->nb * 8;
+  const size_t wgs = (pairs + kBlock - 1) / kBlock;
+  return dim3((unsigned)(wgs < (size_t)kNodePassGridCap ? wgs : kNodePassGridCap));
 }
 
 // Bin the particles of a state (positions x) into `out`.
@@ -1486,7 +1494,7 @@ int launch_fwd(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, 
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
-    hipLaunchKernelGGL(k_grid, grid_launch(ctx), dim3(64, kGridRows), 0, st, M, T, time, dt, B, ctx->tiles,
+    hipLaunchKernelGGL(k_grid, grid_launch(ctx), dim3(kBlock), 0, st, M, T, time, dt, B, ctx->tiles,
                        overflow(ctx), keep, v_out);
   }
   if (M.N > 0) {
@@ -1527,7 +1535,7 @@ int launch_adj(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, 
     }
     {
       ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
-      hipLaunchKernelGGL(k_grid, grid_launch(ctx), dim3(64, kGridRows), 0, st, M, T, time, dt, B, ctx->tiles,
+      hipLaunchKernelGGL(k_grid, grid_launch(ctx), dim3(kBlock), 0, st, M, T, time, dt, B, ctx->tiles,
                          overflow(ctx), keep, v_out);
     }
   }
@@ -1545,7 +1553,7 @@ int launch_adj(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, 
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID_ADJ, st);
-    hipLaunchKernelGGL(k_grid_adj, grid_launch(ctx), dim3(64, kGridRows), 0, st, M, T, time, dt, B, ctx->tiles,
+    hipLaunchKernelGGL(k_grid_adj, grid_launch(ctx), dim3(kBlock), 0, st, M, T, time, dt, B, ctx->tiles,
                        overflow(ctx), keep, ctx->gadj_in, (flags & EMPM_ADJ_DROP_GRID_MASS) ? 1 : 0);
   }
   if (M.N > 0) {
