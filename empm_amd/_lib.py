@@ -12,7 +12,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libempm.so")
 
-EMPM_ABI_VERSION = 3
+EMPM_ABI_VERSION = 4
 EMPM_MAX_COLLIDERS = 16
 
 EMPM_OK = 0
@@ -31,6 +31,7 @@ EXPORTED_SYMBOLS = (
     "empm_abi_version",
     "empm_last_error",
     "empm_create",
+    "empm_create_batch",
     "empm_destroy",
     "empm_mu_lam",
     "empm_mu_lam_adj",
@@ -40,6 +41,10 @@ EXPORTED_SYMBOLS = (
     "empm_rollout_fwd",
     "empm_rollout_fwd_replay",
     "empm_rollout_adj",
+    "empm_batch_rollout_fwd",
+    "empm_batch_rollout_fwd_replay",
+    "empm_batch_rollout_adj",
+    "empm_batch_sizes",
     "empm_sort_by_cell",
     "empm_bin_record_ints",
     "empm_profile_enable",
@@ -100,6 +105,29 @@ class Particles(C.Structure):
                 ("Fe", C.c_void_p)]
 
 
+class Batch(C.Structure):
+    _fields_ = [
+        ("n_cand", C.c_int),
+        ("n_particles", C.c_int),
+        ("models", C.c_void_p),
+        ("mu", C.c_void_p),
+        ("lam", C.c_void_p),
+        ("yield_stress", C.c_void_p),
+        ("mass", C.c_void_p),
+        ("vol", C.c_void_p),
+        ("selection", C.c_void_p),
+    ]
+
+
+class BatchCreateDesc(C.Structure):
+    _fields_ = [
+        ("device", C.c_int),
+        ("max_cand", C.c_int),
+        ("max_particles", C.c_int),
+        ("max_grid", C.c_int),
+    ]
+
+
 class GridRec(C.Structure):
     _fields_ = [("mv_m", C.c_void_p), ("v_out", C.c_void_p), ("bins", C.c_void_p)]
 
@@ -139,6 +167,20 @@ def _declare(lib):
     lib.empm_last_error.argtypes = []
     lib.empm_create.restype = C.c_int
     lib.empm_create.argtypes = [P(vp), P(CreateDesc)]
+    lib.empm_create_batch.restype = C.c_int
+    lib.empm_create_batch.argtypes = [P(vp), P(BatchCreateDesc)]
+    lib.empm_batch_rollout_fwd.restype = C.c_int
+    lib.empm_batch_rollout_fwd.argtypes = [vp, P(Batch), P(Collider), C.c_int, P(C.c_double), C.c_double,
+                                           C.c_int, P(Particles), P(GridRec), P(Collider), P(C.c_double), vp]
+    lib.empm_batch_rollout_fwd_replay.restype = C.c_int
+    lib.empm_batch_rollout_fwd_replay.argtypes = [vp, P(Batch), P(Collider), C.c_int, P(C.c_double), C.c_double,
+                                                  C.c_int, P(Particles), P(GridRec), vp]
+    lib.empm_batch_rollout_adj.restype = C.c_int
+    lib.empm_batch_rollout_adj.argtypes = [vp, P(Batch), P(Collider), C.c_int, P(C.c_double), C.c_double,
+                                           C.c_int, P(Particles), P(GridRec), P(Particles),
+                                           vp, vp, vp, vp, vp, C.c_int, vp]
+    lib.empm_batch_sizes.restype = C.c_int
+    lib.empm_batch_sizes.argtypes = [P(Batch), P(C.c_longlong), P(C.c_longlong)]
     lib.empm_destroy.restype = None
     lib.empm_destroy.argtypes = [vp]
     lib.empm_mu_lam.restype = C.c_int

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -50,9 +51,17 @@ int set_err(int code, const char* fmt, ...) {
   } while (0)
 
 constexpr int kBlock = 256;
+constexpr int kWaveSize = 64;
 
+// One simulation of a batch (a single rollout is a batch of one).  Its
+// particles, blocks and grid nodes are the ranges [p0, p0 + N), [b0, b0 +
+// nb^3) and [g0, g0 + n^3) of the batch's arenas; the per-particle arrays
+// point at the whole particle arena and are indexed by the global particle.
 struct DevModel {
-  int N, n, material, pad0;
+  int N, n, material, nb;
+  int p0, b0;
+  int w0, pad0;  // first wave of this simulation in per-particle launches
+  long long g0;
   float grid_lim, dx, inv_dx, rpic, damp_scale;
   float g[3];
   MatParams mp;
@@ -63,6 +72,58 @@ struct DevModel {
   const float* vol;
   const int* sel;
 };
+
+// The simulations of a launch: a device table ordered by p0 (and b0, g0, w0).
+struct Batch {
+  const DevModel* cand;
+  int ncand;
+  int N;       // particles in the arena
+  int nblk;    // blocks in the arena
+  int nwaves;  // waves of a per-particle launch: sum over simulations of ceil(N / 64)
+};
+
+// The simulation owning global particle p / global block bg (binary search
+// on the table; a batch of one returns at once).
+EMPM_DEV const DevModel& cand_of_particle(const Batch& Bt, int p) {
+  int lo = 0, hi = Bt.ncand - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (Bt.cand[mid].p0 <= p) lo = mid;
+    else hi = mid - 1;
+  }
+  return Bt.cand[lo];
+}
+EMPM_DEV const DevModel& cand_of_block(const Batch& Bt, int bg) {
+  int lo = 0, hi = Bt.ncand - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (Bt.cand[mid].b0 <= bg) lo = mid;
+    else hi = mid - 1;
+  }
+  return Bt.cand[lo];
+}
+
+// Per-particle launches give each simulation its own whole waves (wave w of
+// the launch belongs to the simulation with w0 <= w < w0 + ceil(N / 64)), so
+// the simulation of a wave is uniform: its table entry is read with scalar
+// loads.  body(M, p) runs for the thread's particle p when M.material is
+// `mat` (or mat < 0 for any material).
+template <typename F>
+EMPM_DEV void per_particle(const Batch& Bt, int mat, F body) {
+  const int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) / kWaveSize));
+  if (w >= Bt.nwaves) return;
+  int lo = 0;
+  for (int hi = Bt.ncand - 1; lo < hi;) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (Bt.cand[mid].w0 <= w) lo = mid;
+    else hi = mid - 1;
+  }
+  const DevModel& M = Bt.cand[__builtin_amdgcn_readfirstlane(lo)];
+  if (mat >= 0 && M.material != mat) return;
+  const int k = (w - M.w0) * kWaveSize + (int)(threadIdx.x % kWaveSize);
+  if (k >= M.N) return;
+  body(M, M.p0 + k);
+}
 
 struct ColTable {
   empm_collider c[EMPM_MAX_COLLIDERS];
@@ -322,9 +383,8 @@ struct Binning {
   const int* offs;
   const int* active;   // ids of the non-empty blocks, in no particular order
   const int* nactive;  // their count (device scalar)
-  const int* occ;      // nb^3: block holds particles
-  const int* near;     // nb^3: some tile covering a node of this block (blocks o-2..o+1 per axis) is non-empty
-  int nb;
+  const int* occ;      // per block: holds particles
+  const int* near;     // per block: some non-empty block's tile covers a node this block owns
 };
 
 // ---- cross-lane exchanges (wave64) ----------------------------------------
@@ -464,7 +524,7 @@ EMPM_DEV void group_scatter(float (*tile)[kTileNodes], const int* t0, bool in_gr
 // it, else into the overflow grid with global atomics.
 template <int NC>
 EMPM_DEV void lane_scatter(float (*tile)[kTileNodes], const int* o, float4* __restrict__ ovf, int* ovf_flag, int n,
-                           const ScatterPayload& P, const Stencil& s) {
+                           long long g0, const ScatterPayload& P, const Stencil& s) {
   const bool in_tile = (unsigned)(s.base[0] - o[0]) <= (unsigned)(kTileN - 3) &&
                        (unsigned)(s.base[1] - o[1]) <= (unsigned)(kTileN - 3) &&
                        (unsigned)(s.base[2] - o[2]) <= (unsigned)(kTileN - 3);
@@ -480,7 +540,7 @@ EMPM_DEV void lane_scatter(float (*tile)[kTileNodes], const int* o, float4* __re
 #pragma unroll
       for (int k = 0; k < NC; ++k) atomicAdd(&tile[k][t], out[k]);
     } else if (node_in(ix, iy, iz, n)) {
-      float* g = reinterpret_cast<float*>(&ovf[node_id(ix, iy, iz, n)]);
+      float* g = reinterpret_cast<float*>(&ovf[g0 + node_id(ix, iy, iz, n)]);
 #pragma unroll
       for (int k = 0; k < NC; ++k) atomicAdd(g + k, out[k]);
       *ovf_flag = 1;
@@ -499,16 +559,26 @@ EMPM_DEV void tile_origin(int blk, int nb, int* o) {
 // and payload.  Every active block is taken by one workgroup; its waves take
 // the block's cell chunks round robin.
 template <int NC, typename Load>
-EMPM_DEV void block_scatter(const Binning& B, const int* __restrict__ sel, float inv_dx, int n,
-                            float4* __restrict__ tiles, float4* __restrict__ ovf, int* ovf_flag, Load load) {
+EMPM_DEV void block_scatter(const Batch& Bt, const Binning& B, float4* __restrict__ tiles, float4* __restrict__ ovf,
+                            int* ovf_flag, Load load) {
   __shared__ float tile[NC][kTileNodes];
   const int lane = __lane_id();
   const int wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
   const int nact = *B.nactive;
   for (int ai = blockIdx.x; ai < nact; ai += gridDim.x) {
-    const int blk = B.active[ai];
+    const int blk = __builtin_amdgcn_readfirstlane(B.active[ai]);
+    int ci = 0;
+    for (int hi = Bt.ncand - 1; ci < hi;) {
+      const int mid = (ci + hi + 1) >> 1;
+      if (Bt.cand[mid].b0 <= blk) ci = mid;
+      else hi = mid - 1;
+    }
+    const DevModel& M = Bt.cand[__builtin_amdgcn_readfirstlane(ci)];
+    const int* __restrict__ sel = M.sel;
+    const float inv_dx = M.inv_dx;
+    const int n = M.n;
     int o[3];
-    tile_origin(blk, B.nb, o);
+    tile_origin(blk - M.b0, M.nb, o);
     for (int i = threadIdx.x; i < NC * kTileNodes; i += blockDim.x) (&tile[0][0])[i] = 0.f;
     __syncthreads();
     int cs[9];
@@ -534,12 +604,12 @@ EMPM_DEV void block_scatter(const Binning& B, const int* __restrict__ sel, float
       float xp[3] = {(o[0] + t0[0] + 1.f) * (1.f / inv_dx), (o[1] + t0[1] + 1.f) * (1.f / inv_dx),
                      (o[2] + t0[2] + 1.f) * (1.f / inv_dx)};
       ScatterPayload P = {};
-      if (active) load(p, xp, P);
+      if (active) load(p, xp, P, M);
       Stencil s;
       make_stencil(xp, inv_dx, s);
       const bool home = active && s.base[0] == o[0] + t0[0] && s.base[1] == o[1] + t0[1] && s.base[2] == o[2] + t0[2];
       if (__ballot(home)) group_scatter<NC>(tile, t0, home, P, s, lane);
-      if (active && !home) lane_scatter<NC>(tile, o, ovf, ovf_flag, n, P, s);
+      if (active && !home) lane_scatter<NC>(tile, o, ovf, ovf_flag, n, M.g0, P, s);
     }
     __syncthreads();
     float4* dst = tiles + (size_t)blk * kTileNodes;
@@ -554,10 +624,11 @@ EMPM_DEV void block_scatter(const Binning& B, const int* __restrict__ sel, float
 // lies in the tiles of blocks b = ((i - 2) >> 1) + {0, 1} per axis (a tile
 // spans nodes 2b .. 2b + 3); all 8 occupancy flags and tile entries are
 // loaded before any is used, so the gather costs one memory latency.
-EMPM_DEV float4 gather_node(const Binning& B, const float4* __restrict__ tiles, int ix, int iy, int iz) {
+EMPM_DEV float4 gather_node(const Binning& B, const DevModel& M, const float4* __restrict__ tiles, int ix, int iy,
+                            int iz) {
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  const int nb = B.nb;
-  if (!B.near[((ix >> 1) * nb + (iy >> 1)) * nb + (iz >> 1)]) return acc;
+  const int nb = M.nb;
+  if (!B.near[M.b0 + ((ix >> 1) * nb + (iy >> 1)) * nb + (iz >> 1)]) return acc;
   const int l0 = (ix - 2) >> 1, l1 = (iy - 2) >> 1, l2 = (iz - 2) >> 1;
   bool ok[8];
   int blk[8];
@@ -566,7 +637,7 @@ EMPM_DEV float4 gather_node(const Binning& B, const float4* __restrict__ tiles, 
   for (int j = 0; j < 8; ++j) {
     const int b0 = l0 + (j >> 2), b1 = l1 + ((j >> 1) & 1), b2 = l2 + (j & 1);
     ok[j] = (unsigned)b0 < (unsigned)nb && (unsigned)b1 < (unsigned)nb && (unsigned)b2 < (unsigned)nb;
-    blk[j] = ok[j] ? (b0 * nb + b1) * nb + b2 : 0;
+    blk[j] = ok[j] ? M.b0 + (b0 * nb + b1) * nb + b2 : 0;
     const int lx = ix - 2 * b0, ly = iy - 2 * b1, lz = iz - 2 * b2;
     k[j] = ok[j] ? (size_t)blk[j] * kTileNodes + (lx * kTileN + ly) * kTileN + lz : 0;
   }
@@ -597,7 +668,9 @@ EMPM_DEV float4 gather_node(const Binning& B, const float4* __restrict__ tiles, 
 // payload.
 template <int MAT>
 EMPM_DEV void particle_stress(const DevModel& M, float dt, int p, const float* Ftr, float* Fe_out,
-                              float* __restrict__ S_out, int write_params) {
+                              float* __restrict__ S_out, int allow_write) {
+  // material 5 and hardening rewrite the model arrays in place (forward only)
+  const bool write_params = allow_write && (MAT == 5 || M.mp.hardening == 1.f);
   float mu = M.mu[p], lam = M.lam[p], ys = M.ys[p];
   MatParams matp = M.mp;
   matp.material = MAT;  // compile-time material: one code path, constitutive state stays in registers
@@ -620,29 +693,30 @@ EMPM_DEV void particle_stress(const DevModel& M, float dt, int p, const float* F
 // The stress stage of the first substep of a forward call (later substeps
 // get it from the previous substep's G2P).
 template <int MAT>
-__global__ __launch_bounds__(kBlock) void k_stress(DevModel M, float dt, const float* __restrict__ F,
-                                                   float* __restrict__ Fe_out, float* __restrict__ S_out,
-                                                   int write_params) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= M.N || (M.sel && M.sel[p] != 0)) return;
+__global__ __launch_bounds__(kBlock) void k_stress(Batch Bt, float dt,
+                                                   const float* __restrict__ F, float* __restrict__ Fe_out,
+                                                   float* __restrict__ S_out, int write_params) {
+  per_particle(Bt, MAT, [&](const DevModel& M, const int p) {
+  if (M.sel && M.sel[p] != 0) return;
   float Ftr[9];
   load9(F, p, Ftr);
   particle_stress<MAT>(M, dt, p, Ftr, Fe_out, S_out, write_params);
+  });
 }
 
 // p2g_apic_with_stress (mpm_utils.py:358-423): APIC momentum, mass and the
 // stress impulse into the block tiles.
-__global__ __launch_bounds__(kBlkWG) void k_p2g(DevModel M, const float* __restrict__ x, const float* __restrict__ v,
+__global__ __launch_bounds__(kBlkWG) void k_p2g(Batch Bt, const float* __restrict__ x, const float* __restrict__ v,
                                                 const float* __restrict__ C, const float* __restrict__ S, Binning B,
                                                 float4* __restrict__ tiles, Overflow ov) {
-  const float dx = M.dx, r = M.rpic;
-  block_scatter<4>(B, M.sel, M.inv_dx, M.n, tiles, ov.grid, ov.flag, [&](int p, float* xp, ScatterPayload& P) {
+  block_scatter<4>(Bt, B, tiles, ov.grid, ov.flag, [&](int p, float* xp, ScatterPayload& P, const DevModel& M) {
+    const float dx = M.dx;
 #pragma unroll
     for (int d = 0; d < 3; ++d) xp[d] = x[3 * p + d];
     const float mp = M.mass[p];
     float Cp[9], Ct[9];
     load9(C, p, Cp);
-    rpic_blend(Cp, r, Ct);
+    rpic_blend(Cp, M.rpic, Ct);
 #pragma unroll
     for (int d = 0; d < 3; ++d) P.A[d] = mp * v[3 * p + d];
 #pragma unroll
@@ -664,24 +738,25 @@ __global__ __launch_bounds__(kBlkWG) void k_p2g(DevModel M, const float* __restr
 // pairs, so non-near blocks cost one flag load each.
 constexpr int kNodePassGridCap = 1024;
 template <typename F>
-EMPM_DEV void for_near_nodes(const DevModel& M, const Binning& B, F f) {
-  const int nb = B.nb;
-  const size_t npairs = (size_t)nb * nb * nb * 8;
+EMPM_DEV void for_near_nodes(const Batch& Bt, const Binning& B, F f) {
+  const size_t npairs = (size_t)Bt.nblk * 8;
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < npairs; q += (size_t)gridDim.x * blockDim.x) {
     const int blk = (int)(q >> 3), r = (int)(q & 7);
     if (!B.near[blk]) continue;
-    const int ix = 2 * (blk / (nb * nb)) + (r >> 2), iy = 2 * ((blk / nb) % nb) + ((r >> 1) & 1),
-              iz = 2 * (blk % nb) + (r & 1);
+    const DevModel& M = cand_of_block(Bt, blk);
+    const int nb = M.nb, lb = blk - M.b0;
+    const int ix = 2 * (lb / (nb * nb)) + (r >> 2), iy = 2 * ((lb / nb) % nb) + ((r >> 1) & 1),
+              iz = 2 * (lb % nb) + (r & 1);
     if (ix >= M.n || iy >= M.n || iz >= M.n) continue;
-    f(ix, iy, iz, node_id(ix, iy, iz, M.n));
+    f(M, ix, iy, iz, (size_t)M.g0 + node_id(ix, iy, iz, M.n));
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_grid(DevModel M, ColTable T, float time, float dt, Binning B,
+__global__ __launch_bounds__(kBlock) void k_grid(Batch Bt, ColTable T, float time, float dt, Binning B,
                                                  const float4* __restrict__ tiles, Overflow ov,
                                                  float4* __restrict__ keep, float4* __restrict__ v_out) {
-  for_near_nodes(M, B, [&](int ix, int iy, int iz, size_t id) {
-  float4 g = gather_node(B, tiles, ix, iy, iz);
+  for_near_nodes(Bt, B, [&](const DevModel& M, int ix, int iy, int iz, size_t id) {
+  float4 g = gather_node(B, M, tiles, ix, iy, iz);
   if (*ov.flag) {
     const float4 o = ov.grid[id];
     if (o.x != 0.f || o.y != 0.f || o.z != 0.f || o.w != 0.f) {
@@ -705,13 +780,12 @@ __global__ __launch_bounds__(kBlock) void k_grid(DevModel M, ColTable T, float t
 // Fe and Fe_next may be the same buffer (each particle reads its Fe before
 // writing Fe_next).
 template <int MAT>
-__global__ __launch_bounds__(kBlock) void k_g2p(DevModel M, float dt, const float* __restrict__ x, const float* Fe,
-                                                const float4* __restrict__ v_out, float* __restrict__ x1,
-                                                float* __restrict__ v1, float* __restrict__ C1,
-                                                float* __restrict__ F1, int next_stress, float* Fe_next,
-                                                float* __restrict__ S_next, int write_params) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= M.N) return;
+__global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float* __restrict__ x,
+                                                const float* Fe, const float4* __restrict__ v_out,
+                                                float* __restrict__ x1, float* __restrict__ v1,
+                                                float* __restrict__ C1, float* __restrict__ F1, int next_stress,
+                                                float* Fe_next, float* __restrict__ S_next, int write_params) {
+  per_particle(Bt, MAT, [&](const DevModel& M, const int p) {
   if (M.sel && M.sel[p] != 0) {  // partial_clone defaults (mpm_data_structure.py:356-387)
     float I[9], Z[9];
     m_eye(I);
@@ -737,7 +811,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(DevModel M, float dt, const floa
       for (int c = 0; c < 3; ++c) {
         const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
         const bool in = node_in(ix, iy, iz, M.n);
-        const float4 gv = v_out[in ? node_id(ix, iy, iz, M.n) : 0];
+        const float4 gv = v_out[M.g0 + (in ? node_id(ix, iy, iz, M.n) : 0)];
         const float gvv[3] = {in ? gv.x : 0.f, in ? gv.y : 0.f, in ? gv.z : 0.f};
         const float dpos[3] = {(float)a - s.f[0], (float)b - s.f[1], (float)c - s.f[2]};
         const float wt = s.w[0][a] * s.w[1][b] * s.w[2][c];
@@ -769,6 +843,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(DevModel M, float dt, const floa
   m_mul(IF, Fp, Fn);
   store9(F1, p, Fn);
   if (next_stress) particle_stress<MAT>(M, dt, p, Fn, Fe_next, S_next, write_params);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -777,14 +852,13 @@ __global__ __launch_bounds__(kBlock) void k_g2p(DevModel M, float dt, const floa
 // Adjoint of g2p_differentiable, per particle: dL/dx (partial), dL/dFe, and
 // the payload of the dL/dv_out scatter (A = dL/dv', B = 4 inv_dx dL/dC',
 // S = dt inv_dx dL/dF' Fe^T), stored kAdjPayload floats per particle.
-__global__ __launch_bounds__(kBlock) void k_g2p_adj(DevModel M, float dt, const float* __restrict__ x,
+__global__ __launch_bounds__(kBlock) void k_g2p_adj(Batch Bt, float dt, const float* __restrict__ x,
                                                     const float* __restrict__ Fe, const float4* __restrict__ v_out,
                                                     const float* __restrict__ dx1, const float* __restrict__ dv1,
                                                     const float* __restrict__ dC1, const float* __restrict__ dF1,
                                                     float* __restrict__ xbar, float* __restrict__ Febar,
                                                     float* __restrict__ payload) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= M.N) return;
+  per_particle(Bt, -1, [&](const DevModel& M, const int p) {
   if (M.sel && M.sel[p] != 0) {
     xbar[3 * p] = xbar[3 * p + 1] = xbar[3 * p + 2] = 0.f;
     float Z[9];
@@ -819,7 +893,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(DevModel M, float dt, const 
     const int b = j / 3, c = j % 3;
     const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
     const bool in = node_in(ix, iy, iz, M.n);
-    const float4 g4 = v_out[in ? node_id(ix, iy, iz, M.n) : 0];
+    const float4 g4 = v_out[M.g0 + (in ? node_id(ix, iy, iz, M.n) : 0)];
     const float vi[3] = {in ? g4.x : 0.f, in ? g4.y : 0.f, in ? g4.z : 0.f};
     const float wt = wa * s.w[1][b] * s.w[2][c];
     const float gw[3] = {dwa * s.w[1][b] * s.w[2][c] * inv_dx, wa * s.dw[1][b] * s.w[2][c] * inv_dx,
@@ -872,13 +946,14 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(DevModel M, float dt, const 
   q4[3] = make_float4(c4 * dC[8], inv_dx * Gb[0], inv_dx * Gb[1], inv_dx * Gb[2]);
   q4[4] = make_float4(inv_dx * Gb[3], inv_dx * Gb[4], inv_dx * Gb[5], inv_dx * Gb[6]);
   q4[5] = make_float4(inv_dx * Gb[7], inv_dx * Gb[8], 0.f, 0.f);
+  });
 }
 
 // Scatter of dL/dv_out (the transpose of the G2P gather) into the block tiles.
-__global__ __launch_bounds__(kBlkWG) void k_g2p_adj_scatter(DevModel M, const float* __restrict__ x,
+__global__ __launch_bounds__(kBlkWG) void k_g2p_adj_scatter(Batch Bt, const float* __restrict__ x,
                                                             const float* __restrict__ payload, Binning B,
                                                             float4* __restrict__ tiles, Overflow ov) {
-  block_scatter<3>(B, M.sel, M.inv_dx, M.n, tiles, ov.grid, ov.flag, [&](int p, float* xp, ScatterPayload& P) {
+  block_scatter<3>(Bt, B, tiles, ov.grid, ov.flag, [&](int p, float* xp, ScatterPayload& P, const DevModel&) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) xp[d] = x[3 * p + d];
     const float4* q4 = reinterpret_cast<const float4*>(payload + (size_t)kAdjPayload * p);
@@ -897,12 +972,12 @@ __global__ __launch_bounds__(kBlkWG) void k_g2p_adj_scatter(DevModel M, const fl
 // node's forward (normalisation, damping, colliders) to get dv_out/dv per
 // component, and write dL/d(mv, m).  The grid-mass term is dropped when
 // `drop_mass` is set (reference Tape behaviour).
-__global__ __launch_bounds__(kBlock) void k_grid_adj(DevModel M, ColTable T, float time, float dt, Binning B,
+__global__ __launch_bounds__(kBlock) void k_grid_adj(Batch Bt, ColTable T, float time, float dt, Binning B,
                                                      const float4* __restrict__ tiles, Overflow ov,
                                                      const float4* __restrict__ keep, float4* __restrict__ gadj_in,
                                                      int drop_mass) {
-  for_near_nodes(M, B, [&](int ix, int iy, int iz, size_t id) {
-  float4 b = gather_node(B, tiles, ix, iy, iz);
+  for_near_nodes(Bt, B, [&](const DevModel& M, int ix, int iy, int iz, size_t id) {
+  float4 b = gather_node(B, M, tiles, ix, iy, iz);
   if (*ov.flag) {
     const float4 ob = ov.grid[id];
     if (ob.x != 0.f || ob.y != 0.f || ob.z != 0.f) {
@@ -929,7 +1004,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_adj(DevModel M, ColTable T, flo
 }
 
 template <int MAT>
-__global__ __launch_bounds__(kBlock) void k_p2g_adj(DevModel M, float dt, const float* __restrict__ x,
+__global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const float* __restrict__ x,
                                                     const float* __restrict__ v, const float* __restrict__ C,
                                                     const float* __restrict__ F, const float4* __restrict__ gadj_in,
                                                     const float* __restrict__ xbar, const float* __restrict__ Febar_in,
@@ -938,8 +1013,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(DevModel M, float dt, const 
                                                     float* __restrict__ dmu, float* __restrict__ dlam,
                                                     float* __restrict__ dys, float* __restrict__ dmass,
                                                     float* __restrict__ dvol) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= M.N) return;
+  per_particle(Bt, MAT, [&](const DevModel& M, const int p) {
   if (M.sel && M.sel[p] != 0) {
     float Z[9];
     m_zero(Z);
@@ -979,7 +1053,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(DevModel M, float dt, const 
       for (int c = 0; c < 3; ++c) {
         const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
         if (!node_in(ix, iy, iz, M.n)) continue;
-        const float4 ga = gadj_in[node_id(ix, iy, iz, M.n)];
+        const float4 ga = gadj_in[M.g0 + node_id(ix, iy, iz, M.n)];
         const float av[3] = {ga.x, ga.y, ga.z};
         const float bm = ga.w;
         const float dpos[3] = {((float)a - s.f[0]) * M.dx, ((float)b - s.f[1]) * M.dx, ((float)c - s.f[2]) * M.dx};
@@ -1053,6 +1127,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(DevModel M, float dt, const 
   if (dys) dys[p] += ysb;
   if (dmass) dmass[p] += mb;
   if (dvol) dvol[p] += Vb;
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -1080,14 +1155,16 @@ struct BinBuf {
   int* near;  // nb^3
 };
 
-EMPM_DEV int bin_of(const float* __restrict__ x, int p, float inv_dx, int n, int nb) {
+// Global bin of particle p of simulation M: (M.b0 + block) * 8 + cell.
+EMPM_DEV int bin_of(const float* __restrict__ x, int p, const DevModel& M) {
   int c[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    int v = (int)(x[3 * p + d] * inv_dx - 0.5f);
-    c[d] = v < 0 ? 0 : (v >= n ? n - 1 : v);
+    int v = (int)(x[3 * p + d] * M.inv_dx - 0.5f);
+    c[d] = v < 0 ? 0 : (v >= M.n ? M.n - 1 : v);
   }
-  const int blk = ((c[0] >> 1) * nb + (c[1] >> 1)) * nb + (c[2] >> 1);
+  const int nb = M.nb;
+  const int blk = M.b0 + ((c[0] >> 1) * nb + (c[1] >> 1)) * nb + (c[2] >> 1);
   return blk * 8 + (((c[0] & 1) << 2) | ((c[1] & 1) << 1) | (c[2] & 1));
 }
 
@@ -1096,12 +1173,12 @@ __global__ void k_fill_i32(int* __restrict__ a, size_t n, int v) {
   if (i < n) a[i] = v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_bin_count(int N, const float* __restrict__ x, float inv_dx, int n, int nb,
-                                                      int* __restrict__ count, int2* __restrict__ key_rank) {
+__global__ __launch_bounds__(kBlock) void k_bin_count(Batch Bt, const float* __restrict__ x, int* __restrict__ count,
+                                                      int2* __restrict__ key_rank) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = __lane_id();
-  const bool active = p < N;
-  const int key = active ? bin_of(x, p, inv_dx, n, nb) : -1;
+  const bool active = p < Bt.N;
+  const int key = active ? bin_of(x, p, cand_of_particle(Bt, p)) : -1;
   unsigned long long pending = __ballot(active);
   while (pending) {
     const int leader = __ffsll((unsigned long long)pending) - 1;
@@ -1173,12 +1250,12 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_scan(int nblk, const int* __
   }
 }
 
-__global__ __launch_bounds__(kBinScanWG) void k_bin_finish(int nb, int* __restrict__ count,
+__global__ __launch_bounds__(kBinScanWG) void k_bin_finish(Batch Bt, int* __restrict__ count,
                                                            const int* __restrict__ blkoff,
                                                            const int* __restrict__ wsum, int* __restrict__ offs,
                                                            int* __restrict__ act, int* __restrict__ occ,
                                                            int* __restrict__ near) {
-  const int nblk = nb * nb * nb;
+  const int nblk = Bt.nblk;
   const int b = blockIdx.x * kBinScanWG + threadIdx.x;
   int prefix = 0;
   for (int w = 0; w < (int)blockIdx.x; ++w) prefix += wsum[w];
@@ -1203,12 +1280,15 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_finish(int nb, int* __restri
   occ[b] = full ? 1 : 0;
   if (!full) return;
   act[1 + atomicAdd(&act[0], 1)] = b;
-  // the blocks owning a node of this block's tile (nodes 2b .. 2b + 3)
-  const int b0 = b / (nb * nb), b1 = (b / nb) % nb, b2 = b % nb;
+  // the blocks of the same simulation owning a node of this block's tile
+  // (nodes 2b .. 2b + 3 per axis)
+  const DevModel& M = cand_of_block(Bt, b);
+  const int nb = M.nb, lb = b - M.b0;
+  const int b0 = lb / (nb * nb), b1 = (lb / nb) % nb, b2 = lb % nb;
 #pragma unroll
   for (int d = 0; d < 8; ++d) {
     const int o0 = b0 + (d >> 2), o1 = b1 + ((d >> 1) & 1), o2 = b2 + (d & 1);
-    if (o0 < nb && o1 < nb && o2 < nb) near[(o0 * nb + o1) * nb + o2] = 1;
+    if (o0 < nb && o1 < nb && o2 < nb) near[M.b0 + (o0 * nb + o1) * nb + o2] = 1;
   }
 }
 
@@ -1264,29 +1344,33 @@ struct KernelProfile {
 };
 
 struct empm_ctx {
-  int device;
-  int n_grid;
-  int max_particles;
-  int nb;           // blocks per axis
-  size_t G;
-  size_t nbins;     // nb^3 * 8
-  float4* tiles;    // nb^3 tiles of kTileNodes float4 (block-owned scatter)
-  float4* ovf;      // G: overflow grid for particles outside their tile (kept zero between uses)
-  int* ovf_flag;    // 1: overflow grid may be non-zero
-  float4* keep;     // G: (mv, m) of the substep being differentiated
-  float4* v_out;    // G
-  float4* gadj_in;  // G: dL/d(mv, m)
-  float* Fe;        // N*9 scratch
-  float* S;         // N*9: stress part of the P2G payload (written by k_stress / k_g2p)
-  float* payload;   // N*kAdjPayload: G2P-adjoint scatter payload
-  float* xbar;      // N*3 scratch
-  float* Febar;     // N*9 scratch
-  int* bins;        // the context's own binning record (bin_record_ints)
-  int* count;       // nbins: per-bin counts, zero between binnings
-  int2* key_rank;   // N: bin key and rank of each particle
-  int* blkoff;      // nb^3: workgroup-local exclusive block offsets
-  int* wsum;        // block-scan workgroup totals
-  KernelProfile* prof;
+  int device = 0;
+  int n_grid = 0;            // grid of a single-simulation context (0: batch context)
+  int max_cand = 0;
+  int max_particles = 0;
+  long long max_nodes = 0;   // node arena capacity
+  long long max_blocks = 0;  // block arena capacity
+  float4* tiles = nullptr;   // max_blocks tiles of kTileNodes float4 (block-owned scatter)
+  float4* ovf = nullptr;     // node arena: overflow grid for particles outside their tile (kept zero)
+  int* ovf_flag = nullptr;   // 1: overflow grid may be non-zero
+  float4* keep = nullptr;    // node arena: (mv, m) of the substep being differentiated
+  float4* v_out = nullptr;   // node arena
+  float4* gadj_in = nullptr; // node arena: dL/d(mv, m)
+  float* Fe = nullptr;       // N*9 scratch
+  float* S = nullptr;        // N*9: stress part of the P2G payload (written by k_stress / k_g2p)
+  float* payload = nullptr;  // N*kAdjPayload: G2P-adjoint scatter payload
+  float* xbar = nullptr;     // N*3 scratch
+  float* Febar = nullptr;    // N*9 scratch
+  int* bins = nullptr;       // the context's own binning record
+  int* count = nullptr;      // max_blocks * 8: per-bin counts, zero between binnings
+  int2* key_rank = nullptr;  // N: bin key and rank of each particle
+  int* blkoff = nullptr;     // max_blocks: workgroup-local exclusive block offsets
+  int* wsum = nullptr;       // block-scan workgroup totals
+  DevModel* cand_dev = nullptr;     // device table of the simulations of the current call
+  DevModel* cand_pinned = nullptr;  // its pinned staging copy
+  std::vector<DevModel> cand_last;  // what cand_dev holds
+  hipEvent_t cand_upload = nullptr; // completion of the last table upload
+  KernelProfile* prof = nullptr;
 };
 
 namespace {
@@ -1328,18 +1412,24 @@ struct ProfScope {
   }
 };
 
-int check_model(const empm_ctx* ctx, const empm_model* m) {
-  if (!ctx || !m) return set_err(EMPM_E_INVALID_ARG, "null context or model");
-  if (m->n_particles < 0 || m->n_particles > ctx->max_particles)
-    return set_err(EMPM_E_INVALID_ARG, "n_particles=%d outside [0, %d]", m->n_particles, ctx->max_particles);
-  if (m->n_grid != ctx->n_grid)
-    return set_err(EMPM_E_INVALID_ARG, "model n_grid=%d but context n_grid=%d", m->n_grid, ctx->n_grid);
-  if (m->material < 0 || m->material > 7)
-    return set_err(EMPM_E_UNSUPPORTED_MATERIAL, "Undefined material type %d", m->material);
-  if (m->n_particles > 0 && (!m->mu || !m->lam || !m->yield_stress || !m->mass || !m->vol))
-    return set_err(EMPM_E_INVALID_ARG, "null per-particle model array");
-  if (!(m->inv_dx > 0.f) || !(m->dx > 0.f)) return set_err(EMPM_E_INVALID_ARG, "dx/inv_dx must be positive");
-  return EMPM_OK;
+int blocks_per_axis(int n) { return (n + kBlkCells - 1) / kBlkCells; }
+
+// The simulations of one call on the host: the device-side table entries,
+// arena totals and, per material, the contiguous particle ranges whose
+// simulations use it (the constitutive kernels are launched per material).
+struct HostBatch {
+  std::vector<DevModel> cand;
+  int N = 0;
+  int nblk = 0;
+  long long G = 0;
+  int nwaves = 0;
+  bool has_material[8] = {};
+  Batch dev{};
+};
+
+unsigned particle_grid(const HostBatch& hb) {
+  const size_t t = (size_t)hb.nwaves * kWaveSize;
+  return (unsigned)((t + kBlock - 1) / kBlock > 0 ? (t + kBlock - 1) / kBlock : 1);
 }
 
 int check_particles(const empm_particles* s, int N, const char* what) {
@@ -1348,7 +1438,126 @@ int check_particles(const empm_particles* s, int N, const char* what) {
   return EMPM_OK;
 }
 
-int make_table(const empm_collider* cols, int ncol, ColTable& T, int n) {
+int check_scalars(const empm_model* m, int c) {
+  if (m->material < 0 || m->material > 7)
+    return set_err(EMPM_E_UNSUPPORTED_MATERIAL, "Undefined material type %d (simulation %d)", m->material, c);
+  if (m->n_grid < 4 || m->n_grid > 1024)
+    return set_err(EMPM_E_INVALID_ARG, "simulation %d: n_grid=%d outside [4, 1024]", c, m->n_grid);
+  if (m->n_particles < 0) return set_err(EMPM_E_INVALID_ARG, "simulation %d: n_particles < 0", c);
+  if (!(m->inv_dx > 0.f) || !(m->dx > 0.f))
+    return set_err(EMPM_E_INVALID_ARG, "simulation %d: dx/inv_dx must be positive", c);
+  return EMPM_OK;
+}
+
+// Validate a batch against the context and build its host description.
+int build_batch(const empm_ctx* ctx, const empm_batch* b, HostBatch& hb) {
+  if (!ctx || !b) return set_err(EMPM_E_INVALID_ARG, "null context or batch");
+  if (b->n_cand < 1 || b->n_cand > ctx->max_cand)
+    return set_err(EMPM_E_INVALID_ARG, "n_cand=%d outside [1, %d]", b->n_cand, ctx->max_cand);
+  if (!b->models) return set_err(EMPM_E_INVALID_ARG, "null simulation table");
+  hb.cand.assign((size_t)b->n_cand, DevModel{});
+  long long N = 0, nblk = 0, G = 0;
+  for (int c = 0; c < b->n_cand; ++c) {
+    const empm_model* m = &b->models[c];
+    int rc = check_scalars(m, c);
+    if (rc) return rc;
+    DevModel& d = hb.cand[(size_t)c];
+    d.N = m->n_particles;
+    d.n = m->n_grid;
+    d.material = m->material;
+    d.nb = blocks_per_axis(m->n_grid);
+    d.p0 = (int)N;
+    d.b0 = (int)nblk;
+    d.g0 = G;
+    d.grid_lim = m->grid_lim;
+    d.dx = m->dx;
+    d.inv_dx = m->inv_dx;
+    d.rpic = m->rpic_damping;
+    d.damp_scale = m->grid_v_damping_scale;
+    d.g[0] = m->gravity[0];
+    d.g[1] = m->gravity[1];
+    d.g[2] = m->gravity[2];
+    d.mp.material = m->material;
+    d.mp.alpha = m->alpha;
+    d.mp.hardening = m->hardening;
+    d.mp.xi = m->xi;
+    d.mp.plastic_viscosity = m->plastic_viscosity;
+    d.mp.softening = m->softening;
+    d.mu = b->mu;
+    d.lam = b->lam;
+    d.ys = b->yield_stress;
+    d.mass = b->mass;
+    d.vol = b->vol;
+    d.sel = b->selection;
+    N += m->n_particles;
+    nblk += (long long)d.nb * d.nb * d.nb;
+    G += (long long)m->n_grid * m->n_grid * m->n_grid;
+  }
+  if (N != b->n_particles)
+    return set_err(EMPM_E_INVALID_ARG, "n_particles=%d but the simulations hold %lld", b->n_particles, N);
+  if (N > ctx->max_particles)
+    return set_err(EMPM_E_INVALID_ARG, "%lld particles exceed the context capacity %d", N, ctx->max_particles);
+  if (G > ctx->max_nodes || nblk > ctx->max_blocks)
+    return set_err(EMPM_E_INVALID_ARG, "grids of %lld nodes exceed the context capacity %lld", G, ctx->max_nodes);
+  if (N > 0 && (!b->mu || !b->lam || !b->yield_stress || !b->mass || !b->vol))
+    return set_err(EMPM_E_INVALID_ARG, "null per-particle model array");
+  hb.N = (int)N;
+  hb.nblk = (int)nblk;
+  hb.G = G;
+  int w = 0;
+  for (int m = 0; m < 8; ++m) hb.has_material[m] = false;
+  for (DevModel& d : hb.cand) {
+    d.w0 = w;
+    w += (d.N + kWaveSize - 1) / kWaveSize;
+    if (d.N > 0) hb.has_material[d.material] = true;
+  }
+  hb.nwaves = w;
+  return EMPM_OK;
+}
+
+// Make the device table hold hb's simulations (stream-ordered; uploads only
+// when the table changed since the last call).
+int upload_batch(empm_ctx* ctx, HostBatch& hb, hipStream_t st) {
+  const size_t nc = hb.cand.size();
+  const bool same = ctx->cand_last.size() == nc && memcmp(ctx->cand_last.data(), hb.cand.data(),
+                                                             nc * sizeof(DevModel)) == 0;
+  if (!same) {
+    HIP_TRY(hipEventSynchronize(ctx->cand_upload));  // the staging copy is free again
+    memcpy(ctx->cand_pinned, hb.cand.data(), nc * sizeof(DevModel));
+    HIP_TRY(hipMemcpyAsync(ctx->cand_dev, ctx->cand_pinned, nc * sizeof(DevModel), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(ctx->cand_upload, st));
+    ctx->cand_last = hb.cand;
+  }
+  hb.dev = Batch{ctx->cand_dev, (int)nc, hb.N, hb.nblk, hb.nwaves};
+  return EMPM_OK;
+}
+
+// A single simulation as a batch of one.
+empm_batch batch_of(const empm_model* m) {
+  empm_batch b;
+  memset(&b, 0, sizeof(b));
+  b.n_cand = 1;
+  b.n_particles = m ? m->n_particles : 0;
+  b.models = m;
+  if (m) {
+    b.mu = m->mu;
+    b.lam = m->lam;
+    b.yield_stress = m->yield_stress;
+    b.mass = m->mass;
+    b.vol = m->vol;
+    b.selection = m->selection;
+  }
+  return b;
+}
+
+int prepare(empm_ctx* ctx, const empm_batch* b, HostBatch& hb, hipStream_t st) {
+  int rc = build_batch(ctx, b, hb);
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  return upload_batch(ctx, hb, st);
+}
+
+int make_table(const empm_collider* cols, int ncol, ColTable& T) {
   if (ncol < 0 || ncol > EMPM_MAX_COLLIDERS)
     return set_err(EMPM_E_INVALID_ARG, "ncol=%d outside [0, %d]", ncol, EMPM_MAX_COLLIDERS);
   if (ncol > 0 && !cols) return set_err(EMPM_E_INVALID_ARG, "null collider table");
@@ -1360,37 +1569,7 @@ int make_table(const empm_collider* cols, int ncol, ColTable& T, int n) {
     T.c[k] = cols[k];
   }
   T.n = ncol;
-  (void)n;
   return EMPM_OK;
-}
-
-DevModel dev_model(const empm_model* m) {
-  DevModel d;
-  memset(&d, 0, sizeof(d));
-  d.N = m->n_particles;
-  d.n = m->n_grid;
-  d.material = m->material;
-  d.grid_lim = m->grid_lim;
-  d.dx = m->dx;
-  d.inv_dx = m->inv_dx;
-  d.rpic = m->rpic_damping;
-  d.damp_scale = m->grid_v_damping_scale;
-  d.g[0] = m->gravity[0];
-  d.g[1] = m->gravity[1];
-  d.g[2] = m->gravity[2];
-  d.mp.material = m->material;
-  d.mp.alpha = m->alpha;
-  d.mp.hardening = m->hardening;
-  d.mp.xi = m->xi;
-  d.mp.plastic_viscosity = m->plastic_viscosity;
-  d.mp.softening = m->softening;
-  d.mu = m->mu;
-  d.lam = m->lam;
-  d.ys = m->yield_stress;
-  d.mass = m->mass;
-  d.vol = m->vol;
-  d.sel = m->selection;
-  return d;
 }
 
 // Launch KERNEL<material>(args...): the constitutive kernels are compiled
@@ -1407,63 +1586,61 @@ DevModel dev_model(const empm_model* m) {
     default: hipLaunchKernelGGL(KERNEL<7>, grid, block, 0, stream, __VA_ARGS__); break;        \
   }
 
-// Layout of one binning record: [perm N | offs nbins + 1 | act 1 + nb^3 |
-// occ nb^3 | near nb^3], each section padded to a multiple of 4 ints so that
-// the int4 accesses of the binning kernels stay 16-byte aligned.
+// KERNEL<mat>(Bt, args...) over all particle waves, once per material
+// present (waves of other materials return at once)
+#define EMPM_LAUNCH_RANGES(hb, KERNEL, stream, ...)                                              \
+  for (int mat_ = 0; mat_ < 8; ++mat_)                                                          \
+    if ((hb).has_material[mat_]) {                                                              \
+      EMPM_LAUNCH_MAT(mat_, KERNEL, dim3(particle_grid(hb)), dim3(kBlock), stream, (hb).dev,    \
+                      __VA_ARGS__);                                                             \
+    }
+
 size_t pad4(size_t k) { return (k + 3) & ~(size_t)3; }
 
-size_t bin_record_ints(int n_grid, int n_particles) {
-  const size_t nb = (size_t)(n_grid + kBlkCells - 1) / kBlkCells;
-  const size_t nblk = nb * nb * nb;
-  return pad4((size_t)n_particles) + pad4(nblk * 8 + 1) + pad4(1 + nblk) + 2 * pad4(nblk);
+// Layout of one binning record: [perm N | offs 8 nblk + 1 | act 1 + nblk |
+// occ nblk | near nblk], each section padded to a multiple of 4 ints so that
+// the int4 accesses of the binning kernels stay 16-byte aligned.
+size_t bin_record_ints(long long nblk, long long n_particles) {
+  return pad4((size_t)n_particles) + pad4((size_t)nblk * 8 + 1) + pad4(1 + (size_t)nblk) + 2 * pad4((size_t)nblk);
 }
 
-BinBuf bin_buf(int* rec, int n_grid, int n_particles) {
-  const size_t nb = (size_t)(n_grid + kBlkCells - 1) / kBlkCells;
-  const size_t nblk = nb * nb * nb;
+BinBuf bin_buf(int* rec, int nblk, int n_particles) {
   BinBuf b;
   b.perm = rec;
   b.offs = b.perm + pad4((size_t)n_particles);
-  b.act = b.offs + pad4(nblk * 8 + 1);
-  b.occ = b.act + pad4(1 + nblk);
-  b.near = b.occ + pad4(nblk);
+  b.act = b.offs + pad4((size_t)nblk * 8 + 1);
+  b.occ = b.act + pad4(1 + (size_t)nblk);
+  b.near = b.occ + pad4((size_t)nblk);
   return b;
 }
 
-Binning binning(const empm_ctx* ctx, const BinBuf& b) {
-  return Binning{b.perm, b.offs, b.act + 1, b.act, b.occ, b.near, ctx->nb};
-}
+Binning binning(const BinBuf& b) { return Binning{b.perm, b.offs, b.act + 1, b.act, b.occ, b.near}; }
 
 Overflow overflow(const empm_ctx* ctx) { return Overflow{ctx->ovf, ctx->ovf_flag}; }
 
-unsigned block_grid(const empm_ctx* ctx) {
-  const size_t nblk = (size_t)ctx->nb * ctx->nb * ctx->nb;
-  const size_t cap = (size_t)kBlkGridCap;
-  return (unsigned)(nblk < cap ? nblk : cap);
-}
-
 unsigned nblocks(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-dim3 grid_launch(const empm_ctx* ctx) {
-  const size_t pairs = (size_t)ctx->nb * ctx->nb * ctx->nb * 8;
-  const size_t wgs = (pairs + kBlock - 1) / kBlock;
-  return dim3((unsigned)(wgs < (size_t)kNodePassGridCap ? wgs : kNodePassGridCap));
+unsigned block_grid(int nblk) { return (unsigned)(nblk < kBlkGridCap ? (nblk > 0 ? nblk : 1) : kBlkGridCap); }
+
+dim3 node_grid(int nblk) {
+  const size_t wgs = ((size_t)nblk * 8 + kBlock - 1) / kBlock;
+  return dim3((unsigned)(wgs < (size_t)kNodePassGridCap ? (wgs > 0 ? wgs : 1) : kNodePassGridCap));
 }
 
 // Bin the particles of a state (positions x) into `out`.
-int do_bin(empm_ctx* ctx, int N, const float* x, float inv_dx, const BinBuf& out, hipStream_t st) {
+int do_bin(empm_ctx* ctx, const HostBatch& hb, const float* x, const BinBuf& out, hipStream_t st) {
   ProfScope ps(ctx, EMPM_KERNEL_BIN, st);
-  const int n = ctx->n_grid, nb = ctx->nb;
-  const int nblk = nb * nb * nb;
+  const int nblk = hb.nblk;
   const unsigned wgs = (unsigned)((nblk + kBinScanWG - 1) / kBinScanWG);
-  if (N > 0)
-    hipLaunchKernelGGL(k_bin_count, dim3(nblocks(N)), dim3(kBlock), 0, st, N, x, inv_dx, n, nb, ctx->count,
-                       ctx->key_rank);
+  if (hb.N > 0)
+    hipLaunchKernelGGL(k_bin_count, dim3(nblocks(hb.N)), dim3(kBlock), 0, st, hb.dev, x, ctx->count, ctx->key_rank);
   hipLaunchKernelGGL(k_bin_scan, dim3(wgs), dim3(kBinScanWG), 0, st, nblk, ctx->count, ctx->blkoff, ctx->wsum,
                      out.near, out.act, ctx->ovf_flag);
-  hipLaunchKernelGGL(k_bin_finish, dim3(wgs), dim3(kBinScanWG), 0, st, nb, ctx->count, ctx->blkoff, ctx->wsum,
+  hipLaunchKernelGGL(k_bin_finish, dim3(wgs), dim3(kBinScanWG), 0, st, hb.dev, ctx->count, ctx->blkoff, ctx->wsum,
                      out.offs, out.act, out.occ, out.near);
-  if (N > 0) hipLaunchKernelGGL(k_bin_place, dim3(nblocks(N)), dim3(kBlock), 0, st, N, ctx->key_rank, out.offs, out.perm);
+  if (hb.N > 0)
+    hipLaunchKernelGGL(k_bin_place, dim3(nblocks(hb.N)), dim3(kBlock), 0, st, hb.N, ctx->key_rank, out.offs,
+                       out.perm);
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }
@@ -1471,36 +1648,34 @@ int do_bin(empm_ctx* ctx, int N, const float* x, float inv_dx, const BinBuf& out
 // One forward substep in -> out, binning `in` first (into the record's bins
 // when given).  `first`: run the stress stage here (else the previous
 // substep's G2P did); `last`: no stress stage for a next substep.
-int launch_fwd(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, float dt, const empm_particles* in,
+int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time, float dt, const empm_particles* in,
                const empm_particles* out, const empm_grid_rec* rec, bool first, bool last, hipStream_t st) {
-  const int write_params = (M.material == 5 || M.mp.hardening == 1.f) ? 1 : 0;
   float* Fe = in->Fe ? in->Fe : ctx->Fe;
   float* Fe_next = out->Fe ? out->Fe : ctx->Fe;
   float4* keep = rec ? reinterpret_cast<float4*>(rec->mv_m) : nullptr;
   float4* v_out = rec && rec->v_out ? reinterpret_cast<float4*>(rec->v_out) : ctx->v_out;
-  const BinBuf bb = bin_buf(rec && rec->bins ? rec->bins : ctx->bins, ctx->n_grid, M.N);
-  int rc = do_bin(ctx, M.N, in->x, M.inv_dx, bb, st);
+  const BinBuf bb = bin_buf(rec && rec->bins ? rec->bins : ctx->bins, hb.nblk, hb.N);
+  int rc = do_bin(ctx, hb, in->x, bb, st);
   if (rc) return rc;
-  const Binning B = binning(ctx, bb);
-  if (M.N > 0 && first) {
+  const Binning B = binning(bb);
+  if (hb.N > 0 && first) {
     ProfScope ps(ctx, EMPM_KERNEL_STRESS, st);
-    EMPM_LAUNCH_MAT(M.material, k_stress, dim3(nblocks(M.N)), dim3(kBlock), st, M, dt, in->F, Fe, ctx->S,
-                    write_params);
+    EMPM_LAUNCH_RANGES(hb, k_stress, st, dt, in->F, Fe, ctx->S, 1);
   }
-  if (M.N > 0) {
+  if (hb.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
-    hipLaunchKernelGGL(k_p2g, dim3(block_grid(ctx)), dim3(kBlkWG), 0, st, M, in->x, in->v, in->C, ctx->S, B,
+    hipLaunchKernelGGL(k_p2g, dim3(block_grid(hb.nblk)), dim3(kBlkWG), 0, st, hb.dev, in->x, in->v, in->C, ctx->S, B,
                        ctx->tiles, overflow(ctx));
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
-    hipLaunchKernelGGL(k_grid, grid_launch(ctx), dim3(kBlock), 0, st, M, T, time, dt, B, ctx->tiles,
+    hipLaunchKernelGGL(k_grid, node_grid(hb.nblk), dim3(kBlock), 0, st, hb.dev, T, time, dt, B, ctx->tiles,
                        overflow(ctx), keep, v_out);
   }
-  if (M.N > 0) {
+  if (hb.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_G2P, st);
-    EMPM_LAUNCH_MAT(M.material, k_g2p, dim3(nblocks(M.N)), dim3(kBlock), st, M, dt, in->x, Fe, v_out, out->x, out->v,
-                    out->C, out->F, last ? 0 : 1, Fe_next, ctx->S, write_params);
+    EMPM_LAUNCH_RANGES(hb, k_g2p, st, dt, in->x, Fe, v_out, out->x, out->v, out->C, out->F, last ? 0 : 1, Fe_next,
+                       ctx->S, 1);
   }
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
@@ -1509,7 +1684,7 @@ int launch_fwd(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, 
 // Adjoint of one substep.  With a recorded grid and Fe (rec->mv_m, v_out and
 // in->Fe) the forward stress, P2G and boundary pass are not recomputed; with
 // recorded bins (rec->bins) the particles are not re-binned.
-int launch_adj(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, float dt, const empm_particles* in,
+int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time, float dt, const empm_particles* in,
                const empm_grid_rec* rec, const empm_particles* d_out, const empm_particles* d_in, float* d_mu,
                float* d_lam, float* d_ys, float* d_mass, float* d_vol, int flags, hipStream_t st) {
   const bool recorded = rec && rec->mv_m && rec->v_out && in->Fe;
@@ -1517,62 +1692,64 @@ int launch_adj(empm_ctx* ctx, const DevModel& M, const ColTable& T, float time, 
   float* Fe = recorded ? in->Fe : ctx->Fe;
   float4* keep = recorded ? reinterpret_cast<float4*>(rec->mv_m) : ctx->keep;
   float4* v_out = recorded ? reinterpret_cast<float4*>(rec->v_out) : ctx->v_out;
-  const BinBuf bb = bin_buf(have_bins ? rec->bins : ctx->bins, ctx->n_grid, M.N);
+  const BinBuf bb = bin_buf(have_bins ? rec->bins : ctx->bins, hb.nblk, hb.N);
   if (!have_bins) {
-    int rc = do_bin(ctx, M.N, in->x, M.inv_dx, bb, st);
+    int rc = do_bin(ctx, hb, in->x, bb, st);
     if (rc) return rc;
   }
-  const Binning B = binning(ctx, bb);
+  const Binning B = binning(bb);
   if (!recorded) {
-    if (M.N > 0) {
+    if (hb.N > 0) {
       ProfScope ps(ctx, EMPM_KERNEL_STRESS, st);
-      EMPM_LAUNCH_MAT(M.material, k_stress, dim3(nblocks(M.N)), dim3(kBlock), st, M, dt, in->F, Fe, ctx->S, 0);
+      EMPM_LAUNCH_RANGES(hb, k_stress, st, dt, in->F, Fe, ctx->S, 0);
     }
-    if (M.N > 0) {
+    if (hb.N > 0) {
       ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
-      hipLaunchKernelGGL(k_p2g, dim3(block_grid(ctx)), dim3(kBlkWG), 0, st, M, in->x, in->v, in->C, ctx->S, B,
-                         ctx->tiles, overflow(ctx));
+      hipLaunchKernelGGL(k_p2g, dim3(block_grid(hb.nblk)), dim3(kBlkWG), 0, st, hb.dev, in->x, in->v, in->C, ctx->S,
+                         B, ctx->tiles, overflow(ctx));
     }
     {
       ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
-      hipLaunchKernelGGL(k_grid, grid_launch(ctx), dim3(kBlock), 0, st, M, T, time, dt, B, ctx->tiles,
+      hipLaunchKernelGGL(k_grid, node_grid(hb.nblk), dim3(kBlock), 0, st, hb.dev, T, time, dt, B, ctx->tiles,
                          overflow(ctx), keep, v_out);
     }
   }
-  if (M.N > 0) {
+  if (hb.N > 0) {
     {
       ProfScope ps(ctx, EMPM_KERNEL_G2P_ADJ, st);
-      hipLaunchKernelGGL(k_g2p_adj, dim3(nblocks(M.N)), dim3(kBlock), 0, st, M, dt, in->x, Fe, v_out, d_out->x,
-                         d_out->v, d_out->C, d_out->F, ctx->xbar, ctx->Febar, ctx->payload);
+      hipLaunchKernelGGL(k_g2p_adj, dim3(particle_grid(hb)), dim3(kBlock), 0, st, hb.dev, dt, in->x, Fe, v_out,
+                         d_out->x, d_out->v, d_out->C, d_out->F, ctx->xbar, ctx->Febar, ctx->payload);
     }
     {
       ProfScope ps(ctx, EMPM_KERNEL_G2P_ADJ_SCATTER, st);
-      hipLaunchKernelGGL(k_g2p_adj_scatter, dim3(block_grid(ctx)), dim3(kBlkWG), 0, st, M, in->x, ctx->payload, B,
-                         ctx->tiles, overflow(ctx));
+      hipLaunchKernelGGL(k_g2p_adj_scatter, dim3(block_grid(hb.nblk)), dim3(kBlkWG), 0, st, hb.dev, in->x,
+                         ctx->payload, B, ctx->tiles, overflow(ctx));
     }
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID_ADJ, st);
-    hipLaunchKernelGGL(k_grid_adj, grid_launch(ctx), dim3(kBlock), 0, st, M, T, time, dt, B, ctx->tiles,
+    hipLaunchKernelGGL(k_grid_adj, node_grid(hb.nblk), dim3(kBlock), 0, st, hb.dev, T, time, dt, B, ctx->tiles,
                        overflow(ctx), keep, ctx->gadj_in, (flags & EMPM_ADJ_DROP_GRID_MASS) ? 1 : 0);
   }
-  if (M.N > 0) {
+  if (hb.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G_ADJ, st);
-    EMPM_LAUNCH_MAT(M.material, k_p2g_adj, dim3(nblocks(M.N)), dim3(kBlock), st, M, dt, in->x, in->v, in->C, in->F,
-                    ctx->gadj_in, ctx->xbar, ctx->Febar, d_in->x, d_in->v, d_in->C, d_in->F, d_mu, d_lam, d_ys, d_mass,
-                    d_vol);
+    EMPM_LAUNCH_RANGES(hb, k_p2g_adj, st, dt, in->x, in->v, in->C, in->F, ctx->gadj_in, ctx->xbar, ctx->Febar,
+                       d_in->x, d_in->v, d_in->C, d_in->F, d_mu, d_lam, d_ys, d_mass, d_vol);
   }
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }
 
-int check_adj_supported(const empm_model* m) {
-  if (m->material == 5)
-    return set_err(EMPM_E_UNSUPPORTED,
-                   "adjoint of material 5 (plasticine) is not supported: it rewrites yield stress, mu and lam "
-                   "inside the substep");
-  if (m->hardening == 1.f && (m->material == 1 || m->material == 7))
-    return set_err(EMPM_E_UNSUPPORTED, "adjoint with hardening == 1 is not supported (yield stress is state)");
+int check_adj_supported(const HostBatch& hb) {
+  for (size_t c = 0; c < hb.cand.size(); ++c) {
+    const DevModel& d = hb.cand[c];
+    if (d.material == 5)
+      return set_err(EMPM_E_UNSUPPORTED,
+                     "adjoint of material 5 (plasticine) is not supported: it rewrites yield stress, mu and lam "
+                     "inside the substep");
+    if (d.mp.hardening == 1.f && (d.material == 1 || d.material == 7))
+      return set_err(EMPM_E_UNSUPPORTED, "adjoint with hardening == 1 is not supported (yield stress is state)");
+  }
   return EMPM_OK;
 }
 
@@ -1583,6 +1760,123 @@ void advance_colliders(empm_collider* cols, int ncol, double t, double dt) {
     if (t >= (double)c.start_time && t < (double)c.end_time)
       for (int d = 0; d < 3; ++d) c.point[d] = (float)((double)c.point[d] + dt * (double)c.velocity[d]);
   }
+}
+
+// Allocate every buffer of a context for the given capacities.
+int alloc_ctx(empm_ctx* c) {
+  const size_t np = c->max_particles > 0 ? (size_t)c->max_particles : 1;
+  const size_t nblk = (size_t)c->max_blocks, G = (size_t)c->max_nodes;
+  const size_t nwsum = (nblk + kBinScanWG - 1) / kBinScanWG;
+  hipError_t e = hipSuccess;
+  e = e == hipSuccess ? hipMalloc((void**)&c->tiles, nblk * kTileNodes * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->ovf, G * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->ovf_flag, sizeof(int)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->keep, G * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->v_out, G * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->gadj_in, G * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->Fe, np * 9 * sizeof(float)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->S, np * 9 * sizeof(float)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->payload, np * kAdjPayload * sizeof(float)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->xbar, np * 3 * sizeof(float)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->Febar, np * 9 * sizeof(float)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->bins, bin_record_ints((long long)nblk, (long long)np) * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->count, nblk * 8 * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->key_rank, np * sizeof(int2)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->blkoff, nblk * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->wsum, nwsum * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->cand_dev, (size_t)c->max_cand * sizeof(DevModel)) : e;
+  e = e == hipSuccess ? hipHostMalloc((void**)&c->cand_pinned, (size_t)c->max_cand * sizeof(DevModel)) : e;
+  e = e == hipSuccess ? hipEventCreateWithFlags(&c->cand_upload, hipEventDisableTiming) : e;
+  e = e == hipSuccess ? hipMemset(c->count, 0, nblk * 8 * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMemset(c->ovf, 0, G * sizeof(float4)) : e;
+  e = e == hipSuccess ? hipMemset(c->ovf_flag, 0, sizeof(int)) : e;
+  e = e == hipSuccess ? hipDeviceSynchronize() : e;
+  if (e != hipSuccess)
+    return set_err(e == hipErrorOutOfMemory ? EMPM_E_OOM : EMPM_E_HIP, "empm_create: %s", hipGetErrorString(e));
+  return EMPM_OK;
+}
+
+// Forward rollout of a batch over n_steps substeps (see empm_rollout_fwd).
+int rollout_fwd(empm_ctx* ctx, const empm_batch* batch, empm_collider* cols, int ncol, double* time, double dt,
+                int n_steps, const empm_particles* states, const empm_grid_rec* grids, empm_collider* cols_hist,
+                double* times_hist, hipStream_t st) {
+  HostBatch hb;
+  int rc = build_batch(ctx, batch, hb);
+  if (rc) return rc;
+  if (!time || !states || n_steps < 0) return set_err(EMPM_E_INVALID_ARG, "empm_rollout_fwd: bad arguments");
+  for (int k = 0; k <= n_steps; ++k)
+    if ((rc = check_particles(&states[k], hb.N, "states[k]"))) return rc;
+  for (int k = 0; k < n_steps; ++k)
+    if (hb.N > 0 && states[k].x == states[k + 1].x)
+      return set_err(EMPM_E_INVALID_ARG, "states[%d] and states[%d] alias", k, k + 1);
+  ColTable T;
+  if ((rc = make_table(cols, ncol, T))) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if ((rc = upload_batch(ctx, hb, st))) return rc;
+  for (int k = 0; k < n_steps; ++k) {
+    make_table(cols, ncol, T);
+    if (cols_hist) memcpy(&cols_hist[(size_t)k * ncol], cols, sizeof(empm_collider) * ncol);
+    if (times_hist) times_hist[k] = *time;
+    if ((rc = launch_fwd(ctx, hb, T, (float)*time, (float)dt, &states[k], &states[k + 1], grids ? &grids[k] : nullptr,
+                         k == 0, k == n_steps - 1, st)))
+      return rc;
+    advance_colliders(cols, ncol, *time, dt);
+    *time = *time + dt;
+  }
+  return EMPM_OK;
+}
+
+int rollout_fwd_replay(empm_ctx* ctx, const empm_batch* batch, const empm_collider* cols_hist, int ncol,
+                       const double* times_hist, double dt, int n_steps, const empm_particles* states,
+                       const empm_grid_rec* grids, hipStream_t st) {
+  HostBatch hb;
+  int rc = build_batch(ctx, batch, hb);
+  if (rc) return rc;
+  if (!states || !times_hist || n_steps < 0 || (ncol > 0 && !cols_hist))
+    return set_err(EMPM_E_INVALID_ARG, "empm_rollout_fwd_replay: bad arguments");
+  for (int k = 0; k <= n_steps; ++k)
+    if ((rc = check_particles(&states[k], hb.N, "states[k]"))) return rc;
+  for (int k = 0; k < n_steps; ++k)
+    if (hb.N > 0 && states[k].x == states[k + 1].x)
+      return set_err(EMPM_E_INVALID_ARG, "states[%d] and states[%d] alias", k, k + 1);
+  HIP_TRY(hipSetDevice(ctx->device));
+  if ((rc = upload_batch(ctx, hb, st))) return rc;
+  ColTable T;
+  for (int k = 0; k < n_steps; ++k) {
+    if ((rc = make_table(&cols_hist[(size_t)k * ncol], ncol, T))) return rc;
+    if ((rc = launch_fwd(ctx, hb, T, (float)times_hist[k], (float)dt, &states[k], &states[k + 1],
+                         grids ? &grids[k] : nullptr, k == 0, k == n_steps - 1, st)))
+      return rc;
+  }
+  return EMPM_OK;
+}
+
+int rollout_adj(empm_ctx* ctx, const empm_batch* batch, const empm_collider* cols_hist, int ncol,
+                const double* times_hist, double dt, int n_steps, const empm_particles* states,
+                const empm_grid_rec* grids, const empm_particles* adj, float* d_mu, float* d_lam, float* d_ys,
+                float* d_mass, float* d_vol, int flags, hipStream_t st) {
+  HostBatch hb;
+  int rc = build_batch(ctx, batch, hb);
+  if (rc) return rc;
+  if ((rc = check_adj_supported(hb))) return rc;
+  if (!states || !adj || !times_hist || n_steps < 0 || (ncol > 0 && !cols_hist))
+    return set_err(EMPM_E_INVALID_ARG, "empm_rollout_adj: bad arguments");
+  for (int k = 0; k < n_steps; ++k)
+    if ((rc = check_particles(&states[k], hb.N, "states[k]"))) return rc;
+  if ((rc = check_particles(&adj[0], hb.N, "adj[0]")) || (rc = check_particles(&adj[1], hb.N, "adj[1]"))) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if ((rc = upload_batch(ctx, hb, st))) return rc;
+  ColTable T;
+  for (int j = 0; j < n_steps; ++j) {
+    const int k = n_steps - 1 - j;
+    if ((rc = make_table(&cols_hist[(size_t)k * ncol], ncol, T))) return rc;
+    const empm_particles* dout = &adj[j % 2];
+    const empm_particles* din = &adj[(j + 1) % 2];
+    if ((rc = launch_adj(ctx, hb, T, (float)times_hist[k], (float)dt, &states[k], grids ? &grids[k] : nullptr, dout,
+                         din, d_mu, d_lam, d_ys, d_mass, d_vol, flags, st)))
+      return rc;
+  }
+  return EMPM_OK;
 }
 
 }  // namespace
@@ -1601,40 +1895,46 @@ int empm_create(empm_ctx** out, const empm_create_desc* desc) {
   if (desc->max_particles < 0) return set_err(EMPM_E_INVALID_ARG, "max_particles < 0");
   HIP_TRY(hipSetDevice(desc->device));
   empm_ctx* c = new empm_ctx();
-  memset(c, 0, sizeof(*c));
   c->device = desc->device;
   c->n_grid = desc->n_grid;
+  c->max_cand = 1;
   c->max_particles = desc->max_particles;
-  c->nb = (desc->n_grid + kBlkCells - 1) / kBlkCells;
-  c->G = (size_t)desc->n_grid * desc->n_grid * desc->n_grid;
-  c->nbins = (size_t)c->nb * c->nb * c->nb * 8;
-  const size_t np = desc->max_particles > 0 ? (size_t)desc->max_particles : 1;
-  const size_t ntiles = (size_t)c->nb * c->nb * c->nb;
-  const size_t nwsum = (ntiles + kBinScanWG - 1) / kBinScanWG;
-  hipError_t e = hipSuccess;
-  e = e == hipSuccess ? hipMalloc((void**)&c->tiles, ntiles * kTileNodes * sizeof(float4)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->ovf, c->G * sizeof(float4)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->ovf_flag, sizeof(int)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->keep, c->G * sizeof(float4)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->v_out, c->G * sizeof(float4)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->gadj_in, c->G * sizeof(float4)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->Fe, np * 9 * sizeof(float)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->S, np * 9 * sizeof(float)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->payload, np * kAdjPayload * sizeof(float)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->xbar, np * 3 * sizeof(float)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->Febar, np * 9 * sizeof(float)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->bins, bin_record_ints(c->n_grid, (int)np) * sizeof(int)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->count, c->nbins * sizeof(int)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->key_rank, np * sizeof(int2)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->blkoff, ntiles * sizeof(int)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->wsum, nwsum * sizeof(int)) : e;
-  e = e == hipSuccess ? hipMemset(c->count, 0, c->nbins * sizeof(int)) : e;
-  e = e == hipSuccess ? hipMemset(c->ovf, 0, c->G * sizeof(float4)) : e;
-  e = e == hipSuccess ? hipMemset(c->ovf_flag, 0, sizeof(int)) : e;
-  e = e == hipSuccess ? hipDeviceSynchronize() : e;
-  if (e != hipSuccess) {
+  const long long nb = blocks_per_axis(desc->n_grid);
+  c->max_nodes = (long long)desc->n_grid * desc->n_grid * desc->n_grid;
+  c->max_blocks = nb * nb * nb;
+  int rc = alloc_ctx(c);
+  if (rc) {
     empm_destroy(c);
-    return set_err(e == hipErrorOutOfMemory ? EMPM_E_OOM : EMPM_E_HIP, "empm_create: %s", hipGetErrorString(e));
+    return rc;
+  }
+  *out = c;
+  return EMPM_OK;
+}
+
+int empm_create_batch(empm_ctx** out, const empm_batch_create_desc* desc) {
+  if (!out || !desc) return set_err(EMPM_E_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (desc->max_grid < 4 || desc->max_grid > 1024)
+    return set_err(EMPM_E_INVALID_ARG, "max_grid=%d outside [4, 1024]", desc->max_grid);
+  if (desc->max_cand < 1 || desc->max_particles < 0)
+    return set_err(EMPM_E_INVALID_ARG, "max_cand must be >= 1 and max_particles >= 0");
+  const long long nb = blocks_per_axis(desc->max_grid);
+  const long long nodes = (long long)desc->max_cand * desc->max_grid * desc->max_grid * desc->max_grid;
+  const long long blocks = (long long)desc->max_cand * nb * nb * nb;
+  if (blocks * 8 + 1 > INT32_MAX || nodes > ((long long)1 << 40))
+    return set_err(EMPM_E_INVALID_ARG, "batch capacity too large (%lld blocks)", blocks);
+  HIP_TRY(hipSetDevice(desc->device));
+  empm_ctx* c = new empm_ctx();
+  c->device = desc->device;
+  c->n_grid = 0;
+  c->max_cand = desc->max_cand;
+  c->max_particles = desc->max_particles;
+  c->max_nodes = nodes;
+  c->max_blocks = blocks;
+  int rc = alloc_ctx(c);
+  if (rc) {
+    empm_destroy(c);
+    return rc;
   }
   *out = c;
   return EMPM_OK;
@@ -1642,53 +1942,97 @@ int empm_create(empm_ctx** out, const empm_create_desc* desc) {
 
 void empm_destroy(empm_ctx* c) {
   if (!c) return;
-  hipSetDevice(c->device);
-  hipDeviceSynchronize();
+  (void)hipSetDevice(c->device);
+  (void)hipDeviceSynchronize();
   if (c->prof) {
     for (int k = 0; k < EMPM_NUM_KERNELS; ++k)
       for (auto& pr : c->prof->pending[k]) {
-        hipEventDestroy(pr.first);
-        hipEventDestroy(pr.second);
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
       }
-    for (hipEvent_t e : c->prof->pool) hipEventDestroy(e);
+    for (hipEvent_t e : c->prof->pool) (void)hipEventDestroy(e);
     delete c->prof;
   }
-  hipFree(c->tiles);
-  hipFree(c->ovf);
-  hipFree(c->keep);
-  hipFree(c->v_out);
-  hipFree(c->gadj_in);
-  hipFree(c->Fe);
-  hipFree(c->S);
-  hipFree(c->payload);
-  hipFree(c->xbar);
-  hipFree(c->Febar);
-  hipFree(c->ovf_flag);
-  hipFree(c->bins);
-  hipFree(c->count);
-  hipFree(c->key_rank);
-  hipFree(c->blkoff);
-  hipFree(c->wsum);
+  (void)hipFree(c->tiles);
+  (void)hipFree(c->ovf);
+  (void)hipFree(c->ovf_flag);
+  (void)hipFree(c->keep);
+  (void)hipFree(c->v_out);
+  (void)hipFree(c->gadj_in);
+  (void)hipFree(c->Fe);
+  (void)hipFree(c->S);
+  (void)hipFree(c->payload);
+  (void)hipFree(c->xbar);
+  (void)hipFree(c->Febar);
+  (void)hipFree(c->bins);
+  (void)hipFree(c->count);
+  (void)hipFree(c->key_rank);
+  (void)hipFree(c->blkoff);
+  (void)hipFree(c->wsum);
+  (void)hipFree(c->cand_dev);
+  if (c->cand_pinned) (void)hipHostFree(c->cand_pinned);
+  if (c->cand_upload) (void)hipEventDestroy(c->cand_upload);
   delete c;
 }
 
 int empm_sort_by_cell(empm_ctx* ctx, int n_particles, const float* x, float inv_dx, int* perm, void* stream) {
-  if (!ctx || n_particles < 0 || n_particles > ctx->max_particles || (n_particles > 0 && (!x || !perm)) ||
-      !(inv_dx > 0.f))
+  if (!ctx || ctx->n_grid == 0 || n_particles < 0 || n_particles > ctx->max_particles ||
+      (n_particles > 0 && (!x || !perm)) || !(inv_dx > 0.f))
     return set_err(EMPM_E_INVALID_ARG, "empm_sort_by_cell: bad arguments");
   if (n_particles == 0) return EMPM_OK;
+  empm_model m;
+  memset(&m, 0, sizeof(m));
+  m.n_particles = n_particles;
+  m.n_grid = ctx->n_grid;
+  m.inv_dx = inv_dx;
+  m.dx = 1.f / inv_dx;
+  m.grid_lim = (float)ctx->n_grid / inv_dx;
+  empm_batch b = batch_of(&m);
+  HostBatch hb;
+  hipStream_t st = (hipStream_t)stream;
+  // positions only: the per-particle model arrays are not read by the binning
+  hb.cand.clear();
+  DevModel d{};
+  d.N = n_particles;
+  d.n = ctx->n_grid;
+  d.nb = blocks_per_axis(ctx->n_grid);
+  d.inv_dx = inv_dx;
+  d.dx = m.dx;
+  hb.cand.push_back(d);
+  hb.N = n_particles;
+  hb.nblk = d.nb * d.nb * d.nb;
+  hb.nwaves = (n_particles + kWaveSize - 1) / kWaveSize;
+  (void)b;
   HIP_TRY(hipSetDevice(ctx->device));
-  const BinBuf bb = bin_buf(ctx->bins, ctx->n_grid, n_particles);
-  int rc = do_bin(ctx, n_particles, x, inv_dx, bb, (hipStream_t)stream);
+  int rc = upload_batch(ctx, hb, st);
   if (rc) return rc;
-  HIP_TRY(hipMemcpyAsync(perm, bb.perm, (size_t)n_particles * sizeof(int), hipMemcpyDeviceToDevice,
-                         (hipStream_t)stream));
+  const BinBuf bb = bin_buf(ctx->bins, hb.nblk, n_particles);
+  if ((rc = do_bin(ctx, hb, x, bb, st))) return rc;
+  HIP_TRY(hipMemcpyAsync(perm, bb.perm, (size_t)n_particles * sizeof(int), hipMemcpyDeviceToDevice, st));
   return EMPM_OK;
 }
 
 long long empm_bin_record_ints(int n_grid, int n_particles) {
   if (n_grid < 4 || n_grid > 1024 || n_particles < 0) return -1;
-  return (long long)bin_record_ints(n_grid, n_particles);
+  const long long nb = blocks_per_axis(n_grid);
+  return (long long)bin_record_ints(nb * nb * nb, n_particles);
+}
+
+int empm_batch_sizes(const empm_batch* batch, long long* n_nodes, long long* bin_record_ints_out) {
+  if (!batch || !batch->models || batch->n_cand < 1) return set_err(EMPM_E_INVALID_ARG, "empm_batch_sizes: bad batch");
+  long long G = 0, nblk = 0, N = 0;
+  for (int c = 0; c < batch->n_cand; ++c) {
+    const empm_model* m = &batch->models[c];
+    int rc = check_scalars(m, c);
+    if (rc) return rc;
+    const long long nb = blocks_per_axis(m->n_grid);
+    G += (long long)m->n_grid * m->n_grid * m->n_grid;
+    nblk += nb * nb * nb;
+    N += m->n_particles;
+  }
+  if (n_nodes) *n_nodes = G;
+  if (bin_record_ints_out) *bin_record_ints_out = (long long)bin_record_ints(nblk, N);
+  return EMPM_OK;
 }
 
 int empm_profile_enable(empm_ctx* ctx, unsigned kernel_mask) {
@@ -1767,114 +2111,85 @@ int empm_mass_from_density(int n, const float* density, const float* vol, float*
 
 int empm_substep_fwd(empm_ctx* ctx, const empm_model* model, const empm_collider* cols, int ncol, float time, float dt,
                      const empm_particles* in, const empm_particles* out, void* stream) {
-  int rc = check_model(ctx, model);
+  if (!model) return set_err(EMPM_E_INVALID_ARG, "null model");
+  const empm_batch b = batch_of(model);
+  HostBatch hb;
+  hipStream_t st = (hipStream_t)stream;
+  int rc = prepare(ctx, &b, hb, st);
   if (rc) return rc;
-  if ((rc = check_particles(in, model->n_particles, "in")) || (rc = check_particles(out, model->n_particles, "out")))
-    return rc;
-  if (model->n_particles > 0 && (in->x == out->x || in->F == out->F))
-    return set_err(EMPM_E_INVALID_ARG, "in and out states alias");
+  if ((rc = check_particles(in, hb.N, "in")) || (rc = check_particles(out, hb.N, "out"))) return rc;
+  if (hb.N > 0 && (in->x == out->x || in->F == out->F)) return set_err(EMPM_E_INVALID_ARG, "in and out states alias");
   ColTable T;
-  if ((rc = make_table(cols, ncol, T, model->n_grid))) return rc;
-  HIP_TRY(hipSetDevice(ctx->device));
-  return launch_fwd(ctx, dev_model(model), T, time, dt, in, out, nullptr, true, true, (hipStream_t)stream);
+  if ((rc = make_table(cols, ncol, T))) return rc;
+  return launch_fwd(ctx, hb, T, time, dt, in, out, nullptr, true, true, st);
 }
 
 int empm_substep_adj(empm_ctx* ctx, const empm_model* model, const empm_collider* cols, int ncol, float time, float dt,
                      const empm_particles* in, const empm_particles* d_out, const empm_particles* d_in, float* d_mu,
                      float* d_lam, float* d_ys, float* d_mass, float* d_vol, int flags, void* stream) {
-  int rc = check_model(ctx, model);
+  if (!model) return set_err(EMPM_E_INVALID_ARG, "null model");
+  const empm_batch b = batch_of(model);
+  HostBatch hb;
+  hipStream_t st = (hipStream_t)stream;
+  int rc = build_batch(ctx, &b, hb);
   if (rc) return rc;
-  if ((rc = check_adj_supported(model))) return rc;
-  if ((rc = check_particles(in, model->n_particles, "in")) ||
-      (rc = check_particles(d_out, model->n_particles, "d_out")) ||
-      (rc = check_particles(d_in, model->n_particles, "d_in")))
+  if ((rc = check_adj_supported(hb))) return rc;
+  if ((rc = check_particles(in, hb.N, "in")) || (rc = check_particles(d_out, hb.N, "d_out")) ||
+      (rc = check_particles(d_in, hb.N, "d_in")))
     return rc;
   ColTable T;
-  if ((rc = make_table(cols, ncol, T, model->n_grid))) return rc;
+  if ((rc = make_table(cols, ncol, T))) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
-  return launch_adj(ctx, dev_model(model), T, time, dt, in, nullptr, d_out, d_in, d_mu, d_lam, d_ys, d_mass, d_vol,
-                    flags, (hipStream_t)stream);
+  if ((rc = upload_batch(ctx, hb, st))) return rc;
+  return launch_adj(ctx, hb, T, time, dt, in, nullptr, d_out, d_in, d_mu, d_lam, d_ys, d_mass, d_vol, flags, st);
 }
 
 int empm_rollout_fwd(empm_ctx* ctx, const empm_model* model, empm_collider* cols, int ncol, double* time, double dt,
                      int n_steps, const empm_particles* states, const empm_grid_rec* grids, empm_collider* cols_hist,
                      double* times_hist, void* stream) {
-  int rc = check_model(ctx, model);
-  if (rc) return rc;
-  if (!time || !states || n_steps < 0) return set_err(EMPM_E_INVALID_ARG, "empm_rollout_fwd: bad arguments");
-  for (int k = 0; k <= n_steps; ++k)
-    if ((rc = check_particles(&states[k], model->n_particles, "states[k]"))) return rc;
-  for (int k = 0; k < n_steps; ++k)
-    if (model->n_particles > 0 && states[k].x == states[k + 1].x)
-      return set_err(EMPM_E_INVALID_ARG, "states[%d] and states[%d] alias", k, k + 1);
-  ColTable T;
-  if ((rc = make_table(cols, ncol, T, model->n_grid))) return rc;
-  HIP_TRY(hipSetDevice(ctx->device));
-  const DevModel M = dev_model(model);
-  for (int k = 0; k < n_steps; ++k) {
-    make_table(cols, ncol, T, model->n_grid);
-    if (cols_hist) memcpy(&cols_hist[(size_t)k * ncol], cols, sizeof(empm_collider) * ncol);
-    if (times_hist) times_hist[k] = *time;
-    if ((rc = launch_fwd(ctx, M, T, (float)*time, (float)dt, &states[k], &states[k + 1], grids ? &grids[k] : nullptr,
-                         k == 0, k == n_steps - 1, (hipStream_t)stream)))
-      return rc;
-    advance_colliders(cols, ncol, *time, dt);
-    *time = *time + dt;
-  }
-  return EMPM_OK;
+  if (!model) return set_err(EMPM_E_INVALID_ARG, "null model");
+  const empm_batch b = batch_of(model);
+  return rollout_fwd(ctx, &b, cols, ncol, time, dt, n_steps, states, grids, cols_hist, times_hist,
+                     (hipStream_t)stream);
 }
 
 int empm_rollout_fwd_replay(empm_ctx* ctx, const empm_model* model, const empm_collider* cols_hist, int ncol,
                             const double* times_hist, double dt, int n_steps, const empm_particles* states,
                             const empm_grid_rec* grids, void* stream) {
-  int rc = check_model(ctx, model);
-  if (rc) return rc;
-  if (!states || !times_hist || n_steps < 0 || (ncol > 0 && !cols_hist))
-    return set_err(EMPM_E_INVALID_ARG, "empm_rollout_fwd_replay: bad arguments");
-  for (int k = 0; k <= n_steps; ++k)
-    if ((rc = check_particles(&states[k], model->n_particles, "states[k]"))) return rc;
-  for (int k = 0; k < n_steps; ++k)
-    if (model->n_particles > 0 && states[k].x == states[k + 1].x)
-      return set_err(EMPM_E_INVALID_ARG, "states[%d] and states[%d] alias", k, k + 1);
-  HIP_TRY(hipSetDevice(ctx->device));
-  const DevModel M = dev_model(model);
-  ColTable T;
-  for (int k = 0; k < n_steps; ++k) {
-    if ((rc = make_table(&cols_hist[(size_t)k * ncol], ncol, T, model->n_grid))) return rc;
-    if ((rc = launch_fwd(ctx, M, T, (float)times_hist[k], (float)dt, &states[k], &states[k + 1],
-                         grids ? &grids[k] : nullptr, k == 0, k == n_steps - 1, (hipStream_t)stream)))
-      return rc;
-  }
-  return EMPM_OK;
+  if (!model) return set_err(EMPM_E_INVALID_ARG, "null model");
+  const empm_batch b = batch_of(model);
+  return rollout_fwd_replay(ctx, &b, cols_hist, ncol, times_hist, dt, n_steps, states, grids, (hipStream_t)stream);
 }
 
 int empm_rollout_adj(empm_ctx* ctx, const empm_model* model, const empm_collider* cols_hist, int ncol,
                      const double* times_hist, double dt, int n_steps, const empm_particles* states,
                      const empm_grid_rec* grids, const empm_particles* adj, float* d_mu, float* d_lam, float* d_ys,
                      float* d_mass, float* d_vol, int flags, void* stream) {
-  int rc = check_model(ctx, model);
-  if (rc) return rc;
-  if ((rc = check_adj_supported(model))) return rc;
-  if (!states || !adj || !times_hist || n_steps < 0 || (ncol > 0 && !cols_hist))
-    return set_err(EMPM_E_INVALID_ARG, "empm_rollout_adj: bad arguments");
-  for (int k = 0; k < n_steps; ++k)
-    if ((rc = check_particles(&states[k], model->n_particles, "states[k]"))) return rc;
-  if ((rc = check_particles(&adj[0], model->n_particles, "adj[0]")) ||
-      (rc = check_particles(&adj[1], model->n_particles, "adj[1]")))
-    return rc;
-  HIP_TRY(hipSetDevice(ctx->device));
-  const DevModel M = dev_model(model);
-  ColTable T;
-  for (int j = 0; j < n_steps; ++j) {
-    const int k = n_steps - 1 - j;
-    if ((rc = make_table(&cols_hist[(size_t)k * ncol], ncol, T, model->n_grid))) return rc;
-    const empm_particles* dout = &adj[j % 2];
-    const empm_particles* din = &adj[(j + 1) % 2];
-    if ((rc = launch_adj(ctx, M, T, (float)times_hist[k], (float)dt, &states[k], grids ? &grids[k] : nullptr, dout,
-                         din, d_mu, d_lam, d_ys, d_mass, d_vol, flags, (hipStream_t)stream)))
-      return rc;
-  }
-  return EMPM_OK;
+  if (!model) return set_err(EMPM_E_INVALID_ARG, "null model");
+  const empm_batch b = batch_of(model);
+  return rollout_adj(ctx, &b, cols_hist, ncol, times_hist, dt, n_steps, states, grids, adj, d_mu, d_lam, d_ys, d_mass,
+                     d_vol, flags, (hipStream_t)stream);
+}
+
+int empm_batch_rollout_fwd(empm_ctx* ctx, const empm_batch* batch, empm_collider* cols, int ncol, double* time,
+                           double dt, int n_steps, const empm_particles* states, const empm_grid_rec* grids,
+                           empm_collider* cols_hist, double* times_hist, void* stream) {
+  return rollout_fwd(ctx, batch, cols, ncol, time, dt, n_steps, states, grids, cols_hist, times_hist,
+                     (hipStream_t)stream);
+}
+
+int empm_batch_rollout_fwd_replay(empm_ctx* ctx, const empm_batch* batch, const empm_collider* cols_hist, int ncol,
+                                  const double* times_hist, double dt, int n_steps, const empm_particles* states,
+                                  const empm_grid_rec* grids, void* stream) {
+  return rollout_fwd_replay(ctx, batch, cols_hist, ncol, times_hist, dt, n_steps, states, grids, (hipStream_t)stream);
+}
+
+int empm_batch_rollout_adj(empm_ctx* ctx, const empm_batch* batch, const empm_collider* cols_hist, int ncol,
+                           const double* times_hist, double dt, int n_steps, const empm_particles* states,
+                           const empm_grid_rec* grids, const empm_particles* adj, float* d_mu, float* d_lam,
+                           float* d_ys, float* d_mass, float* d_vol, int flags, void* stream) {
+  return rollout_adj(ctx, batch, cols_hist, ncol, times_hist, dt, n_steps, states, grids, adj, d_mu, d_lam, d_ys,
+                     d_mass, d_vol, flags, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define EMPM_ABI_VERSION 3
+#define EMPM_ABI_VERSION 4
 #define EMPM_MAX_COLLIDERS 16
 
 enum {
@@ -143,6 +143,36 @@ typedef struct {
     const int *mask; /* device, n^3 int32, kind 5 only */
 } empm_collider;
 
+/* A batch: n_cand independent simulations advanced by the same kernel
+ * launches over concatenated arenas (the population of a CMA-ES step or a
+ * gradient batch; no counterpart in the reference, which runs candidates one
+ * after another, phys_sim/engine/trainer_warp_mpm.py:598-663).  Simulation c owns the next
+ * models[c].n_particles particles of the arena (in order) and a grid of its
+ * own models[c].n_grid^3 nodes; grids may differ in size and material.  The
+ * per-particle arrays below span the whole arena; the per-particle pointers
+ * inside models[] are ignored.  All simulations share one time, dt and
+ * collider table (each applies it in its own grid). */
+typedef struct {
+    int n_cand;
+    int n_particles; /* arena total: sum of models[c].n_particles */
+    const empm_model *models; /* host array of n_cand */
+    float *mu;
+    float *lam;
+    float *yield_stress;
+    const float *mass;
+    const float *vol;
+    const int *selection; /* or NULL */
+} empm_batch;
+
+/* Capacity of a batch context: at most max_cand simulations, max_particles
+ * particles in all, each grid at most max_grid^3 nodes. */
+typedef struct {
+    int device;
+    int max_cand;
+    int max_particles;
+    int max_grid;
+} empm_batch_create_desc;
+
 /* ABI version (EMPM_ABI_VERSION). */
 int empm_abi_version(void);
 
@@ -153,6 +183,10 @@ const char *empm_last_error(void);
  * Replaces the per-rollout/per-substep allocations of MPMStateStruct.init /
  * init_grid / partial_clone (mpm_data_structure.py:41-117,356-387). */
 int empm_create(empm_ctx **out, const empm_create_desc *desc);
+
+/* Allocate a context for batches (empm_batch_rollout_*).  No counterpart
+ * in the reference. */
+int empm_create_batch(empm_ctx **out, const empm_batch_create_desc *desc);
 
 /* Free a context (synchronises its device first). */
 void empm_destroy(empm_ctx *ctx);
@@ -232,6 +266,27 @@ int empm_rollout_adj(empm_ctx *ctx, const empm_model *model, const empm_collider
  * the order.  No counterpart in the reference (it scatters in caller order,
  * mpm_utils.py:358-423). */
 int empm_sort_by_cell(empm_ctx *ctx, int n_particles, const float *x, float inv_dx, int *perm, void *stream);
+
+/* empm_rollout_fwd / _replay / _adj over a batch: every substep runs each
+ * stage once for all simulations (one launch per stage, per material for the
+ * constitutive stages).  States, grid records and parameter adjoints span
+ * the arenas (grid records: sum of n_grid^3 nodes; bins:
+ * empm_batch_sizes).  Replaces the per-candidate rollout loop of the
+ * reference's CMA-ES step (phys_sim/engine/trainer_warp_mpm.py:598-663). */
+int empm_batch_rollout_fwd(empm_ctx *ctx, const empm_batch *batch, empm_collider *cols, int ncol, double *time,
+                           double dt, int n_steps, const empm_particles *states, const empm_grid_rec *grids,
+                           empm_collider *cols_hist, double *times_hist, void *stream);
+int empm_batch_rollout_fwd_replay(empm_ctx *ctx, const empm_batch *batch, const empm_collider *cols_hist, int ncol,
+                                  const double *times_hist, double dt, int n_steps, const empm_particles *states,
+                                  const empm_grid_rec *grids, void *stream);
+int empm_batch_rollout_adj(empm_ctx *ctx, const empm_batch *batch, const empm_collider *cols_hist, int ncol,
+                           const double *times_hist, double dt, int n_steps, const empm_particles *states,
+                           const empm_grid_rec *grids, const empm_particles *adj, float *d_mu, float *d_lam,
+                           float *d_ys, float *d_mass, float *d_vol, int flags, void *stream);
+
+/* Sizes of a batch's arenas: total grid nodes and the ints of one binning
+ * record. */
+int empm_batch_sizes(const empm_batch *batch, long long *n_nodes, long long *bin_record_ints);
 
 /* Size in ints of one substep's binning record (empm_grid_rec.bins) for a
  * grid of resolution n_grid and n_particles particles; -1 on bad arguments.

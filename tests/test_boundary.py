@@ -43,6 +43,9 @@ STRUCTS = {
     "empm_model": _lib.Model,
     "empm_particles": _lib.Particles,
     "empm_collider": _lib.Collider,
+    "empm_grid_rec": _lib.GridRec,
+    "empm_batch": _lib.Batch,
+    "empm_batch_create_desc": _lib.BatchCreateDesc,
 }
 
 
