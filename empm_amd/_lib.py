@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "empm_batch_rollout_fwd_replay",
     "empm_batch_rollout_adj",
     "empm_batch_sizes",
+    "empm_batch_sort_by_cell",
     "empm_sort_by_cell",
     "empm_bin_record_ints",
     "empm_profile_enable",
@@ -179,6 +180,8 @@ def _declare(lib):
     lib.empm_batch_rollout_adj.argtypes = [vp, P(Batch), P(Collider), C.c_int, P(C.c_double), C.c_double,
                                            C.c_int, P(Particles), P(GridRec), P(Particles),
                                            vp, vp, vp, vp, vp, C.c_int, vp]
+    lib.empm_batch_sort_by_cell.restype = C.c_int
+    lib.empm_batch_sort_by_cell.argtypes = [vp, P(Batch), vp, vp, vp]
     lib.empm_batch_sizes.restype = C.c_int
     lib.empm_batch_sizes.argtypes = [P(Batch), P(C.c_longlong), P(C.c_longlong)]
     lib.empm_destroy.restype = None

@@ -2012,6 +2012,19 @@ int empm_sort_by_cell(empm_ctx* ctx, int n_particles, const float* x, float inv_
   return EMPM_OK;
 }
 
+int empm_batch_sort_by_cell(empm_ctx* ctx, const empm_batch* batch, const float* x, int* perm, void* stream) {
+  HostBatch hb;
+  hipStream_t st = (hipStream_t)stream;
+  int rc = prepare(ctx, batch, hb, st);
+  if (rc) return rc;
+  if (hb.N == 0) return EMPM_OK;
+  if (!x || !perm) return set_err(EMPM_E_INVALID_ARG, "empm_batch_sort_by_cell: null array");
+  const BinBuf bb = bin_buf(ctx->bins, hb.nblk, hb.N);
+  if ((rc = do_bin(ctx, hb, x, bb, st))) return rc;
+  HIP_TRY(hipMemcpyAsync(perm, bb.perm, (size_t)hb.N * sizeof(int), hipMemcpyDeviceToDevice, st));
+  return EMPM_OK;
+}
+
 long long empm_bin_record_ints(int n_grid, int n_particles) {
   if (n_grid < 4 || n_grid > 1024 || n_particles < 0) return -1;
   const long long nb = blocks_per_axis(n_grid);

@@ -13,7 +13,7 @@ from dataclasses import dataclass, field
 import torch
 
 from . import _lib
-from ._lib import Collider, CreateDesc, GridRec, Model, Particles, check
+from ._lib import Batch, BatchCreateDesc, Collider, CreateDesc, GridRec, Model, Particles, check
 
 F32 = torch.float32
 
@@ -35,9 +35,11 @@ def stream_handle(device) -> int:
 
 
 class Context:
-    """An empm_ctx: grid buffers + per-particle scratch on one device."""
+    """An empm_ctx: grid buffers + per-particle scratch on one device.  With
+    max_cand given it is a batch context for up to max_cand simulations of at
+    most max_particles particles in all, each grid at most n_grid^3."""
 
-    def __init__(self, n_grid: int, max_particles: int, device="cuda:0"):
+    def __init__(self, n_grid: int, max_particles: int, device="cuda:0", max_cand: int | None = None):
         self.lib = _lib.load()
         self.device = torch.device(device)
         if self.device.type != "cuda":
@@ -46,9 +48,15 @@ class Context:
         self.device = torch.device("cuda", idx)
         self.n_grid = int(n_grid)
         self.max_particles = int(max_particles)
-        desc = CreateDesc(device=idx, n_grid=self.n_grid, max_particles=self.max_particles)
+        self.max_cand = max_cand
         h = C.c_void_p()
-        check(self.lib.empm_create(C.byref(h), C.byref(desc)))
+        if max_cand is None:
+            desc = CreateDesc(device=idx, n_grid=self.n_grid, max_particles=self.max_particles)
+            check(self.lib.empm_create(C.byref(h), C.byref(desc)))
+        else:
+            desc = BatchCreateDesc(device=idx, max_cand=int(max_cand), max_particles=self.max_particles,
+                                   max_grid=self.n_grid)
+            check(self.lib.empm_create_batch(C.byref(h), C.byref(desc)))
         self._h = h
 
     @property
@@ -150,6 +158,80 @@ class ModelArrays:
         return m
 
 
+class BatchModel:
+    """Several simulations advanced together: per-simulation scalars (the
+    ModelArrays in `sims`, whose own per-particle arrays are ignored) and
+    per-particle arrays over the concatenated particle arena, simulation c
+    owning the next sims[c].n particles."""
+
+    def __init__(self, sims: list[ModelArrays], counts: list[int], mu, lam, yield_stress, mass, vol,
+                 selection=None):
+        self.sims = list(sims)
+        self.counts = [int(k) for k in counts]
+        self.mu, self.lam, self.yield_stress, self.mass, self.vol = mu, lam, yield_stress, mass, vol
+        self.selection = selection
+        self.n = int(sum(self.counts))
+        self.offsets = [0]
+        for k in self.counts:
+            self.offsets.append(self.offsets[-1] + k)
+        self._models = None
+
+    @staticmethod
+    def concat(models: list[ModelArrays]) -> "BatchModel":
+        """A batch from single-simulation models, copying their per-particle
+        arrays into the arena in order."""
+        cat = lambda ts: torch.cat(ts).contiguous()
+        sel = None
+        if any(m.selection is not None for m in models):
+            sel = cat([m.selection if m.selection is not None else torch.zeros_like(m.mu, dtype=torch.int32)
+                       for m in models])
+        return BatchModel(models, [int(m.mu.shape[0]) for m in models], cat([m.mu for m in models]),
+                          cat([m.lam for m in models]), cat([m.yield_stress for m in models]),
+                          cat([m.mass for m in models]), cat([m.vol for m in models]), sel)
+
+    def struct(self) -> Batch:
+        arr = (Model * len(self.sims))()
+        for i, (m, k) in enumerate(zip(self.sims, self.counts)):
+            st = Model()
+            st.n_particles = k
+            st.n_grid = int(m.n_grid)
+            st.material = int(m.material)
+            st.grid_lim = float(m.grid_lim)
+            st.dx = float(m.dx)
+            st.inv_dx = float(m.inv_dx)
+            st.rpic_damping = float(m.rpic_damping)
+            st.grid_v_damping_scale = float(m.grid_v_damping_scale)
+            for d in range(3):
+                st.gravity[d] = float(m.gravity[d])
+            st.alpha = float(m.alpha)
+            st.hardening = float(m.hardening)
+            st.xi = float(m.xi)
+            st.plastic_viscosity = float(m.plastic_viscosity)
+            st.softening = float(m.softening)
+            arr[i] = st
+        self._models = arr  # keep the host table alive with the struct
+        b = Batch()
+        b.n_cand = len(self.sims)
+        b.n_particles = self.n
+        b.models = C.cast(arr, C.c_void_p)
+        b.mu, b.lam, b.yield_stress = _ptr(self.mu), _ptr(self.lam), _ptr(self.yield_stress)
+        b.mass, b.vol = _ptr(self.mass), _ptr(self.vol)
+        b.selection = _ptr(self.selection) if self.selection is not None else None
+        return b
+
+    def sizes(self):
+        """(total grid nodes, ints of one binning record)."""
+        g, nb = C.c_longlong(), C.c_longlong()
+        st = self.struct()
+        check(_lib.load().empm_batch_sizes(C.byref(st), C.byref(g), C.byref(nb)))
+        return int(g.value), int(nb.value)
+
+    def permuted(self, perm) -> "BatchModel":
+        g = lambda t: t.index_select(0, perm).contiguous() if t is not None else None
+        return BatchModel(self.sims, self.counts, g(self.mu), g(self.lam), g(self.yield_stress), g(self.mass),
+                          g(self.vol), g(self.selection))
+
+
 def collider_array(cols: list[Collider]):
     n = len(cols)
     if n > _lib.EMPM_MAX_COLLIDERS:
@@ -238,15 +320,14 @@ class Records:
     ((m v, m) and v_out, float4 per node) and their particle binnings, one
     tensor per field."""
 
-    def __init__(self, n: int, n_particles: int, n_grid: int, device):
-        N, G = int(n_particles), int(n_grid) ** 3
+    def __init__(self, n: int, n_particles: int, n_nodes: int, bin_ints: int, device):
+        N, G = int(n_particles), int(n_nodes)
         e = lambda *shape: torch.empty(shape, dtype=F32, device=device)
         self.n = int(n)
         self.x, self.v = e(n + 1, N, 3), e(n + 1, N, 3)
         self.C, self.F, self.Fe = e(n + 1, N, 3, 3), e(n + 1, N, 3, 3), e(n + 1, N, 3, 3)
         self.grid = e(max(n, 1), 2, G, 4)
-        nbi = int(_lib.load().empm_bin_record_ints(int(n_grid), N))
-        self.bins = torch.empty((max(n, 1), nbi), dtype=torch.int32, device=device)
+        self.bins = torch.empty((max(n, 1), int(bin_ints)), dtype=torch.int32, device=device)
 
     def state(self, k: int) -> ParticleState:
         return ParticleState(self.x[k], self.v[k], self.C[k], self.F[k], self.Fe[k])
@@ -272,26 +353,55 @@ def _grid_array(recs: list):
 
 
 class Rollout:
-    """Forward and reverse rollouts of one scene on one context.
+    """Forward and reverse rollouts of one scene (ModelArrays) or of a batch
+    of scenes (BatchModel, on a batch context) on one context.
 
     ``forward`` advances ``state`` (ParticleState, in place via two ping-pong
     buffers) by ``n_steps`` substeps, returning the copies of x requested by
     ``record`` (a sorted list of substep counts k >= 1: x after k substeps) and,
-    when ``tape`` is requested, a RolloutTape for ``backward``."""
+    when ``tape`` is requested, a RolloutTape for ``backward``.  For a batch
+    the state spans the particle arena (simulation c owns rows
+    offsets[c] .. offsets[c + 1]) and so do the parameter gradients."""
 
-    def __init__(self, ctx: Context, model: ModelArrays, n_particles: int, sort: bool = True):
+    def __init__(self, ctx: Context, model, n_particles: int, sort: bool = True):
         self.ctx = ctx
         self.model = model
+        self.batch = isinstance(model, BatchModel)
         self.n = int(n_particles)
         self.sort = sort
         dev = ctx.device
         self._buf = [ParticleState.empty(self.n, dev), ParticleState.empty(self.n, dev)]
+        if self.batch:
+            if self.n != model.n:
+                raise ValueError(f"n_particles={self.n} but the batch holds {model.n}")
+            self._nodes, self._bin_ints = model.sizes()
+        else:
+            self._nodes = int(model.n_grid) ** 3
+            self._bin_ints = int(_lib.load().empm_bin_record_ints(int(model.n_grid), self.n))
+
+    # ------------------------------------------------------------ C-ABI dispatch
+    def _fwd(self, m, *args):
+        f = self.ctx.lib.empm_batch_rollout_fwd if self.batch else self.ctx.lib.empm_rollout_fwd
+        check(f(self.ctx.handle, C.byref(m), *args))
+
+    def _replay(self, m, *args):
+        f = self.ctx.lib.empm_batch_rollout_fwd_replay if self.batch else self.ctx.lib.empm_rollout_fwd_replay
+        check(f(self.ctx.handle, C.byref(m), *args))
+
+    def _adj(self, m, *args):
+        f = self.ctx.lib.empm_batch_rollout_adj if self.batch else self.ctx.lib.empm_rollout_adj
+        check(f(self.ctx.handle, C.byref(m), *args))
 
     # ------------------------------------------------------------ ordering
     def cell_order(self, x: torch.Tensor) -> torch.Tensor:
         perm = torch.empty(self.n, dtype=torch.int32, device=x.device)
-        check(self.ctx.lib.empm_sort_by_cell(self.ctx.handle, self.n, _ptr(x.contiguous()),
-                                             C.c_float(self.model.inv_dx), _ptr(perm), stream_handle(x.device)))
+        if self.batch:
+            b = self.model.struct()
+            check(self.ctx.lib.empm_batch_sort_by_cell(self.ctx.handle, C.byref(b), _ptr(x.contiguous()), _ptr(perm),
+                                                       stream_handle(x.device)))
+        else:
+            check(self.ctx.lib.empm_sort_by_cell(self.ctx.handle, self.n, _ptr(x.contiguous()),
+                                                 C.c_float(self.model.inv_dx), _ptr(perm), stream_handle(x.device)))
         return perm.long()
 
     @staticmethod
@@ -304,8 +414,10 @@ class Rollout:
         out.index_copy_(0, perm, t)
         return out
 
-    def _permuted_model(self, perm) -> ModelArrays:
+    def _permuted_model(self, perm):
         m = self.model
+        if self.batch:
+            return m.permuted(perm)
         g = lambda t: self._gather(perm, t)
         return ModelArrays(material=m.material, n_grid=m.n_grid, grid_lim=m.grid_lim, dx=m.dx, inv_dx=m.inv_dx,
                            mu=g(m.mu), lam=g(m.lam), yield_stress=g(m.yield_stress), mass=g(m.mass), vol=g(m.vol),
@@ -333,7 +445,8 @@ class Rollout:
         s_sorted = self._perm_state(perm, state)
         final, t1, xs, tp = self._forward_core(pm, s_sorted, cols, time, dt, n_steps, record, tape, ckpt_every,
                                                cols_out)
-        if pm.material == 5 or pm.hardening == 1.0:  # in-place parameter updates go back to the caller's arrays
+        sims = pm.sims if self.batch else [pm]
+        if any(s.material == 5 or s.hardening == 1.0 for s in sims):  # in-place updates go back to the caller
             for a, b in ((self.model.mu, pm.mu), (self.model.lam, pm.lam), (self.model.yield_stress, pm.yield_stress)):
                 a.index_copy_(0, perm, b)
         if tp is not None:
@@ -342,7 +455,6 @@ class Rollout:
         return self._perm_state(perm, final, inverse=True), t1, [self._scatter(perm, x) for x in xs], tp
 
     def _forward_core(self, model, state, cols, time, dt, n_steps, record, tape, ckpt_every, cols_out):
-        lib = self.ctx.lib
         arr, ncol = collider_array(cols)
         m = model.struct()
         t = C.c_double(time)
@@ -355,13 +467,12 @@ class Rollout:
         stream = stream_handle(self.ctx.device)
         if tape and tp.ckpt_every == 1:
             # recorded mode: every substep writes its own output slot, Fe and grid
-            R = Records(n_steps, self.n, model.n_grid, self.ctx.device)
+            R = Records(n_steps, self.n, self._nodes, self._bin_ints, self.ctx.device)
             R.state(0).copy_(state)
             states = [R.state(k) for k in range(n_steps + 1)]
             grids = [R.record(k) for k in range(n_steps)]
-            check(lib.empm_rollout_fwd(self.ctx.handle, C.byref(m), arr, ncol, C.byref(t), C.c_double(dt), n_steps,
-                                       _states_array(states), _grid_array(grids), tp.cols_hist, tp.times_hist,
-                                       stream))
+            self._fwd(m, arr, ncol, C.byref(t), C.c_double(dt), n_steps, _states_array(states), _grid_array(grids),
+                      tp.cols_hist, tp.times_hist, stream)
             tp.rec_states = states[:n_steps]
             tp.rec_grids = grids
             tp.checkpoints[0] = states[0]
@@ -389,8 +500,7 @@ class Rollout:
             sarr = _states_array(seq)
             ch = C.cast(C.byref(tp.cols_hist, k * ncol * C.sizeof(Collider)), C.POINTER(Collider)) if tape else None
             th = C.cast(C.byref(tp.times_hist, k * C.sizeof(C.c_double)), C.POINTER(C.c_double)) if tape else None
-            check(lib.empm_rollout_fwd(self.ctx.handle, C.byref(m), arr, ncol, C.byref(t), C.c_double(dt), n,
-                                       sarr, None, ch, th, stream))
+            self._fwd(m, arr, ncol, C.byref(t), C.c_double(dt), n, sarr, None, ch, th, stream)
             cur = seq[-1]
             k = stop
             if k in rec:
@@ -423,7 +533,6 @@ class Rollout:
         return self._perm_state(perm, d0, inverse=True), gu
 
     def _backward_core(self, model, tp, d_final, seeds, flags, grads):
-        lib = self.ctx.lib
         dev = self.ctx.device
         n = self.n
         if grads is None:
@@ -455,8 +564,8 @@ class Rollout:
                 garr = _grid_array(grids[j - s0:k - s0])
                 chj = C.cast(C.byref(tp.cols_hist, j * ncol * C.sizeof(Collider)), C.POINTER(Collider))
                 thj = C.cast(C.byref(tp.times_hist, j * C.sizeof(C.c_double)), C.POINTER(C.c_double))
-                check(lib.empm_rollout_adj(self.ctx.handle, C.byref(m), chj, ncol, thj, C.c_double(tp.dt), cnt,
-                                           sub, garr, _states_array(adj), *grads.ptrs(), int(flags), stream))
+                self._adj(m, chj, ncol, thj, C.c_double(tp.dt), cnt, sub, garr, _states_array(adj), *grads.ptrs(),
+                          int(flags), stream)
                 if cnt % 2 == 1:
                     adj = [adj[1], adj[0]]
                 k = j
@@ -465,7 +574,7 @@ class Rollout:
             reverse(0, tp.n_steps, tp.rec_states, tp.rec_grids)
         else:
             K = tp.ckpt_every
-            seg = Records(min(K, tp.n_steps), n, model.n_grid, dev)
+            seg = Records(min(K, tp.n_steps), n, self._nodes, self._bin_ints, dev)
             seg_starts = sorted(tp.checkpoints.keys())
             k_hi = tp.n_steps
             for s0 in reversed(seg_starts):
@@ -478,8 +587,8 @@ class Rollout:
                 ch = C.cast(C.byref(tp.cols_hist, s0 * ncol * C.sizeof(Collider)), C.POINTER(Collider))
                 th = C.cast(C.byref(tp.times_hist, s0 * C.sizeof(C.c_double)), C.POINTER(C.c_double))
                 # replay the segment, recording every substep's Fe and grid
-                check(lib.empm_rollout_fwd_replay(self.ctx.handle, C.byref(m), ch, ncol, th, C.c_double(tp.dt), L,
-                                                  _states_array(states), _grid_array(grids), stream))
+                self._replay(m, ch, ncol, th, C.c_double(tp.dt), L, _states_array(states), _grid_array(grids),
+                             stream)
                 reverse(s0, k_hi, states[:L], grids)
                 k_hi = s0
         if 0 in seeds:

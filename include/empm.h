@@ -284,6 +284,11 @@ int empm_batch_rollout_adj(empm_ctx *ctx, const empm_batch *batch, const empm_co
                            const empm_grid_rec *grids, const empm_particles *adj, float *d_mu, float *d_lam,
                            float *d_ys, float *d_mass, float *d_vol, int flags, void *stream);
 
+/* empm_sort_by_cell for a batch: perm (device, n_particles ints) orders the
+ * arena by (simulation, block, cell), so the particles of simulation c keep
+ * the range its models[c].n_particles gives them. */
+int empm_batch_sort_by_cell(empm_ctx *ctx, const empm_batch *batch, const float *x, int *perm, void *stream);
+
 /* Sizes of a batch's arenas: total grid nodes and the ints of one binning
  * record. */
 int empm_batch_sizes(const empm_batch *batch, long long *n_nodes, long long *bin_record_ints);

@@ -50,3 +50,20 @@ def rel_err(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def gpu_collider(c):
+    """The empm_collider of an oracle collider (same fields, no mask)."""
+    from empm_amd._lib import Collider
+
+    g = Collider()
+    for f, _ in Collider._fields_:
+        if f == "mask":
+            continue
+        val = getattr(c, f)
+        if isinstance(val, (int, float)):
+            setattr(g, f, val)
+        else:
+            for i in range(3):
+                getattr(g, f)[i] = val[i]
+    return g

@@ -15,25 +15,9 @@ import torch
 from empm_amd import runtime as rt
 from empm_amd import scenes
 from oracle import oracle as orc
-from tests.helpers import rel_err
+from tests.helpers import gpu_collider, rel_err
 
 pytestmark = pytest.mark.gpu
-
-
-def gpu_collider(c):
-    from empm_amd._lib import Collider
-
-    g = Collider()
-    for f, _ in Collider._fields_:
-        if f == "mask":
-            continue
-        val = getattr(c, f)
-        if isinstance(val, (int, float)):
-            setattr(g, f, val)
-        else:
-            for i in range(3):
-                getattr(g, f)[i] = val[i]
-    return g
 
 
 def setup(sc, dev, E=None, nu=None, ys=None, v0=None):
