@@ -337,6 +337,20 @@ class Records:
         return (self.grid[k], self.bins[k])
 
 
+def tape_bytes(n_particles: int, n_nodes: int, bin_ints: int, n_steps: int, ckpt_every: int) -> int:
+    """Device bytes a reverse-mode rollout holds for its tape: every
+    substep's record (state with Fe 132 B/particle, grid 32 B/node, binning)
+    when ckpt_every == 1, else a 96 B/particle checkpoint every ckpt_every
+    substeps plus one segment of records; plus the reverse sweep's
+    cotangents and parameter gradients."""
+    N, G, T, K = int(n_particles), int(n_nodes), int(n_steps), max(1, int(ckpt_every))
+    rec = lambda k: (k + 1) * N * 132 + k * (32 * G + 4 * int(bin_ints))
+    sweep = 2 * 96 * N + 20 * N
+    if K == 1:
+        return rec(T) + sweep
+    return (-(-T // K)) * 96 * N + rec(min(K, T)) + sweep
+
+
 def _states_array(states: list[ParticleState]):
     arr = (Particles * len(states))()
     for i, st in enumerate(states):

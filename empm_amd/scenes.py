@@ -165,3 +165,72 @@ def small_scene(n_particles=512, n_grid=16, material=0, seed=0, E=2e4, nu=0.3, y
     v0 = np.asarray(vel)[None, :] + np.cross(w[None, :], x - c[None, :])
     sc.params["v0"] = v0.astype(np.float32)
     return sc
+
+
+C3_SHAPES = ("cube", "rod", "ellipsoid")
+C3_HALF_Z = {"cube": 0.1, "rod": 0.03, "ellipsoid": 0.15}
+C3_NOMINAL = {"E": 1e5, "nu": 0.3, "friction_angle": 30.0, "yield_stress": 5000.0}
+
+
+def shape_points(shape, n, rng, center):
+    """n points uniform in a cube of side 0.2, a rod 0.4 x 0.06 x 0.06 along
+    x, or an ellipsoid with radii (0.2, 0.1, 0.15) (half heights along z in
+    C3_HALF_Z); returns (points, volume, half height along z)."""
+    c = np.asarray(center, np.float64)
+    if shape == "cube":
+        ext = np.array([0.2, 0.2, 0.2])
+    elif shape == "rod":
+        ext = np.array([0.4, 0.06, 0.06])
+    elif shape == "ellipsoid":
+        radii = (0.2, 0.1, 0.15)
+        return ellipsoid_points(c, radii, n, rng), 4.0 / 3.0 * np.pi * np.prod(radii), radii[2]
+    else:
+        raise ValueError(f"unknown shape {shape}")
+    return c + (rng.random((n, 3)) - 0.5) * ext, float(np.prod(ext)), ext[2] / 2
+
+
+def population_c3(n_cand=32, seed=4, n_grid=64, n_steps=300, dt=1e-4, differentiable=False, nominal=False,
+                  drop_height=0.3, n_range=(50_000, 150_000)):
+    """A CMA-ES population: n_cand candidates, each an object of N ~ U{50k..150k}
+    particles shaped as a cube (side 0.2), a rod (0.4 x 0.06 x 0.06) or an
+    ellipsoid (radii 0.2, 0.1, 0.15), drawn from the seed, on its own 64^3
+    grid with 3-cell sticky walls.  Materials alternate Drucker-Prager sand
+    (id 2) and von Mises plasticine (id 5, yield stress 5000, softening 0.1,
+    the reference's gallery preset, warp_mpm_wrapper.py:43-47); with
+    differentiable=True the second material is Neo-Hookean + von Mises (id 7,
+    yield stress 5000), whose adjoint exists.  Parameters are log-normal
+    around the nominal E = 1e5, nu = 0.3, friction angle 30 degrees:
+    E = 1e5 exp(0.3 z0), nu = min(0.3 exp(0.1 z1), 0.489),
+    angle = 30 exp(0.1 z2); nominal=True gives the same objects with the
+    nominal parameters (the fitness target).  As in the single-rollout
+    workload, each object starts at the moment of impact after a drop of
+    `drop_height`: bottom a quarter cell above the sticky band, moving down
+    at sqrt(2 g h).  rho = 1000, dt = 1e-4, 300 substeps."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dx = 1.0 / n_grid
+    out = []
+    for c in range(n_cand):
+        N = int(rng.integers(n_range[0], n_range[1] + 1))
+        shape = C3_SHAPES[int(rng.integers(len(C3_SHAPES)))]
+        z = rng.standard_normal(3)
+        material = (2, 7 if differentiable else 5)[c % 2]
+        E = 1e5 * np.exp(0.3 * z[0])
+        nu = min(0.3 * np.exp(0.1 * z[1]), 0.489)
+        phi = 30.0 * np.exp(0.1 * z[2])
+        if nominal:
+            E, nu, phi = C3_NOMINAL["E"], C3_NOMINAL["nu"], C3_NOMINAL["friction_angle"]
+        ys = C3_NOMINAL["yield_stress"]
+        srng = np.random.Generator(np.random.PCG64([seed, c]))
+        x, vol_total, _ = shape_points(shape, N, srng, (0.5, 0.5, 3 * dx + C3_HALF_Z[shape] + 0.25 * dx))
+        f = lambda v: np.full(N, v, np.float32)
+        sc = Scene(name=f"c{c}_{shape}_{N}_m{material}", n_grid=n_grid, grid_lim=1.0, dt=dt, n_steps=n_steps,
+                   material=material, x=x.astype(np.float32), vol=f(vol_total / N), rho=f(1000.0), E=f(E),
+                   nu=f(nu), yield_stress=f(ys), friction_angle=float(phi), softening=0.1,
+                   colliders=sticky_walls(n_grid, 3),
+                   params={"E": float(E), "nu": float(nu), "friction_angle": float(phi), "yield_stress": ys,
+                           "shape": shape})
+        v0 = np.zeros((N, 3), np.float32)
+        v0[:, 2] = -np.sqrt(2 * 9.8 * drop_height)
+        sc.params["v0"] = v0
+        out.append(sc)
+    return out
