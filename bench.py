@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--substeps", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--only", choices=("all", "single", "population"), default="all",
+                    help="run only the single-rollout workload or only the population workload")
+    ap.add_argument("--pop-steps", type=int, default=2, help="timed evaluations of the population workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (3 times): the known-size kernel that "
                          "calibrates FETCH_SIZE / WRITE_SIZE counter readings")
@@ -119,6 +122,9 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written per launch")
         del src, dst
+    if args.only == "population":
+        print(json.dumps({"population": bench_population(args, dev)}), flush=True)
+        return
     sc = scenes.ellipsoid_c2(n_particles=args.particles, n_grid=args.grid, n_steps=args.substeps, param_seed=3)
     tprm = scenes.sample_params_c2(2)
     N, n, T, dt = sc.n, sc.n_grid, sc.n_steps, sc.dt
@@ -237,6 +243,11 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, args.cpu_seconds)
+    population = None
+    if args.only == "all":
+        del ro, tro, ctx
+        torch.cuda.empty_cache()
+        population = bench_population(args, dev)
 
     out = {
         "metric": "particle-substeps/s (fwd+bwd) @200k pts/64^3",
@@ -267,8 +278,68 @@ def main():
                      "traffic_source": TRAFFIC_PROFILE, "algorithmic_bytes": kb,
                      "avg_launch_us": avg_ms * 1000.0, "launches": l.value},
         "cpu_baseline": cpu,
+        "batched": population,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_population(args, dev):
+    """Batched vs sequential evaluation of a 32-candidate CMA-ES population
+    (50k-150k particles each, cube / rod / ellipsoid, own 64^3 grids, 300
+    substeps, dt 1e-4): forward-only fitness (sand / plasticine) and
+    fitness + per-candidate gradients (sand / Neo-Hookean + von Mises).
+    Throughput = sum of particles * 300 / wall time of one evaluation of the
+    whole population; the sequential leg runs the same candidates as plain
+    rollouts one after another on the same GPU."""
+    import torch
+
+    from empm_amd import population as pop
+    from empm_amd import scenes
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, r
+
+    out = {"workload": "CMA-ES population: 32 candidates x 50k-150k particles (cube/rod/ellipsoid), own 64^3 grids, "
+                       "300 substeps, dt 1e-4"}
+    for leg, diff in (("fwd", False), ("fwd_bwd", True)):
+        sc = scenes.population_c3(differentiable=diff)
+        nom = scenes.population_c3(differentiable=diff, nominal=True)
+        ev = pop.PopulationEvaluator(sc, dev)
+        ev.set_targets([pop.Candidate.of_scene(s) for s in nom])
+        cands = [pop.Candidate.of_scene(s) for s in sc]
+        work = ev.n * ev.n_steps
+        info = {"materials": sorted(set(s.material for s in sc)), "n_particles": ev.n, "n_cand": ev.n_cand}
+        if not diff:
+            tb, fb = timed(lambda: ev.fitness(cands), args.pop_steps)
+            ts, fs = timed(lambda: ev.fitness_sequential(cands), args.pop_steps)
+        else:
+            free, _ = torch.cuda.mem_get_info(dev)
+            K = next(k for k in (1, 2, 3, 4, 5, 6, 10, 15, 20, 30, 50, 100, 300)
+                     if ev.tape_bytes(k) <= 0.7 * free)
+            info["ckpt_every_batched"] = K
+            info["ckpt_every_sequential"] = 1
+            info["tape_gb_batched"] = ev.tape_bytes(K) / 1e9
+            tb, (fb, gb) = timed(lambda: ev.fitness_and_grad(cands, ckpt_every=K), args.pop_steps)
+            ts, (fs, gs) = timed(lambda: ev.fitness_and_grad_sequential(cands, ckpt_every=1), args.pop_steps)
+            gerr = (gb - gs).abs().max(1).values / gs.abs().max(1).values
+            info["grad_rel_err_batched_vs_sequential"] = float(gerr.max())
+        ferr = ((fb - fs).abs() / fs.abs()).max()
+        info.update({"batched": work / tb, "sequential": work / ts, "speedup": ts / tb,
+                     "ms_batched": tb * 1000.0, "ms_sequential": ts * 1000.0,
+                     "fitness_rel_err_batched_vs_sequential": float(ferr)})
+        log(f"[bench] population {leg}: batched {tb * 1000:.1f} ms ({work / tb:.3e} p-substeps/s), sequential "
+            f"{ts * 1000:.1f} ms ({work / ts:.3e}), speedup {ts / tb:.2f}x; {info}")
+        out[leg] = info
+        del ev
+        torch.cuda.empty_cache()
+    out["unit"] = "particle-substeps/s"
+    return out
 
 
 def cpu_baseline(sc, seconds):

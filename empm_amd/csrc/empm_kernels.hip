@@ -385,6 +385,8 @@ struct Binning {
   const int* nactive;  // their count (device scalar)
   const int* occ;      // per block: holds particles
   const int* near;     // per block: some non-empty block's tile covers a node this block owns
+  const int* nlist;    // ids of the near blocks, in no particular order
+  const int* nnear;    // their count (device scalar)
 };
 
 // ---- cross-lane exchanges (wave64) ----------------------------------------
@@ -734,15 +736,15 @@ __global__ __launch_bounds__(kBlkWG) void k_p2g(Batch Bt, const float* __restric
 // nodes owned by each block (nodes 2o .. 2o + 1 per axis) whose `near` flag
 // is set.  Other nodes keep stale values, and no later kernel reads them:
 // G2P and the P2G adjoint read only the stencil nodes of particles, which
-// lie in their block's tile.  A capped grid strides over (block, owned node)
-// pairs, so non-near blocks cost one flag load each.
+// lie in their block's tile.  A capped grid strides over (near block, owned
+// node) pairs of the binning's near list, so the cost follows the occupied
+// part of the grid, not its size.
 constexpr int kNodePassGridCap = 1024;
 template <typename F>
 EMPM_DEV void for_near_nodes(const Batch& Bt, const Binning& B, F f) {
-  const size_t npairs = (size_t)Bt.nblk * 8;
+  const size_t npairs = (size_t)(*B.nnear) * 8;
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < npairs; q += (size_t)gridDim.x * blockDim.x) {
-    const int blk = (int)(q >> 3), r = (int)(q & 7);
-    if (!B.near[blk]) continue;
+    const int blk = B.nlist[q >> 3], r = (int)(q & 7);
     const DevModel& M = cand_of_block(Bt, blk);
     const int nb = M.nb, lb = blk - M.b0;
     const int ix = 2 * (lb / (nb * nb)) + (r >> 2), iy = 2 * ((lb / nb) % nb) + ((r >> 1) & 1),
@@ -1138,10 +1140,11 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
 //                 lanes of a wave share one atomic); counts are zero on entry
 //   k_bin_scan    one thread per block, kBinScanWG blocks per workgroup:
 //                 block totals and their workgroup-local exclusive scan;
-//                 clears `near` and the active count
-//   k_bin_finish  one thread per block: global block offset, per-bin
-//                 offsets, occupancy, active list, near marks; re-zeroes the
-//                 counts for the next binning
+//                 clears `near` and the active and near counts
+//   k_bin_finish  one thread per block: global block offset (the sum of the
+//                 earlier workgroups' totals, reduced by the workgroup),
+//                 per-bin offsets, occupancy, active list, near marks and
+//                 near list; re-zeroes the counts for the next binning
 //   k_bin_place   per particle: perm[offs[key] + rank] = p
 // ---------------------------------------------------------------------------
 constexpr int kBinScanWG = 1024;
@@ -1153,6 +1156,7 @@ struct BinBuf {
   int* act;   // 1 + nb^3: count, then the ids of the non-empty blocks
   int* occ;   // nb^3
   int* near;  // nb^3
+  int* nlist; // 1 + nb^3: count, then the ids of the near blocks
 };
 
 // Global bin of particle p of simulation M: (M.b0 + block) * 8 + cell.
@@ -1233,7 +1237,7 @@ EMPM_DEV int block_count(const int* __restrict__ count, int b) {
 __global__ __launch_bounds__(kBinScanWG) void k_bin_scan(int nblk, const int* __restrict__ count,
                                                          int* __restrict__ blkoff, int* __restrict__ wsum,
                                                          int* __restrict__ near, int* __restrict__ act,
-                                                         int* __restrict__ ovf_flag) {
+                                                         int* __restrict__ nlist, int* __restrict__ ovf_flag) {
   __shared__ int smem[kBinScanWG / 64 + 1];
   const int b = blockIdx.x * kBinScanWG + threadIdx.x;
   const int tot = b < nblk ? block_count(count, b) : 0;
@@ -1246,6 +1250,7 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_scan(int nblk, const int* __
   if (threadIdx.x == 0) wsum[blockIdx.x] = wtot;
   if (b == 0) {
     act[0] = 0;
+    nlist[0] = 0;
     *ovf_flag = 0;
   }
 }
@@ -1254,11 +1259,20 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_finish(Batch Bt, int* __rest
                                                            const int* __restrict__ blkoff,
                                                            const int* __restrict__ wsum, int* __restrict__ offs,
                                                            int* __restrict__ act, int* __restrict__ occ,
-                                                           int* __restrict__ near) {
+                                                           int* __restrict__ near, int* __restrict__ nlist) {
+  __shared__ int red[kBinScanWG / 64];
   const int nblk = Bt.nblk;
   const int b = blockIdx.x * kBinScanWG + threadIdx.x;
+  // this workgroup's offset: the totals of the workgroups before it
+  int part = 0;
+  for (int w = threadIdx.x; w < (int)blockIdx.x; w += kBinScanWG) part += wsum[w];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+  __syncthreads();
   int prefix = 0;
-  for (int w = 0; w < (int)blockIdx.x; ++w) prefix += wsum[w];
+#pragma unroll
+  for (int k = 0; k < kBinScanWG / 64; ++k) prefix += red[k];
   if (b >= nblk) return;
   int4* c4 = reinterpret_cast<int4*>(count + (size_t)b * 8);
   const int4 lo = c4[0], hi = c4[1];
@@ -1288,7 +1302,10 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_finish(Batch Bt, int* __rest
 #pragma unroll
   for (int d = 0; d < 8; ++d) {
     const int o0 = b0 + (d >> 2), o1 = b1 + ((d >> 1) & 1), o2 = b2 + (d & 1);
-    if (o0 < nb && o1 < nb && o2 < nb) near[M.b0 + (o0 * nb + o1) * nb + o2] = 1;
+    if (o0 < nb && o1 < nb && o2 < nb) {
+      const int nbk = M.b0 + (o0 * nb + o1) * nb + o2;
+      if (atomicExch(&near[nbk], 1) == 0) nlist[1 + atomicAdd(&nlist[0], 1)] = nbk;
+    }
   }
 }
 
@@ -1598,10 +1615,12 @@ int make_table(const empm_collider* cols, int ncol, ColTable& T) {
 size_t pad4(size_t k) { return (k + 3) & ~(size_t)3; }
 
 // Layout of one binning record: [perm N | offs 8 nblk + 1 | act 1 + nblk |
-// occ nblk | near nblk], each section padded to a multiple of 4 ints so that
-// the int4 accesses of the binning kernels stay 16-byte aligned.
+// occ nblk | near nblk | nlist 1 + nblk], each section padded to a multiple
+// of 4 ints so that the int4 accesses of the binning kernels stay 16-byte
+// aligned.
 size_t bin_record_ints(long long nblk, long long n_particles) {
-  return pad4((size_t)n_particles) + pad4((size_t)nblk * 8 + 1) + pad4(1 + (size_t)nblk) + 2 * pad4((size_t)nblk);
+  return pad4((size_t)n_particles) + pad4((size_t)nblk * 8 + 1) + 2 * pad4(1 + (size_t)nblk) +
+         2 * pad4((size_t)nblk);
 }
 
 BinBuf bin_buf(int* rec, int nblk, int n_particles) {
@@ -1611,10 +1630,13 @@ BinBuf bin_buf(int* rec, int nblk, int n_particles) {
   b.act = b.offs + pad4((size_t)nblk * 8 + 1);
   b.occ = b.act + pad4(1 + (size_t)nblk);
   b.near = b.occ + pad4((size_t)nblk);
+  b.nlist = b.near + pad4((size_t)nblk);
   return b;
 }
 
-Binning binning(const BinBuf& b) { return Binning{b.perm, b.offs, b.act + 1, b.act, b.occ, b.near}; }
+Binning binning(const BinBuf& b) {
+  return Binning{b.perm, b.offs, b.act + 1, b.act, b.occ, b.near, b.nlist + 1, b.nlist};
+}
 
 Overflow overflow(const empm_ctx* ctx) { return Overflow{ctx->ovf, ctx->ovf_flag}; }
 
@@ -1635,9 +1657,9 @@ int do_bin(empm_ctx* ctx, const HostBatch& hb, const float* x, const BinBuf& out
   if (hb.N > 0)
     hipLaunchKernelGGL(k_bin_count, dim3(nblocks(hb.N)), dim3(kBlock), 0, st, hb.dev, x, ctx->count, ctx->key_rank);
   hipLaunchKernelGGL(k_bin_scan, dim3(wgs), dim3(kBinScanWG), 0, st, nblk, ctx->count, ctx->blkoff, ctx->wsum,
-                     out.near, out.act, ctx->ovf_flag);
+                     out.near, out.act, out.nlist, ctx->ovf_flag);
   hipLaunchKernelGGL(k_bin_finish, dim3(wgs), dim3(kBinScanWG), 0, st, hb.dev, ctx->count, ctx->blkoff, ctx->wsum,
-                     out.offs, out.act, out.occ, out.near);
+                     out.offs, out.act, out.occ, out.near, out.nlist);
   if (hb.N > 0)
     hipLaunchKernelGGL(k_bin_place, dim3(nblocks(hb.N)), dim3(kBlock), 0, st, hb.N, ctx->key_rank, out.offs,
                        out.perm);

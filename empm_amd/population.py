@@ -96,6 +96,7 @@ class PopulationEvaluator:
         self.cand_id = torch.repeat_interleave(torch.arange(self.n_cand, device=self.device),
                                                torch.as_tensor(self.counts, device=self.device))
         self.counts_t = torch.as_tensor(self.counts, dtype=torch.float64, device=self.device)
+        self._ends = torch.as_tensor(self.offsets[1:] - 1, device=self.device)
         self.cols = surface_colliders(s0.colliders)
         self.ctx = rt.Context(max(sc.n_grid for sc in scenes), self.n, self.device, max_cand=self.n_cand)
         self._solo_ctx = None
@@ -134,8 +135,13 @@ class PopulationEvaluator:
                                 torch.eye(3, device=self.device).repeat(N, 1, 1).contiguous())
 
     def _seg_sum(self, v: torch.Tensor) -> torch.Tensor:
-        out = torch.zeros(self.n_cand, dtype=torch.float64, device=self.device)
-        return out.index_add_(0, self.cand_id, v.double())
+        """Per-candidate sums of a per-particle array (float64): differences of
+        an inclusive scan at the candidates' range ends (candidates own
+        contiguous ranges; a scatter-add into n_cand slots would serialise
+        on a few addresses)."""
+        cs = torch.cumsum(v.double(), 0)
+        ends = cs[self._ends]
+        return torch.diff(ends, prepend=torch.zeros(1, dtype=torch.float64, device=self.device))
 
     # ------------------------------------------------------------ batched
     def rollout(self, cands: list[Candidate]) -> rt.ParticleState:
