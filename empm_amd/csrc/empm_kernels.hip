@@ -364,7 +364,9 @@ constexpr int kTileMargin = 0;                                  // binning is ex
 constexpr int kTileN = kBlkCells + 2 + 2 * kTileMargin;         // 4 tile nodes per edge
 constexpr int kTileNodes = kTileN * kTileN * kTileN;            // 64
 constexpr int kBlkWG = 256;       // workgroup size of the block-owned scatter kernels
-constexpr int kBlkGridCap = 1024; // max workgroups of a block-owned scatter launch (4 per CU)
+constexpr int kBlkGridCap = 1024; // workgroups of a block-owned scatter launch when the occupancy query fails
+constexpr int kHeavyChunks = 8;   // blocks with more chunks than this are scattered first
+constexpr int kDynamicRounds = 4; // more blocks per scatter workgroup than this: handed out dynamically
 constexpr int kWave = 64;
 constexpr int kAdjPayload = 24;   // floats per particle of the G2P-adjoint scatter payload
 
@@ -381,8 +383,11 @@ struct Overflow {
 struct Binning {
   const int* perm;
   const int* offs;
-  const int* active;   // ids of the non-empty blocks, in no particular order
-  const int* nactive;  // their count (device scalar)
+  // the non-empty blocks: act[0] heavy ones (more than kHeavyChunks chunks)
+  // at act[2 ..], act[1] light ones at act[2 + nblk - 1 ...] downwards; the
+  // scatter takes the heavy ones first
+  const int* act;
+  int nblk;
   const int* occ;      // per block: holds particles
   const int* near;     // per block: some non-empty block's tile covers a node this block owns
   const int* nlist;    // ids of the near blocks, in no particular order
@@ -558,17 +563,40 @@ EMPM_DEV void tile_origin(int blk, int nb, int* o) {
 }
 
 // The block-owned scatter driver.  load(p, xp, P) fills a particle's position
-// and payload.  Every active block is taken by one workgroup; its waves take
-// the block's cell chunks round robin.
+// and payload.  Each active block is taken by one workgroup, which draws its
+// next block from the launch's work counter (zero at launch); its waves take
+// the block's cell chunks round robin.  Launched with as many workgroups as
+// fit on the device at once, so no workgroup waits for another to finish
+// before it starts.
 template <int NC, typename Load>
 EMPM_DEV void block_scatter(const Batch& Bt, const Binning& B, float4* __restrict__ tiles, float4* __restrict__ ovf,
-                            int* ovf_flag, Load load) {
+                            int* ovf_flag, int* __restrict__ work, Load load) {
   __shared__ float tile[NC][kTileNodes];
+  __shared__ int next[2][2];
   const int lane = __lane_id();
   const int wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
-  const int nact = *B.nactive;
-  for (int ai = blockIdx.x; ai < nact; ai += gridDim.x) {
-    const int blk = __builtin_amdgcn_readfirstlane(B.active[ai]);
+  const int last = B.nblk - 1;
+  // List entry w goes to workgroup w (read ahead of the counts, as if
+  // heavy).  With up to kDynamicRounds entries per workgroup the rest are
+  // taken in strides of gridDim.x, the next id read at the top of each
+  // round.  With more, workgroups take entries gridDim.x + (work counter)
+  // one at a time: thread 0 requests the next entry and resolves its block
+  // id while the current block is scattered (a returning device atomic
+  // takes microseconds) and hands both over through alternating LDS slots.
+  int ai = blockIdx.x;
+  const int first = B.act[2 + (ai < last ? ai : last)];
+  const int nh = B.act[0], nact = nh + B.act[1];
+  const bool dynamic = nact > kDynamicRounds * (int)gridDim.x;
+  const auto entry = [&](int k) { return k < nh ? B.act[2 + k] : B.act[2 + last - (k - nh)]; };
+  int blk = ai < nh ? first : (ai < nact ? entry(ai) : 0);
+  for (int it = 0; ai < nact; ++it) {
+    int fetched = 0, next_static = 0;
+    if (dynamic) {
+      if (threadIdx.x == 0) fetched = (int)gridDim.x + atomicAdd(work, 1);
+    } else {
+      next_static = ai + (int)gridDim.x < nact ? entry(ai + (int)gridDim.x) : 0;
+    }
+    blk = __builtin_amdgcn_readfirstlane(blk);
     int ci = 0;
     for (int hi = Bt.ncand - 1; ci < hi;) {
       const int mid = (ci + hi + 1) >> 1;
@@ -613,11 +641,24 @@ EMPM_DEV void block_scatter(const Batch& Bt, const Binning& B, float4* __restric
       if (__ballot(home)) group_scatter<NC>(tile, t0, home, P, s, lane);
       if (active && !home) lane_scatter<NC>(tile, o, ovf, ovf_flag, n, M.g0, P, s);
     }
+    int fetched_blk = 0;
+    if (dynamic && threadIdx.x == 0 && fetched < nact) fetched_blk = entry(fetched);
     __syncthreads();
     float4* dst = tiles + (size_t)blk * kTileNodes;
     for (int k = threadIdx.x; k < kTileNodes; k += blockDim.x)
       dst[k] = make_float4(tile[0][k], tile[1][k], tile[2][k], NC == 4 ? tile[NC - 1][k] : 0.f);
+    if (dynamic && threadIdx.x == 0) {
+      next[it & 1][0] = fetched;
+      next[it & 1][1] = fetched_blk;
+    }
     __syncthreads();
+    if (dynamic) {
+      ai = next[it & 1][0];
+      blk = next[it & 1][1];
+    } else {
+      ai += (int)gridDim.x;
+      blk = next_static;
+    }
   }
 }
 
@@ -710,8 +751,8 @@ __global__ __launch_bounds__(kBlock) void k_stress(Batch Bt, float dt,
 // stress impulse into the block tiles.
 __global__ __launch_bounds__(kBlkWG) void k_p2g(Batch Bt, const float* __restrict__ x, const float* __restrict__ v,
                                                 const float* __restrict__ C, const float* __restrict__ S, Binning B,
-                                                float4* __restrict__ tiles, Overflow ov) {
-  block_scatter<4>(Bt, B, tiles, ov.grid, ov.flag, [&](int p, float* xp, ScatterPayload& P, const DevModel& M) {
+                                                float4* __restrict__ tiles, Overflow ov, int* __restrict__ work) {
+  block_scatter<4>(Bt, B, tiles, ov.grid, ov.flag, work, [&](int p, float* xp, ScatterPayload& P, const DevModel& M) {
     const float dx = M.dx;
 #pragma unroll
     for (int d = 0; d < 3; ++d) xp[d] = x[3 * p + d];
@@ -742,9 +783,15 @@ __global__ __launch_bounds__(kBlkWG) void k_p2g(Batch Bt, const float* __restric
 constexpr int kNodePassGridCap = 1024;
 template <typename F>
 EMPM_DEV void for_near_nodes(const Batch& Bt, const Binning& B, F f) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x, last = (size_t)B.nblk - 1;
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // list entries are read ahead of the count test (the list has nblk slots)
+  int blk_next = B.nlist[(q >> 3) < last ? (q >> 3) : last];
   const size_t npairs = (size_t)(*B.nnear) * 8;
-  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < npairs; q += (size_t)gridDim.x * blockDim.x) {
-    const int blk = B.nlist[q >> 3], r = (int)(q & 7);
+  for (; q < npairs; q += stride) {
+    const int blk = blk_next, r = (int)(q & 7);
+    const size_t qn = (q + stride) >> 3;
+    blk_next = B.nlist[qn < last ? qn : last];
     const DevModel& M = cand_of_block(Bt, blk);
     const int nb = M.nb, lb = blk - M.b0;
     const int ix = 2 * (lb / (nb * nb)) + (r >> 2), iy = 2 * ((lb / nb) % nb) + ((r >> 1) & 1),
@@ -859,7 +906,8 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(Batch Bt, float dt, const fl
                                                     const float* __restrict__ dx1, const float* __restrict__ dv1,
                                                     const float* __restrict__ dC1, const float* __restrict__ dF1,
                                                     float* __restrict__ xbar, float* __restrict__ Febar,
-                                                    float* __restrict__ payload) {
+                                                    float* __restrict__ payload, int* __restrict__ scatter_work) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *scatter_work = 0;  // for the k_g2p_adj_scatter launch that follows
   per_particle(Bt, -1, [&](const DevModel& M, const int p) {
   if (M.sel && M.sel[p] != 0) {
     xbar[3 * p] = xbar[3 * p + 1] = xbar[3 * p + 2] = 0.f;
@@ -954,8 +1002,9 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(Batch Bt, float dt, const fl
 // Scatter of dL/dv_out (the transpose of the G2P gather) into the block tiles.
 __global__ __launch_bounds__(kBlkWG) void k_g2p_adj_scatter(Batch Bt, const float* __restrict__ x,
                                                             const float* __restrict__ payload, Binning B,
-                                                            float4* __restrict__ tiles, Overflow ov) {
-  block_scatter<3>(Bt, B, tiles, ov.grid, ov.flag, [&](int p, float* xp, ScatterPayload& P, const DevModel&) {
+                                                            float4* __restrict__ tiles, Overflow ov,
+                                                            int* __restrict__ work) {
+  block_scatter<3>(Bt, B, tiles, ov.grid, ov.flag, work, [&](int p, float* xp, ScatterPayload& P, const DevModel&) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) xp[d] = x[3 * p + d];
     const float4* q4 = reinterpret_cast<const float4*>(payload + (size_t)kAdjPayload * p);
@@ -1140,12 +1189,15 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
 //                 lanes of a wave share one atomic); counts are zero on entry
 //   k_bin_scan    one thread per block, kBinScanWG blocks per workgroup:
 //                 block totals and their workgroup-local exclusive scan;
-//                 clears `near` and the active and near counts
+//                 clears the active and near counts and the scatter's
+//                 work counter
 //   k_bin_finish  one thread per block: global block offset (the sum of the
 //                 earlier workgroups' totals, reduced by the workgroup),
-//                 per-bin offsets, occupancy, active list, near marks and
-//                 near list; re-zeroes the counts for the next binning
-//   k_bin_place   per particle: perm[offs[key] + rank] = p
+//                 per-bin offsets, occupancy, active list; re-zeroes the
+//                 counts for the next binning
+//   k_bin_place   per particle: perm[offs[key] + rank] = p; per block: the
+//                 near flag (a gather of the 8 occupancy flags whose tiles
+//                 reach the block) and the near list
 // ---------------------------------------------------------------------------
 constexpr int kBinScanWG = 1024;
 
@@ -1153,7 +1205,7 @@ constexpr int kBinScanWG = 1024;
 struct BinBuf {
   int* perm;  // N
   int* offs;  // nb^3 * 8 + 1
-  int* act;   // 1 + nb^3: count, then the ids of the non-empty blocks
+  int* act;   // 2 + nb^3: heavy and light counts, then the non-empty block ids
   int* occ;   // nb^3
   int* near;  // nb^3
   int* nlist; // 1 + nb^3: count, then the ids of the near blocks
@@ -1236,30 +1288,28 @@ EMPM_DEV int block_count(const int* __restrict__ count, int b) {
 
 __global__ __launch_bounds__(kBinScanWG) void k_bin_scan(int nblk, const int* __restrict__ count,
                                                          int* __restrict__ blkoff, int* __restrict__ wsum,
-                                                         int* __restrict__ near, int* __restrict__ act,
-                                                         int* __restrict__ nlist, int* __restrict__ ovf_flag) {
+                                                         int* __restrict__ act, int* __restrict__ nlist,
+                                                         int* __restrict__ ovf_flag,
+                                                         int* __restrict__ p2g_work) {
   __shared__ int smem[kBinScanWG / 64 + 1];
   const int b = blockIdx.x * kBinScanWG + threadIdx.x;
   const int tot = b < nblk ? block_count(count, b) : 0;
   int wtot;
   const int ex = wg_exclusive_scan(tot, smem, wtot);
-  if (b < nblk) {
-    blkoff[b] = ex;
-    near[b] = 0;
-  }
+  if (b < nblk) blkoff[b] = ex;
   if (threadIdx.x == 0) wsum[blockIdx.x] = wtot;
   if (b == 0) {
-    act[0] = 0;
+    act[0] = act[1] = 0;
     nlist[0] = 0;
     *ovf_flag = 0;
+    *p2g_work = 0;  // for the k_p2g launch that follows
   }
 }
 
 __global__ __launch_bounds__(kBinScanWG) void k_bin_finish(Batch Bt, int* __restrict__ count,
                                                            const int* __restrict__ blkoff,
                                                            const int* __restrict__ wsum, int* __restrict__ offs,
-                                                           int* __restrict__ act, int* __restrict__ occ,
-                                                           int* __restrict__ near, int* __restrict__ nlist) {
+                                                           int* __restrict__ act, int* __restrict__ occ) {
   __shared__ int red[kBinScanWG / 64];
   const int nblk = Bt.nblk;
   const int b = blockIdx.x * kBinScanWG + threadIdx.x;
@@ -1293,28 +1343,61 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_finish(Batch Bt, int* __rest
   const bool full = o != run[0];
   occ[b] = full ? 1 : 0;
   if (!full) return;
-  act[1 + atomicAdd(&act[0], 1)] = b;
-  // the blocks of the same simulation owning a node of this block's tile
-  // (nodes 2b .. 2b + 3 per axis)
-  const DevModel& M = cand_of_block(Bt, b);
-  const int nb = M.nb, lb = b - M.b0;
-  const int b0 = lb / (nb * nb), b1 = (lb / nb) % nb, b2 = lb % nb;
+  // list appends are aggregated per wave (one atomic per list)
+  const unsigned long long lt = (1ull << __lane_id()) - 1ull;
+  const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+  int nch = 0;
 #pragma unroll
-  for (int d = 0; d < 8; ++d) {
-    const int o0 = b0 + (d >> 2), o1 = b1 + ((d >> 1) & 1), o2 = b2 + (d & 1);
-    if (o0 < nb && o1 < nb && o2 < nb) {
-      const int nbk = M.b0 + (o0 * nb + o1) * nb + o2;
-      if (atomicExch(&near[nbk], 1) == 0) nlist[1 + atomicAdd(&nlist[0], 1)] = nbk;
-    }
+  for (int k = 0; k < 8; ++k) nch += (c[k] + kWave - 1) / kWave;
+  const bool heavy = nch > kHeavyChunks;
+  const unsigned long long hmask = __ballot(heavy), lmask = __ballot(!heavy);
+  int hbase = 0, lbase = 0;
+  if (__lane_id() == leader) {
+    if (hmask) hbase = atomicAdd(&act[0], __popcll(hmask));
+    if (lmask) lbase = atomicAdd(&act[1], __popcll(lmask));
   }
+  hbase = __shfl(hbase, leader);
+  lbase = __shfl(lbase, leader);
+  if (heavy)
+    act[2 + hbase + __popcll(hmask & lt)] = b;
+  else
+    act[2 + nblk - 1 - (lbase + __popcll(lmask & lt))] = b;
 }
 
-__global__ void k_bin_place(int N, const int2* __restrict__ key_rank, const int* __restrict__ offs,
-                            int* __restrict__ perm) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= N) return;
-  const int2 kr = key_rank[p];
-  perm[offs[kr.x] + kr.y] = p;
+__global__ __launch_bounds__(kBlock) void k_bin_place(Batch Bt, const int2* __restrict__ key_rank,
+                                                      const int* __restrict__ offs, int* __restrict__ perm,
+                                                      const int* __restrict__ occ, int* __restrict__ near,
+                                                      int* __restrict__ nlist) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < Bt.N) {
+    const int2 kr = key_rank[i];
+    perm[offs[kr.x] + kr.y] = i;
+  }
+  // block i is near when a non-empty block of its simulation has it in its
+  // tile (blocks i - {0,1}^3 per axis); near blocks are appended to the
+  // near list, one atomic per wave
+  bool is_near = false;
+  if (i < Bt.nblk) {
+    const DevModel& M = cand_of_block(Bt, i);
+    const int nb = M.nb, lb = i - M.b0;
+    const int b0 = lb / (nb * nb), b1 = (lb / nb) % nb, b2 = lb % nb;
+    int f[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      const int o0 = b0 - (d >> 2), o1 = b1 - ((d >> 1) & 1), o2 = b2 - (d & 1);
+      f[d] = (o0 >= 0 && o1 >= 0 && o2 >= 0) ? occ[M.b0 + (o0 * nb + o1) * nb + o2] : 0;
+    }
+#pragma unroll
+    for (int d = 0; d < 8; ++d) is_near = is_near || f[d] != 0;
+    near[i] = is_near ? 1 : 0;
+  }
+  const unsigned long long m = __ballot(is_near);
+  if (m == 0) return;
+  const int leader = __ffsll(m) - 1;
+  int base = 0;
+  if (__lane_id() == leader) base = atomicAdd(&nlist[0], __popcll(m));
+  base = __shfl(base, leader);
+  if (is_near) nlist[1 + base + __popcll(m & ((1ull << __lane_id()) - 1ull))] = i;
 }
 
 __global__ void k_mu_lam(int n,const float* __restrict__ E, const float* __restrict__ nu, float* __restrict__ mu,
@@ -1383,6 +1466,8 @@ struct empm_ctx {
   int2* key_rank = nullptr;  // N: bin key and rank of each particle
   int* blkoff = nullptr;     // max_blocks: workgroup-local exclusive block offsets
   int* wsum = nullptr;       // block-scan workgroup totals
+  int* work = nullptr;       // 2: work counters of the k_p2g and k_g2p_adj_scatter launches
+  unsigned scatter_wgs[2] = {kBlkGridCap, kBlkGridCap};  // their launch widths: workgroups resident at once
   DevModel* cand_dev = nullptr;     // device table of the simulations of the current call
   DevModel* cand_pinned = nullptr;  // its pinned staging copy
   std::vector<DevModel> cand_last;  // what cand_dev holds
@@ -1614,13 +1699,13 @@ int make_table(const empm_collider* cols, int ncol, ColTable& T) {
 
 size_t pad4(size_t k) { return (k + 3) & ~(size_t)3; }
 
-// Layout of one binning record: [perm N | offs 8 nblk + 1 | act 1 + nblk |
+// Layout of one binning record: [perm N | offs 8 nblk + 1 | act 2 + nblk |
 // occ nblk | near nblk | nlist 1 + nblk], each section padded to a multiple
 // of 4 ints so that the int4 accesses of the binning kernels stay 16-byte
 // aligned.
 size_t bin_record_ints(long long nblk, long long n_particles) {
-  return pad4((size_t)n_particles) + pad4((size_t)nblk * 8 + 1) + 2 * pad4(1 + (size_t)nblk) +
-         2 * pad4((size_t)nblk);
+  return pad4((size_t)n_particles) + pad4((size_t)nblk * 8 + 1) + pad4(2 + (size_t)nblk) +
+         pad4(1 + (size_t)nblk) + 2 * pad4((size_t)nblk);
 }
 
 BinBuf bin_buf(int* rec, int nblk, int n_particles) {
@@ -1628,21 +1713,20 @@ BinBuf bin_buf(int* rec, int nblk, int n_particles) {
   b.perm = rec;
   b.offs = b.perm + pad4((size_t)n_particles);
   b.act = b.offs + pad4((size_t)nblk * 8 + 1);
-  b.occ = b.act + pad4(1 + (size_t)nblk);
+  b.occ = b.act + pad4(2 + (size_t)nblk);
   b.near = b.occ + pad4((size_t)nblk);
   b.nlist = b.near + pad4((size_t)nblk);
   return b;
 }
 
-Binning binning(const BinBuf& b) {
-  return Binning{b.perm, b.offs, b.act + 1, b.act, b.occ, b.near, b.nlist + 1, b.nlist};
+Binning binning(const BinBuf& b, int nblk) {
+  return Binning{b.perm, b.offs, b.act, nblk, b.occ, b.near, b.nlist + 1, b.nlist};
 }
 
 Overflow overflow(const empm_ctx* ctx) { return Overflow{ctx->ovf, ctx->ovf_flag}; }
 
 unsigned nblocks(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-unsigned block_grid(int nblk) { return (unsigned)(nblk < kBlkGridCap ? (nblk > 0 ? nblk : 1) : kBlkGridCap); }
 
 dim3 node_grid(int nblk) {
   const size_t wgs = ((size_t)nblk * 8 + kBlock - 1) / kBlock;
@@ -1657,12 +1741,11 @@ int do_bin(empm_ctx* ctx, const HostBatch& hb, const float* x, const BinBuf& out
   if (hb.N > 0)
     hipLaunchKernelGGL(k_bin_count, dim3(nblocks(hb.N)), dim3(kBlock), 0, st, hb.dev, x, ctx->count, ctx->key_rank);
   hipLaunchKernelGGL(k_bin_scan, dim3(wgs), dim3(kBinScanWG), 0, st, nblk, ctx->count, ctx->blkoff, ctx->wsum,
-                     out.near, out.act, out.nlist, ctx->ovf_flag);
+                     out.act, out.nlist, ctx->ovf_flag, ctx->work);
   hipLaunchKernelGGL(k_bin_finish, dim3(wgs), dim3(kBinScanWG), 0, st, hb.dev, ctx->count, ctx->blkoff, ctx->wsum,
-                     out.offs, out.act, out.occ, out.near, out.nlist);
-  if (hb.N > 0)
-    hipLaunchKernelGGL(k_bin_place, dim3(nblocks(hb.N)), dim3(kBlock), 0, st, hb.N, ctx->key_rank, out.offs,
-                       out.perm);
+                     out.offs, out.act, out.occ);
+  hipLaunchKernelGGL(k_bin_place, dim3(nblocks(hb.N > nblk ? hb.N : nblk)), dim3(kBlock), 0, st, hb.dev,
+                     ctx->key_rank, out.offs, out.perm, out.occ, out.near, out.nlist);
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }
@@ -1679,15 +1762,15 @@ int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
   const BinBuf bb = bin_buf(rec && rec->bins ? rec->bins : ctx->bins, hb.nblk, hb.N);
   int rc = do_bin(ctx, hb, in->x, bb, st);
   if (rc) return rc;
-  const Binning B = binning(bb);
+  const Binning B = binning(bb, hb.nblk);
   if (hb.N > 0 && first) {
     ProfScope ps(ctx, EMPM_KERNEL_STRESS, st);
     EMPM_LAUNCH_RANGES(hb, k_stress, st, dt, in->F, Fe, ctx->S, 1);
   }
   if (hb.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
-    hipLaunchKernelGGL(k_p2g, dim3(block_grid(hb.nblk)), dim3(kBlkWG), 0, st, hb.dev, in->x, in->v, in->C, ctx->S, B,
-                       ctx->tiles, overflow(ctx));
+    hipLaunchKernelGGL(k_p2g, dim3(ctx->scatter_wgs[0]), dim3(kBlkWG), 0, st, hb.dev, in->x, in->v, in->C, ctx->S,
+                       B, ctx->tiles, overflow(ctx), ctx->work);
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
@@ -1719,7 +1802,7 @@ int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
     int rc = do_bin(ctx, hb, in->x, bb, st);
     if (rc) return rc;
   }
-  const Binning B = binning(bb);
+  const Binning B = binning(bb, hb.nblk);
   if (!recorded) {
     if (hb.N > 0) {
       ProfScope ps(ctx, EMPM_KERNEL_STRESS, st);
@@ -1727,8 +1810,8 @@ int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
     }
     if (hb.N > 0) {
       ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
-      hipLaunchKernelGGL(k_p2g, dim3(block_grid(hb.nblk)), dim3(kBlkWG), 0, st, hb.dev, in->x, in->v, in->C, ctx->S,
-                         B, ctx->tiles, overflow(ctx));
+      hipLaunchKernelGGL(k_p2g, dim3(ctx->scatter_wgs[0]), dim3(kBlkWG), 0, st, hb.dev, in->x, in->v, in->C,
+                         ctx->S, B, ctx->tiles, overflow(ctx), ctx->work);
     }
     {
       ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
@@ -1740,12 +1823,12 @@ int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
     {
       ProfScope ps(ctx, EMPM_KERNEL_G2P_ADJ, st);
       hipLaunchKernelGGL(k_g2p_adj, dim3(particle_grid(hb)), dim3(kBlock), 0, st, hb.dev, dt, in->x, Fe, v_out,
-                         d_out->x, d_out->v, d_out->C, d_out->F, ctx->xbar, ctx->Febar, ctx->payload);
+                         d_out->x, d_out->v, d_out->C, d_out->F, ctx->xbar, ctx->Febar, ctx->payload, ctx->work + 1);
     }
     {
       ProfScope ps(ctx, EMPM_KERNEL_G2P_ADJ_SCATTER, st);
-      hipLaunchKernelGGL(k_g2p_adj_scatter, dim3(block_grid(hb.nblk)), dim3(kBlkWG), 0, st, hb.dev, in->x,
-                         ctx->payload, B, ctx->tiles, overflow(ctx));
+      hipLaunchKernelGGL(k_g2p_adj_scatter, dim3(ctx->scatter_wgs[1]), dim3(kBlkWG), 0, st, hb.dev, in->x,
+                         ctx->payload, B, ctx->tiles, overflow(ctx), ctx->work + 1);
     }
   }
   {
@@ -1806,6 +1889,7 @@ int alloc_ctx(empm_ctx* c) {
   e = e == hipSuccess ? hipMalloc((void**)&c->key_rank, np * sizeof(int2)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->blkoff, nblk * sizeof(int)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->wsum, nwsum * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->work, 2 * sizeof(int)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->cand_dev, (size_t)c->max_cand * sizeof(DevModel)) : e;
   e = e == hipSuccess ? hipHostMalloc((void**)&c->cand_pinned, (size_t)c->max_cand * sizeof(DevModel)) : e;
   e = e == hipSuccess ? hipEventCreateWithFlags(&c->cand_upload, hipEventDisableTiming) : e;
@@ -1813,6 +1897,16 @@ int alloc_ctx(empm_ctx* c) {
   e = e == hipSuccess ? hipMemset(c->ovf, 0, G * sizeof(float4)) : e;
   e = e == hipSuccess ? hipMemset(c->ovf_flag, 0, sizeof(int)) : e;
   e = e == hipSuccess ? hipDeviceSynchronize() : e;
+  // the block-owned scatters are launched as wide as the device holds them
+  int cus = 0;
+  e = e == hipSuccess ? hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) : e;
+  const void* scatter_kernels[2] = {reinterpret_cast<const void*>(k_p2g),
+                                    reinterpret_cast<const void*>(k_g2p_adj_scatter)};
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, scatter_kernels[i], kBlkWG, 0);
+    if (e == hipSuccess && per_cu > 0 && cus > 0) c->scatter_wgs[i] = (unsigned)(per_cu * cus);
+  }
   if (e != hipSuccess)
     return set_err(e == hipErrorOutOfMemory ? EMPM_E_OOM : EMPM_E_HIP, "empm_create: %s", hipGetErrorString(e));
   return EMPM_OK;
@@ -1991,6 +2085,7 @@ void empm_destroy(empm_ctx* c) {
   (void)hipFree(c->key_rank);
   (void)hipFree(c->blkoff);
   (void)hipFree(c->wsum);
+  (void)hipFree(c->work);
   (void)hipFree(c->cand_dev);
   if (c->cand_pinned) (void)hipHostFree(c->cand_pinned);
   if (c->cand_upload) (void)hipEventDestroy(c->cand_upload);
