@@ -1092,55 +1092,86 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
   Stencil s;
   make_stencil(xp, M.inv_dx, s);
   const float mp = M.mass[p], Vp = M.vol[p];
-  float vb[3] = {0.f, 0.f, 0.f}, Ctb[9], taub[9], fb[3] = {0.f, 0.f, 0.f};
-  m_zero(Ctb);
-  m_zero(taub);
-  float mb = 0.f, Vb = 0.f;
+  // The node loop accumulates, in grid units (d = node - f, gf = dw/df):
+  //   S0 = sum w a,  S1 = sum w a (x) d,  T = sum a (x) gf,  W = sum w b,
+  //   fA = sum (a.u + b) gf  (u = v + h Ct d),  fB = sum H (tau^T a)
+  // with (a, b) = dL/d(mv, m) of the node and H its weight Hessian; every
+  // per-particle factor (m, V, dt, h = dx) is applied once afterwards.
+  float S0[3] = {0.f, 0.f, 0.f}, S1[9], T[9], fA[3] = {0.f, 0.f, 0.f}, fB[3] = {0.f, 0.f, 0.f};
+  m_zero(S1);
+  m_zero(T);
+  float W = 0.f;
+  const float h = M.dx;
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int b = 0; b < 3; ++b)
+    for (int b = 0; b < 3; ++b) {
+      const float da = (float)a - s.f[0], db = (float)b - s.f[1];
+      const float X = s.w[0][a], Y = s.w[1][b], dX = s.dw[0][a], dY = s.dw[1][b];
+      const float ddX = a == 1 ? -2.f : 1.f, ddY = b == 1 ? -2.f : 1.f;
+      const float XY = X * Y;
+      float uab[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) uab[i] = vp[i] + h * (Ct[i * 3 + 0] * da + Ct[i * 3 + 1] * db);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
         if (!node_in(ix, iy, iz, M.n)) continue;
         const float4 ga = gadj_in[M.g0 + node_id(ix, iy, iz, M.n)];
         const float av[3] = {ga.x, ga.y, ga.z};
-        const float bm = ga.w;
-        const float dpos[3] = {((float)a - s.f[0]) * M.dx, ((float)b - s.f[1]) * M.dx, ((float)c - s.f[2]) * M.dx};
-        const float wt = sw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c);
-        const float gw[3] = {sdw(s, 0, a) * sw(s, 1, b) * sw(s, 2, c) * M.inv_dx, sw(s, 0, a) * sdw(s, 1, b) * sw(s, 2, c) * M.inv_dx,
-                             sw(s, 0, a) * sw(s, 1, b) * sdw(s, 2, c) * M.inv_dx};
-        float u[3], tg[3];
+        const float dc = (float)c - s.f[2];
+        const float Z = s.w[2][c], dZ = s.dw[2][c], ddZ = c == 1 ? -2.f : 1.f;
+        const float wt = XY * Z;
+        const float gf[3] = {dX * Y * Z, X * dY * Z, XY * dZ};
+        float au = 0.f;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          u[i] = vp[i] + Ct[i * 3 + 0] * dpos[0] + Ct[i * 3 + 1] * dpos[1] + Ct[i * 3 + 2] * dpos[2];
-          tg[i] = tau[i * 3 + 0] * gw[0] + tau[i * 3 + 1] * gw[1] + tau[i * 3 + 2] * gw[2];
+          const float wa = wt * av[i];
+          S0[i] += wa;
+          S1[i * 3 + 0] += wa * da;
+          S1[i * 3 + 1] += wa * db;
+          S1[i * 3 + 2] += wa * dc;
+          T[i * 3 + 0] += av[i] * gf[0];
+          T[i * 3 + 1] += av[i] * gf[1];
+          T[i * 3 + 2] += av[i] * gf[2];
+          au += av[i] * (uab[i] + h * Ct[i * 3 + 2] * dc);
         }
-        const float au = av[0] * u[0] + av[1] * u[1] + av[2] * u[2];
-        const float atg = av[0] * tg[0] + av[1] * tg[1] + av[2] * tg[2];
-        const float wm = wt * mp;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          vb[i] += wm * av[i];
-#pragma unroll
-          for (int j = 0; j < 3; ++j) {
-            Ctb[i * 3 + j] += wm * av[i] * dpos[j];
-            taub[i * 3 + j] += -dt * Vp * av[i] * gw[j];
-          }
-        }
-        mb += wt * au + wt * bm;
-        Vb += -dt * atg;
-        const float wb = mp * au + mp * bm;
-        float gwb[3];
+        W += wt * ga.w;
+        const float ab = au + ga.w;
+        float ta[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-          const float db = wm * (Ct[0 * 3 + j] * av[0] + Ct[1 * 3 + j] * av[1] + Ct[2 * 3 + j] * av[2]);
-          fb[j] -= M.dx * db;
-          gwb[j] = -dt * Vp * (tau[0 * 3 + j] * av[0] + tau[1 * 3 + j] * av[1] + tau[2 * 3 + j] * av[2]);
+          fA[j] += ab * gf[j];
+          ta[j] = tau[0 * 3 + j] * av[0] + tau[1 * 3 + j] * av[1] + tau[2 * 3 + j] * av[2];
         }
-        weight_adj(s, a, b, c, M.inv_dx, wb, gwb, fb);
+        // weight Hessian in grid units (symmetric)
+        const float hxx = ddX * Y * Z, hyy = X * ddY * Z, hzz = XY * ddZ;
+        const float hxy = dX * dY * Z, hxz = dX * Y * dZ, hyz = X * dY * dZ;
+        fB[0] += hxx * ta[0] + hxy * ta[1] + hxz * ta[2];
+        fB[1] += hxy * ta[0] + hyy * ta[1] + hyz * ta[2];
+        fB[2] += hxz * ta[0] + hyz * ta[1] + hzz * ta[2];
       }
+    }
+  // dL/d(v, C~, tau, m, V) of the particle and dL/df (f = x inv_dx - base)
+  const float sV = -dt * Vp * M.inv_dx;  // d(stress impulse)/d(tau . grad w) in grid units
+  float vb[3], Ctb[9], taub[9], fb[3];
+  float mb = W, Vb = 0.f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    vb[i] = mp * S0[i];
+    mb += S0[i] * vp[i];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      Ctb[i * 3 + j] = mp * h * S1[i * 3 + j];
+      taub[i * 3 + j] = sV * T[i * 3 + j];
+      mb += h * Ct[i * 3 + j] * S1[i * 3 + j];
+      Vb += -dt * M.inv_dx * tau[i * 3 + j] * T[i * 3 + j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    fb[j] = mp * fA[j] - mp * h * (Ct[0 * 3 + j] * S0[0] + Ct[1 * 3 + j] * S0[1] + Ct[2 * 3 + j] * S0[2]) +
+            sV * fB[j];
   float xo[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) xo[i] = xbar[3 * p + i] + M.inv_dx * fb[i];
