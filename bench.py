@@ -319,13 +319,25 @@ def bench_population(args, dev):
             tb, fb = timed(lambda: ev.fitness(cands), args.pop_steps)
             ts, fs = timed(lambda: ev.fitness_sequential(cands), args.pop_steps)
         else:
-            free, _ = torch.cuda.mem_get_info(dev)
-            K = next(k for k in (1, 2, 3, 4, 5, 6, 10, 15, 20, 30, 50, 100, 300)
-                     if ev.tape_bytes(k) <= 0.7 * free)
+            # the shortest checkpoint interval whose tape fits the free HBM
+            # (every substep recorded when it fits: no replay)
+            free, total = torch.cuda.mem_get_info(dev)
+            Ks = [k for k in (1, 2, 3, 4, 5, 6, 10, 15, 20, 30, 50, 100, 300) if ev.tape_bytes(k) <= 0.9 * free]
+            tb = None
+            for K in Ks:
+                try:
+                    tb, (fb, gb) = timed(lambda: ev.fitness_and_grad(cands, ckpt_every=K), args.pop_steps)
+                    break
+                except torch.OutOfMemoryError:
+                    log(f"[bench] population fwd_bwd: tape of {ev.tape_bytes(K) / 1e9:.0f} GB at ckpt_every={K} "
+                        f"did not fit; next interval")
+                    torch.cuda.empty_cache()
+            if tb is None:
+                raise RuntimeError("no checkpoint interval fits the device memory")
             info["ckpt_every_batched"] = K
             info["ckpt_every_sequential"] = 1
             info["tape_gb_batched"] = ev.tape_bytes(K) / 1e9
-            tb, (fb, gb) = timed(lambda: ev.fitness_and_grad(cands, ckpt_every=K), args.pop_steps)
+            info["hbm_free_gb_at_start"] = free / 1e9
             ts, (fs, gs) = timed(lambda: ev.fitness_and_grad_sequential(cands, ckpt_every=1), args.pop_steps)
             gerr = (gb - gs).abs().max(1).values / gs.abs().max(1).values
             info["grad_rel_err_batched_vs_sequential"] = float(gerr.max())
