@@ -94,8 +94,10 @@ def main():
     ap.add_argument("--substeps", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--only", choices=("all", "single", "population"), default="all",
-                    help="run only the single-rollout workload or only the population workload")
+    ap.add_argument("--only", choices=("all", "single", "population", "serving"), default="all",
+                    help="run only the single-rollout, the population or the serving workload")
+    ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving workload")
+    ap.add_argument("--serve-qps", type=float, default=10.0, help="Poisson arrival rate of the serving workload")
     ap.add_argument("--pop-steps", type=int, default=2, help="timed evaluations of the population workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (3 times): the known-size kernel that "
@@ -124,6 +126,9 @@ def main():
         del src, dst
     if args.only == "population":
         print(json.dumps({"population": bench_population(args, dev)}), flush=True)
+        return
+    if args.only == "serving":
+        print(json.dumps({"serving": bench_serving(args, dev)}), flush=True)
         return
     sc = scenes.ellipsoid_c2(n_particles=args.particles, n_grid=args.grid, n_steps=args.substeps, param_seed=3)
     tprm = scenes.sample_params_c2(2)
@@ -243,11 +248,13 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, args.cpu_seconds)
-    population = None
+    population = serving = None
     if args.only == "all":
         del ro, tro, ctx
         torch.cuda.empty_cache()
         population = bench_population(args, dev)
+        torch.cuda.empty_cache()
+        serving = bench_serving(args, dev)
 
     out = {
         "metric": "particle-substeps/s (fwd+bwd) @200k pts/64^3",
@@ -279,6 +286,7 @@ def main():
                      "avg_launch_us": avg_ms * 1000.0, "launches": l.value},
         "cpu_baseline": cpu,
         "batched": population,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 
@@ -351,6 +359,39 @@ def bench_population(args, dev):
         del ev
         torch.cuda.empty_cache()
     out["unit"] = "particle-substeps/s"
+    return out
+
+
+def bench_serving(args, dev):
+    """Continuous batching under Poisson arrivals: a seeded mix of rollout
+    requests (10k-500k particles, 32^3 / 64^3 / 128^3 grids, 100-2000
+    substeps at dt 5e-5, 60% forward only and 40% forward + gradient, jelly /
+    Neo-Hookean + von Mises / sand) submitted at --serve-qps into one
+    engine; reports p50/p99 latency (arrival to result) and the aggregate
+    particle-substeps/s (forward substeps of all requests over the wall
+    time)."""
+    import torch
+
+    from empm_amd import scenes
+    from empm_amd import serving as sv
+
+    mix = scenes.serving_requests_c5(n_requests=args.serve_requests)
+    reqs = [sv.RolloutRequest(**{k: v for k, v in m.items() if k != "shape"}) for m in mix]
+    eng = sv.ServingEngine(dev, max_particles=8_000_000, max_cand=24, max_grid=128, dt=5e-5, chunk=50)
+    # warm-up: one small request of each mode through the engine
+    for r in reqs[:2]:
+        w = sv.RolloutRequest(**{**r.__dict__, "n_steps": 60})
+        eng.submit(w)
+    eng.run_until_idle()
+    eng.ticks = eng.calls = 0
+    torch.cuda.synchronize()
+    out = sv.serve_poisson(eng, reqs, args.serve_qps, log=log)
+    out["workload"] = (f"{len(reqs)} rollout requests, Poisson arrivals at {args.serve_qps}/s: N log-uniform 10k-500k, "
+                       f"grids 32/64/128, 100-2000 substeps (dt 5e-5), 40% with gradient")
+    out["unit"] = "particle-substeps/s"
+    out["hbm_budget_gb"] = eng.sched.budget / 1e9
+    log(f"[bench] serving: {out}")
+    eng.close()
     return out
 
 

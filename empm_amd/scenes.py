@@ -234,3 +234,37 @@ def population_c3(n_cand=32, seed=4, n_grid=64, n_steps=300, dt=1e-4, differenti
         sc.params["v0"] = v0
         out.append(sc)
     return out
+
+
+def serving_requests_c5(n_requests=60, seed=6, grids=(32, 64, 128), n_range=(10_000, 500_000),
+                        steps_range=(100, 2000), grad_fraction=0.4, drop_height=0.3):
+    """A serving request mix: per request, N log-uniform over n_range, a grid
+    from `grids`, a horizon uniform over steps_range, forward only or (with
+    probability grad_fraction) forward + gradient, a material from {jelly
+    (0), Neo-Hookean + von Mises (7), sand (2)} with the single-rollout
+    workload's parameter distribution (log-uniform E in [1e4, 1e6],
+    uniform nu in [0.2, 0.45], log-uniform yield stress in [1e2, 1e4]), and
+    an object shaped as in the population workload (cube, rod or ellipsoid)
+    starting at the moment of impact on the 3-cell floor band.  Gradient
+    requests carry a target 1 cm below the initial positions.  Returns dicts
+    of RolloutRequest fields."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n_requests):
+        N = int(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
+        grid = int(grids[int(rng.integers(len(grids)))])
+        T = int(rng.integers(steps_range[0], steps_range[1] + 1))
+        grad = bool(rng.random() < grad_fraction)
+        material = (0, 7, 2)[int(rng.integers(3))]
+        prm = sample_params_c2(int(rng.integers(1 << 30)))
+        shape = C3_SHAPES[int(rng.integers(len(C3_SHAPES)))]
+        dx = 1.0 / grid
+        srng = np.random.Generator(np.random.PCG64([seed, i]))
+        x, vol_total, _ = shape_points(shape, N, srng, (0.5, 0.5, 3 * dx + C3_HALF_Z[shape] + 0.25 * dx))
+        x = x.astype(np.float32)
+        v = np.zeros((N, 3), np.float32)
+        v[:, 2] = -np.sqrt(2 * 9.8 * drop_height)
+        out.append(dict(x=x, v=v, vol=np.full(N, vol_total / N, np.float32), material=material, E=prm["E"],
+                        nu=prm["nu"], yield_stress=prm["yield_stress"], n_grid=grid, n_steps=T, want_grad=grad,
+                        target=(x - np.array([0.0, 0.0, 0.01], np.float32)) if grad else None, shape=shape))
+    return out

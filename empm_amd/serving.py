@@ -408,7 +408,7 @@ class ServingEngine:
         now = time.perf_counter()
         done = []
         for s in self.sched.slots.values():
-            if s.phase == DONE and s.rid not in self.results:
+            if s.phase == DONE and s.dev is not None:
                 self.results[s.rid] = self._result(s, now)
                 s.dev = None
                 done.append(s.rid)
