@@ -268,3 +268,45 @@ def serving_requests_c5(n_requests=60, seed=6, grids=(32, 64, 128), n_range=(10_
                         nu=prm["nu"], yield_stress=prm["yield_stress"], n_grid=grid, n_steps=T, want_grad=grad,
                         target=(x - np.array([0.0, 0.0, 0.01], np.float32)) if grad else None, shape=shape))
     return out
+
+
+def slab_c4(n_particles=300_000, n_grid=96, seed=5, dims=(0.5, 0.05, 0.2), n_frames=30, frame_steps=100, dt=5e-5,
+            vmax=0.5, gripper_half=(0.05, 0.05, 0.03)):
+    """Long-horizon embodied workload: a slab of dims (x, y, z) = 0.5 x 0.05 x
+    0.2 (standing on the floor, z up) of uniformly sampled particles on an
+    n_grid^3 grid with 3-cell sticky walls, Neo-Hookean + von Mises
+    (material 7), rho = 1000, dt = 5e-5, n_frames frames of frame_steps
+    substeps (3000 substeps).  A kinematic box gripper
+    (set_velocity_on_cuboid, half extents gripper_half) starts just above the
+    slab's top at its centre and follows a seeded piecewise-linear path:
+    pieces of 5 frames, the first pressing down, the others dragging in the
+    x-z plane, every speed at most vmax (0.5 m/s).  params["gripper"] holds
+    the start point, the half extents and the (n_frames, 3) per-frame
+    velocities."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dx = 1.0 / n_grid
+    d = np.asarray(dims, np.float64)
+    lo = np.array([0.5 - d[0] / 2, 0.5 - d[1] / 2, 3 * dx + 0.25 * dx])
+    x = (lo + rng.random((n_particles, 3)) * d).astype(np.float32)
+    N = x.shape[0]
+    f = lambda v: np.full(N, v, np.float32)
+    prm = sample_params_c2(int(rng.integers(1 << 30)))
+    vel = np.zeros((n_frames, 3))
+    for p0 in range(0, n_frames, 5):
+        if p0 == 0:
+            v = np.array([0.0, 0.0, -rng.uniform(0.2, vmax)])
+        else:
+            ang = rng.uniform(-0.25 * np.pi, 0.25 * np.pi) + (np.pi if rng.random() < 0.5 else 0.0)
+            s = rng.uniform(0.1, vmax)
+            v = np.array([s * np.cos(ang), 0.0, -abs(s * np.sin(ang)) * 0.3])
+        vel[p0:p0 + 5] = v
+    top = lo[2] + d[2]
+    grip = {"point": (0.5, 0.5, top + gripper_half[2] + 0.5 * dx), "half": tuple(gripper_half), "velocity": vel}
+    sc = Scene(name=f"slab_{N}_{n_grid}^3_nh_vm", n_grid=n_grid, grid_lim=1.0, dt=dt, n_steps=n_frames * frame_steps,
+               material=7, x=x, vol=f(float(np.prod(d)) / N), rho=f(1000.0), E=f(prm["E"]), nu=f(prm["nu"]),
+               yield_stress=f(prm["yield_stress"]), colliders=sticky_walls(n_grid, 3), params=prm)
+    sc.params["gripper"] = grip
+    sc.params["frame_steps"] = frame_steps
+    sc.params["n_frames"] = n_frames
+    sc.params["v0"] = np.zeros((N, 3), np.float32)
+    return sc
