@@ -94,8 +94,8 @@ def main():
     ap.add_argument("--substeps", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--only", choices=("all", "single", "population", "serving"), default="all",
-                    help="run only the single-rollout, the population or the serving workload")
+    ap.add_argument("--only", choices=("all", "single", "population", "serving", "long_horizon"), default="all",
+                    help="run only the single-rollout, the population, the serving or the long-horizon workload")
     ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving workload")
     ap.add_argument("--serve-qps", type=float, default=10.0, help="Poisson arrival rate of the serving workload")
     ap.add_argument("--pop-steps", type=int, default=2, help="timed evaluations of the population workload")
@@ -129,6 +129,9 @@ def main():
         return
     if args.only == "serving":
         print(json.dumps({"serving": bench_serving(args, dev)}), flush=True)
+        return
+    if args.only == "long_horizon":
+        print(json.dumps({"long_horizon": bench_long_horizon(args, dev)}), flush=True)
         return
     sc = scenes.ellipsoid_c2(n_particles=args.particles, n_grid=args.grid, n_steps=args.substeps, param_seed=3)
     tprm = scenes.sample_params_c2(2)
@@ -248,13 +251,15 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, args.cpu_seconds)
-    population = serving = None
+    population = serving = long_horizon = None
     if args.only == "all":
         del ro, tro, ctx
         torch.cuda.empty_cache()
         population = bench_population(args, dev)
         torch.cuda.empty_cache()
         serving = bench_serving(args, dev)
+        torch.cuda.empty_cache()
+        long_horizon = bench_long_horizon(args, dev)
 
     out = {
         "metric": "particle-substeps/s (fwd+bwd) @200k pts/64^3",
@@ -287,6 +292,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": population,
         "serving": serving,
+        "long_horizon": long_horizon,
     }
     print(json.dumps(out), flush=True)
 
@@ -392,6 +398,38 @@ def bench_serving(args, dev):
     out["hbm_budget_gb"] = eng.sched.budget / 1e9
     log(f"[bench] serving: {out}")
     eng.close()
+    return out
+
+
+def bench_long_horizon(args, dev):
+    """Long-horizon embodied gradient: 8 parameter sets of a 300k-particle
+    slab on 96^3 grids pressed and dragged by a box gripper, 3000 substeps
+    (30 frames of 100, dt 5e-5), position loss every frame, checkpoint every
+    50 substeps with recompute; one step = forward of all sets + reverse
+    sweep; throughput = 8 * N * 3000 / step time."""
+    import torch
+
+    from empm_amd import long_horizon as lh
+    from empm_amd import scenes
+    from empm_amd.population import Candidate
+
+    sc = scenes.slab_c4()
+    sets = [Candidate(**{k: scenes.sample_params_c2(100 + i)[k] for k in ("E", "nu", "yield_stress")})
+            for i in range(8)]
+    b = lh.LongHorizonBatch(sc, len(sets), dev, ckpt_every=50)
+    b.set_targets([Candidate(**{k: sc.params[k] for k in ("E", "nu", "yield_stress")})] * len(sets))
+    b.loss_and_grad(sets)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss, grad = b.loss_and_grad(sets)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    T = sc.n_steps
+    out = {"workload": f"8 parameter sets x slab of {sc.n} particles, 96^3 grids, box gripper, {T} substeps "
+                       f"(dt 5e-5), position loss every 100 substeps, checkpoint every 50",
+           "value": len(sets) * sc.n * T / el, "unit": "particle-substeps/s (fwd+bwd)", "seconds_per_step": el,
+           "loss": loss.tolist(), "grad_logE_nu_logY": grad.tolist()}
+    log(f"[bench] long horizon: {el:.2f} s per gradient batch, {out['value']:.3e} p-substeps/s")
     return out
 
 
