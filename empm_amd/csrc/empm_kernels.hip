@@ -1266,18 +1266,26 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(Batch Bt, const float* __r
   const int lane = __lane_id();
   const bool active = p < Bt.N;
   const int key = active ? bin_of(x, p, cand_of_particle(Bt, p)) : -1;
+  // group the lanes by key first (ballots only), then every group leader
+  // issues its atomic in the same instruction: one atomic round trip per wave
   unsigned long long pending = __ballot(active);
+  int my_leader = lane, my_rank = 0, group_size = 0;
   while (pending) {
     const int leader = __ffsll((unsigned long long)pending) - 1;
     const int lkey = __shfl(key, leader);
     const bool in = ((pending >> lane) & 1ull) && key == lkey;
     const unsigned long long grp = __ballot(in);
     pending &= ~grp;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&count[lkey], __popcll(grp));
-    base = __shfl(base, leader);
-    if (in) key_rank[p] = make_int2(key, base + __popcll(grp & ((1ull << lane) - 1ull)));
+    if (in) {
+      my_leader = leader;
+      my_rank = __popcll(grp & ((1ull << lane) - 1ull));
+    }
+    if (lane == leader) group_size = __popcll(grp);
   }
+  int base = 0;
+  if (group_size > 0) base = atomicAdd(&count[key], group_size);
+  base = __shfl(base, my_leader);
+  if (active) key_rank[p] = make_int2(key, base + my_rank);
 }
 
 // Exclusive scan of one value per thread over the workgroup (kBinScanWG
