@@ -474,6 +474,37 @@ EMPM_DEV void node_contrib(const ScatterPayload& P, const Stencil& s, int a, int
   if (NC == 4) out[3] = wt * P.m;
 }
 
+// The same contribution factored by stencil column: the terms shared by the
+// three nodes (a, b, 0..2) are formed once, and each node then costs three
+// operations per component: out[k] = wc (X_k + Y_k d2) + dwc Z_k,
+// out[3] = wc Wm.
+struct ColumnTerms {
+  float X[3], Y[3], Z[3], Wm;
+};
+
+template <int NC>
+EMPM_DEV ColumnTerms column_terms(const ScatterPayload& P, const Stencil& s, int a, int b, float wa, float wb,
+                                  float dwa, float dwb) {
+  const float d0 = (float)a - s.f[0], d1 = (float)b - s.f[1];
+  const float wab = wa * wb, ga = dwa * wb, gb = wa * dwb;
+  ColumnTerms t;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    t.X[k] = wab * (P.A[k] + P.B[3 * k] * d0 + P.B[3 * k + 1] * d1) + P.S[3 * k] * ga + P.S[3 * k + 1] * gb;
+    t.Y[k] = wab * P.B[3 * k + 2];
+    t.Z[k] = wab * P.S[3 * k + 2];
+  }
+  t.Wm = NC == 4 ? wab * P.m : 0.f;
+  return t;
+}
+
+template <int NC>
+EMPM_DEV void column_node(const ColumnTerms& t, float d2, float wc, float dwc, float* out) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) out[k] = wc * (t.X[k] + t.Y[k] * d2) + dwc * t.Z[k];
+  if (NC == 4) out[3] = wc * t.Wm;
+}
+
 // Wave-level scatter of the lanes flagged `in_group` (all with stencil base =
 // the chunk's cell base) into the LDS tile at tile-local node offset t0.
 // Nodes are taken NPR per round (NPR * NC <= 32 values); every lane of the
@@ -505,7 +536,8 @@ EMPM_DEV void group_scatter(float (*tile)[kTileNodes], const int* t0, bool in_gr
       if (node < 27) {
         const int a = node / 9, b = (node / 3) % 3, c = node % 3;
         float o[4];
-        node_contrib<NC>(P, s, a, b, c, w0[a], w1[b], w2[c], g0[a], g1[b], g2[c], o);
+        const ColumnTerms t = column_terms<NC>(P, s, a, b, w0[a], w1[b], g0[a], g1[b]);
+        column_node<NC>(t, (float)c - s.f[2], w2[c], g2[c], o);
 #pragma unroll
         for (int k = 0; k < NC; ++k) v[j * NC + k] = o[k];
       }
