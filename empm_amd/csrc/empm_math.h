@@ -87,26 +87,24 @@ EMPM_DEV void m_udvt(const float* U, const float* d, const float* V, float* o) {
 // Gram-Schmidt with u2 = u0 x u1 so that det U = +1 and s2 = u2 . b2 carries
 // the sign of det F.
 // ---------------------------------------------------------------------------
+// One Jacobi rotation zeroing a_pq of the symmetric A = F^T F, held as its
+// diagonal and upper triangle (app, aqq, apq; arp, arq for the third index
+// r): with t = tan of the rotation angle, a_pp -= t a_pq, a_qq += t a_pq,
+// a_pq = 0 and (a_rp, a_rq) rotate by (c, s); the rotation is accumulated
+// into the columns p, q of W.
 template <int P, int Q>
-EMPM_DEV void jacobi_rot(float* A, float* W) {
-  const float apq = A[P * 3 + Q];
+EMPM_DEV void jacobi_rot(float& app, float& aqq, float& apq, float& arp, float& arq, float* W) {
   if (fabsf(apq) < 1e-30f) return;
-  const float theta = (A[Q * 3 + Q] - A[P * 3 + P]) / (2.f * apq);
+  const float theta = (aqq - app) / (2.f * apq);
   const float t = copysignf(1.f, theta) / (fabsf(theta) + sqrtf(theta * theta + 1.f));
   const float c = rsqrtf(t * t + 1.f);
   const float s = t * c;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const float akp = A[k * 3 + P], akq = A[k * 3 + Q];
-    A[k * 3 + P] = c * akp - s * akq;
-    A[k * 3 + Q] = s * akp + c * akq;
-  }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const float apk = A[P * 3 + k], aqk = A[Q * 3 + k];
-    A[P * 3 + k] = c * apk - s * aqk;
-    A[Q * 3 + k] = s * apk + c * aqk;
-  }
+  app -= t * apq;
+  aqq += t * apq;
+  apq = 0.f;
+  const float rp = arp, rq = arq;
+  arp = c * rp - s * rq;
+  arq = s * rp + c * rq;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const float wkp = W[k * 3 + P], wkq = W[k * 3 + Q];
@@ -131,20 +129,18 @@ EMPM_DEV void swap_cols_neg(float* W, float* ev) {
 }
 
 EMPM_DEV void svd3(const float* F, float* U, float* s, float* V) {
-  float A[9];
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      A[r * 3 + c] = F[0 * 3 + r] * F[0 * 3 + c] + F[1 * 3 + r] * F[1 * 3 + c] + F[2 * 3 + r] * F[2 * 3 + c];
+  const auto ata = [&](int r, int c) {
+    return F[0 * 3 + r] * F[0 * 3 + c] + F[1 * 3 + r] * F[1 * 3 + c] + F[2 * 3 + r] * F[2 * 3 + c];
+  };
+  float a00 = ata(0, 0), a11 = ata(1, 1), a22 = ata(2, 2), a01 = ata(0, 1), a02 = ata(0, 2), a12 = ata(1, 2);
   m_eye(V);
 #pragma unroll
   for (int sweep = 0; sweep < 4; ++sweep) {
-    jacobi_rot<0, 1>(A, V);
-    jacobi_rot<0, 2>(A, V);
-    jacobi_rot<1, 2>(A, V);
+    jacobi_rot<0, 1>(a00, a11, a01, a02, a12, V);
+    jacobi_rot<0, 2>(a00, a22, a02, a01, a12, V);
+    jacobi_rot<1, 2>(a11, a22, a12, a01, a02, V);
   }
-  float ev[3] = {A[0], A[4], A[8]};
+  float ev[3] = {a00, a11, a22};
   swap_cols_neg<0>(V, ev);
   swap_cols_neg<1>(V, ev);
   swap_cols_neg<0>(V, ev);
