@@ -880,34 +880,53 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
   const float xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
   Stencil s;
   make_stencil(xp, M.inv_dx, s);
+  // v' = sum w v_i, C' = 4 inv_dx sum w v_i (x) d_i, grad v = sum v_i (x) grad w_i,
+  // factored by stencil column (a, b): per column the sums over c of v Z,
+  // v Z d_z and v dZ, then the column's x and y weights once
   float nv[3] = {0.f, 0.f, 0.f}, nC[9], nG[9];
   m_zero(nC);
   m_zero(nG);
   const float c4 = M.inv_dx * 4.f;
+  float zd[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) zd[c] = s.w[2][c] * ((float)c - s.f[2]);
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int b = 0; b < 3; ++b)
+    for (int b = 0; b < 3; ++b) {
+      float P[3] = {0.f, 0.f, 0.f}, Q[3] = {0.f, 0.f, 0.f}, R[3] = {0.f, 0.f, 0.f};
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
         const bool in = node_in(ix, iy, iz, M.n);
         const float4 gv = v_out[M.g0 + (in ? node_id(ix, iy, iz, M.n) : 0)];
         const float gvv[3] = {in ? gv.x : 0.f, in ? gv.y : 0.f, in ? gv.z : 0.f};
-        const float dpos[3] = {(float)a - s.f[0], (float)b - s.f[1], (float)c - s.f[2]};
-        const float wt = s.w[0][a] * s.w[1][b] * s.w[2][c];
-        const float gw[3] = {s.dw[0][a] * s.w[1][b] * s.w[2][c] * M.inv_dx, s.w[0][a] * s.dw[1][b] * s.w[2][c] * M.inv_dx,
-                             s.w[0][a] * s.w[1][b] * s.dw[2][c] * M.inv_dx};
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          nv[i] += gvv[i] * wt;
-#pragma unroll
-          for (int j = 0; j < 3; ++j) {
-            nC[i * 3 + j] += gvv[i] * dpos[j] * (wt * c4);
-            nG[i * 3 + j] += gvv[i] * gw[j];
-          }
+          P[i] += gvv[i] * s.w[2][c];
+          Q[i] += gvv[i] * zd[c];
+          R[i] += gvv[i] * s.dw[2][c];
         }
       }
+      const float X = s.w[0][a], Y = s.w[1][b];
+      const float XY = X * Y, XYd0 = XY * ((float)a - s.f[0]), XYd1 = XY * ((float)b - s.f[1]);
+      const float dXY = s.dw[0][a] * Y, XdY = X * s.dw[1][b];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        nv[i] += XY * P[i];
+        nC[i * 3 + 0] += XYd0 * P[i];
+        nC[i * 3 + 1] += XYd1 * P[i];
+        nC[i * 3 + 2] += XY * Q[i];
+        nG[i * 3 + 0] += dXY * P[i];
+        nG[i * 3 + 1] += XdY * P[i];
+        nG[i * 3 + 2] += XY * R[i];
+      }
+    }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    nC[k] *= c4;
+    nG[k] *= M.inv_dx;
+  }
   const float amin = M.dx * 2.f;
   const float amax = M.grid_lim - M.dx * 2.f;
 #pragma unroll
