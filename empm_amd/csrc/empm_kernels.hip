@@ -1145,64 +1145,95 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
   const float mp = M.mass[p], Vp = M.vol[p];
   // The node loop accumulates, in grid units (d = node - f, gf = dw/df):
   //   S0 = sum w a,  S1 = sum w a (x) d,  T = sum a (x) gf,  W = sum w b,
-  //   fA = sum (a.u + b) gf  (u = v + h Ct d),  fB = sum H (tau^T a)
+  //   fA = sum (a.u + b) gf  (u = v + h Ct d),  Hs_jk = sum H_jk a
   // with (a, b) = dL/d(mv, m) of the node and H its weight Hessian; every
-  // per-particle factor (m, V, dt, h = dx) is applied once afterwards.
-  float S0[3] = {0.f, 0.f, 0.f}, S1[9], T[9], fA[3] = {0.f, 0.f, 0.f}, fB[3] = {0.f, 0.f, 0.f};
+  // per-particle factor (m, V, dt, h = dx, tau) is applied once afterwards.
+  // All are linear in (a, b) with products of per-axis weights, so each
+  // stencil column (nodes (a, b, 0..2)) is first summed over c with the z
+  // weights, then folded in with the column's x and y weights.
+  float S0[3] = {0.f, 0.f, 0.f}, S1[9], T[9], fA[3] = {0.f, 0.f, 0.f}, Hs[6][3];
   m_zero(S1);
   m_zero(T);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) Hs[j][0] = Hs[j][1] = Hs[j][2] = 0.f;
   float W = 0.f;
   const float h = M.dx;
+  float zd[3], dzd[3];
 #pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    zd[c] = s.w[2][c] * ((float)c - s.f[2]);
+    dzd[c] = s.dw[2][c] * ((float)c - s.f[2]);
+  }
+  const float Ct2[3] = {Ct[2], Ct[5], Ct[8]};
+#pragma unroll 1
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-      const float da = (float)a - s.f[0], db = (float)b - s.f[1];
-      const float X = s.w[0][a], Y = s.w[1][b], dX = s.dw[0][a], dY = s.dw[1][b];
-      const float ddX = a == 1 ? -2.f : 1.f, ddY = b == 1 ? -2.f : 1.f;
-      const float XY = X * Y;
-      float uab[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) uab[i] = vp[i] + h * (Ct[i * 3 + 0] * da + Ct[i * 3 + 1] * db);
+      // column sums over c: Z a, Z d_z a, dZ a, dZ d_z a, ddZ a, Z b, dZ b
+      float A1[3] = {0.f, 0.f, 0.f}, A2[3] = {0.f, 0.f, 0.f}, A3[3] = {0.f, 0.f, 0.f}, A4[3] = {0.f, 0.f, 0.f},
+            A5[3] = {0.f, 0.f, 0.f};
+      float B1 = 0.f, B3 = 0.f;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
         if (!node_in(ix, iy, iz, M.n)) continue;
         const float4 ga = gadj_in[M.g0 + node_id(ix, iy, iz, M.n)];
         const float av[3] = {ga.x, ga.y, ga.z};
-        const float dc = (float)c - s.f[2];
-        const float Z = s.w[2][c], dZ = s.dw[2][c], ddZ = c == 1 ? -2.f : 1.f;
-        const float wt = XY * Z;
-        const float gf[3] = {dX * Y * Z, X * dY * Z, XY * dZ};
-        float au = 0.f;
+        const float bm = ga.w;
+        const float Z = s.w[2][c], dZ = s.dw[2][c];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          const float wa = wt * av[i];
-          S0[i] += wa;
-          S1[i * 3 + 0] += wa * da;
-          S1[i * 3 + 1] += wa * db;
-          S1[i * 3 + 2] += wa * dc;
-          T[i * 3 + 0] += av[i] * gf[0];
-          T[i * 3 + 1] += av[i] * gf[1];
-          T[i * 3 + 2] += av[i] * gf[2];
-          au += av[i] * (uab[i] + h * Ct[i * 3 + 2] * dc);
+          A1[i] += Z * av[i];
+          A2[i] += zd[c] * av[i];
+          A3[i] += dZ * av[i];
+          A4[i] += dzd[c] * av[i];
+          A5[i] += c == 1 ? -2.f * av[i] : av[i];
         }
-        W += wt * ga.w;
-        const float ab = au + ga.w;
-        float ta[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          fA[j] += ab * gf[j];
-          ta[j] = tau[0 * 3 + j] * av[0] + tau[1 * 3 + j] * av[1] + tau[2 * 3 + j] * av[2];
-        }
-        // weight Hessian in grid units (symmetric)
-        const float hxx = ddX * Y * Z, hyy = X * ddY * Z, hzz = XY * ddZ;
-        const float hxy = dX * dY * Z, hxz = dX * Y * dZ, hyz = X * dY * dZ;
-        fB[0] += hxx * ta[0] + hxy * ta[1] + hxz * ta[2];
-        fB[1] += hxy * ta[0] + hyy * ta[1] + hyz * ta[2];
-        fB[2] += hxz * ta[0] + hyz * ta[1] + hzz * ta[2];
+        B1 += Z * bm;
+        B3 += dZ * bm;
       }
+      const float X = sw(s, 0, a), Y = s.w[1][b], dX = sdw(s, 0, a), dY = s.dw[1][b];
+      const float ddX = a == 1 ? -2.f : 1.f, ddY = b == 1 ? -2.f : 1.f;
+      const float da = (float)a - s.f[0], db = (float)b - s.f[1];
+      const float XY = X * Y, XYd0 = XY * da, XYd1 = XY * db, dXY = dX * Y, XdY = X * dY;
+      const float ddXY = ddX * Y, XddY = X * ddY, dXdY = dX * dY;
+      float e1 = B1, e3 = B3;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const float uab = vp[i] + h * (Ct[i * 3 + 0] * da + Ct[i * 3 + 1] * db);
+        e1 += uab * A1[i] + h * Ct2[i] * A2[i];
+        e3 += uab * A3[i] + h * Ct2[i] * A4[i];
+        S0[i] += XY * A1[i];
+        S1[i * 3 + 0] += XYd0 * A1[i];
+        S1[i * 3 + 1] += XYd1 * A1[i];
+        S1[i * 3 + 2] += XY * A2[i];
+        T[i * 3 + 0] += dXY * A1[i];
+        T[i * 3 + 1] += XdY * A1[i];
+        T[i * 3 + 2] += XY * A3[i];
+        Hs[0][i] += ddXY * A1[i];  // xx
+        Hs[1][i] += dXdY * A1[i];  // xy
+        Hs[2][i] += dXY * A3[i];   // xz
+        Hs[3][i] += XddY * A1[i];  // yy
+        Hs[4][i] += XdY * A3[i];   // yz
+        Hs[5][i] += XY * A5[i];    // zz
+      }
+      W += XY * B1;
+      fA[0] += dXY * e1;
+      fA[1] += XdY * e1;
+      fA[2] += XY * e3;
     }
+  // fB_j = sum_k H_jk (tau^T a)_k
+  const int hm[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  float fB[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) acc += tau[i * 3 + k] * Hs[hm[j][k]][i];
+    fB[j] = acc;
+  }
   // dL/d(v, C~, tau, m, V) of the particle and dL/df (f = x inv_dx - base)
   const float sV = -dt * Vp * M.inv_dx;  // d(stress impulse)/d(tau . grad w) in grid units
   float vb[3], Ctb[9], taub[9], fb[3];
