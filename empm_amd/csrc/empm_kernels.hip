@@ -865,7 +865,8 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
                                                 const float* Fe, const float4* __restrict__ v_out,
                                                 float* __restrict__ x1, float* __restrict__ v1,
                                                 float* __restrict__ C1, float* __restrict__ F1, int next_stress,
-                                                float* Fe_next, float* __restrict__ S_next, int write_params) {
+                                                float* Fe_next, float* __restrict__ S_next, int write_params,
+                                                int write_F) {
   per_particle(Bt, MAT, [&](const DevModel& M, const int p) {
   if (M.sel && M.sel[p] != 0) {  // partial_clone defaults (mpm_data_structure.py:356-387)
     float I[9], Z[9];
@@ -874,7 +875,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
     x1[3 * p] = x1[3 * p + 1] = x1[3 * p + 2] = 0.f;
     v1[3 * p] = v1[3 * p + 1] = v1[3 * p + 2] = 0.f;
     store9(C1, p, Z);
-    store9(F1, p, I);
+    if (write_F) store9(F1, p, I);
     return;
   }
   const float xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
@@ -941,7 +942,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
   for (int i = 0; i < 9; ++i) IF[i] = ((i % 4 == 0) ? 1.f : 0.f) + nG[i] * dt;
   load9(Fe, p, Fp);
   m_mul(IF, Fp, Fn);
-  store9(F1, p, Fn);
+  if (write_F) store9(F1, p, Fn);
   if (next_stress) particle_stress<MAT>(M, dt, p, Fn, Fe_next, S_next, write_params);
   });
 }
@@ -1900,8 +1901,11 @@ int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
   }
   if (hb.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_G2P, st);
+    // F_trial of an intermediate state is read only through a recorded tape
+    // (the next substep starts from Fe), so it is written for the call's
+    // last substep and when recording
     EMPM_LAUNCH_RANGES(hb, k_g2p, st, dt, in->x, Fe, v_out, out->x, out->v, out->C, out->F, last ? 0 : 1, Fe_next,
-                       ctx->S, 1);
+                       ctx->S, 1, (last || rec) ? 1 : 0);
   }
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
