@@ -50,10 +50,10 @@ def kernel_bytes(N, G):
         # reads F_trial (36 B) and mu, lam, yield, vol (16 B); writes Fe and S (72 B)
         "k_stress": 124 * N,
         # reads x, Fe (48), the adjoint of the next state (96) and v_out; writes dL/dx partial (12), dL/dFe (36)
-        # and the scatter payload (84)
-        "k_g2p_adj": 276 * N + 12 * G,
-        # reads x and the payload (96 B); writes dL/dv_out once per node (12 B)
-        "k_g2p_adj_scatter": 96 * N + 12 * G,
+        # and dL/dv' for the scatter (16)
+        "k_g2p_adj": 208 * N + 12 * G,
+        # reads x, dL/dv' (28 B), dL/dC', dL/dF' and Fe (108 B); writes dL/dv_out once per node (12 B)
+        "k_g2p_adj_scatter": 136 * N + 12 * G,
         # reads (mv, m) and dL/dv_out, writes dL/d(mv, m)
         "k_grid_adj": 44 * G,
         # reads state + constants (116), partial adjoints (48), dL/d(mv, m) once per node;

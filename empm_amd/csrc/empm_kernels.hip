@@ -368,7 +368,7 @@ constexpr int kBlkGridCap = 1024; // workgroups of a block-owned scatter launch 
 constexpr int kHeavyChunks = 8;   // blocks with more chunks than this are scattered first
 constexpr int kDynamicRounds = 4; // more blocks per scatter workgroup than this: handed out dynamically
 constexpr int kWave = 64;
-constexpr int kAdjPayload = 24;   // floats per particle of the G2P-adjoint scatter payload
+constexpr int kAdjPayload = 4;    // floats per particle of the G2P-adjoint scatter payload (dL/dv', pad)
 
 // Particle binning of the current segment: perm[k] = k-th particle in
 // block-major order, offs[b * 8 + c] = first position of cell c of block b
@@ -952,7 +952,9 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
 // ---------------------------------------------------------------------------
 // Adjoint of g2p_differentiable, per particle: dL/dx (partial), dL/dFe, and
 // the payload of the dL/dv_out scatter (A = dL/dv', B = 4 inv_dx dL/dC',
-// S = dt inv_dx dL/dF' Fe^T), stored kAdjPayload floats per particle.
+// S = dt inv_dx dL/dF' Fe^T): k_g2p_adj stores A, the only part that needs
+// the gather (the clamp mask), and the scatter forms B and S from the
+// state's cotangents itself.
 __global__ __launch_bounds__(kBlock) void k_g2p_adj(Batch Bt, float dt, const float* __restrict__ x,
                                                     const float* __restrict__ Fe, const float4* __restrict__ v_out,
                                                     const float* __restrict__ dx1, const float* __restrict__ dv1,
@@ -1040,33 +1042,37 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(Batch Bt, float dt, const fl
 #pragma unroll
   for (int l = 0; l < 3; ++l)
     xbar[3 * p + l] = xb[l] + inv_dx * fb[l] + (nvb[0] * nG[l] + nvb[1] * nG[3 + l] + nvb[2] * nG[6 + l]);
-  float* q = payload + (size_t)kAdjPayload * p;
-  float4* q4 = reinterpret_cast<float4*>(q);
-  q4[0] = make_float4(nvb[0], nvb[1], nvb[2], 0.f);
-  q4[1] = make_float4(c4 * dC[0], c4 * dC[1], c4 * dC[2], c4 * dC[3]);
-  q4[2] = make_float4(c4 * dC[4], c4 * dC[5], c4 * dC[6], c4 * dC[7]);
-  q4[3] = make_float4(c4 * dC[8], inv_dx * Gb[0], inv_dx * Gb[1], inv_dx * Gb[2]);
-  q4[4] = make_float4(inv_dx * Gb[3], inv_dx * Gb[4], inv_dx * Gb[5], inv_dx * Gb[6]);
-  q4[5] = make_float4(inv_dx * Gb[7], inv_dx * Gb[8], 0.f, 0.f);
+  reinterpret_cast<float4*>(payload)[p] = make_float4(nvb[0], nvb[1], nvb[2], 0.f);
   });
 }
 
 // Scatter of dL/dv_out (the transpose of the G2P gather) into the block tiles.
-__global__ __launch_bounds__(kBlkWG) void k_g2p_adj_scatter(Batch Bt, const float* __restrict__ x,
-                                                            const float* __restrict__ payload, Binning B,
+// Per particle: A = dL/dv' (clamp mask applied, from k_g2p_adj), and from
+// the state's cotangents B = 4 inv_dx dL/dC' and S = dt inv_dx dL/dF' Fe^T
+// (F' = (I + dt grad v) Fe).
+__global__ __launch_bounds__(kBlkWG) void k_g2p_adj_scatter(Batch Bt, float dt, const float* __restrict__ x,
+                                                            const float* __restrict__ payload,
+                                                            const float* __restrict__ dC1,
+                                                            const float* __restrict__ dF1,
+                                                            const float* __restrict__ Fe, Binning B,
                                                             float4* __restrict__ tiles, Overflow ov,
                                                             int* __restrict__ work) {
-  block_scatter<3>(Bt, B, tiles, ov.grid, ov.flag, work, [&](int p, float* xp, ScatterPayload& P, const DevModel&) {
+  block_scatter<3>(Bt, B, tiles, ov.grid, ov.flag, work, [&](int p, float* xp, ScatterPayload& P, const DevModel& M) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) xp[d] = x[3 * p + d];
-    const float4* q4 = reinterpret_cast<const float4*>(payload + (size_t)kAdjPayload * p);
-    const float4 a = q4[0], b = q4[1], c = q4[2], d = q4[3], e = q4[4], f = q4[5];
+    const float4 a = reinterpret_cast<const float4*>(payload)[p];
     P.A[0] = a.x, P.A[1] = a.y, P.A[2] = a.z;
-    P.B[0] = b.x, P.B[1] = b.y, P.B[2] = b.z, P.B[3] = b.w;
-    P.B[4] = c.x, P.B[5] = c.y, P.B[6] = c.z, P.B[7] = c.w;
-    P.B[8] = d.x, P.S[0] = d.y, P.S[1] = d.z, P.S[2] = d.w;
-    P.S[3] = e.x, P.S[4] = e.y, P.S[5] = e.z, P.S[6] = e.w;
-    P.S[7] = f.x, P.S[8] = f.y;
+    float dC[9], dF[9], Fp[9], Gb[9];
+    load9(dC1, p, dC);
+    load9(dF1, p, dF);
+    load9(Fe, p, Fp);
+    m_mul_bt(dF, Fp, Gb);
+    const float c4 = M.inv_dx * 4.f, sg = dt * M.inv_dx;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      P.B[k] = c4 * dC[k];
+      P.S[k] = sg * Gb[k];
+    }
     P.m = 0.f;
   });
 }
@@ -1952,8 +1958,8 @@ int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
     }
     {
       ProfScope ps(ctx, EMPM_KERNEL_G2P_ADJ_SCATTER, st);
-      hipLaunchKernelGGL(k_g2p_adj_scatter, dim3(ctx->scatter_wgs[1]), dim3(kBlkWG), 0, st, hb.dev, in->x,
-                         ctx->payload, B, ctx->tiles, overflow(ctx), ctx->work + 1);
+      hipLaunchKernelGGL(k_g2p_adj_scatter, dim3(ctx->scatter_wgs[1]), dim3(kBlkWG), 0, st, hb.dev, dt, in->x,
+                         ctx->payload, d_out->C, d_out->F, Fe, B, ctx->tiles, overflow(ctx), ctx->work + 1);
     }
   }
   {
