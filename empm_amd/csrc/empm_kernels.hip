@@ -1132,16 +1132,6 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
     store9(dFo, p, Z);
     return;
   }
-  float Ftr[9];
-  load9(F, p, Ftr);
-  float mu = M.mu[p], lam = M.lam[p], ys = M.ys[p];
-  const float mu0 = mu, lam0 = lam, ys0 = ys;
-  MatParams matp = M.mp;
-  matp.material = MAT;
-  ConstitState st;
-  float tau[9];
-  constitutive_fwd(st, Ftr, mu, lam, ys, matp, dt, tau);
-
   const float xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
   const float vp[3] = {v[3 * p], v[3 * p + 1], v[3 * p + 2]};
   float Cp[9], Ct[9];
@@ -1229,6 +1219,17 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
       fA[1] += XdY * e1;
       fA[2] += XY * e3;
     }
+  // the substep's return map and stress, recomputed (after the node loop,
+  // which does not need them, so that their state is not live during it)
+  float Ftr[9];
+  load9(F, p, Ftr);
+  float mu = M.mu[p], lam = M.lam[p], ys = M.ys[p];
+  const float mu0 = mu, lam0 = lam, ys0 = ys;
+  MatParams matp = M.mp;
+  matp.material = MAT;
+  ConstitState st;
+  float tau[9];
+  constitutive_fwd(st, Ftr, mu, lam, ys, matp, dt, tau);
   // fB_j = sum_k H_jk (tau^T a)_k
   const int hm[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
   float fB[3];
