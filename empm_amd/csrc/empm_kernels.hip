@@ -1425,7 +1425,7 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_scan(int nblk, const int* __
   const int tot = b < nblk ? block_count(count, b) : 0;
   int wtot;
   const int ex = wg_exclusive_scan(tot, smem, wtot);
-  if (b < nblk) blkoff[b] = ex;
+  if (b < nblk) blkoff[b] = tot > 0 ? ex : -1 - ex;  // empty blocks flagged by the sign
   if (threadIdx.x == 0) wsum[blockIdx.x] = wtot;
   if (b == 0) {
     act[0] = act[1] = 0;
@@ -1453,10 +1453,20 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_finish(Batch Bt, int* __rest
 #pragma unroll
   for (int k = 0; k < kBinScanWG / 64; ++k) prefix += red[k];
   if (b >= nblk) return;
+  // an empty block's counts are already zero and its offsets are never read
+  // (a block's end is the next block's first offset, which every non-empty
+  // block writes itself): only its occupancy flag is written
+  const int e = blkoff[b];
+  const bool full = e >= 0;
+  occ[b] = full ? 1 : 0;
+  if (!full) {
+    if (b == nblk - 1) offs[(size_t)nblk * 8] = prefix - 1 - e;
+    return;
+  }
   int4* c4 = reinterpret_cast<int4*>(count + (size_t)b * 8);
   const int4 lo = c4[0], hi = c4[1];
   const int c[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-  int o = prefix + blkoff[b];
+  int o = prefix + e;
   int run[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
@@ -1466,12 +1476,9 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_finish(Batch Bt, int* __rest
   int4* o4 = reinterpret_cast<int4*>(offs + (size_t)b * 8);
   o4[0] = make_int4(run[0], run[1], run[2], run[3]);
   o4[1] = make_int4(run[4], run[5], run[6], run[7]);
+  offs[(size_t)(b + 1) * 8] = o;  // this block's end (the next block writes the same value if non-empty)
   c4[0] = make_int4(0, 0, 0, 0);
   c4[1] = make_int4(0, 0, 0, 0);
-  if (b == nblk - 1) offs[(size_t)nblk * 8] = o;
-  const bool full = o != run[0];
-  occ[b] = full ? 1 : 0;
-  if (!full) return;
   // list appends are aggregated per wave (one atomic per list)
   const unsigned long long lt = (1ull << __lane_id()) - 1ull;
   const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
