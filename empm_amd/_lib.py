@@ -36,6 +36,7 @@ EXPORTED_SYMBOLS = (
     "empm_mu_lam",
     "empm_mu_lam_adj",
     "empm_mass_from_density",
+    "empm_nn_sqdist",
     "empm_substep_fwd",
     "empm_substep_adj",
     "empm_rollout_fwd",
@@ -192,6 +193,8 @@ def _declare(lib):
     lib.empm_mu_lam_adj.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp]
     lib.empm_mass_from_density.restype = C.c_int
     lib.empm_mass_from_density.argtypes = [C.c_int, vp, vp, vp, vp]
+    lib.empm_nn_sqdist.restype = C.c_int
+    lib.empm_nn_sqdist.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp]
     lib.empm_substep_fwd.restype = C.c_int
     lib.empm_substep_fwd.argtypes = [vp, P(Model), P(Collider), C.c_int, C.c_float, C.c_float,
                                      P(Particles), P(Particles), vp]

@@ -1563,6 +1563,49 @@ __global__ void k_mass(int n, const float* __restrict__ rho, const float* __rest
   mass[p] = rho[p] * vol[p];
 }
 
+// Nearest neighbour in B of every point of A: min_j |a_i - b_j|^2 and the
+// smallest j attaining it (the knn_points(K = 1) step of the Chamfer loss).
+// Each thread owns one point of A; B passes through LDS in tiles that every
+// lane reads at the same address (a broadcast).
+constexpr int kNnTile = 2048;
+__global__ __launch_bounds__(kBlock) void k_nn_sqdist(int n, const float* __restrict__ a, int m,
+                                                      const float* __restrict__ b, float* __restrict__ dist,
+                                                      int* __restrict__ idx) {
+  __shared__ float4 tile[kNnTile];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  if (i < n) {
+    ax = a[3 * (size_t)i];
+    ay = a[3 * (size_t)i + 1];
+    az = a[3 * (size_t)i + 2];
+  }
+  float best = __builtin_huge_valf();
+  int bj = 0;
+  for (int t0 = 0; t0 < m; t0 += kNnTile) {
+    const int cnt = m - t0 < kNnTile ? m - t0 : kNnTile;
+    __syncthreads();
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+      const size_t q = 3 * (size_t)(t0 + k);
+      tile[k] = make_float4(b[q], b[q + 1], b[q + 2], 0.f);
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < cnt; ++k) {
+      const float4 q = tile[k];
+      const float dx = ax - q.x, dy = ay - q.y, dz = az - q.z;
+      const float d = dx * dx + dy * dy + dz * dz;
+      if (d < best) {  // strict: the first (smallest) index of a tie is kept
+        best = d;
+        bj = t0 + k;
+      }
+    }
+  }
+  if (i < n) {
+    dist[i] = best;
+    idx[i] = bj;
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -2374,6 +2417,15 @@ int empm_mass_from_density(int n, const float* density, const float* vol, float*
     return set_err(EMPM_E_INVALID_ARG, "empm_mass_from_density: bad arguments");
   if (n == 0) return EMPM_OK;
   hipLaunchKernelGGL(k_mass, dim3(nblocks(n)), dim3(kBlock), 0, (hipStream_t)stream, n, density, vol, mass);
+  HIP_TRY(hipGetLastError());
+  return EMPM_OK;
+}
+
+int empm_nn_sqdist(int n, const float* a, int m, const float* b, float* dist, int* idx, void* stream) {
+  if (n < 0 || m < 0 || (n > 0 && (m < 1 || !a || !b || !dist || !idx)))
+    return set_err(EMPM_E_INVALID_ARG, "empm_nn_sqdist: bad arguments (n=%d, m=%d)", n, m);
+  if (n == 0) return EMPM_OK;
+  hipLaunchKernelGGL(k_nn_sqdist, dim3(nblocks(n)), dim3(kBlock), 0, (hipStream_t)stream, n, a, m, b, dist, idx);
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }

@@ -202,6 +202,14 @@ int empm_mu_lam_adj(int n, const float *E, const float *nu, const float *dmu, co
  * interface.py:125-134 (mpm_data_structure.py:719-726). */
 int empm_mass_from_density(int n, const float *density, const float *vol, float *mass, void *stream);
 
+/* Nearest neighbour in B of every point of A: dist[i] = min_j |a_i - b_j|^2
+ * and idx[i] = the smallest j attaining it.  This is the knn_points(K = 1)
+ * step of pytorch3d.loss.chamfer_distance, which the training loss calls per
+ * frame on the visible points (phys_sim/engine/trainer_warp_mpm.py:746-752).
+ * a: n x 3, b: m x 3, float32 device arrays; dist: n floats, idx: n ints;
+ * m >= 1 when n >= 1. */
+int empm_nn_sqdist(int n, const float *a, int m, const float *b, float *dist, int *idx, void *stream);
+
 /* One substep state `in` -> state `out` (`out` must not alias `in`):
  * MPMWARPDiff.p2g2p_differentiable (mpm_solver_diff.py:165-291), i.e.
  * zero_grid, compute_stress_from_F_trial, p2g_apic_with_stress,
