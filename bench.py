@@ -69,12 +69,13 @@ def kernel_bytes(N, G):
 # HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE counter runs of
 # this workload (calibrated on a plain copy); the file gives the commands
 TRAFFIC_PROFILE = "profiles/c2_hbm_traffic_pmc.md"
+TRAFFIC_PROFILE_LBS = "profiles/lbs_hbm_traffic_pmc.md"
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, profile=None):
     """Counter-measured HBM bytes per launch of `kernel`, or None."""
     try:
-        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+        with open(os.path.join(ROOT, profile or TRAFFIC_PROFILE)) as f:
             for line in f:
                 parts = line.split()
                 if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
@@ -98,8 +99,9 @@ def main():
     ap.add_argument("--substeps", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--only", choices=("all", "single", "population", "serving", "long_horizon"), default="all",
-                    help="run only the single-rollout, the population, the serving or the long-horizon workload")
+    ap.add_argument("--only", choices=("all", "single", "population", "serving", "long_horizon", "lbs"),
+                    default="all", help="run only the single-rollout, the population, the serving, the long-horizon "
+                                        "or the Gaussian interpolation (lbs) workload")
     ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving workload")
     ap.add_argument("--serve-qps", type=float, default=10.0, help="Poisson arrival rate of the serving workload")
     ap.add_argument("--pop-steps", type=int, default=2, help="timed evaluations of the population workload")
@@ -138,6 +140,9 @@ def main():
         return
     if args.only == "long_horizon":
         print(json.dumps({"long_horizon": bench_long_horizon(args, dev)}), flush=True)
+        return
+    if args.only == "lbs":
+        print(json.dumps({"lbs": bench_lbs(args, dev)}), flush=True)
         return
     sc = scenes.ellipsoid_c2(n_particles=args.particles, n_grid=args.grid, n_steps=args.substeps, param_seed=3)
     tprm = scenes.sample_params_c2(2)
@@ -258,7 +263,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, args.cpu_seconds)
-    population = serving = long_horizon = None
+    population = serving = long_horizon = lbs = None
     if args.only == "all":
         del ro, tro, ctx
         torch.cuda.empty_cache()
@@ -267,6 +272,8 @@ def main():
         serving = bench_serving(args, dev)
         torch.cuda.empty_cache()
         long_horizon = bench_long_horizon(args, dev)
+        torch.cuda.empty_cache()
+        lbs = bench_lbs(args, dev)
 
     out = {
         "metric": "particle-substeps/s (fwd+bwd) @200k pts/64^3",
@@ -300,6 +307,7 @@ def main():
         "batched": population,
         "serving": serving,
         "long_horizon": long_horizon,
+        "lbs": lbs,
     }
     print(json.dumps(out), flush=True)
 
@@ -441,6 +449,108 @@ def bench_long_horizon(args, dev):
            "value": len(sets) * sc.n * T / el, "unit": "particle-substeps/s (fwd+bwd)", "seconds_per_step": el,
            "loss": loss.tolist(), "grad_logE_nu_logY": grad.tolist()}
     log(f"[bench] long horizon: {el:.2f} s per gradient batch, {out['value']:.3e} p-substeps/s")
+    return out
+
+
+def bench_lbs(args, dev):
+    """Per-frame motion interpolation of Gaussians on the simulated particles
+    (linear blend skinning, third_party/gaussian_splatting/dynamic_utils.py:305,
+    once per frame in scripts/interactive_viser.py:259-280): 50k particles as
+    bones, 1M Gaussians, K = 16 bones per Gaussian and 16 neighbours per bone,
+    a smooth non-rigid motion.  One frame = the bone transforms (k_lbs_bones)
+    + the per-Gaussian blend (k_lbs_points) with the inverse-distance weights
+    computed in the blend; the reference's two-call sequence (weights, then
+    interpolation with the weights) is timed beside it."""
+    import torch
+
+    from empm_amd import _lib
+    from empm_amd import gaussian_splatting as gs
+    from empm_amd.runtime import stream_handle
+
+    nb, ng, K = 50_000, 1_000_000, 16
+    rng = np.random.default_rng(0)
+    bones = rng.random((nb, 3)) * 0.2
+    xyz = bones[rng.integers(0, nb, ng)] + 0.004 * rng.standard_normal((ng, 3))
+    quat = rng.standard_normal((ng, 4))
+    c = bones.mean(0)
+    ang = 0.1
+    Q = np.array([[np.cos(ang), -np.sin(ang), 0.0], [np.sin(ang), np.cos(ang), 0.0], [0.0, 0.0, 1.0]])
+    x, y, z = (bones - c).T
+    motions = (bones - c) @ Q.T + c - bones + 0.01 * np.stack([np.sin(20 * y), np.sin(20 * z), np.sin(20 * x)], 1)
+
+    def t(a):
+        return torch.tensor(a, dtype=torch.float32, device=dev)
+
+    tb, tm, tx, tq = t(bones), t(motions), t(xyz), t(quat)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rel = gs.get_topk_indices(tb, K=K)
+    _, idx = gs.knn_weights_sparse(tb, tx, K=K)
+    torch.cuda.synchronize()
+    setup_ms = (time.perf_counter() - t0) * 1e3
+    interp = gs.LBSInterpolator()
+    xo, qo = torch.empty_like(tx), torch.empty_like(tq)
+
+    def fused():
+        interp(tb, tm, rel, tx, quat=tq, weights=None, weights_indices=idx, xyz_out=xo, quat_out=qo)
+
+    def two_call():
+        w = gs.calc_weights_vals_from_indices(tb, tx, idx)
+        gs.interpolate_motions_speedup(bones=tb, motions=tm, relations=rel, weights=w, weights_indices=idx, xyz=tx,
+                                       quat=tq)
+
+    frames = 200
+
+    def timed(fn):
+        for _ in range(10):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / frames
+
+    ms_fused = timed(fused)
+    ms_two = timed(two_call)
+    # the blend kernel alone, HIP events on its stream around each launch
+    lib = _lib.load()
+    rec, degen = interp.bones(tb, tm, rel)
+    st = stream_handle(dev)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(frames)]
+    for e0, e1 in ev:
+        e0.record()
+        _lib.check(lib.empm_lbs_points(ng, tx.data_ptr(), tq.data_ptr(), K, idx.data_ptr(), 4, None, nb,
+                                       rec.data_ptr(), degen.data_ptr(), xo.data_ptr(), qo.data_ptr(), st))
+        e1.record()
+    torch.cuda.synchronize()
+    us = sorted(e0.elapsed_time(e1) * 1e3 for e0, e1 in ev)
+    avg_us = sum(us) / len(us)
+    # per Gaussian: x (12) and quaternion (16) read, K int32 indices (64), x' (12) and q' (16) written;
+    # per bone: its 80-byte record, read once
+    algo = ng * (12 + 16 + 4 * K + 12 + 16) + nb * 80
+    achieved = algo / (avg_us * 1e-6) / 1e9
+    out = {"workload": f"linear-blend-skinning of {ng} Gaussians on {nb} particles, K={K} bones per Gaussian, "
+                       f"{K} neighbours per bone, per-frame bone transforms + blend",
+           "value": ng / (ms_fused * 1e-3), "unit": "Gaussians/s (per-frame interpolation)",
+           "ms_per_frame": ms_fused, "ms_per_frame_two_call": ms_two, "setup_ms_knn": setup_ms,
+           "roofline": {"bound": "hbm", "kernel": "k_lbs_points", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_lbs_points", TRAFFIC_PROFILE_LBS),
+                        "traffic_source": TRAFFIC_PROFILE_LBS, "algorithmic_bytes": algo, "avg_launch_us": avg_us,
+                        "median_launch_us": us[len(us) // 2], "launches": frames}}
+    if not args.no_cpu_baseline:
+        from oracle import lbs as olbs
+
+        idx_np = idx[:50_000].long().cpu().numpy()
+        rel_np = rel.long().cpu().numpy()
+        t0 = time.perf_counter()
+        olbs.interpolate(bones, motions, rel_np, xyz[:50_000], quat[:50_000], None, idx_np)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 50_000 / el, "unit": "Gaussians/s", "cores": 1, "kind": "port",
+                               "sample": f"50000 of the Gaussians, all {nb} bone transforms, float64 NumPy oracle, "
+                                         f"{el:.1f} s"}
+    log(f"[bench] lbs: {ms_fused:.3f} ms/frame fused, {ms_two:.3f} ms two-call, setup {setup_ms:.1f} ms; "
+        f"k_lbs_points avg {avg_us:.1f} us ({achieved:.0f} GB/s)")
     return out
 
 

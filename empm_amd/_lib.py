@@ -53,6 +53,10 @@ EXPORTED_SYMBOLS = (
     "empm_profile_read",
     "empm_profile_reset",
     "empm_kernel_name",
+    "empm_knn",
+    "empm_lbs_weights",
+    "empm_lbs_bones",
+    "empm_lbs_points",
 )
 
 KERNEL_NAMES = ("k_p2g", "k_grid", "k_g2p", "k_g2p_adj", "k_grid_adj", "k_p2g_adj", "k_stress", "k_g2p_adj_scatter",
@@ -195,6 +199,14 @@ def _declare(lib):
     lib.empm_mass_from_density.argtypes = [C.c_int, vp, vp, vp, vp]
     lib.empm_nn_sqdist.restype = C.c_int
     lib.empm_nn_sqdist.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp]
+    lib.empm_knn.restype = C.c_int
+    lib.empm_knn.argtypes = [C.c_int, vp, C.c_int, vp, C.c_int, C.c_int, vp, vp, vp]
+    lib.empm_lbs_weights.restype = C.c_int
+    lib.empm_lbs_weights.argtypes = [C.c_int, vp, C.c_int, vp, C.c_int, C.c_int, vp, vp, vp]
+    lib.empm_lbs_bones.restype = C.c_int
+    lib.empm_lbs_bones.argtypes = [C.c_int, vp, vp, C.c_int, vp, C.c_int, vp, vp, vp]
+    lib.empm_lbs_points.restype = C.c_int
+    lib.empm_lbs_points.argtypes = [C.c_int, vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp, vp, vp, vp]
     lib.empm_substep_fwd.restype = C.c_int
     lib.empm_substep_fwd.argtypes = [vp, P(Model), P(Collider), C.c_int, C.c_float, C.c_float,
                                      P(Particles), P(Particles), vp]

@@ -8,7 +8,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "empm_kernels.hip")
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("empm_kernels.hip", "empm_lbs.hip")]
 OUT = os.path.join(HERE, "libempm.so")
 
 HIPCC_FLAGS = [
@@ -37,7 +37,7 @@ def hipcc():
 
 
 def build(verbose: bool = False) -> str:
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", OUT + ".tmp", SRC]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", OUT + ".tmp", *SRCS]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

@@ -2179,6 +2179,12 @@ int rollout_adj(empm_ctx* ctx, const empm_batch* batch, const empm_collider* col
 
 }  // namespace
 
+// Error reporting for the library's other translation units: stores msg as
+// the thread's last error and returns code.
+__attribute__((visibility("hidden"))) int empm_internal_error(int code, const char* msg) {
+  return set_err(code, "%s", msg);
+}
+
 extern "C" {
 
 int empm_abi_version(void) { return EMPM_ABI_VERSION; }

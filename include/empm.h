@@ -339,6 +339,52 @@ int empm_profile_reset(empm_ctx *ctx);
 /* Name of a kernel id ("" if out of range). */
 const char *empm_kernel_name(int kernel_id);
 
+/* ---- Linear-blend skinning of Gaussians on the simulated particles -------
+ * (third_party/gaussian_splatting/dynamic_utils.py).  The particles are the
+ * "bones".  All arrays are float32 device arrays, points as n x 3 rows;
+ * index arrays are n x k rows of idx_bytes = 4 (int32) or 8 (int64, what
+ * torch.topk returns) byte integers. */
+
+/* The k nearest rows of r (m x 3) for every row of q (n x 3), nearest first:
+ * idx (n x k ints) and the Euclidean distances dist (n x k); equal distances
+ * keep the smaller index first.  skip_first = 1 finds k + 1 and drops the
+ * first (the query itself when q is r).  1 <= k + skip_first <= 32 and
+ * m >= k + skip_first.  Replaces the torch.topk over torch.cdist / a dense
+ * distance matrix of get_topk_indices (dynamic_utils.py:248-265, with
+ * skip_first = 1) and knn_weights / knn_weights_sparse (:268-302). */
+int empm_knn(int n, const float *q, int m, const float *r, int k, int skip_first, int *idx, float *dist,
+             void *stream);
+
+/* weights[i, j] = 1 / (|xyz_i - bones_idx[i,j]| + 1e-6), normalised over j:
+ * calc_weights_vals_from_indices (dynamic_utils.py:282-292).  k <= 32;
+ * bones: n_bones x 3.  A row holding an index outside [0, n_bones) is NaN. */
+int empm_lbs_weights(int n, const float *xyz, int k, const void *idx, int idx_bytes, int n_bones,
+                     const float *bones, float *weights, void *stream);
+
+/* Per-bone transforms of interpolate_motions_speedup (dynamic_utils.py:
+ * 321-416): for bone b with adjacent bones relations[b, :] (n_bones x n_adj),
+ * F = sum_j ((bones + motions)[a_j] - (bones + motions)[b]) (bones[a_j] -
+ * bones[b])^T, R = the rotation closest to F, q = its unit quaternion
+ * (w, x, y, z).  bone_rec: 20 floats per bone, 16-byte aligned:
+ * [R00 R01 R02 b.x R10 R11 R12 b.y R20 R21 R22 b.z m.x m.y m.z 0 q.w q.x q.y q.z];
+ * a bone with a relation index outside [0, n_bones) gets a NaN rotation.
+ * *degenerate (one device int) receives the number of bones whose F has rank
+ * below 2; when it is non-zero the reference leaves every rotation at the
+ * identity, and empm_lbs_points does the same. */
+int empm_lbs_bones(int n_bones, const float *bones, const float *motions, int n_adj, const void *relations,
+                   int idx_bytes, float *bone_rec, int *degenerate, void *stream);
+
+/* Per-Gaussian blend of interpolate_motions_speedup (dynamic_utils.py:
+ * 434-485): xyz_out = sum_j w_j (R_j (xyz - b_j) + m_j + b_j) and, when quat
+ * (n x 4, w x y z, 16-byte aligned) is given, quat_out = normalize(sum_j w_j
+ * q_j) * quat (Hamilton product).  weights (n x k) may be null: the weights
+ * of empm_lbs_weights are then computed from the bones in bone_rec and xyz.
+ * bone_rec holds n_bones records; a row holding an index outside
+ * [0, n_bones) is NaN.  Outputs must not alias inputs. */
+int empm_lbs_points(int n, const float *xyz, const float *quat, int k, const void *idx, int idx_bytes,
+                    const float *weights, int n_bones, const float *bone_rec, const int *degenerate,
+                    float *xyz_out, float *quat_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
