@@ -99,9 +99,9 @@ def main():
     ap.add_argument("--substeps", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--only", choices=("all", "single", "population", "serving", "long_horizon", "lbs"),
-                    default="all", help="run only the single-rollout, the population, the serving, the long-horizon "
-                                        "or the Gaussian interpolation (lbs) workload")
+    ap.add_argument("--only", choices=("all", "single", "population", "serving", "long_horizon", "loss", "lbs"),
+                    default="all", help="run only the single-rollout, the population, the serving, the long-horizon, "
+                                        "the frame-loss or the Gaussian interpolation (lbs) workload")
     ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving workload")
     ap.add_argument("--serve-qps", type=float, default=10.0, help="Poisson arrival rate of the serving workload")
     ap.add_argument("--pop-steps", type=int, default=2, help="timed evaluations of the population workload")
@@ -140,6 +140,9 @@ def main():
         return
     if args.only == "long_horizon":
         print(json.dumps({"long_horizon": bench_long_horizon(args, dev)}), flush=True)
+        return
+    if args.only == "loss":
+        print(json.dumps({"loss": bench_loss(args, dev)}), flush=True)
         return
     if args.only == "lbs":
         print(json.dumps({"lbs": bench_lbs(args, dev)}), flush=True)
@@ -263,7 +266,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, args.cpu_seconds)
-    population = serving = long_horizon = lbs = None
+    population = serving = long_horizon = frame_loss = lbs = None
     if args.only == "all":
         del ro, tro, ctx
         torch.cuda.empty_cache()
@@ -272,6 +275,8 @@ def main():
         serving = bench_serving(args, dev)
         torch.cuda.empty_cache()
         long_horizon = bench_long_horizon(args, dev)
+        torch.cuda.empty_cache()
+        frame_loss = bench_loss(args, dev)
         torch.cuda.empty_cache()
         lbs = bench_lbs(args, dev)
 
@@ -307,6 +312,7 @@ def main():
         "batched": population,
         "serving": serving,
         "long_horizon": long_horizon,
+        "loss": frame_loss,
         "lbs": lbs,
     }
     print(json.dumps(out), flush=True)
@@ -452,6 +458,101 @@ def bench_long_horizon(args, dev):
     return out
 
 
+def bench_loss(args, dev):
+    """The training loss on frames with its gradient
+    (phys_sim/engine/trainer_warp_mpm.py:727-766): 30 frames of 20k
+    simulated points against 20k observed points, 80% of them visible and 90%
+    with a valid motion per frame; one step = Chamfer + tracking summed over
+    the frames and the backward to the points.  The nearest-neighbour kernel
+    (k_nn_sqdist, brute force, two directions per frame) is VALU-bound:
+    9 operations per pair (3 differences, 3 multiply-adds, compare, 2 selects)."""
+    import torch
+
+    from empm_amd import _lib
+    from empm_amd import loss
+    from empm_amd.runtime import stream_handle
+
+    T, N = 30, 20_000
+    rng = np.random.default_rng(7)
+    pts = rng.random((T, N, 3)) * 0.2
+    tgt = pts + 0.005 * rng.standard_normal((T, N, 3))
+    vis = rng.random((T, N)) < 0.8
+    mv = rng.random((T, N)) < 0.9
+    tp = torch.tensor(pts, dtype=torch.float32, device=dev, requires_grad=True)
+    tt = torch.tensor(tgt, dtype=torch.float32, device=dev)
+    tv = torch.tensor(vis, device=dev)
+    tm = torch.tensor(mv, device=dev)
+
+    def step():
+        tp.grad = None
+        L = loss.frame_loss(tp, tt, tv, tm)
+        L.backward()
+        return L
+
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    # the kernel alone on one frame's visible points, HIP events on its stream
+    a = tp[0][tv[0]].detach().contiguous()
+    b = tt[0][tv[0]].contiguous()
+    n = a.shape[0]
+    dist = torch.empty(n, dtype=torch.float32, device=dev)
+    idx = torch.empty(n, dtype=torch.int32, device=dev)
+    lib = _lib.load()
+    st = stream_handle(dev)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(50)]
+    for e0, e1 in ev:
+        e0.record()
+        _lib.check(lib.empm_nn_sqdist(n, a.data_ptr(), n, b.data_ptr(), dist.data_ptr(), idx.data_ptr(), st))
+        e1.record()
+    torch.cuda.synchronize()
+    us = sum(e0.elapsed_time(e1) for e0, e1 in ev) / len(ev) * 1e3
+    tflops = 9.0 * n * n / (us * 1e-6) / 1e12
+    out = {"workload": f"frame loss (Chamfer + tracking) and its gradient: {T} frames of {N} simulated points vs "
+                       f"{N} observed points, 80% visible, 90% with valid motion",
+           "value": T / (ms * 1e-3), "unit": "frames/s (loss + gradient)", "ms_per_step": ms, "loss": float(L),
+           "roofline": {"bound": "valu", "kernel": "k_nn_sqdist", "achieved": tflops, "peak": 157.3,
+                        "unit": "TFLOP/s", "frac": tflops / 157.3, "traffic": None,
+                        "note": f"{n} x {n} pairs, 9 FP32 operations per pair", "avg_launch_us": us}}
+    if not args.no_cpu_baseline:
+        from oracle import chamfer as och
+
+        # a 4000-point subsample of each frame's visible points (the brute
+        # force is quadratic: pairs/s scale to the full frame's 2 n^2 pairs)
+        sub, f, pairs = 4000, 0, 0
+        t0 = time.perf_counter()
+        while f < T and time.perf_counter() - t0 < min(args.cpu_seconds, 10.0):
+            x, y = pts[f][vis[f]][:sub], tgt[f][vis[f]][:sub]
+            och.chamfer(x, y)
+            pairs += 2 * len(x) * len(y)
+            f += 1
+        el = time.perf_counter() - t0
+        frame_pairs = float(np.mean([2 * int(v.sum()) ** 2 for v in vis]))
+        out["cpu_baseline"] = {"value": pairs / el / frame_pairs * 1.0, "unit": "frames/s (Chamfer value)",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{f} frames cut to {sub} visible points each ({pairs:.3g} point pairs), "
+                                         f"float64 NumPy brute force, {el:.1f} s, scaled by pairs to full frames"}
+    log(f"[bench] loss: {ms:.2f} ms per {T}-frame loss + gradient; k_nn_sqdist {us:.1f} us for {n}x{n} "
+        f"({tflops:.1f} TFLOP/s)")
+    return out
+
+
+def _morton(p, lo, hi, bits=10):
+    """Morton (Z-order) keys of points p in the box [lo, hi]."""
+    q = np.clip(((p - lo) / (hi - lo) * (1 << bits)).astype(np.int64), 0, (1 << bits) - 1)
+    key = np.zeros(len(p), np.int64)
+    for b in range(bits):
+        for a in range(3):
+            key |= ((q[:, a] >> b) & 1) << (3 * b + a)
+    return key
+
+
 def bench_lbs(args, dev):
     """Per-frame motion interpolation of Gaussians on the simulated particles
     (linear blend skinning, third_party/gaussian_splatting/dynamic_utils.py:305,
@@ -460,7 +561,10 @@ def bench_lbs(args, dev):
     a smooth non-rigid motion.  One frame = the bone transforms (k_lbs_bones)
     + the per-Gaussian blend (k_lbs_points) with the inverse-distance weights
     computed in the blend; the reference's two-call sequence (weights, then
-    interpolation with the weights) is timed beside it."""
+    interpolation with the weights) is timed beside it.  Measured twice: with
+    particles and Gaussians in random order (every bone-record gather a cache
+    miss; `value`) and in Z-order (as a grid-filling particle sampler and a
+    splat initialised from a scanned point cloud store them)."""
     import torch
 
     from empm_amd import _lib
@@ -472,34 +576,19 @@ def bench_lbs(args, dev):
     bones = rng.random((nb, 3)) * 0.2
     xyz = bones[rng.integers(0, nb, ng)] + 0.004 * rng.standard_normal((ng, 3))
     quat = rng.standard_normal((ng, 4))
-    c = bones.mean(0)
-    ang = 0.1
-    Q = np.array([[np.cos(ang), -np.sin(ang), 0.0], [np.sin(ang), np.cos(ang), 0.0], [0.0, 0.0, 1.0]])
-    x, y, z = (bones - c).T
-    motions = (bones - c) @ Q.T + c - bones + 0.01 * np.stack([np.sin(20 * y), np.sin(20 * z), np.sin(20 * x)], 1)
+    lib = _lib.load()
+    st = stream_handle(dev)
+    frames = 200
+
+    def motions_of(b):
+        c = np.full(3, 0.1)
+        ang = 0.1
+        Q = np.array([[np.cos(ang), -np.sin(ang), 0.0], [np.sin(ang), np.cos(ang), 0.0], [0.0, 0.0, 1.0]])
+        x, y, z = (b - c).T
+        return (b - c) @ Q.T + c - b + 0.01 * np.stack([np.sin(20 * y), np.sin(20 * z), np.sin(20 * x)], 1)
 
     def t(a):
         return torch.tensor(a, dtype=torch.float32, device=dev)
-
-    tb, tm, tx, tq = t(bones), t(motions), t(xyz), t(quat)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rel = gs.get_topk_indices(tb, K=K)
-    _, idx = gs.knn_weights_sparse(tb, tx, K=K)
-    torch.cuda.synchronize()
-    setup_ms = (time.perf_counter() - t0) * 1e3
-    interp = gs.LBSInterpolator()
-    xo, qo = torch.empty_like(tx), torch.empty_like(tq)
-
-    def fused():
-        interp(tb, tm, rel, tx, quat=tq, weights=None, weights_indices=idx, xyz_out=xo, quat_out=qo)
-
-    def two_call():
-        w = gs.calc_weights_vals_from_indices(tb, tx, idx)
-        gs.interpolate_motions_speedup(bones=tb, motions=tm, relations=rel, weights=w, weights_indices=idx, xyz=tx,
-                                       quat=tq)
-
-    frames = 200
 
     def timed(fn):
         for _ in range(10):
@@ -511,46 +600,76 @@ def bench_lbs(args, dev):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) * 1e3 / frames
 
-    ms_fused = timed(fused)
-    ms_two = timed(two_call)
-    # the blend kernel alone, HIP events on its stream around each launch
-    lib = _lib.load()
-    rec, degen = interp.bones(tb, tm, rel)
-    st = stream_handle(dev)
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(frames)]
-    for e0, e1 in ev:
-        e0.record()
-        _lib.check(lib.empm_lbs_points(ng, tx.data_ptr(), tq.data_ptr(), K, idx.data_ptr(), 4, None, nb,
-                                       rec.data_ptr(), degen.data_ptr(), xo.data_ptr(), qo.data_ptr(), st))
-        e1.record()
-    torch.cuda.synchronize()
-    us = sorted(e0.elapsed_time(e1) * 1e3 for e0, e1 in ev)
-    avg_us = sum(us) / len(us)
+    def case(bones, xyz, quat):
+        tb, tm, tx, tq = t(bones), t(motions_of(bones)), t(xyz), t(quat)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rel = gs.get_topk_indices(tb, K=K)
+        _, idx = gs.knn_weights_sparse(tb, tx, K=K)
+        torch.cuda.synchronize()
+        setup_ms = (time.perf_counter() - t0) * 1e3
+        interp = gs.LBSInterpolator()
+        xo, qo = torch.empty_like(tx), torch.empty_like(tq)
+
+        def fused():
+            interp(tb, tm, rel, tx, quat=tq, weights=None, weights_indices=idx, xyz_out=xo, quat_out=qo)
+
+        def two_call():
+            w = gs.calc_weights_vals_from_indices(tb, tx, idx)
+            gs.interpolate_motions_speedup(bones=tb, motions=tm, relations=rel, weights=w, weights_indices=idx,
+                                           xyz=tx, quat=tq)
+
+        ms_fused = timed(fused)
+        ms_two = timed(two_call)
+        # the blend kernel alone, HIP events on its stream around each launch
+        rec, degen = interp.bones(tb, tm, rel)
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(frames)]
+        for e0, e1 in ev:
+            e0.record()
+            _lib.check(lib.empm_lbs_points(ng, tx.data_ptr(), tq.data_ptr(), K, idx.data_ptr(), 4, None, nb,
+                                           rec.data_ptr(), degen.data_ptr(), xo.data_ptr(), qo.data_ptr(), st))
+            e1.record()
+        torch.cuda.synchronize()
+        us = sorted(e0.elapsed_time(e1) * 1e3 for e0, e1 in ev)
+        return {"ms_per_frame": ms_fused, "ms_per_frame_two_call": ms_two, "setup_ms_knn": setup_ms,
+                "avg_launch_us": sum(us) / len(us), "median_launch_us": us[len(us) // 2]}, rel, idx
+
+    rnd, rel, idx = case(bones, xyz, quat)
+    ob = np.argsort(_morton(bones, 0.0, 0.2), kind="stable")
+    og = np.argsort(_morton(xyz, -0.02, 0.22), kind="stable")
+    zo, _, _ = case(bones[ob], xyz[og], quat[og])
     # per Gaussian: x (12) and quaternion (16) read, K int32 indices (64), x' (12) and q' (16) written;
-    # per bone: its 80-byte record, read once
-    algo = ng * (12 + 16 + 4 * K + 12 + 16) + nb * 80
-    achieved = algo / (avg_us * 1e-6) / 1e9
+    # per bone: its 64-byte record, read once
+    algo = ng * (12 + 16 + 4 * K + 12 + 16) + nb * 64
+    achieved = algo / (rnd["avg_launch_us"] * 1e-6) / 1e9
     out = {"workload": f"linear-blend-skinning of {ng} Gaussians on {nb} particles, K={K} bones per Gaussian, "
-                       f"{K} neighbours per bone, per-frame bone transforms + blend",
-           "value": ng / (ms_fused * 1e-3), "unit": "Gaussians/s (per-frame interpolation)",
-           "ms_per_frame": ms_fused, "ms_per_frame_two_call": ms_two, "setup_ms_knn": setup_ms,
+                       f"{K} neighbours per bone, per-frame bone transforms + blend; random storage order",
+           "value": ng / (rnd["ms_per_frame"] * 1e-3), "unit": "Gaussians/s (per-frame interpolation)", **rnd,
            "roofline": {"bound": "hbm", "kernel": "k_lbs_points", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_lbs_points", TRAFFIC_PROFILE_LBS),
-                        "traffic_source": TRAFFIC_PROFILE_LBS, "algorithmic_bytes": algo, "avg_launch_us": avg_us,
-                        "median_launch_us": us[len(us) // 2], "launches": frames}}
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": measured_traffic("k_lbs_points", TRAFFIC_PROFILE_LBS),
+                        "traffic_source": TRAFFIC_PROFILE_LBS, "algorithmic_bytes": algo,
+                        "avg_launch_us": rnd["avg_launch_us"], "launches": frames},
+           "z_order": {"value": ng / (zo["ms_per_frame"] * 1e-3), **zo,
+                       "roofline_frac": algo / (zo["avg_launch_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS}}
     if not args.no_cpu_baseline:
         from oracle import lbs as olbs
 
-        idx_np = idx[:50_000].long().cpu().numpy()
         rel_np = rel.long().cpu().numpy()
-        t0 = time.perf_counter()
-        olbs.interpolate(bones, motions, rel_np, xyz[:50_000], quat[:50_000], None, idx_np)
+        idx_np = idx.long().cpu().numpy()
+        mo = motions_of(bones)
+        n_cpu, t0 = 0, time.perf_counter()
+        while n_cpu < ng and time.perf_counter() - t0 < args.cpu_seconds:
+            sl = slice(n_cpu, n_cpu + 100_000)
+            olbs.interpolate(bones, mo, rel_np, xyz[sl], quat[sl], None, idx_np[sl])
+            n_cpu += len(xyz[sl])
         el = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": 50_000 / el, "unit": "Gaussians/s", "cores": 1, "kind": "port",
-                               "sample": f"50000 of the Gaussians, all {nb} bone transforms, float64 NumPy oracle, "
-                                         f"{el:.1f} s"}
-    log(f"[bench] lbs: {ms_fused:.3f} ms/frame fused, {ms_two:.3f} ms two-call, setup {setup_ms:.1f} ms; "
-        f"k_lbs_points avg {avg_us:.1f} us ({achieved:.0f} GB/s)")
+        out["cpu_baseline"] = {"value": n_cpu / el, "unit": "Gaussians/s", "cores": 1, "kind": "port",
+                               "sample": f"{n_cpu} of the Gaussians in chunks of 100000 (bone transforms of all "
+                                         f"{nb} particles per chunk), float64 NumPy oracle, {el:.1f} s"}
+    log(f"[bench] lbs: {rnd['ms_per_frame']:.3f} ms/frame fused ({zo['ms_per_frame']:.3f} in Z-order), "
+        f"{rnd['ms_per_frame_two_call']:.3f} two-call, setup {rnd['setup_ms_knn']:.1f} ms; k_lbs_points avg "
+        f"{rnd['avg_launch_us']:.1f} us ({achieved:.0f} GB/s), Z-order {zo['avg_launch_us']:.1f} us")
     return out
 
 

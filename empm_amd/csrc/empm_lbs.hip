@@ -16,7 +16,7 @@
 //   k_lbs_weights  inverse-distance weights of the K chosen bones;
 //   k_lbs_bones    per bone: F = sum_j (new_j - new_b)(old_j - old_b)^T over
 //                  its adjacent bones, the closest rotation R (Kabsch), the
-//                  rotation's quaternion, written as an 80-byte record;
+//                  rotation's quaternion, written as a 64-byte record;
 //   k_lbs_points   per Gaussian: gather of its K bone records, blended
 //                  position and quaternion, product with its own quaternion.
 // The per-frame path is k_lbs_bones (n_bones threads) + k_lbs_points (one
@@ -38,6 +38,10 @@ using namespace empm;
 namespace {
 
 constexpr int kLbsBlock = 256;
+
+// native 4-vectors for the non-temporal builtins (HIP's int4 / float4 are structs)
+typedef int iv4 __attribute__((ext_vector_type(4)));
+typedef float fv4 __attribute__((ext_vector_type(4)));
 constexpr int kKnnTile = 2048;
 constexpr int kMaxK = 32;
 
@@ -130,40 +134,66 @@ __global__ __launch_bounds__(kLbsBlock) void k_knn(int n, const float* __restric
   }
 }
 
-// Index rows are int32 or int64 (torch.topk's dtype), read as given.  An
-// index outside [0, nb) is replaced by 0 and reported (the caller then
-// writes NaN for that row, as no memory outside the bones may be read).
-template <typename I>
-__device__ __forceinline__ bool load_row(const I* __restrict__ row, int k, int nb, int* out) {
+// Index rows are int32 or int64 (torch.topk's dtype), read as given; KC = 16
+// with int32 reads the 64-byte row as four 16-byte loads (plain loads: the
+// four pieces of a row share a cache line, which a streaming hint would let
+// go between them).  An index outside [0, nb) is
+// replaced by 0 and reported (the caller then writes NaN for that row, as no
+// memory outside the bones may be read).
+template <int KC, typename I>
+__device__ __forceinline__ bool load_row(const I* __restrict__ idx, int i, int k, int nb, int* out) {
+  constexpr int KM = KC ? KC : kMaxK;
   bool ok = true;
+  if constexpr (KC == 16 && sizeof(I) == 4) {
+    const iv4* row = reinterpret_cast<const iv4*>(idx + (size_t)i * 16);
 #pragma unroll
-  for (int j = 0; j < kMaxK; ++j) {
-    if (j < k) {
-      const long long v = (long long)row[j];
-      const bool in = v >= 0 && v < nb;
+    for (int c = 0; c < 4; ++c) {
+      const iv4 v = row[c];
+      out[4 * c] = v.x;
+      out[4 * c + 1] = v.y;
+      out[4 * c + 2] = v.z;
+      out[4 * c + 3] = v.w;
+    }
+#pragma unroll
+    for (int j = 0; j < KM; ++j) {
+      const bool in = (unsigned)out[j] < (unsigned)nb;
       ok = ok && in;
-      out[j] = in ? (int)v : 0;
+      out[j] = in ? out[j] : 0;
+    }
+  } else {
+    const I* row = idx + (size_t)i * k;
+#pragma unroll
+    for (int j = 0; j < KM; ++j) {
+      if (j < k) {
+        const long long v = (long long)row[j];
+        const bool in = v >= 0 && v < nb;
+        ok = ok && in;
+        out[j] = in ? (int)v : 0;
+      }
     }
   }
   return ok;
 }
 
 // calc_weights_vals_from_indices (dynamic_utils.py:282-292):
-// w_j = 1 / (|x - b_idx_j| + 1e-6), normalised to sum 1 over the row.
-template <typename I>
-__global__ __launch_bounds__(kLbsBlock) void k_lbs_weights(int n, const float* __restrict__ xyz, int k,
+// w_j = 1 / (|x - b_idx_j| + 1e-6), normalised to sum 1 over the row.  KC = 16
+// writes the row as four 16-byte stores.
+template <int KC, typename I>
+__global__ __launch_bounds__(kLbsBlock) void k_lbs_weights(int n, const float* __restrict__ xyz, int k_rt,
                                                            const I* __restrict__ idx, int nb,
                                                            const float* __restrict__ bones,
                                                            float* __restrict__ w) {
+  constexpr int KM = KC ? KC : kMaxK;
+  const int k = KC ? KC : k_rt;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float x = xyz[3 * (size_t)i], y = xyz[3 * (size_t)i + 1], z = xyz[3 * (size_t)i + 2];
-  int nbr[kMaxK];
-  const bool ok = load_row(idx + (size_t)i * k, k, nb, nbr);
-  float v[kMaxK];
+  int nbr[KM];
+  const bool ok = load_row<KC, I>(idx, i, k, nb, nbr);
+  float v[KM];
   float sum = 0.f;
 #pragma unroll
-  for (int j = 0; j < kMaxK; ++j) {
+  for (int j = 0; j < KM; ++j) {
     if (j < k) {
       const size_t b = 3 * (size_t)nbr[j];
       const float dx = x - bones[b], dy = y - bones[b + 1], dz = z - bones[b + 2];
@@ -172,9 +202,22 @@ __global__ __launch_bounds__(kLbsBlock) void k_lbs_weights(int n, const float* _
     }
   }
   const float inv = ok ? 1.f / sum : __builtin_nanf("");
+  if constexpr (KC == 16) {
+    fv4* row = reinterpret_cast<fv4*>(w + (size_t)i * 16);
 #pragma unroll
-  for (int j = 0; j < kMaxK; ++j)
-    if (j < k) w[(size_t)i * k + j] = v[j] * inv;
+    for (int c = 0; c < 4; ++c) {
+      fv4 o;
+      o.x = v[4 * c] * inv;
+      o.y = v[4 * c + 1] * inv;
+      o.z = v[4 * c + 2] * inv;
+      o.w = v[4 * c + 3] * inv;
+      row[c] = o;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < KM; ++j)
+      if (j < k) w[(size_t)i * k + j] = v[j] * inv;
+  }
 }
 
 // Quaternion (w, x, y, z) of a rotation matrix by Shepperd's method with the
@@ -217,9 +260,11 @@ __device__ __forceinline__ float4 normalize4(float4 q) {
 // determinant corrections produce.  svd3 returns U, V in SO(3) with the sign
 // of det F on the last singular value, so that rotation is U V^T.
 //
-// The record of bone b (5 float4):
-//   [R00 R01 R02 b.x] [R10 R11 R12 b.y] [R20 R21 R22 b.z] [m.x m.y m.z 0] [q]
-// q = normalize(quaternion of R).
+// The record of bone b is 64 bytes (4 float4, one 64-byte line segment for
+// a gather):
+//   [q.w q.x q.y q.z] [b.x b.y b.z 0] [m.x m.y m.z 0] [0 0 0 0]
+// q = normalize(quaternion of R); the blend rotates by q, whose matrix is R
+// to float32 round-off (the conversion's eps = 1e-8 is below it).
 //
 // A bone whose F has rank < 2 (torch.linalg.matrix_rank with its default
 // tolerance: singular values above 3 * eps_f32 * s_max count) sends the
@@ -291,12 +336,11 @@ __global__ __launch_bounds__(kLbsBlock) void k_lbs_bones(int nb, const float* __
     m_mul_bt(U, V, R);
   }
   const float4 qn = normalize4(rot_to_quat(R));
-  float4* o = rec + 5 * (size_t)b;
-  o[0] = make_float4(R[0], R[1], R[2], bx);
-  o[1] = make_float4(R[3], R[4], R[5], by);
-  o[2] = make_float4(R[6], R[7], R[8], bz);
-  o[3] = make_float4(mx, my, mz, 0.f);
-  o[4] = qn;
+  float4* o = rec + 4 * (size_t)b;
+  o[0] = qn;
+  o[1] = make_float4(bx, by, bz, 0.f);
+  o[2] = make_float4(mx, my, mz, 0.f);
+  o[3] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // Hamilton product q1 * q2, (w, x, y, z) (dynamic_utils.py:452-460)
@@ -305,85 +349,116 @@ __device__ __forceinline__ float4 quat_mul(float4 a, float4 b) {
                      a.x * b.z - a.y * b.w + a.z * b.x + a.w * b.y, a.x * b.w + a.y * b.z - a.z * b.y + a.w * b.x);
 }
 
+// v rotated by the unit quaternion q = (w, u): v + 2w (u x v) + 2 u x (u x v)
+__device__ __forceinline__ void quat_rotate(float4 q, float vx, float vy, float vz, float& ox, float& oy, float& oz) {
+  const float tx = 2.f * (q.z * vz - q.w * vy), ty = 2.f * (q.w * vx - q.y * vz), tz = 2.f * (q.y * vy - q.z * vx);
+  ox = vx + q.x * tx + (q.z * tz - q.w * ty);
+  oy = vy + q.x * ty + (q.w * tx - q.y * tz);
+  oz = vz + q.x * tz + (q.y * ty - q.z * tx);
+}
+
 // Per Gaussian (dynamic_utils.py:434-485):
 //   x' = sum_j w_j (R_j (x - b_j) + m_j + b_j)
 //   q' = normalize(sum_j w_j q_j) * quat   (when quat is given)
 // with w either given (n x k) or, when null, the inverse-distance weights of
 // calc_weights_vals_from_indices computed from the same bones and x.  With
 // *degenerate != 0 every R_j is the identity and every q_j = (1, 0, 0, 0).
-template <typename I>
+// One thread per Gaussian; KC = 16 is the reference's K with its index row
+// read as four 16-byte loads, KC = 0 any K <= 32; W: weights given, Q:
+// quaternions given.  Every branch inside the neighbour loop is resolved at
+// compile time (the identity fallback is a select of q = (1, 0, 0, 0), whose
+// rotation is exact), so the gathers of all neighbours are in flight
+// together.  The per-Gaussian streams
+// (x, quat, indices, weights, outputs) are read and written non-temporally so
+// that the bone records stay in L2 for the gathers.
+template <int KC, typename I, bool W, bool Q>
 __global__ __launch_bounds__(kLbsBlock) void k_lbs_points(int n, const float* __restrict__ xyz,
-                                                          const float* __restrict__ quat, int k,
+                                                          const float* __restrict__ quat, int k_rt,
                                                           const I* __restrict__ idx,
                                                           const float* __restrict__ weights, int nbones,
                                                           const float4* __restrict__ rec,
                                                           const int* __restrict__ degenerate,
                                                           float* __restrict__ xyz_out,
                                                           float* __restrict__ quat_out) {
+  constexpr int KM = KC ? KC : kMaxK;
+  const int k = KC ? KC : k_rt;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const bool ident = *degenerate != 0;
   const float x = xyz[3 * (size_t)i], y = xyz[3 * (size_t)i + 1], z = xyz[3 * (size_t)i + 2];
-  int nb[kMaxK];
-  const bool ok = load_row(idx + (size_t)i * k, k, nbones, nb);
-  float w[kMaxK];
-  if (weights) {
+  int nb[KM];
+  const bool ok = load_row<KC, I>(idx, i, k, nbones, nb);
+  float wv[KM];
+  if constexpr (W) {
+    if constexpr (KC == 16) {
+      const fv4* row = reinterpret_cast<const fv4*>(weights + (size_t)i * 16);
 #pragma unroll
-    for (int j = 0; j < kMaxK; ++j)
-      if (j < k) w[j] = weights[(size_t)i * k + j];
-  } else {
-    float sum = 0.f;
-#pragma unroll
-    for (int j = 0; j < kMaxK; ++j) {
-      if (j < k) {
-        const float4* r = rec + 5 * (size_t)nb[j];
-        const float dx = x - r[0].w, dy = y - r[1].w, dz = z - r[2].w;
-        w[j] = 1.f / (sqrtf(dx * dx + dy * dy + dz * dz) + 1e-6f);
-        sum += w[j];
+      for (int c = 0; c < 4; ++c) {
+        const fv4 v = row[c];
+        wv[4 * c] = v.x;
+        wv[4 * c + 1] = v.y;
+        wv[4 * c + 2] = v.z;
+        wv[4 * c + 3] = v.w;
       }
-    }
-    const float inv = 1.f / sum;
+    } else {
 #pragma unroll
-    for (int j = 0; j < kMaxK; ++j)
-      if (j < k) w[j] *= inv;
+      for (int j = 0; j < KM; ++j)
+        if (j < k) wv[j] = weights[(size_t)i * k + j];
+    }
   }
-  float px = 0.f, py = 0.f, pz = 0.f;
+  // One pass over the neighbours.  Without given weights the blend uses the
+  // unnormalised v_j = 1 / (|x - b_j| + 1e-6) and divides by their sum at
+  // the end (the quaternion is normalised anyway), which is the reference's
+  // w_j = v_j / sum_j v_j up to rounding.
+  float px = 0.f, py = 0.f, pz = 0.f, vsum = 0.f;
   float4 qa = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-  for (int j = 0; j < kMaxK; ++j) {
+  for (int j = 0; j < KM; ++j) {
     if (j < k) {
-      const float4* r = rec + 5 * (size_t)nb[j];
-      const float4 r0 = r[0], r1 = r[1], r2 = r[2], t = r[3];
-      const float lx = x - r0.w, ly = y - r1.w, lz = z - r2.w;
-      float ox, oy, oz;
-      if (ident) {
-        ox = lx;
-        oy = ly;
-        oz = lz;
+      const float4* r = rec + 4 * (size_t)nb[j];
+      const float4 b = r[1], m = r[2];
+      const float4 rq = r[0];
+      const float4 bq = ident ? make_float4(1.f, 0.f, 0.f, 0.f) : rq;
+      const float lx = x - b.x, ly = y - b.y, lz = z - b.z;
+      float wj;
+      if constexpr (W) {
+        wj = wv[j];
       } else {
-        ox = r0.x * lx + r0.y * ly + r0.z * lz;
-        oy = r1.x * lx + r1.y * ly + r1.z * lz;
-        oz = r2.x * lx + r2.y * ly + r2.z * lz;
+        wj = 1.f / (sqrtf(lx * lx + ly * ly + lz * lz) + 1e-6f);
+        vsum += wj;
       }
-      px += ((ox + t.x) + r0.w) * w[j];
-      py += ((oy + t.y) + r1.w) * w[j];
-      pz += ((oz + t.z) + r2.w) * w[j];
-      if (quat) {
-        const float4 bq = ident ? make_float4(1.f, 0.f, 0.f, 0.f) : r[4];
-        qa.x += bq.x * w[j];
-        qa.y += bq.y * w[j];
-        qa.z += bq.z * w[j];
-        qa.w += bq.w * w[j];
+      float ox, oy, oz;
+      quat_rotate(bq, lx, ly, lz, ox, oy, oz);
+      px += ((ox + m.x) + b.x) * wj;
+      py += ((oy + m.y) + b.y) * wj;
+      pz += ((oz + m.z) + b.z) * wj;
+      if constexpr (Q) {
+        qa.x += bq.x * wj;
+        qa.y += bq.y * wj;
+        qa.z += bq.z * wj;
+        qa.w += bq.w * wj;
       }
     }
   }
+  if constexpr (!W) {
+    const float inv = 1.f / vsum;
+    px *= inv;
+    py *= inv;
+    pz *= inv;
+  }
   if (!ok) px = py = pz = qa.x = __builtin_nanf("");
-  xyz_out[3 * (size_t)i] = px;
-  xyz_out[3 * (size_t)i + 1] = py;
-  xyz_out[3 * (size_t)i + 2] = pz;
-  if (quat) {
-    const float4 g = reinterpret_cast<const float4*>(quat)[i];
-    reinterpret_cast<float4*>(quat_out)[i] = quat_mul(normalize4(qa), g);
+  __builtin_nontemporal_store(px, xyz_out + 3 * (size_t)i);
+  __builtin_nontemporal_store(py, xyz_out + 3 * (size_t)i + 1);
+  __builtin_nontemporal_store(pz, xyz_out + 3 * (size_t)i + 2);
+  if constexpr (Q) {
+    const fv4 gv = __builtin_nontemporal_load(reinterpret_cast<const fv4*>(quat) + i);
+    const float4 r = quat_mul(normalize4(qa), make_float4(gv.x, gv.y, gv.z, gv.w));
+    fv4 rv;
+    rv.x = r.x;
+    rv.y = r.y;
+    rv.z = r.z;
+    rv.w = r.w;
+    __builtin_nontemporal_store(rv, reinterpret_cast<fv4*>(quat_out) + i);
   }
 }
 
@@ -423,12 +498,20 @@ int empm_lbs_weights(int n, const float* xyz, int k, const void* idx, int idx_by
   if (n_bones < 1) return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_weights: n_bones=%lld", n_bones);
   if (!xyz || !idx || !bones || !weights) return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_weights: null array");
   hipStream_t st = (hipStream_t)stream;
-  if (idx_bytes == 4)
-    hipLaunchKernelGGL(k_lbs_weights<int32_t>, dim3(lbs_blocks(n)), dim3(kLbsBlock), 0, st, n, xyz, k,
-                       (const int32_t*)idx, n_bones, bones, weights);
+  const dim3 g(lbs_blocks(n)), blk(kLbsBlock);
+  const bool v16 = k == 16 && (idx_bytes == 8 || aligned16(idx)) && aligned16(weights);
+  if (idx_bytes == 4 && v16)
+    hipLaunchKernelGGL((k_lbs_weights<16, int32_t>), g, blk, 0, st, n, xyz, k, (const int32_t*)idx, n_bones, bones,
+                       weights);
+  else if (idx_bytes == 4)
+    hipLaunchKernelGGL((k_lbs_weights<0, int32_t>), g, blk, 0, st, n, xyz, k, (const int32_t*)idx, n_bones, bones,
+                       weights);
+  else if (v16)
+    hipLaunchKernelGGL((k_lbs_weights<16, int64_t>), g, blk, 0, st, n, xyz, k, (const int64_t*)idx, n_bones, bones,
+                       weights);
   else
-    hipLaunchKernelGGL(k_lbs_weights<int64_t>, dim3(lbs_blocks(n)), dim3(kLbsBlock), 0, st, n, xyz, k,
-                       (const int64_t*)idx, n_bones, bones, weights);
+    hipLaunchKernelGGL((k_lbs_weights<0, int64_t>), g, blk, 0, st, n, xyz, k, (const int64_t*)idx, n_bones, bones,
+                       weights);
   LBS_HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }
@@ -440,7 +523,8 @@ int empm_lbs_bones(int n_bones, const float* bones, const float* motions, int n_
   if (!degenerate) return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_bones: null degenerate counter");
   if (n_bones > 0 && (!bones || !motions || !bone_rec || (n_adj > 0 && !relations)))
     return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_bones: null array");
-  if (n_bones > 0 && !aligned16(bone_rec)) return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_bones: bone_rec not 16-byte aligned");
+  if (n_bones > 0 && ((uintptr_t)bone_rec & 63u))
+    return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_bones: bone_rec not 64-byte aligned");
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, degenerate);
   if (n_bones > 0) {
@@ -466,15 +550,35 @@ int empm_lbs_points(int n, const float* xyz, const float* quat, int k, const voi
     return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_points: null array");
   if (xyz_out == xyz || (quat && quat_out == quat))
     return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_points: outputs alias inputs");
-  if (!aligned16(bone_rec) || (quat && (!aligned16(quat) || !aligned16(quat_out))))
-    return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_points: bone_rec and quaternions must be 16-byte aligned");
+  if (((uintptr_t)bone_rec & 63u) || (quat && (!aligned16(quat) || !aligned16(quat_out))))
+    return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_points: bone_rec must be 64-byte, quaternions 16-byte aligned");
   hipStream_t st = (hipStream_t)stream;
-  if (idx_bytes == 4)
-    hipLaunchKernelGGL(k_lbs_points<int32_t>, dim3(lbs_blocks(n)), dim3(kLbsBlock), 0, st, n, xyz, quat, k,
-                       (const int32_t*)idx, weights, n_bones, (const float4*)bone_rec, degenerate, xyz_out, quat_out);
-  else
-    hipLaunchKernelGGL(k_lbs_points<int64_t>, dim3(lbs_blocks(n)), dim3(kLbsBlock), 0, st, n, xyz, quat, k,
-                       (const int64_t*)idx, weights, n_bones, (const float4*)bone_rec, degenerate, xyz_out, quat_out);
+  const dim3 g(lbs_blocks(n)), blk(kLbsBlock);
+  const float4* r4 = (const float4*)bone_rec;
+  const bool v16 = k == 16 && (idx_bytes == 8 || aligned16(idx)) && (!weights || aligned16(weights));
+  const int sel = (v16 ? 8 : 0) | (idx_bytes == 8 ? 4 : 0) | (weights ? 2 : 0) | (quat ? 1 : 0);
+#define EMPM_LBS_POINTS(KC, I, W, Q)                                                                          \
+  hipLaunchKernelGGL((k_lbs_points<KC, I, W, Q>), g, blk, 0, st, n, xyz, quat, k, (const I*)idx, weights, n_bones, \
+                     r4, degenerate, xyz_out, quat_out)
+  switch (sel) {
+    case 0: EMPM_LBS_POINTS(0, int32_t, false, false); break;
+    case 1: EMPM_LBS_POINTS(0, int32_t, false, true); break;
+    case 2: EMPM_LBS_POINTS(0, int32_t, true, false); break;
+    case 3: EMPM_LBS_POINTS(0, int32_t, true, true); break;
+    case 4: EMPM_LBS_POINTS(0, int64_t, false, false); break;
+    case 5: EMPM_LBS_POINTS(0, int64_t, false, true); break;
+    case 6: EMPM_LBS_POINTS(0, int64_t, true, false); break;
+    case 7: EMPM_LBS_POINTS(0, int64_t, true, true); break;
+    case 8: EMPM_LBS_POINTS(16, int32_t, false, false); break;
+    case 9: EMPM_LBS_POINTS(16, int32_t, false, true); break;
+    case 10: EMPM_LBS_POINTS(16, int32_t, true, false); break;
+    case 11: EMPM_LBS_POINTS(16, int32_t, true, true); break;
+    case 12: EMPM_LBS_POINTS(16, int64_t, false, false); break;
+    case 13: EMPM_LBS_POINTS(16, int64_t, false, true); break;
+    case 14: EMPM_LBS_POINTS(16, int64_t, true, false); break;
+    default: EMPM_LBS_POINTS(16, int64_t, true, true); break;
+  }
+#undef EMPM_LBS_POINTS
   LBS_HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }

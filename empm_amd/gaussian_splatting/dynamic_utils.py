@@ -127,8 +127,8 @@ def calc_weights_vals_from_indices(bones, pts, indices):
 
 class LBSInterpolator:
     """Per-frame interpolation with persistent device buffers: the bone
-    records (80 bytes per bone) and the degenerate-bone counter are reused
-    across frames."""
+    records (64 bytes per bone: unit quaternion, position, motion) and the
+    degenerate-bone counter are reused across frames."""
 
     def __init__(self, device=None):
         self.device = device
@@ -137,7 +137,9 @@ class LBSInterpolator:
 
     def _buffers(self, n_bones: int, device):
         if self._rec is None or self._rec.shape[0] < n_bones or self._rec.device != device:
-            self._rec = torch.empty((max(n_bones, 1), 20), dtype=torch.float32, device=device)
+            self._rec = torch.empty((max(n_bones, 1), 16), dtype=torch.float32, device=device)
+            if self._rec.data_ptr() % 64:
+                raise RuntimeError("bone records must be 64-byte aligned")
             self._degen = torch.empty(1, dtype=torch.int32, device=device)
         return self._rec, self._degen
 

@@ -365,8 +365,8 @@ int empm_lbs_weights(int n, const float *xyz, int k, const void *idx, int idx_by
  * 321-416): for bone b with adjacent bones relations[b, :] (n_bones x n_adj),
  * F = sum_j ((bones + motions)[a_j] - (bones + motions)[b]) (bones[a_j] -
  * bones[b])^T, R = the rotation closest to F, q = its unit quaternion
- * (w, x, y, z).  bone_rec: 20 floats per bone, 16-byte aligned:
- * [R00 R01 R02 b.x R10 R11 R12 b.y R20 R21 R22 b.z m.x m.y m.z 0 q.w q.x q.y q.z];
+ * (w, x, y, z).  bone_rec: 16 floats per bone, 64-byte aligned:
+ * [q.w q.x q.y q.z  b.x b.y b.z 0  m.x m.y m.z 0  0 0 0 0];
  * a bone with a relation index outside [0, n_bones) gets a NaN rotation.
  * *degenerate (one device int) receives the number of bones whose F has rank
  * below 2; when it is non-zero the reference leaves every rotation at the
