@@ -506,10 +506,13 @@ def bench_loss(args, dev):
     idx = torch.empty(n, dtype=torch.int32, device=dev)
     lib = _lib.load()
     st = stream_handle(dev)
+    ws_bytes = int(lib.empm_nn_sqdist_workspace(n, n))
+    ws = torch.empty(max(ws_bytes // 4, 1), dtype=torch.int32, device=dev)
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(50)]
     for e0, e1 in ev:
         e0.record()
-        _lib.check(lib.empm_nn_sqdist(n, a.data_ptr(), n, b.data_ptr(), dist.data_ptr(), idx.data_ptr(), st))
+        _lib.check(lib.empm_nn_sqdist(n, a.data_ptr(), n, b.data_ptr(), dist.data_ptr(), idx.data_ptr(),
+                                      ws.data_ptr(), ws_bytes, st))
         e1.record()
     torch.cuda.synchronize()
     us = sum(e0.elapsed_time(e1) for e0, e1 in ev) / len(ev) * 1e3

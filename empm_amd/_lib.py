@@ -37,6 +37,7 @@ EXPORTED_SYMBOLS = (
     "empm_mu_lam_adj",
     "empm_mass_from_density",
     "empm_nn_sqdist",
+    "empm_nn_sqdist_workspace",
     "empm_substep_fwd",
     "empm_substep_adj",
     "empm_rollout_fwd",
@@ -198,7 +199,9 @@ def _declare(lib):
     lib.empm_mass_from_density.restype = C.c_int
     lib.empm_mass_from_density.argtypes = [C.c_int, vp, vp, vp, vp]
     lib.empm_nn_sqdist.restype = C.c_int
-    lib.empm_nn_sqdist.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp]
+    lib.empm_nn_sqdist.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, C.c_size_t, vp]
+    lib.empm_nn_sqdist_workspace.restype = C.c_size_t
+    lib.empm_nn_sqdist_workspace.argtypes = [C.c_int, C.c_int]
     lib.empm_knn.restype = C.c_int
     lib.empm_knn.argtypes = [C.c_int, vp, C.c_int, vp, C.c_int, C.c_int, vp, vp, vp]
     lib.empm_lbs_weights.restype = C.c_int

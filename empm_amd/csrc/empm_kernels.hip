@@ -1566,13 +1566,19 @@ __global__ void k_mass(int n, const float* __restrict__ rho, const float* __rest
 // Nearest neighbour in B of every point of A: min_j |a_i - b_j|^2 and the
 // smallest j attaining it (the knn_points(K = 1) step of the Chamfer loss).
 // Each thread owns one point of A; B passes through LDS in tiles that every
-// lane reads at the same address (a broadcast).
+// lane reads at the same address (a broadcast).  A few thousand points of A
+// are too few threads for 256 CUs, so B is split over blockIdx.y in chunks
+// of `chunk` points: split s writes its best (distance, index) to
+// dist/idx + s * n and k_nn_reduce takes the minimum over the splits, in
+// split order with a strict comparison, which keeps the smallest index.
 constexpr int kNnTile = 2048;
 __global__ __launch_bounds__(kBlock) void k_nn_sqdist(int n, const float* __restrict__ a, int m,
-                                                      const float* __restrict__ b, float* __restrict__ dist,
-                                                      int* __restrict__ idx) {
+                                                      const float* __restrict__ b, int chunk,
+                                                      float* __restrict__ dist, int* __restrict__ idx) {
   __shared__ float4 tile[kNnTile];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j0 = blockIdx.y * chunk;
+  const int j1 = min(m, j0 + chunk);
   float ax = 0.f, ay = 0.f, az = 0.f;
   if (i < n) {
     ax = a[3 * (size_t)i];
@@ -1580,9 +1586,9 @@ __global__ __launch_bounds__(kBlock) void k_nn_sqdist(int n, const float* __rest
     az = a[3 * (size_t)i + 2];
   }
   float best = __builtin_huge_valf();
-  int bj = 0;
-  for (int t0 = 0; t0 < m; t0 += kNnTile) {
-    const int cnt = m - t0 < kNnTile ? m - t0 : kNnTile;
+  int bj = j0;
+  for (int t0 = j0; t0 < j1; t0 += kNnTile) {
+    const int cnt = j1 - t0 < kNnTile ? j1 - t0 : kNnTile;
     __syncthreads();
     for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
       const size_t q = 3 * (size_t)(t0 + k);
@@ -1601,9 +1607,39 @@ __global__ __launch_bounds__(kBlock) void k_nn_sqdist(int n, const float* __rest
     }
   }
   if (i < n) {
-    dist[i] = best;
-    idx[i] = bj;
+    const size_t o = (size_t)blockIdx.y * n + i;
+    dist[o] = best;
+    idx[o] = bj;
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_nn_reduce(int n, int splits, const float* __restrict__ pd,
+                                                      const int* __restrict__ pi, float* __restrict__ dist,
+                                                      int* __restrict__ idx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float best = pd[i];
+  int bj = pi[i];
+  for (int s = 1; s < splits; ++s) {
+    const float d = pd[(size_t)s * n + i];
+    if (d < best) {
+      best = d;
+      bj = pi[(size_t)s * n + i];
+    }
+  }
+  dist[i] = best;
+  idx[i] = bj;
+}
+
+// Splits of B for n points of A: enough blocks for 8 per CU, at least 512
+// points of B per split.
+int nn_splits(int n, int m) {
+  const int qb = (n + kBlock - 1) / kBlock;
+  int s = (2048 + qb - 1) / qb;
+  s = std::min(s, std::max(1, m / 512));
+  if (s < 1) s = 1;
+  const int chunk = (m + s - 1) / s;
+  return chunk > 0 ? (m + chunk - 1) / chunk : 1;
 }
 
 }  // namespace
@@ -2427,11 +2463,29 @@ int empm_mass_from_density(int n, const float* density, const float* vol, float*
   return EMPM_OK;
 }
 
-int empm_nn_sqdist(int n, const float* a, int m, const float* b, float* dist, int* idx, void* stream) {
+size_t empm_nn_sqdist_workspace(int n, int m) {
+  if (n <= 0 || m <= 0) return 0;
+  const int s = nn_splits(n, m);
+  return s > 1 ? (size_t)s * (size_t)n * 8 : 0;
+}
+
+int empm_nn_sqdist(int n, const float* a, int m, const float* b, float* dist, int* idx, void* workspace,
+                   size_t workspace_bytes, void* stream) {
   if (n < 0 || m < 0 || (n > 0 && (m < 1 || !a || !b || !dist || !idx)))
     return set_err(EMPM_E_INVALID_ARG, "empm_nn_sqdist: bad arguments (n=%d, m=%d)", n, m);
   if (n == 0) return EMPM_OK;
-  hipLaunchKernelGGL(k_nn_sqdist, dim3(nblocks(n)), dim3(kBlock), 0, (hipStream_t)stream, n, a, m, b, dist, idx);
+  hipStream_t st = (hipStream_t)stream;
+  int s = nn_splits(n, m);
+  if (s > 1 && (!workspace || workspace_bytes < (size_t)s * (size_t)n * 8 || ((uintptr_t)workspace & 3u))) s = 1;
+  const int chunk = (m + s - 1) / s;
+  if (s == 1) {
+    hipLaunchKernelGGL(k_nn_sqdist, dim3(nblocks(n), 1), dim3(kBlock), 0, st, n, a, m, b, m, dist, idx);
+  } else {
+    float* pd = (float*)workspace;
+    int* pi = (int*)((char*)workspace + (size_t)s * n * 4);
+    hipLaunchKernelGGL(k_nn_sqdist, dim3(nblocks(n), s), dim3(kBlock), 0, st, n, a, m, b, chunk, pd, pi);
+    hipLaunchKernelGGL(k_nn_reduce, dim3(nblocks(n)), dim3(kBlock), 0, st, n, s, pd, pi, dist, idx);
+  }
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }

@@ -41,7 +41,11 @@ def nn_sqdist(a: torch.Tensor, b: torch.Tensor):
         raise ValueError("nearest neighbour in an empty cloud")
     dist = torch.empty(n, dtype=torch.float32, device=a.device)
     idx = torch.empty(n, dtype=torch.int32, device=a.device)
-    _lib.check(_lib.load().empm_nn_sqdist(n, _ptr(a), m, _ptr(b), _ptr(dist), _ptr(idx), stream_handle(a.device)))
+    lib = _lib.load()
+    ws_bytes = int(lib.empm_nn_sqdist_workspace(n, m))
+    ws = torch.empty(max(ws_bytes // 4, 1), dtype=torch.int32, device=a.device)
+    _lib.check(lib.empm_nn_sqdist(n, _ptr(a), m, _ptr(b), _ptr(dist), _ptr(idx), ws.data_ptr(), ws_bytes,
+                                  stream_handle(a.device)))
     return dist, idx.long()
 
 

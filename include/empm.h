@@ -207,8 +207,15 @@ int empm_mass_from_density(int n, const float *density, const float *vol, float 
  * step of pytorch3d.loss.chamfer_distance, which the training loss calls per
  * frame on the visible points (phys_sim/engine/trainer_warp_mpm.py:746-752).
  * a: n x 3, b: m x 3, float32 device arrays; dist: n floats, idx: n ints;
- * m >= 1 when n >= 1. */
-int empm_nn_sqdist(int n, const float *a, int m, const float *b, float *dist, int *idx, void *stream);
+ * m >= 1 when n >= 1.  workspace (device, 4-byte aligned, may be NULL) of
+ * empm_nn_sqdist_workspace(n, m) bytes lets the search split B over more
+ * workgroups when A alone is too few threads to fill the GPU; without it
+ * the search runs unsplit (same result, slower for small n). */
+int empm_nn_sqdist(int n, const float *a, int m, const float *b, float *dist, int *idx, void *workspace,
+                   size_t workspace_bytes, void *stream);
+
+/* Workspace bytes empm_nn_sqdist uses for n points against m (0: none). */
+size_t empm_nn_sqdist_workspace(int n, int m);
 
 /* One substep state `in` -> state `out` (`out` must not alias `in`):
  * MPMWARPDiff.p2g2p_differentiable (mpm_solver_diff.py:165-291), i.e.

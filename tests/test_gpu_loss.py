@@ -40,6 +40,24 @@ def test_nearest_neighbour_ties_take_the_smallest_index(dev):
     assert i.tolist() == [0, 0] and d.tolist() == [1.0, 1.0]
 
 
+def test_nearest_neighbour_ties_across_splits_of_the_search(dev):
+    # 300 points of A are too few threads, so B is searched in several
+    # splits; equal nearest points in different splits: the smaller index wins
+    from empm_amd import _lib
+    m = 4096
+    assert int(_lib.load().empm_nn_sqdist_workspace(300, m)) > 0
+    rng = np.random.default_rng(8)
+    b = rng.random((m, 3)).astype(np.float32) + 2.0
+    a = rng.random((300, 3)).astype(np.float32)
+    b[100] = a[7] + np.float32(0.001)
+    b[3900] = b[100]
+    d, i = loss.nn_sqdist(torch.as_tensor(a, device=dev), torch.as_tensor(b, device=dev))
+    rd, ri = och.nn_sqdist(a, b)
+    assert int(i[7]) == 100
+    np.testing.assert_array_equal(i.cpu().numpy(), ri)
+    np.testing.assert_allclose(d.cpu().numpy(), rd, rtol=1e-5)
+
+
 def test_chamfer_value_and_gradient(dev):
     rng = np.random.default_rng(4)
     x, y = rng.random((2, 700, 3)).astype(np.float32), rng.random((2, 500, 3)).astype(np.float32)
