@@ -35,6 +35,55 @@ from .. import _lib
 from ..runtime import stream_handle
 
 
+def quat2mat(q):
+    """Rotation matrices (n, 3, 3) of quaternions (n, 4) in (w, x, y, z)
+    order, normalised first (dynamic_utils.py:5-22).  Elementwise torch on
+    the quaternions' device."""
+    q = q / q.norm(dim=1, keepdim=True)
+    w, x, y, z = q.unbind(1)
+    return torch.stack([
+        1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+        2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+        2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], 1).reshape(-1, 3, 3)
+
+
+def mat2quat(rot):
+    """Quaternions (n, 4), (w, x, y, z), of rotation matrices (n, 3, 3) by
+    the reference's own branch rule (dynamic_utils.py:25-64): the trace
+    branch whenever the trace (clamped at -1) exceeds -1, otherwise the
+    largest diagonal entry, ties going to the earlier axis for x, then y.
+    In the three diagonal branches the reference sets the branch's own
+    component to 0.25 / sqrt(1 + 2 r_aa - tr) instead of
+    0.5 * sqrt(1 + 2 r_aa - tr) (:38-64; the function is its fallback when
+    kornia is missing); here every branch returns the quaternion of the
+    matrix."""
+    m = rot
+    d0, d1, d2 = m[:, 0, 0], m[:, 1, 1], m[:, 2, 2]
+    t = torch.clamp(d0 + d1 + d2, min=-1)
+    q = torch.zeros((m.shape[0], 4), dtype=m.dtype, device=m.device)
+    b0 = t > -1
+    b1 = ~b0 & (d0 >= d1) & (d0 >= d2)
+    b2 = ~b0 & (d1 >= d2) & (d1 > d0)
+    b3 = ~b0 & (d2 > d0) & (d2 > d1)
+    if bool(b0.any()):
+        r = torch.sqrt(t[b0] + 1)
+        h = 0.5 / r
+        mm = m[b0]
+        q[b0] = torch.stack([0.5 * r, (mm[:, 2, 1] - mm[:, 1, 2]) * h, (mm[:, 0, 2] - mm[:, 2, 0]) * h,
+                             (mm[:, 1, 0] - mm[:, 0, 1]) * h], 1)
+    for sel, a, b, c in ((b1, 0, 1, 2), (b2, 1, 2, 0), (b3, 2, 0, 1)):
+        if bool(sel.any()):
+            mm = m[sel]
+            h = 0.5 / torch.sqrt(1 + mm[:, a, a] - mm[:, b, b] - mm[:, c, c])
+            out = torch.empty((mm.shape[0], 4), dtype=m.dtype, device=m.device)
+            out[:, 0] = (mm[:, c, b] - mm[:, b, c]) * h
+            out[:, 1 + a] = 0.25 / h
+            out[:, 1 + b] = (mm[:, b, a] + mm[:, a, b]) * h
+            out[:, 1 + c] = (mm[:, c, a] + mm[:, a, c]) * h
+            q[sel] = out
+    return q
+
+
 def _dev(t: torch.Tensor, what: str) -> None:
     if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
         raise RuntimeError(f"{what} must be a tensor on a GPU (the interpolation runs only on the GPU)")

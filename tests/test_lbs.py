@@ -152,6 +152,23 @@ def test_degenerate_bones_fall_back_to_identity():
     np.testing.assert_allclose(q, [[1.0, 0, 0, 0]], atol=1e-15)
 
 
+def test_quat2mat_and_mat2quat():
+    R = Rotation.random(300, random_state=21)
+    rs = [Rotation.from_rotvec(v) for v in ([np.pi * 0.999, 0, 0], [0, np.pi * 0.999, 0], [0, 0, np.pi * 0.999],
+                                             [np.pi, 0, 0], [0.2, 0.1, -0.3])]
+    M = np.concatenate([R.as_matrix(), np.stack([r.as_matrix() for r in rs])])
+    q = gs.mat2quat(torch.tensor(M)).numpy()
+    # every branch of the reference's rule is reached and gives a unit
+    # quaternion of the matrix (quat2mat restated in the oracle)
+    tr = np.trace(M, axis1=1, axis2=2)
+    assert np.any(tr <= -1 + 1e-9) or np.any(tr < -0.99)
+    np.testing.assert_allclose(np.linalg.norm(q, axis=1), 1.0, atol=1e-9)
+    np.testing.assert_allclose(lbs.quat_to_rot(q), M, atol=1e-9)
+    # quat2mat normalises and matches scipy
+    raw = _wxyz(R) * 3.0
+    np.testing.assert_allclose(gs.quat2mat(torch.tensor(raw)).numpy(), R.as_matrix(), atol=1e-12)
+
+
 def test_mirror_refuses_cpu_tensors():
     b = torch.rand(20, 3)
     with pytest.raises(RuntimeError, match="GPU"):
