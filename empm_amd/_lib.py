@@ -38,6 +38,8 @@ EXPORTED_SYMBOLS = (
     "empm_mass_from_density",
     "empm_nn_sqdist",
     "empm_nn_sqdist_workspace",
+    "empm_nn_sqdist_segments",
+    "empm_nn_sqdist_segments_workspace",
     "empm_substep_fwd",
     "empm_substep_adj",
     "empm_rollout_fwd",
@@ -202,6 +204,11 @@ def _declare(lib):
     lib.empm_nn_sqdist.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, C.c_size_t, vp]
     lib.empm_nn_sqdist_workspace.restype = C.c_size_t
     lib.empm_nn_sqdist_workspace.argtypes = [C.c_int, C.c_int]
+    lib.empm_nn_sqdist_segments.restype = C.c_int
+    lib.empm_nn_sqdist_segments.argtypes = [C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp,
+                                            C.c_size_t, vp]
+    lib.empm_nn_sqdist_segments_workspace.restype = C.c_size_t
+    lib.empm_nn_sqdist_segments_workspace.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
     lib.empm_knn.restype = C.c_int
     lib.empm_knn.argtypes = [C.c_int, vp, C.c_int, vp, C.c_int, C.c_int, vp, vp, vp]
     lib.empm_lbs_weights.restype = C.c_int

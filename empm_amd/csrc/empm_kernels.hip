@@ -1631,10 +1631,62 @@ __global__ __launch_bounds__(kBlock) void k_nn_reduce(int n, int splits, const f
   idx[i] = bj;
 }
 
+// The same search over segments: segment g pairs A[a_off[g], a_off[g+1])
+// with B[b_off[g], b_off[g+1]) (blockIdx.z = g; the frames of the training
+// loss in one launch).  Indices are global into B; a query whose B segment
+// is empty gets +inf and -1.  Blocks past their segment's end exit together.
+__global__ __launch_bounds__(kBlock) void k_nn_sqdist_seg(const int* __restrict__ a_off,
+                                                          const int* __restrict__ b_off,
+                                                          const float* __restrict__ a,
+                                                          const float* __restrict__ b, int splits, int n_a,
+                                                          float* __restrict__ dist, int* __restrict__ idx) {
+  __shared__ float4 tile[kNnTile];
+  const int g = blockIdx.z;
+  const int a0 = a_off[g], a1 = a_off[g + 1];
+  if (a0 + (int)(blockIdx.x * blockDim.x) >= a1) return;  // uniform over the block
+  const int b0 = b_off[g], b1 = b_off[g + 1];
+  const int chunk = (b1 - b0 + splits - 1) / splits;
+  const int j0 = b0 + (int)blockIdx.y * chunk;
+  const int j1 = min(b1, j0 + chunk);
+  const int i = a0 + blockIdx.x * blockDim.x + threadIdx.x;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  if (i < a1) {
+    ax = a[3 * (size_t)i];
+    ay = a[3 * (size_t)i + 1];
+    az = a[3 * (size_t)i + 2];
+  }
+  float best = __builtin_huge_valf();
+  int bj = -1;
+  for (int t0 = j0; t0 < j1; t0 += kNnTile) {
+    const int cnt = j1 - t0 < kNnTile ? j1 - t0 : kNnTile;
+    __syncthreads();
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+      const size_t q = 3 * (size_t)(t0 + k);
+      tile[k] = make_float4(b[q], b[q + 1], b[q + 2], 0.f);
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < cnt; ++k) {
+      const float4 q = tile[k];
+      const float dx = ax - q.x, dy = ay - q.y, dz = az - q.z;
+      const float d = dx * dx + dy * dy + dz * dz;
+      if (d < best) {
+        best = d;
+        bj = t0 + k;
+      }
+    }
+  }
+  if (i < a1) {
+    const size_t o = (size_t)blockIdx.y * n_a + i;
+    dist[o] = best;
+    idx[o] = bj;
+  }
+}
+
 // Splits of B for n points of A: enough blocks for 8 per CU, at least 512
 // points of B per split.
-int nn_splits(int n, int m) {
-  const int qb = (n + kBlock - 1) / kBlock;
+int nn_splits(int n, int m, int n_seg = 1) {
+  const int qb = n_seg * ((n + kBlock - 1) / kBlock);
   int s = (2048 + qb - 1) / qb;
   s = std::min(s, std::max(1, m / 512));
   if (s < 1) s = 1;
@@ -2459,6 +2511,35 @@ int empm_mass_from_density(int n, const float* density, const float* vol, float*
     return set_err(EMPM_E_INVALID_ARG, "empm_mass_from_density: bad arguments");
   if (n == 0) return EMPM_OK;
   hipLaunchKernelGGL(k_mass, dim3(nblocks(n)), dim3(kBlock), 0, (hipStream_t)stream, n, density, vol, mass);
+  HIP_TRY(hipGetLastError());
+  return EMPM_OK;
+}
+
+size_t empm_nn_sqdist_segments_workspace(int n_seg, int n_a, int max_a, int max_b) {
+  if (n_seg <= 0 || n_a <= 0 || max_a <= 0 || max_b <= 0) return 0;
+  const int s = nn_splits(max_a, max_b, n_seg);
+  return s > 1 ? (size_t)s * (size_t)n_a * 8 : 0;
+}
+
+int empm_nn_sqdist_segments(int n_seg, const int* a_off, const int* b_off, int n_a, int max_a, int max_b,
+                            const float* a, const float* b, float* dist, int* idx, void* workspace,
+                            size_t workspace_bytes, void* stream) {
+  if (n_seg < 0 || n_a < 0 || max_a < 0 || max_b < 0 || n_seg > 65535 ||
+      (n_seg > 0 && n_a > 0 && (!a_off || !b_off || !a || !b || !dist || !idx)))
+    return set_err(EMPM_E_INVALID_ARG, "empm_nn_sqdist_segments: bad arguments (n_seg=%d, n_a=%d)", n_seg, n_a);
+  if (n_seg == 0 || n_a == 0 || max_a == 0) return EMPM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  int s = max_b > 0 ? nn_splits(max_a, max_b, n_seg) : 1;
+  if (s > 1 && (!workspace || workspace_bytes < (size_t)s * (size_t)n_a * 8 || ((uintptr_t)workspace & 3u))) s = 1;
+  const dim3 grid(nblocks(max_a), s, n_seg);
+  if (s == 1) {
+    hipLaunchKernelGGL(k_nn_sqdist_seg, grid, dim3(kBlock), 0, st, a_off, b_off, a, b, 1, n_a, dist, idx);
+  } else {
+    float* pd = (float*)workspace;
+    int* pi = (int*)((char*)workspace + (size_t)s * n_a * 4);
+    hipLaunchKernelGGL(k_nn_sqdist_seg, grid, dim3(kBlock), 0, st, a_off, b_off, a, b, s, n_a, pd, pi);
+    hipLaunchKernelGGL(k_nn_reduce, dim3(nblocks(n_a)), dim3(kBlock), 0, st, n_a, s, pd, pi, dist, idx);
+  }
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }

@@ -49,6 +49,54 @@ def nn_sqdist(a: torch.Tensor, b: torch.Tensor):
     return dist, idx.long()
 
 
+def nn_sqdist_segments(a: torch.Tensor, b: torch.Tensor, a_off: torch.Tensor, b_off: torch.Tensor, max_a: int,
+                       max_b: int):
+    """nn_sqdist within segments in one launch: rows [a_off[g], a_off[g+1])
+    of a against rows [b_off[g], b_off[g+1]) of b; (dist, idx) with idx
+    global into b.  a_off, b_off: int32 device tensors of n_seg + 1 offsets;
+    max_a, max_b: the longest segments."""
+    if a.device.type != "cuda" or b.device.type != "cuda":
+        raise RuntimeError("nn_sqdist_segments runs on the GPU; got tensors on " + f"{a.device} and {b.device}")
+    a = a.detach().float().contiguous()
+    b = b.detach().float().contiguous()
+    a_off = a_off.to(device=a.device, dtype=torch.int32).contiguous()
+    b_off = b_off.to(device=a.device, dtype=torch.int32).contiguous()
+    n_seg, n = int(a_off.shape[0]) - 1, int(a.shape[0])
+    dist = torch.empty(n, dtype=torch.float32, device=a.device)
+    idx = torch.empty(n, dtype=torch.int32, device=a.device)
+    lib = _lib.load()
+    ws_bytes = int(lib.empm_nn_sqdist_segments_workspace(n_seg, n, max_a, max_b))
+    ws = torch.empty(max(ws_bytes // 4, 1), dtype=torch.int32, device=a.device)
+    _lib.check(lib.empm_nn_sqdist_segments(n_seg, a_off.data_ptr(), b_off.data_ptr(), n, max_a, max_b,
+                                           a.data_ptr(), b.data_ptr(), dist.data_ptr(), idx.data_ptr(),
+                                           ws.data_ptr(), ws_bytes, stream_handle(a.device)))
+    return dist, idx.long()
+
+
+class _SegmentChamfer(torch.autograd.Function):
+    """Sum over segments of the Chamfer distance of x_g against y_g (equal
+    counts c_g per segment): sum_i w_i (d(x_i, y) + d(y_i, x)) with
+    w_i = 1 / c_g of the row's segment."""
+
+    @staticmethod
+    def forward(ctx, x, y, off, w, max_len):
+        dx, ix = nn_sqdist_segments(x, y, off, off, max_len, max_len)
+        dy, iy = nn_sqdist_segments(y, x, off, off, max_len, max_len)
+        ctx.save_for_backward(x, y, ix, iy, w)
+        return (w * (dx + dy)).sum()
+
+    @staticmethod
+    def backward(ctx, g):
+        x, y, ix, iy, w = ctx.saved_tensors
+        rx = x.float() - y.float()[ix]   # x_i - y_nn(i)
+        ry = y.float() - x.float()[iy]   # y_j - x_nn(j)
+        gx = (2.0 * w)[:, None] * rx
+        gx = gx.index_add(0, iy, (-2.0 * w)[:, None] * ry)
+        gy = (2.0 * w)[:, None] * ry
+        gy = gy.index_add(0, ix, (-2.0 * w)[:, None] * rx)
+        return (g * gx).to(x.dtype), (g * gy).to(y.dtype), None, None, None
+
+
 class _Chamfer(torch.autograd.Function):
     """Chamfer distance of one pair of clouds x (N, 3), y (M, 3)."""
 
@@ -110,11 +158,32 @@ def frame_loss(points, target_points, visibilities, motions_valid, n_track: int 
     points and target_points (T, N, 3); visibilities and motions_valid (T, N)
     boolean masks; the first n_track points are the tracked ones.  Returns
     1e3 * (sum of per-frame Chamfer distances of the visible points + sum of
-    per-frame tracking terms of the points with a valid motion)."""
+    per-frame tracking terms of the points with a valid motion).  The frames
+    are evaluated together: one segmented nearest-neighbour launch per
+    direction, one host synchronisation for the per-frame counts."""
     if n_track is not None:
         points = points[:, :n_track, :]
-    T = points.shape[0]
-    chamfer = sum(chamfer_distance(points[t][visibilities[t]].unsqueeze(0),
-                                   target_points[t][visibilities[t]].unsqueeze(0))[0] for t in range(T))
-    tracking = sum(tracking_loss(points[t][motions_valid[t]], target_points[t][motions_valid[t]]) for t in range(T))
+    T, N = points.shape[0], points.shape[1]
+    if T == 0:
+        return points.sum() * 0.0
+    vis = visibilities.to(device=points.device, dtype=torch.bool)
+    valid = motions_valid.to(device=points.device, dtype=torch.bool)
+    if tuple(vis.shape) != (T, N) or tuple(valid.shape) != (T, N) or target_points.shape[0] != T:
+        raise ValueError(f"frame_loss: points {tuple(points.shape)}, targets {tuple(target_points.shape)}, masks "
+                         f"{tuple(vis.shape)} and {tuple(valid.shape)} do not match")
+    # all frames at once: the visible rows of every frame, frame after frame
+    counts = vis.sum(1)
+    cnt = counts.tolist()
+    tf, ti = torch.nonzero(vis, as_tuple=True)
+    x = points[tf, ti]
+    y = target_points[tf, ti]
+    off = torch.zeros(T + 1, dtype=torch.int32, device=points.device)
+    off[1:] = torch.cumsum(counts, 0)
+    w = torch.repeat_interleave(1.0 / counts.clamp(min=1).float(), counts, output_size=int(sum(cnt)))
+    chamfer = _SegmentChamfer.apply(x, y, off, w, max(cnt))
+    if min(cnt) == 0:
+        chamfer = chamfer + float("nan")   # an empty frame: the reference's mean over no points
+    # tracking: |pred - target| over the valid rows of each frame / their number
+    diff = (points - target_points[:, :N]) * valid[..., None]
+    tracking = (torch.linalg.vector_norm(diff, dim=(1, 2)) / valid.sum(1)).sum()
     return chamfer * 1e3 + tracking * 1e3

@@ -217,6 +217,20 @@ int empm_nn_sqdist(int n, const float *a, int m, const float *b, float *dist, in
 /* Workspace bytes empm_nn_sqdist uses for n points against m (0: none). */
 size_t empm_nn_sqdist_workspace(int n, int m);
 
+/* empm_nn_sqdist over n_seg segments in one launch (the frames of the
+ * training loss, trainer_warp_mpm.py:743-752): segment g pairs the rows
+ * [a_off[g], a_off[g+1]) of a with the rows [b_off[g], b_off[g+1]) of b.
+ * a_off, b_off: DEVICE arrays of n_seg + 1 ints; n_a = a_off[n_seg];
+ * max_a, max_b: the longest segment of a and of b (host values, they size
+ * the grid; a longer segment is searched only up to max_a rows).  idx is
+ * global into b; a row whose b segment is empty gets +inf and -1.
+ * workspace as for empm_nn_sqdist, of
+ * empm_nn_sqdist_segments_workspace(n_seg, n_a, max_a, max_b) bytes. */
+int empm_nn_sqdist_segments(int n_seg, const int *a_off, const int *b_off, int n_a, int max_a, int max_b,
+                            const float *a, const float *b, float *dist, int *idx, void *workspace,
+                            size_t workspace_bytes, void *stream);
+size_t empm_nn_sqdist_segments_workspace(int n_seg, int n_a, int max_a, int max_b);
+
 /* One substep state `in` -> state `out` (`out` must not alias `in`):
  * MPMWARPDiff.p2g2p_differentiable (mpm_solver_diff.py:165-291), i.e.
  * zero_grid, compute_stress_from_F_trial, p2g_apic_with_stress,

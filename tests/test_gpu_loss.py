@@ -58,6 +58,59 @@ def test_nearest_neighbour_ties_across_splits_of_the_search(dev):
     np.testing.assert_allclose(d.cpu().numpy(), rd, rtol=1e-5)
 
 
+def test_segmented_search_matches_each_segment(dev):
+    rng = np.random.default_rng(9)
+    na = [700, 0, 1500, 3, 2600]
+    nb = [900, 50, 1200, 0, 2600]
+    a = rng.random((sum(na), 3)).astype(np.float32)
+    b = rng.random((sum(nb), 3)).astype(np.float32)
+    ao = np.concatenate([[0], np.cumsum(na)]).astype(np.int32)
+    bo = np.concatenate([[0], np.cumsum(nb)]).astype(np.int32)
+    d, i = loss.nn_sqdist_segments(torch.as_tensor(a, device=dev), torch.as_tensor(b, device=dev),
+                                   torch.as_tensor(ao, device=dev), torch.as_tensor(bo, device=dev), max(na), max(nb))
+    d, i = d.cpu().numpy(), i.cpu().numpy()
+    for g in range(len(na)):
+        sa, sb = slice(ao[g], ao[g + 1]), slice(bo[g], bo[g + 1])
+        if na[g] == 0:
+            continue
+        if nb[g] == 0:
+            assert np.all(np.isinf(d[sa])) and np.all(i[sa] == -1)
+            continue
+        rd, ri = och.nn_sqdist(a[sa], b[sb])
+        np.testing.assert_array_equal(i[sa], ri + bo[g])
+        np.testing.assert_allclose(d[sa], rd, rtol=1e-5, atol=1e-9)
+
+
+def test_batched_frame_loss_matches_per_frame_calls(dev):
+    """frame_loss evaluates all frames together; the reference's per-frame
+    calls (chamfer_distance of each frame's visible points, tracking of its
+    valid points) give the same value and gradient, and so does the float64
+    restatement."""
+    rng = np.random.default_rng(10)
+    T, N = 6, 3000
+    p = rng.random((T, N, 3)).astype(np.float32)
+    q = (p + 0.01 * rng.standard_normal((T, N, 3))).astype(np.float32)
+    vis = rng.random((T, N)) < 0.7
+    valid = rng.random((T, N)) < 0.8
+    valid[2] = True
+    q[2][valid[2]] = p[2][valid[2]]           # a frame with zero tracking error: the gradient stays finite
+    pt = torch.as_tensor(p, device=dev).requires_grad_()
+    qt, vt, mt = torch.as_tensor(q, device=dev), torch.as_tensor(vis, device=dev), torch.as_tensor(valid, device=dev)
+    L = loss.frame_loss(pt, qt, vt, mt)
+    L.backward()
+    g_batched = pt.grad.clone()
+    pt.grad = None
+    L2 = sum(loss.chamfer_distance(pt[t][vt[t]].unsqueeze(0), qt[t][vt[t]].unsqueeze(0))[0] for t in range(T)) * 1e3
+    L2 = L2 + sum(torch.linalg.vector_norm(pt[t][mt[t]] - qt[t][mt[t]]) / int(mt[t].sum()) for t in range(T)) * 1e3
+    L2.backward()
+    ref = och.frame_loss(p, q, vis, valid)
+    print(f"[frame loss] batched {float(L):.8e}, per frame {float(L2):.8e}, oracle {ref:.8e}", flush=True)
+    assert abs(float(L) - float(L2)) < 1e-5 * abs(float(L2))
+    assert abs(float(L) - ref) < 1e-5 * abs(ref)
+    assert torch.isfinite(g_batched).all()
+    torch.testing.assert_close(g_batched, pt.grad, rtol=1e-4, atol=1e-6)
+
+
 def test_chamfer_value_and_gradient(dev):
     rng = np.random.default_rng(4)
     x, y = rng.random((2, 700, 3)).astype(np.float32), rng.random((2, 500, 3)).astype(np.float32)
