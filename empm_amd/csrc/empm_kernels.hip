@@ -1162,8 +1162,23 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
     dzd[c] = s.dw[2][c] * ((float)c - s.f[2]);
   }
   const float Ct2[3] = {Ct[2], Ct[5], Ct[8]};
+  // the nodes of plane a (9 of them, b-major) are loaded one plane ahead:
+  // out-of-grid nodes read node 0 of the grid and count as zero
+  const auto load_plane = [&](int a, float4* pl) {
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+      const int ix = s.base[0] + a, iy = s.base[1] + j / 3, iz = s.base[2] + j % 3;
+      const bool in = node_in(ix, iy, iz, M.n);
+      const float4 ga = gadj_in[M.g0 + (in ? node_id(ix, iy, iz, M.n) : 0)];
+      pl[j] = in ? ga : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  };
+  float4 cur[9];
+  load_plane(0, cur);
 #pragma unroll 1
-  for (int a = 0; a < 3; ++a)
+  for (int a = 0; a < 3; ++a) {
+    float4 nxt[9];
+    if (a < 2) load_plane(a + 1, nxt);
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
       // column sums over c: Z a, Z d_z a, dZ a, dZ d_z a, ddZ a, Z b, dZ b
@@ -1172,9 +1187,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
       float B1 = 0.f, B3 = 0.f;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
-        if (!node_in(ix, iy, iz, M.n)) continue;
-        const float4 ga = gadj_in[M.g0 + node_id(ix, iy, iz, M.n)];
+        const float4 ga = cur[b * 3 + c];
         const float av[3] = {ga.x, ga.y, ga.z};
         const float bm = ga.w;
         const float Z = s.w[2][c], dZ = s.dw[2][c];
@@ -1219,6 +1232,9 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
       fA[1] += XdY * e1;
       fA[2] += XY * e3;
     }
+#pragma unroll
+    for (int j = 0; j < 9; ++j) cur[j] = nxt[j];
+  }
   // the substep's return map and stress, recomputed (after the node loop,
   // which does not need them, so that their state is not live during it)
   float Ftr[9];
