@@ -20,7 +20,7 @@
 //   k_lbs_points   per Gaussian: gather of its K bone records, blended
 //                  position and quaternion, product with its own quaternion.
 // The per-frame path is k_lbs_bones (n_bones threads) + k_lbs_points (one
-// thread per Gaussian, HBM-bound on its index row and state).
+// thread per Gaussian, bound by gathering its 16 bone records).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -368,9 +368,10 @@ __device__ __forceinline__ void quat_rotate(float4 q, float vx, float vy, float 
 // quaternions given.  Every branch inside the neighbour loop is resolved at
 // compile time (the identity fallback is a select of q = (1, 0, 0, 0), whose
 // rotation is exact), so the gathers of all neighbours are in flight
-// together.  The per-Gaussian streams
-// (x, quat, indices, weights, outputs) are read and written non-temporally so
-// that the bone records stay in L2 for the gathers.
+// together.  The quaternion stream and the outputs, touched once per
+// frame, are read and written non-temporally so that the bone records stay
+// in L2 for the gathers; the 64-byte index and weight rows use plain loads
+// (their four 16-byte pieces share a cache line).
 template <int KC, typename I, bool W, bool Q>
 __global__ __launch_bounds__(kLbsBlock) void k_lbs_points(int n, const float* __restrict__ xyz,
                                                           const float* __restrict__ quat, int k_rt,
