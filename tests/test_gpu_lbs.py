@@ -156,6 +156,24 @@ def _block_rotations(bones, seed):
     return _f32(out)
 
 
+def test_default_k_and_no_quaternions_match_oracle(dev):
+    """The signatures' default K = 5 (the blend's generic path) and a blend
+    without quaternions."""
+    print("[lbs] K=5, no quaternions", flush=True)
+    bones, xyz, _ = _scene(1200, 8000, 25)
+    motions = _deform(bones, 26)
+    tb = _t(bones, dev)
+    rel = gs.get_topk_indices(tb, K=5)
+    w, idx = gs.knn_weights_sparse(tb, _t(xyz, dev), K=5)
+    out, rot, _ = gs.interpolate_motions_speedup(bones=tb, motions=_t(motions, dev), relations=rel,
+                                                 xyz=_t(xyz, dev), weights=w, weights_indices=idx)
+    torch.cuda.synchronize()
+    assert rot is None
+    rel_np, idx_np = rel.long().cpu().numpy(), idx.long().cpu().numpy()
+    out_o, _ = lbs.interpolate(bones, motions, rel_np, xyz, None, w.double().cpu().numpy(), idx_np)
+    np.testing.assert_allclose(out.cpu().numpy(), out_o, atol=POS_TOL)
+
+
 def test_bone_records_match_oracle(dev):
     print("[lbs] bone records", flush=True)
     bones, _, _ = _scene(3000, 1, 13)

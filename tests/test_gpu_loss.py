@@ -111,6 +111,24 @@ def test_batched_frame_loss_matches_per_frame_calls(dev):
     torch.testing.assert_close(g_batched, pt.grad, rtol=1e-4, atol=1e-6)
 
 
+def test_frame_loss_tracks_the_first_points(dev):
+    """n_track keeps the first points of the simulated frames (the trainer's
+    points[:, :n_track_points]); the targets and masks cover those points."""
+    rng = np.random.default_rng(11)
+    T, N, n_track = 4, 2500, 1800
+    p = rng.random((T, N, 3)).astype(np.float32)
+    q = (p[:, :n_track] + 0.01 * rng.standard_normal((T, n_track, 3))).astype(np.float32)
+    vis = rng.random((T, n_track)) < 0.75
+    valid = rng.random((T, n_track)) < 0.8
+    pt = torch.as_tensor(p, device=dev).requires_grad_()
+    L = loss.frame_loss(pt, torch.as_tensor(q, device=dev), torch.as_tensor(vis, device=dev),
+                        torch.as_tensor(valid, device=dev), n_track=n_track)
+    L.backward()
+    ref = och.frame_loss(p[:, :n_track], q, vis, valid)
+    assert abs(float(L) - ref) < 1e-5 * abs(ref)
+    assert torch.all(pt.grad[:, n_track:] == 0) and torch.any(pt.grad[:, :n_track] != 0)
+
+
 def test_chamfer_value_and_gradient(dev):
     rng = np.random.default_rng(4)
     x, y = rng.random((2, 700, 3)).astype(np.float32), rng.random((2, 500, 3)).astype(np.float32)
