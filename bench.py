@@ -99,9 +99,10 @@ def main():
     ap.add_argument("--substeps", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--only", choices=("all", "single", "population", "serving", "long_horizon", "loss", "lbs"),
+    ap.add_argument("--only", choices=("all", "single", "population", "serving", "long_horizon", "loss", "lbs", "c1"),
                     default="all", help="run only the single-rollout, the population, the serving, the long-horizon, "
-                                        "the frame-loss or the Gaussian interpolation (lbs) workload")
+                                        "the frame-loss, the Gaussian interpolation (lbs) or the 32^3 cube (c1) "
+                                        "workload")
     ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving workload")
     ap.add_argument("--serve-qps", type=float, default=10.0, help="Poisson arrival rate of the serving workload")
     ap.add_argument("--pop-steps", type=int, default=2, help="timed evaluations of the population workload")
@@ -146,6 +147,9 @@ def main():
         return
     if args.only == "lbs":
         print(json.dumps({"lbs": bench_lbs(args, dev)}), flush=True)
+        return
+    if args.only == "c1":
+        print(json.dumps({"c1": bench_c1(args, dev)}), flush=True)
         return
     sc = scenes.ellipsoid_c2(n_particles=args.particles, n_grid=args.grid, n_steps=args.substeps, param_seed=3)
     tprm = scenes.sample_params_c2(2)
@@ -266,7 +270,9 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, args.cpu_seconds)
-    population = serving = long_horizon = frame_loss = lbs = None
+    population = serving = long_horizon = frame_loss = lbs = c1 = None
+    copy_gbs = copy_bandwidth(dev)
+    log(f"[bench] device copy: {copy_gbs:.0f} GB/s read + write")
     if args.only == "all":
         del ro, tro, ctx
         torch.cuda.empty_cache()
@@ -279,6 +285,8 @@ def main():
         frame_loss = bench_loss(args, dev)
         torch.cuda.empty_cache()
         lbs = bench_lbs(args, dev)
+        torch.cuda.empty_cache()
+        c1 = bench_c1(args, dev)
 
     out = {
         "metric": "particle-substeps/s (fwd+bwd) @200k pts/64^3",
@@ -304,6 +312,7 @@ def main():
         "loss": float(loss),
         "per_kernel_ms_one_step": {k: v[1] for k, v in per_kernel.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "peak_measured_copy": copy_gbs,
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(dom) if (N, n) == (200_000, 64) else None,
                      "traffic_source": TRAFFIC_PROFILE, "algorithmic_bytes": kb,
@@ -314,6 +323,7 @@ def main():
         "long_horizon": long_horizon,
         "loss": frame_loss,
         "lbs": lbs,
+        "c1": c1,
     }
     print(json.dumps(out), flush=True)
 
@@ -456,6 +466,106 @@ def bench_long_horizon(args, dev):
            "loss": loss.tolist(), "grad_logE_nu_logY": grad.tolist()}
     log(f"[bench] long horizon: {el:.2f} s per gradient batch, {out['value']:.3e} p-substeps/s")
     return out
+
+
+def bench_c1(args, dev):
+    """The reference's CPU-runnable case at full size: a 32^3 cube of 8
+    particles per covered cell (about 7.1k), FCR, 3-cell sticky walls, dt
+    1e-4, 200 forward substeps; the GPU rollout (median of 5 after 2
+    warm-ups) and the FP32 CPU oracle on 1 and on all host threads (up to
+    16)."""
+    import torch
+
+    from empm_amd import _lib
+    from empm_amd import runtime as rt
+    from empm_amd import scenes
+    from empm_amd.diff_simulator.warp_mpm.mpm_solver_diff import KIND_SURFACE
+
+    sc = scenes.cube_c1()
+    N, n, T, dt = sc.n, sc.n_grid, sc.n_steps, sc.dt
+
+    def t(a):
+        return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=dev).clone()
+
+    Et, nut = t(sc.E), t(sc.nu)
+    mu, lam = torch.empty_like(Et), torch.empty_like(Et)
+    rt.mu_lam(Et, nut, mu, lam)
+    model = rt.ModelArrays(material=sc.material, n_grid=n, grid_lim=1.0, dx=float(np.float32(1.0 / n)),
+                           inv_dx=float(np.float32(n)), mu=mu, lam=lam, yield_stress=t(sc.yield_stress),
+                           mass=t(sc.vol * sc.rho), vol=t(sc.vol), gravity=sc.gravity)
+
+    def cols():
+        out = []
+        for c in sc.colliders:
+            g = _lib.Collider()
+            g.kind = KIND_SURFACE
+            g.start_time, g.end_time = 0.0, 999.0
+            for i in range(3):
+                g.point[i] = c["point"][i]
+                g.normal[i] = c["normal"][i]
+            out.append(g)
+        return out
+
+    s0 = rt.ParticleState(t(sc.x), torch.zeros((N, 3), device=dev), torch.zeros((N, 3, 3), device=dev),
+                          torch.eye(3, device=dev).repeat(N, 1, 1).contiguous())
+    ro = rt.Rollout(rt.Context(n, N, dev), model, N)
+    for _ in range(2):
+        ro.forward(s0, cols(), 0.0, dt, T)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ro.forward(s0, cols(), 0.0, dt, T)
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    med = sorted(ms)[len(ms) // 2]
+    out = {"workload": f"cube 32^3, {N} particles (8 per covered cell), FCR, 3-cell sticky walls, dt 1e-4, "
+                       f"{T} forward substeps", "value": N * T / (med * 1e-3), "unit": "particle-substeps/s (forward)",
+           "ms_per_rollout_median": med}
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+
+        cpu = {}
+        mu64, lam64 = orc.mu_lam(sc.E.astype(np.float64), sc.nu.astype(np.float64), "f32")
+        P = orc.make_params(N, n, material=sc.material, gravity=sc.gravity)
+        ocols = [orc.surface_collider(c["point"], c["normal"]) for c in sc.colliders]
+        for threads in sorted({1, min(os.cpu_count() or 1, 16)}):
+            orc.set_threads(threads)
+            o = orc.OracleState(sc.x, np.zeros((N, 3)), np.zeros((N, 3, 3)), np.tile(np.eye(3), (N, 1, 1)), mu64,
+                                lam64, sc.yield_stress, sc.vol * sc.rho, sc.vol, precision="f32")
+            t0 = time.perf_counter()
+            orc.rollout(P, ocols, 0.0, dt, T, o)
+            cpu[threads] = N * T / (time.perf_counter() - t0)
+        out["cpu_baseline"] = {"value": cpu[max(cpu)], "unit": "particle-substeps/s (forward)", "cores": max(cpu),
+                               "kind": "port", "value_1_thread": cpu[1],
+                               "sample": f"the whole workload ({T} substeps) on the FP32 CPU oracle"}
+    log(f"[bench] C1 cube: {med:.2f} ms per {T}-substep rollout on the GPU, {out['value']:.3e} p-substeps/s"
+        + (f"; CPU oracle {out['cpu_baseline']['value']:.3e} on {out['cpu_baseline']['cores']} threads, "
+           f"{out['cpu_baseline']['value_1_thread']:.3e} on 1" if "cpu_baseline" in out else ""))
+    return out
+
+
+def copy_bandwidth(dev, mib=1024, reps=10):
+    """Achieved HBM bandwidth of a plain device copy (read + write bytes per
+    second), the measured counterpart of the 8 TB/s published peak."""
+    import torch
+
+    src = torch.ones(mib * (1 << 18), device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * mib * (1 << 20) * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return gbs
 
 
 def bench_loss(args, dev):
