@@ -168,17 +168,6 @@ EMPM_DEV bool node_in(int ix, int iy, int iz, int n) {
 }
 EMPM_DEV size_t node_id(int ix, int iy, int iz, int n) { return ((size_t)ix * n + iy) * n + iz; }
 
-// d(w, grad w)/df chain for node (a,b,c): fb += dL/df given wb = dL/dw and
-// gwb = dL/d(grad w).  grad w = inv_dx (N'a Nb Nc, Na N'b Nc, Na Nb N'c).
-EMPM_DEV void weight_adj(const Stencil& s, int a, int b, int c, float inv_dx, float wb, const float* gwb, float* fb) {
-  const float X = sw(s, 0, a), Y = sw(s, 1, b), Z = sw(s, 2, c);
-  const float dX = sdw(s, 0, a), dY = sdw(s, 1, b), dZ = sdw(s, 2, c);
-  const float ddX = a == 1 ? -2.f : 1.f, ddY = b == 1 ? -2.f : 1.f, ddZ = c == 1 ? -2.f : 1.f;
-  fb[0] += wb * dX * Y * Z + inv_dx * (gwb[0] * ddX * Y * Z + gwb[1] * dX * dY * Z + gwb[2] * dX * Y * dZ);
-  fb[1] += wb * X * dY * Z + inv_dx * (gwb[0] * dX * dY * Z + gwb[1] * X * ddY * Z + gwb[2] * X * dY * dZ);
-  fb[2] += wb * X * Y * dZ + inv_dx * (gwb[0] * dX * Y * dZ + gwb[1] * X * dY * dZ + gwb[2] * X * Y * ddZ);
-}
-
 EMPM_DEV void load9(const float* __restrict__ a, size_t p, float* o) {
 #pragma unroll
   for (int i = 0; i < 9; ++i) o[i] = a[9 * p + i];
@@ -987,42 +976,78 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(Batch Bt, float dt, const fl
   // adjoint) and every d(w, grad w, d)/dx term.  The term of dL/dv' through
   // the weights, sum_j (dL/dv' . v_j) dw_j/df, is (dL/dv')^T grad v / inv_dx,
   // so it is added after the pass, once dL/dv' (which needs v') is known.
+  // Per node j with v_j, w_j = X Y Z and d_j = node - f the pass needs
+  //   dL/dw_j  = 4 inv_dx v_j . (dC d_j)            (through C')
+  //   dL/dgw_j = Gb^T v_j                           (through grad v)
+  // times the first and second derivatives of w_j, and 4 inv_dx w_j dC^T v_j
+  // (through d_j).  All are linear in v_j with products of per-axis weights,
+  // so each stencil column (nodes (a, b, 0..2)) is summed over c first (P, Q,
+  // R, R2, A5 below) and folded in with its x and y weights; the d_j term is
+  // 4 inv_dx dC^T v' after the pass.
   float nv[3] = {0.f, 0.f, 0.f}, nG[9], fb[3] = {0.f, 0.f, 0.f};
   m_zero(nG);
+  float zd[3], dzd[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    zd[c] = s.w[2][c] * ((float)c - s.f[2]);
+    dzd[c] = s.dw[2][c] * ((float)c - s.f[2]);
+  }
+  const float q[3] = {dC[2], dC[5], dC[8]};  // dC e_z
 #pragma unroll 1
   for (int a = 0; a < 3; ++a) {
-    const float wa = sw(s, 0, a), dwa = sdw(s, 0, a);
+    const float X = sw(s, 0, a), dX = sdw(s, 0, a), ddX = a == 1 ? -2.f : 1.f;
+    const float da = (float)a - s.f[0];
 #pragma unroll
-  for (int j = 0; j < 9; ++j) {
-    const int b = j / 3, c = j % 3;
-    const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
-    const bool in = node_in(ix, iy, iz, M.n);
-    const float4 g4 = v_out[M.g0 + (in ? node_id(ix, iy, iz, M.n) : 0)];
-    const float vi[3] = {in ? g4.x : 0.f, in ? g4.y : 0.f, in ? g4.z : 0.f};
-    const float wt = wa * s.w[1][b] * s.w[2][c];
-    const float gw[3] = {dwa * s.w[1][b] * s.w[2][c] * inv_dx, wa * s.dw[1][b] * s.w[2][c] * inv_dx,
-                         wa * s.w[1][b] * s.dw[2][c] * inv_dx};
+    for (int b = 0; b < 3; ++b) {
+      // column sums over c: Z v, Z d_z v, dZ v, dZ d_z v, ddZ v
+      float P[3] = {0.f, 0.f, 0.f}, Q[3] = {0.f, 0.f, 0.f}, R[3] = {0.f, 0.f, 0.f}, R2[3] = {0.f, 0.f, 0.f},
+            A5[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      nv[k] += vi[k] * wt;
+      for (int c = 0; c < 3; ++c) {
+        const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
+        const bool in = node_in(ix, iy, iz, M.n);
+        const float4 g4 = v_out[M.g0 + (in ? node_id(ix, iy, iz, M.n) : 0)];
+        const float vi[3] = {in ? g4.x : 0.f, in ? g4.y : 0.f, in ? g4.z : 0.f};
+        const float Z = s.w[2][c], dZ = s.dw[2][c];
 #pragma unroll
-      for (int l = 0; l < 3; ++l) nG[k * 3 + l] += vi[k] * gw[l];
+        for (int m = 0; m < 3; ++m) {
+          P[m] += Z * vi[m];
+          Q[m] += zd[c] * vi[m];
+          R[m] += dZ * vi[m];
+          R2[m] += dzd[c] * vi[m];
+          A5[m] += c == 1 ? -2.f * vi[m] : vi[m];
+        }
+      }
+      const float Y = s.w[1][b], dY = s.dw[1][b], ddY = b == 1 ? -2.f : 1.f;
+      const float db = (float)b - s.f[1];
+      const float XY = X * Y, dXY = dX * Y, XdY = X * dY;
+      float WZ = 0.f, WdZ = 0.f;
+      float gP0 = 0.f, gP1 = 0.f, gR0 = 0.f, gR1 = 0.f, gR2 = 0.f, gA2 = 0.f;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        nv[m] += XY * P[m];
+        nG[m * 3 + 0] += dXY * P[m];
+        nG[m * 3 + 1] += XdY * P[m];
+        nG[m * 3 + 2] += XY * R[m];
+        const float pab = dC[m * 3 + 0] * da + dC[m * 3 + 1] * db;
+        WZ += pab * P[m] + q[m] * Q[m];
+        WdZ += pab * R[m] + q[m] * R2[m];
+        gP0 += Gb[m * 3 + 0] * P[m];
+        gP1 += Gb[m * 3 + 1] * P[m];
+        gR0 += Gb[m * 3 + 0] * R[m];
+        gR1 += Gb[m * 3 + 1] * R[m];
+        gR2 += Gb[m * 3 + 2] * R[m];
+        gA2 += Gb[m * 3 + 2] * A5[m];
+      }
+      fb[0] += c4 * dXY * WZ + inv_dx * (gP0 * ddX * Y + gP1 * dX * dY + gR2 * dXY);
+      fb[1] += c4 * XdY * WZ + inv_dx * (gP0 * dX * dY + gP1 * X * ddY + gR2 * XdY);
+      fb[2] += c4 * XY * WdZ + inv_dx * (gR0 * dXY + gR1 * XdY + gA2 * XY);
     }
-    const float d[3] = {(float)a - s.f[0], (float)b - s.f[1], (float)c - s.f[2]};
-    float Cd[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) Cd[k] = dC[k * 3 + 0] * d[0] + dC[k * 3 + 1] * d[1] + dC[k * 3 + 2] * d[2];
-    const float wb = c4 * (vi[0] * Cd[0] + vi[1] * Cd[1] + vi[2] * Cd[2]);
-    float gwb[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      // d = off - f: dd_k = 4 inv_dx w sum_i dC_ik v_i
-      fb[k] -= c4 * wt * (dC[0 * 3 + k] * vi[0] + dC[1 * 3 + k] * vi[1] + dC[2 * 3 + k] * vi[2]);
-      gwb[k] = Gb[0 * 3 + k] * vi[0] + Gb[1 * 3 + k] * vi[1] + Gb[2 * 3 + k] * vi[2];
-    }
-    weight_adj(s, a, b, c, inv_dx, wb, gwb, fb);
   }
-  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) nG[k] *= inv_dx;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) fb[k] -= c4 * (dC[0 * 3 + k] * nv[0] + dC[1 * 3 + k] * nv[1] + dC[2 * 3 + k] * nv[2]);
   const float amin = M.dx * 2.f, amax = M.grid_lim - M.dx * 2.f;
   float xb[3], nvb[3];
 #pragma unroll
