@@ -272,6 +272,64 @@ def test_reference_frame_loop_matches_oracle(dev):
     np.testing.assert_allclose(gs_rot.cpu().numpy(), q_o, atol=QUAT_TOL * np.abs(quat0).max())
 
 
+def test_interactive_loop_on_simulated_frames(dev):
+    """Both halves of the interactive loop (scripts/interactive_viser.py:
+    240-291) together: the simulator mirror advances a spinning jelly
+    ellipsoid frame by frame, and its particles drive the Gaussians through
+    the reference's LBS call sequence; the oracle replays the same float32
+    frames on the same neighbour lists."""
+    print("[lbs] loop on simulated frames", flush=True)
+    from empm_amd import scenes
+    from empm_amd.diff_simulator import WarpMPMWrapper
+    from empm_amd.diff_simulator.warp_mpm.warp_mpm_wrapper import material_gallery
+    sc = scenes.small_scene(n_particles=800, n_grid=16, material=0, spin=6.0)
+    mpm = WarpMPMWrapper(grid_size=sc.n_grid, grid_lim=1.0, frame_dt=sc.dt * 40, sim_dt=sc.dt, device=dev)
+    mpm.material_params = dict(material_gallery["jelly"])
+    mpm.add_particles(sc.x, sc.params["v0"], sc.vol.astype(np.float32), sc.E.astype(np.float32),
+                      sc.nu.astype(np.float32), sc.rho.astype(np.float32))
+    for c in sc.colliders:
+        mpm.surface_colliders.append([c["point"], c["normal"], "sticky", 0.0])
+    mpm.initialize()
+    rng = np.random.default_rng(29)
+    xyz0 = _f32(sc.x[rng.integers(0, sc.n, 4000)] + 0.01 * rng.standard_normal((4000, 3)))
+    quat0 = _f32(rng.standard_normal((4000, 4)))
+    gs_pos, gs_rot = _t(xyz0, dev), _t(quat0, dev)
+    relations = weights = weights_indices = prev_x = None
+    frames = []
+    for f in range(5):
+        world_pts = mpm.advance_to(n_frames=1)[-1]
+        frames.append(world_pts.double().cpu().numpy())
+        if prev_x is not None:
+            if relations is None:
+                relations = gs.get_topk_indices(prev_x, K=16)
+            if weights is None:
+                weights, weights_indices = gs.knn_weights_sparse(prev_x, gs_pos, K=16)
+            weights = gs.calc_weights_vals_from_indices(prev_x, gs_pos, weights_indices)
+            gs_pos, gs_rot, _ = gs.interpolate_motions_speedup(bones=prev_x, motions=world_pts - prev_x,
+                                                               relations=relations, weights=weights,
+                                                               weights_indices=weights_indices, xyz=gs_pos,
+                                                               quat=gs_rot)
+        prev_x = world_pts.clone()
+    torch.cuda.synchronize()
+    # the particles moved: ~1 mm a frame, rotating about z
+    step = np.linalg.norm(frames[-1] - frames[0], axis=1)
+    assert step.mean() > 2e-3
+    rel_o, _ = lbs.knn(frames[0], frames[0], 16, skip_first=1)
+    idx_o, _ = lbs.knn(xyz0, frames[0], 16)
+    assert np.mean(relations.cpu().numpy() == rel_o) > 0.999
+    assert np.mean(weights_indices.cpu().numpy() == idx_o) > 0.999
+    rel_o = relations.long().cpu().numpy()
+    idx_o = weights_indices.long().cpu().numpy()
+    x_o, q_o = xyz0, quat0
+    for f in range(1, 5):
+        w_o = lbs.inv_dist_weights(x_o, frames[f - 1], idx_o)
+        x_o, q_o = lbs.interpolate(frames[f - 1], frames[f] - frames[f - 1], rel_o, x_o, q_o, w_o, idx_o)
+    np.testing.assert_allclose(gs_pos.cpu().numpy(), x_o, atol=POS_TOL)
+    np.testing.assert_allclose(gs_rot.cpu().numpy(), q_o, atol=QUAT_TOL * np.abs(quat0).max())
+    # the Gaussians followed the body: their mean moved as the particles' did
+    np.testing.assert_allclose(x_o.mean(0) - xyz0.mean(0), frames[-1].mean(0) - frames[0].mean(0), atol=2e-3)
+
+
 def test_rigid_motion_full_size(dev):
     """50k bones and 1M Gaussians, the interpolation's benchmark size: a
     rigid motion moves every Gaussian rigidly whatever its weights."""
