@@ -80,6 +80,13 @@ struct Batch {
   int N;       // particles in the arena
   int nblk;    // blocks in the arena
   int nwaves;  // waves of a per-particle launch: sum over simulations of ceil(N / 64)
+  // Per-material launches cover only that material's waves: entries
+  // moff[m] .. moff[m + 1] - 1 of mlist are (simulation, its first wave in
+  // material m's launch) for the non-empty simulations of material m, in
+  // table order; mwaves[m] is that launch's wave count.
+  const int2* mlist;
+  int moff[9];
+  int mwaves[8];
 };
 
 // The simulation owning global particle p / global block bg (binary search
@@ -106,21 +113,37 @@ EMPM_DEV const DevModel& cand_of_block(const Batch& Bt, int bg) {
 // Per-particle launches give each simulation its own whole waves (wave w of
 // the launch belongs to the simulation with w0 <= w < w0 + ceil(N / 64)), so
 // the simulation of a wave is uniform: its table entry is read with scalar
-// loads.  body(M, p) runs for the thread's particle p when M.material is
-// `mat` (or mat < 0 for any material).
+// loads.  With mat >= 0 the launch covers the waves of material `mat` only
+// (Batch::mlist; launch with material_grid), else every simulation's waves
+// (particle_grid).  body(M, p) runs for the thread's particle p.
 template <typename F>
 EMPM_DEV void per_particle(const Batch& Bt, int mat, F body) {
   const int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) / kWaveSize));
-  if (w >= Bt.nwaves) return;
-  int lo = 0;
-  for (int hi = Bt.ncand - 1; lo < hi;) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (Bt.cand[mid].w0 <= w) lo = mid;
-    else hi = mid - 1;
+  int ci, wl;
+  if (mat >= 0) {
+    if (w >= Bt.mwaves[mat]) return;
+    int lo = Bt.moff[mat];
+    for (int hi = Bt.moff[mat + 1] - 1; lo < hi;) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (Bt.mlist[mid].y <= w) lo = mid;
+      else hi = mid - 1;
+    }
+    const int2 e = Bt.mlist[lo];
+    ci = e.x;
+    wl = w - e.y;
+  } else {
+    if (w >= Bt.nwaves) return;
+    int lo = 0;
+    for (int hi = Bt.ncand - 1; lo < hi;) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (Bt.cand[mid].w0 <= w) lo = mid;
+      else hi = mid - 1;
+    }
+    ci = lo;
+    wl = w - Bt.cand[lo].w0;
   }
-  const DevModel& M = Bt.cand[__builtin_amdgcn_readfirstlane(lo)];
-  if (mat >= 0 && M.material != mat) return;
-  const int k = (w - M.w0) * kWaveSize + (int)(threadIdx.x % kWaveSize);
+  const DevModel& M = Bt.cand[__builtin_amdgcn_readfirstlane(ci)];
+  const int k = __builtin_amdgcn_readfirstlane(wl) * kWaveSize + (int)(threadIdx.x % kWaveSize);
   if (k >= M.N) return;
   body(M, M.p0 + k);
 }
@@ -723,6 +746,59 @@ EMPM_DEV float4 gather_node(const Binning& B, const DevModel& M, const float4* _
   return acc;
 }
 
+// Bin of a particle of simulation M at position xp: (M.b0 + block) * 8 +
+// cell, the cell being the one its stencil starts at (clamped to the grid).
+EMPM_DEV int bin_key(const float* xp, const DevModel& M) {
+  int c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int v = (int)(xp[d] * M.inv_dx - 0.5f);
+    c[d] = v < 0 ? 0 : (v >= M.n ? M.n - 1 : v);
+  }
+  const int nb = M.nb;
+  const int blk = M.b0 + ((c[0] >> 1) * nb + (c[1] >> 1)) * nb + (c[2] >> 1);
+  return blk * 8 + (((c[0] & 1) << 2) | ((c[1] & 1) << 1) | (c[2] & 1));
+}
+
+// Counting step of the binning for the calling lanes: count[key] grows by
+// one per active lane and key_rank[p] = (key, the lane's rank in its bin).
+// Lanes are grouped by key first (ballots only), then every group leader
+// issues its atomic in the same instruction: one atomic round trip per wave.
+// Issued in two halves so that a caller can overlap the atomic's round trip
+// with other work: bin_count_issue returns the group leaders' atomic results,
+// bin_count_finish spreads them and stores key_rank.
+struct BinCount {
+  int leader, rank, base;
+};
+EMPM_DEV BinCount bin_count_issue(bool active, int key, int* __restrict__ count) {
+  const int lane = __lane_id();
+  unsigned long long pending = __ballot(active);
+  BinCount bc{lane, 0, 0};
+  int group_size = 0;
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int lkey = __shfl(key, leader);
+    const bool in = ((pending >> lane) & 1ull) && key == lkey;
+    const unsigned long long grp = __ballot(in);
+    pending &= ~grp;
+    if (in) {
+      bc.leader = leader;
+      bc.rank = __popcll(grp & ((1ull << lane) - 1ull));
+    }
+    if (lane == leader) group_size = __popcll(grp);
+  }
+  if (group_size > 0) bc.base = atomicAdd(&count[key], group_size);
+  return bc;
+}
+EMPM_DEV void bin_count_finish(bool active, int key, int p, const BinCount& bc, int2* __restrict__ key_rank) {
+  const int base = __shfl(bc.base, bc.leader);
+  if (active) key_rank[p] = make_int2(key, base + bc.rank);
+}
+EMPM_DEV void count_bin(bool active, int key, int p, int* __restrict__ count, int2* __restrict__ key_rank) {
+  const BinCount bc = bin_count_issue(active, key, count);
+  bin_count_finish(active, key, p, bc, key_rank);
+}
+
 // ---------------------------------------------------------------------------
 // Forward kernels
 // ---------------------------------------------------------------------------
@@ -855,7 +931,9 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
                                                 float* __restrict__ x1, float* __restrict__ v1,
                                                 float* __restrict__ C1, float* __restrict__ F1, int next_stress,
                                                 float* Fe_next, float* __restrict__ S_next, int write_params,
-                                                int write_F) {
+                                                int write_F, int* __restrict__ count, int2* __restrict__ key_rank) {
+  // with count set, the counting step of the next substep's binning is done
+  // here on the new positions (k_bin_count's work)
   per_particle(Bt, MAT, [&](const DevModel& M, const int p) {
   if (M.sel && M.sel[p] != 0) {  // partial_clone defaults (mpm_data_structure.py:356-387)
     float I[9], Z[9];
@@ -865,6 +943,10 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
     v1[3 * p] = v1[3 * p + 1] = v1[3 * p + 2] = 0.f;
     store9(C1, p, Z);
     if (write_F) store9(F1, p, I);
+    if (count) {
+      const float z[3] = {0.f, 0.f, 0.f};
+      count_bin(true, bin_key(z, M), p, count, key_rank);
+    }
     return;
   }
   const float xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
@@ -919,11 +1001,19 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
   }
   const float amin = M.dx * 2.f;
   const float amax = M.grid_lim - M.dx * 2.f;
+  float xo[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     v1[3 * p + i] = nv[i];
     const float xn = xp[i] + dt * nv[i];
-    x1[3 * p + i] = fminf(fmaxf(xn, amin), amax);
+    xo[i] = fminf(fmaxf(xn, amin), amax);
+    x1[3 * p + i] = xo[i];
+  }
+  int key = 0;
+  BinCount bc{};
+  if (count) {
+    key = bin_key(xo, M);
+    bc = bin_count_issue(true, key, count);
   }
   store9(C1, p, nC);
   float IF[9], Fp[9], Fn[9];
@@ -933,6 +1023,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
   m_mul(IF, Fp, Fn);
   if (write_F) store9(F1, p, Fn);
   if (next_stress) particle_stress<MAT>(M, dt, p, Fn, Fe_next, S_next, write_params);
+  if (count) bin_count_finish(true, key, p, bc, key_rank);
   });
 }
 
@@ -1373,19 +1464,6 @@ struct BinBuf {
   int* nlist; // 1 + nb^3: count, then the ids of the near blocks
 };
 
-// Global bin of particle p of simulation M: (M.b0 + block) * 8 + cell.
-EMPM_DEV int bin_of(const float* __restrict__ x, int p, const DevModel& M) {
-  int c[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    int v = (int)(x[3 * p + d] * M.inv_dx - 0.5f);
-    c[d] = v < 0 ? 0 : (v >= M.n ? M.n - 1 : v);
-  }
-  const int nb = M.nb;
-  const int blk = M.b0 + ((c[0] >> 1) * nb + (c[1] >> 1)) * nb + (c[2] >> 1);
-  return blk * 8 + (((c[0] & 1) << 2) | ((c[1] & 1) << 1) | (c[2] & 1));
-}
-
 __global__ void k_fill_i32(int* __restrict__ a, size_t n, int v) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
@@ -1394,29 +1472,14 @@ __global__ void k_fill_i32(int* __restrict__ a, size_t n, int v) {
 __global__ __launch_bounds__(kBlock) void k_bin_count(Batch Bt, const float* __restrict__ x, int* __restrict__ count,
                                                       int2* __restrict__ key_rank) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = __lane_id();
   const bool active = p < Bt.N;
-  const int key = active ? bin_of(x, p, cand_of_particle(Bt, p)) : -1;
-  // group the lanes by key first (ballots only), then every group leader
-  // issues its atomic in the same instruction: one atomic round trip per wave
-  unsigned long long pending = __ballot(active);
-  int my_leader = lane, my_rank = 0, group_size = 0;
-  while (pending) {
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const int lkey = __shfl(key, leader);
-    const bool in = ((pending >> lane) & 1ull) && key == lkey;
-    const unsigned long long grp = __ballot(in);
-    pending &= ~grp;
-    if (in) {
-      my_leader = leader;
-      my_rank = __popcll(grp & ((1ull << lane) - 1ull));
-    }
-    if (lane == leader) group_size = __popcll(grp);
+  float xp[3] = {0.f, 0.f, 0.f};
+  if (active) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) xp[d] = x[3 * p + d];
   }
-  int base = 0;
-  if (group_size > 0) base = atomicAdd(&count[key], group_size);
-  base = __shfl(base, my_leader);
-  if (active) key_rank[p] = make_int2(key, base + my_rank);
+  const int key = active ? bin_key(xp, cand_of_particle(Bt, p)) : -1;
+  count_bin(active, key, p, count, key_rank);
 }
 
 // Exclusive scan of one value per thread over the workgroup (kBinScanWG
@@ -1771,6 +1834,7 @@ struct empm_ctx {
   float* Febar = nullptr;    // N*9 scratch
   int* bins = nullptr;       // the context's own binning record
   int* count = nullptr;      // max_blocks * 8: per-bin counts, zero between binnings
+  bool counts_pending = false;  // a G2P counted the next substep's bins, not yet binned
   int2* key_rank = nullptr;  // N: bin key and rank of each particle
   int* blkoff = nullptr;     // max_blocks: workgroup-local exclusive block offsets
   int* wsum = nullptr;       // block-scan workgroup totals
@@ -1834,13 +1898,18 @@ struct HostBatch {
   long long G = 0;
   int nwaves = 0;
   bool has_material[8] = {};
+  std::vector<int2> mlist;  // Batch::mlist, material-major
+  int moff[9] = {};
+  int mwaves[8] = {};
   Batch dev{};
 };
 
-unsigned particle_grid(const HostBatch& hb) {
-  const size_t t = (size_t)hb.nwaves * kWaveSize;
+unsigned waves_grid(int nwaves) {
+  const size_t t = (size_t)nwaves * kWaveSize;
   return (unsigned)((t + kBlock - 1) / kBlock > 0 ? (t + kBlock - 1) / kBlock : 1);
 }
+unsigned particle_grid(const HostBatch& hb) { return waves_grid(hb.nwaves); }
+unsigned material_grid(const HostBatch& hb, int mat) { return waves_grid(hb.mwaves[mat]); }
 
 int check_particles(const empm_particles* s, int N, const char* what) {
   if (!s) return set_err(EMPM_E_INVALID_ARG, "null %s", what);
@@ -1860,6 +1929,8 @@ int check_scalars(const empm_model* m, int c) {
 }
 
 // Validate a batch against the context and build its host description.
+void index_waves(HostBatch& hb);
+
 int build_batch(const empm_ctx* ctx, const empm_batch* b, HostBatch& hb) {
   if (!ctx || !b) return set_err(EMPM_E_INVALID_ARG, "null context or batch");
   if (b->n_cand < 1 || b->n_cand > ctx->max_cand)
@@ -1914,6 +1985,13 @@ int build_batch(const empm_ctx* ctx, const empm_batch* b, HostBatch& hb) {
   hb.N = (int)N;
   hb.nblk = (int)nblk;
   hb.G = G;
+  index_waves(hb);
+  return EMPM_OK;
+}
+
+// Wave ranges of per-particle launches: each simulation's first wave (w0),
+// the total, and the per-material lists.
+void index_waves(HostBatch& hb) {
   int w = 0;
   for (int m = 0; m < 8; ++m) hb.has_material[m] = false;
   for (DevModel& d : hb.cand) {
@@ -1922,7 +2000,19 @@ int build_batch(const empm_ctx* ctx, const empm_batch* b, HostBatch& hb) {
     if (d.N > 0) hb.has_material[d.material] = true;
   }
   hb.nwaves = w;
-  return EMPM_OK;
+  hb.mlist.clear();
+  for (int m = 0; m < 8; ++m) {
+    hb.moff[m] = (int)hb.mlist.size();
+    int mw = 0;
+    for (size_t c = 0; c < hb.cand.size(); ++c) {
+      const DevModel& d = hb.cand[c];
+      if (d.material != m || d.N == 0) continue;
+      hb.mlist.push_back(make_int2((int)c, mw));
+      mw += (d.N + kWaveSize - 1) / kWaveSize;
+    }
+    hb.mwaves[m] = mw;
+  }
+  hb.moff[8] = (int)hb.mlist.size();
 }
 
 // Make the device table hold hb's simulations (stream-ordered; uploads only
@@ -1931,14 +2021,23 @@ int upload_batch(empm_ctx* ctx, HostBatch& hb, hipStream_t st) {
   const size_t nc = hb.cand.size();
   const bool same = ctx->cand_last.size() == nc && memcmp(ctx->cand_last.data(), hb.cand.data(),
                                                              nc * sizeof(DevModel)) == 0;
+  // the material lists follow the table in the same buffers (they are a
+  // function of it, so an unchanged table means unchanged lists)
+  int2* mlist_dev = reinterpret_cast<int2*>(ctx->cand_dev + ctx->max_cand);
   if (!same) {
     HIP_TRY(hipEventSynchronize(ctx->cand_upload));  // the staging copy is free again
     memcpy(ctx->cand_pinned, hb.cand.data(), nc * sizeof(DevModel));
+    int2* mlist_pinned = reinterpret_cast<int2*>(ctx->cand_pinned + ctx->max_cand);
+    memcpy(mlist_pinned, hb.mlist.data(), hb.mlist.size() * sizeof(int2));
     HIP_TRY(hipMemcpyAsync(ctx->cand_dev, ctx->cand_pinned, nc * sizeof(DevModel), hipMemcpyHostToDevice, st));
+    if (!hb.mlist.empty())
+      HIP_TRY(hipMemcpyAsync(mlist_dev, mlist_pinned, hb.mlist.size() * sizeof(int2), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(ctx->cand_upload, st));
     ctx->cand_last = hb.cand;
   }
-  hb.dev = Batch{ctx->cand_dev, (int)nc, hb.N, hb.nblk, hb.nwaves};
+  hb.dev = Batch{ctx->cand_dev, (int)nc, hb.N, hb.nblk, hb.nwaves, mlist_dev, {}, {}};
+  for (int m = 0; m <= 8; ++m) hb.dev.moff[m] = hb.moff[m];
+  for (int m = 0; m < 8; ++m) hb.dev.mwaves[m] = hb.mwaves[m];
   return EMPM_OK;
 }
 
@@ -1996,13 +2095,13 @@ int make_table(const empm_collider* cols, int ncol, ColTable& T) {
     default: hipLaunchKernelGGL(KERNEL<7>, grid, block, 0, stream, __VA_ARGS__); break;        \
   }
 
-// KERNEL<mat>(Bt, args...) over all particle waves, once per material
-// present (waves of other materials return at once)
+// KERNEL<mat>(Bt, args...) over the particle waves of material mat, once
+// per material present
 #define EMPM_LAUNCH_RANGES(hb, KERNEL, stream, ...)                                              \
   for (int mat_ = 0; mat_ < 8; ++mat_)                                                          \
     if ((hb).has_material[mat_]) {                                                              \
-      EMPM_LAUNCH_MAT(mat_, KERNEL, dim3(particle_grid(hb)), dim3(kBlock), stream, (hb).dev,    \
-                      __VA_ARGS__);                                                             \
+      EMPM_LAUNCH_MAT(mat_, KERNEL, dim3(material_grid(hb, mat_)), dim3(kBlock), stream,        \
+                      (hb).dev, __VA_ARGS__);                                                   \
     }
 
 size_t pad4(size_t k) { return (k + 3) & ~(size_t)3; }
@@ -2042,12 +2141,26 @@ dim3 node_grid(int nblk) {
 }
 
 // Bin the particles of a state (positions x) into `out`.
-int do_bin(empm_ctx* ctx, const HostBatch& hb, const float* x, const BinBuf& out, hipStream_t st) {
+// With `counted`, the counting step was done by the previous substep's G2P
+// on these positions (ctx->counts_pending); else it runs here, after
+// clearing the counts of a G2P count whose binning never ran (a call that
+// stopped between the two).
+int do_bin(empm_ctx* ctx, const HostBatch& hb, const float* x, const BinBuf& out, hipStream_t st,
+           bool counted = false) {
   ProfScope ps(ctx, EMPM_KERNEL_BIN, st);
   const int nblk = hb.nblk;
   const unsigned wgs = (unsigned)((nblk + kBinScanWG - 1) / kBinScanWG);
-  if (hb.N > 0)
-    hipLaunchKernelGGL(k_bin_count, dim3(nblocks(hb.N)), dim3(kBlock), 0, st, hb.dev, x, ctx->count, ctx->key_rank);
+  if (!(counted && ctx->counts_pending)) {
+    if (ctx->counts_pending) {
+      const size_t nc = (size_t)ctx->max_blocks * 8;
+      hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nc + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, ctx->count,
+                         nc, 0);
+    }
+    if (hb.N > 0)
+      hipLaunchKernelGGL(k_bin_count, dim3(nblocks(hb.N)), dim3(kBlock), 0, st, hb.dev, x, ctx->count,
+                         ctx->key_rank);
+  }
+  ctx->counts_pending = false;
   hipLaunchKernelGGL(k_bin_scan, dim3(wgs), dim3(kBinScanWG), 0, st, nblk, ctx->count, ctx->blkoff, ctx->wsum,
                      out.act, out.nlist, ctx->ovf_flag, ctx->work);
   hipLaunchKernelGGL(k_bin_finish, dim3(wgs), dim3(kBinScanWG), 0, st, hb.dev, ctx->count, ctx->blkoff, ctx->wsum,
@@ -2068,7 +2181,7 @@ int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
   float4* keep = rec ? reinterpret_cast<float4*>(rec->mv_m) : nullptr;
   float4* v_out = rec && rec->v_out ? reinterpret_cast<float4*>(rec->v_out) : ctx->v_out;
   const BinBuf bb = bin_buf(rec && rec->bins ? rec->bins : ctx->bins, hb.nblk, hb.N);
-  int rc = do_bin(ctx, hb, in->x, bb, st);
+  int rc = do_bin(ctx, hb, in->x, bb, st, !first);
   if (rc) return rc;
   const Binning B = binning(bb, hb.nblk);
   if (hb.N > 0 && first) {
@@ -2090,8 +2203,11 @@ int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
     // F_trial of an intermediate state is read only through a recorded tape
     // (the next substep starts from Fe), so it is written for the call's
     // last substep and when recording
+    // a next substep also gets its binning's counting step from here
     EMPM_LAUNCH_RANGES(hb, k_g2p, st, dt, in->x, Fe, v_out, out->x, out->v, out->C, out->F, last ? 0 : 1, Fe_next,
-                       ctx->S, 1, (last || rec) ? 1 : 0);
+                       ctx->S, 1, (last || rec) ? 1 : 0, last ? nullptr : ctx->count,
+                       last ? nullptr : ctx->key_rank);
+    if (!last) ctx->counts_pending = true;
   }
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
@@ -2201,8 +2317,9 @@ int alloc_ctx(empm_ctx* c) {
   e = e == hipSuccess ? hipMalloc((void**)&c->blkoff, nblk * sizeof(int)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->wsum, nwsum * sizeof(int)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->work, 2 * sizeof(int)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->cand_dev, (size_t)c->max_cand * sizeof(DevModel)) : e;
-  e = e == hipSuccess ? hipHostMalloc((void**)&c->cand_pinned, (size_t)c->max_cand * sizeof(DevModel)) : e;
+  const size_t cand_bytes = (size_t)c->max_cand * (sizeof(DevModel) + sizeof(int2));  // table + material lists
+  e = e == hipSuccess ? hipMalloc((void**)&c->cand_dev, cand_bytes) : e;
+  e = e == hipSuccess ? hipHostMalloc((void**)&c->cand_pinned, cand_bytes) : e;
   e = e == hipSuccess ? hipEventCreateWithFlags(&c->cand_upload, hipEventDisableTiming) : e;
   e = e == hipSuccess ? hipMemset(c->count, 0, nblk * 8 * sizeof(int)) : e;
   e = e == hipSuccess ? hipMemset(c->ovf, 0, G * sizeof(float4)) : e;
@@ -2435,7 +2552,7 @@ int empm_sort_by_cell(empm_ctx* ctx, int n_particles, const float* x, float inv_
   hb.cand.push_back(d);
   hb.N = n_particles;
   hb.nblk = d.nb * d.nb * d.nb;
-  hb.nwaves = (n_particles + kWaveSize - 1) / kWaveSize;
+  index_waves(hb);
   (void)b;
   HIP_TRY(hipSetDevice(ctx->device));
   int rc = upload_batch(ctx, hb, st);
