@@ -764,17 +764,10 @@ EMPM_DEV int bin_key(const float* xp, const DevModel& M) {
 // one per active lane and key_rank[p] = (key, the lane's rank in its bin).
 // Lanes are grouped by key first (ballots only), then every group leader
 // issues its atomic in the same instruction: one atomic round trip per wave.
-// Issued in two halves so that a caller can overlap the atomic's round trip
-// with other work: bin_count_issue returns the group leaders' atomic results,
-// bin_count_finish spreads them and stores key_rank.
-struct BinCount {
-  int leader, rank, base;
-};
-EMPM_DEV BinCount bin_count_issue(bool active, int key, int* __restrict__ count) {
+EMPM_DEV void count_bin(bool active, int key, int p, int* __restrict__ count, int2* __restrict__ key_rank) {
   const int lane = __lane_id();
   unsigned long long pending = __ballot(active);
-  BinCount bc{lane, 0, 0};
-  int group_size = 0;
+  int my_leader = lane, my_rank = 0, group_size = 0;
   while (pending) {
     const int leader = __ffsll((unsigned long long)pending) - 1;
     const int lkey = __shfl(key, leader);
@@ -782,21 +775,15 @@ EMPM_DEV BinCount bin_count_issue(bool active, int key, int* __restrict__ count)
     const unsigned long long grp = __ballot(in);
     pending &= ~grp;
     if (in) {
-      bc.leader = leader;
-      bc.rank = __popcll(grp & ((1ull << lane) - 1ull));
+      my_leader = leader;
+      my_rank = __popcll(grp & ((1ull << lane) - 1ull));
     }
     if (lane == leader) group_size = __popcll(grp);
   }
-  if (group_size > 0) bc.base = atomicAdd(&count[key], group_size);
-  return bc;
-}
-EMPM_DEV void bin_count_finish(bool active, int key, int p, const BinCount& bc, int2* __restrict__ key_rank) {
-  const int base = __shfl(bc.base, bc.leader);
-  if (active) key_rank[p] = make_int2(key, base + bc.rank);
-}
-EMPM_DEV void count_bin(bool active, int key, int p, int* __restrict__ count, int2* __restrict__ key_rank) {
-  const BinCount bc = bin_count_issue(active, key, count);
-  bin_count_finish(active, key, p, bc, key_rank);
+  int base = 0;
+  if (group_size > 0) base = atomicAdd(&count[key], group_size);
+  base = __shfl(base, my_leader);
+  if (active) key_rank[p] = make_int2(key, base + my_rank);
 }
 
 // ---------------------------------------------------------------------------
@@ -933,7 +920,8 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
                                                 float* Fe_next, float* __restrict__ S_next, int write_params,
                                                 int write_F, int* __restrict__ count, int2* __restrict__ key_rank) {
   // with count set, the counting step of the next substep's binning is done
-  // here on the new positions (k_bin_count's work)
+  // here on the new positions (k_bin_count's work), last: counting in the
+  // middle keeps the gather's state live across the stress stage
   per_particle(Bt, MAT, [&](const DevModel& M, const int p) {
   if (M.sel && M.sel[p] != 0) {  // partial_clone defaults (mpm_data_structure.py:356-387)
     float I[9], Z[9];
@@ -1009,12 +997,6 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
     xo[i] = fminf(fmaxf(xn, amin), amax);
     x1[3 * p + i] = xo[i];
   }
-  int key = 0;
-  BinCount bc{};
-  if (count) {
-    key = bin_key(xo, M);
-    bc = bin_count_issue(true, key, count);
-  }
   store9(C1, p, nC);
   float IF[9], Fp[9], Fn[9];
 #pragma unroll
@@ -1023,7 +1005,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
   m_mul(IF, Fp, Fn);
   if (write_F) store9(F1, p, Fn);
   if (next_stress) particle_stress<MAT>(M, dt, p, Fn, Fe_next, S_next, write_params);
-  if (count) bin_count_finish(true, key, p, bc, key_rank);
+  if (count) count_bin(true, bin_key(xo, M), p, count, key_rank);
   });
 }
 

@@ -234,6 +234,10 @@ EMPM_DEV bool is_plastic(int m) { return m == 1 || m == 2 || m == 3 || m == 5 ||
 // (mpm_utils.py:133-186); material 7 uses the former.
 EMPM_DEV void rm_von_mises(ConstitState& st, const float* Ftr, float& mu, float& lam, float& ys, const MatParams& P,
                            bool damage) {
+  // Every branch only sets locals; the state and the in/out parameters are
+  // written once at the end (stores to different fields on different
+  // branches otherwise merge into one store through a selected address,
+  // which keeps the fields out of registers).
   float e[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) e[i] = logf(fmaxf(st.s[i], 0.01f));
@@ -245,48 +249,48 @@ EMPM_DEV void rm_von_mises(ConstitState& st, const float* Ftr, float& mu, float&
   const float st3 = (tau[0] + tau[1] + tau[2]) / 3.f;
   const float c0 = tau[0] - st3, c1 = tau[1] - st3, c2 = tau[2] - st3;
   const float cn = sqrtf(c0 * c0 + c1 * c1 + c2 * c2);
-  st.lo = 0.01f;
-  st.use_abs = 0;
+  float g[3] = {st.s[0], st.s[1], st.s[2]};
+  float mu_n = mu, lam_n = lam, ys_n = ys, k = 1.f;
+  int branch = RM_IDENTITY;
   if (cn > ys) {
     if (damage && ys <= 0.f) {
-      st.branch = RM_DAMAGED;
-      m_copy(Ftr, st.Fe);
-      st.se[0] = st.s[0];
-      st.se[1] = st.s[1];
-      st.se[2] = st.s[2];
-      return;
-    }
-    float eh[3] = {e[0] - temp, e[1] - temp, e[2] - temp};
-    const float ehn = sqrtf(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]) + 1e-6f;
-    const float dg = ehn - ys / (2.f * mu);
-    float en[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) en[i] = e[i] - (dg / ehn) * eh[i];
-    if (damage) {
-      const float q0 = (dg / ehn) * eh[0], q1 = (dg / ehn) * eh[1], q2 = (dg / ehn) * eh[2];
-      ys = ys - P.softening * sqrtf(q0 * q0 + q1 * q1 + q2 * q2);
-      if (ys <= 0.f) {
-        mu = 0.f;
-        lam = 0.f;
+      branch = RM_DAMAGED;
+    } else {
+      float eh[3] = {e[0] - temp, e[1] - temp, e[2] - temp};
+      const float ehn = sqrtf(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]) + 1e-6f;
+      const float dg = ehn - ys / (2.f * mu);
+      if (damage) {
+        const float q0 = (dg / ehn) * eh[0], q1 = (dg / ehn) * eh[1], q2 = (dg / ehn) * eh[2];
+        ys_n = ys - P.softening * sqrtf(q0 * q0 + q1 * q1 + q2 * q2);
+        if (ys_n <= 0.f) {
+          mu_n = 0.f;
+          lam_n = 0.f;
+        }
       }
-    }
 #pragma unroll
-    for (int i = 0; i < 3; ++i) st.se[i] = expf(en[i]);
-    m_udvt(st.U, st.se, st.V, st.Fe);
-    if (P.hardening == 1.f) ys = ys + 2.f * mu * P.xi * dg;
-    st.branch = RM_POWER;
-    st.k = 1.f - dg / ehn;
-    return;
+      for (int i = 0; i < 3; ++i) g[i] = expf(e[i] - (dg / ehn) * eh[i]);
+      if (P.hardening == 1.f) ys_n = ys_n + 2.f * mu_n * P.xi * dg;
+      branch = RM_POWER;
+      k = 1.f - dg / ehn;
+    }
   }
-  st.branch = RM_IDENTITY;
-  m_copy(Ftr, st.Fe);
-  st.se[0] = st.s[0];
-  st.se[1] = st.s[1];
-  st.se[2] = st.s[2];
+  st.lo = 0.01f;
+  st.use_abs = 0;
+  st.branch = branch;
+  st.k = k;
+  st.se[0] = g[0];
+  st.se[1] = g[1];
+  st.se[2] = g[2];
+  if (branch == RM_POWER) m_udvt(st.U, g, st.V, st.Fe);
+  else m_copy(Ftr, st.Fe);
+  mu = mu_n;
+  lam = lam_n;
+  ys = ys_n;
 }
 
 // viscoplasticity_return_mapping_with_StVK (mpm_utils.py:189-234)
 EMPM_DEV void rm_visco(ConstitState& st, const float* Ftr, float mu, float ys, const MatParams& P, float dt) {
+  // locals per branch, state written once (as in rm_von_mises)
   float sg[3], b[3], e[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -299,27 +303,31 @@ EMPM_DEV void rm_visco(ConstitState& st, const float* Ftr, float mu, float ys, c
   float s[3] = {2.f * mu * eh[0], 2.f * mu * eh[1], 2.f * mu * eh[2]};
   const float sn = sqrtf(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
   const float y = sn - sqrtf(2.f / 3.f) * ys;
-  st.lo = 0.01f;
-  st.use_abs = 0;
+  float g[3] = {st.s[0], st.s[1], st.s[2]};
+  float k = 1.f;
+  int branch = RM_IDENTITY;
   if (y > 0.f) {
     const float muh = mu * (b[0] + b[1] + b[2]) / 3.f;
     const float snew = sn - y / (1.f + P.plastic_viscosity / (2.f * muh * dt));
 #pragma unroll
-    for (int i = 0; i < 3; ++i) st.se[i] = expf(1.f / (2.f * mu) * ((snew / sn) * s[i]) + tr / 3.f);
-    m_udvt(st.U, st.se, st.V, st.Fe);
-    st.branch = RM_POWER;
-    st.k = snew / sn;
-    return;
+    for (int i = 0; i < 3; ++i) g[i] = expf(1.f / (2.f * mu) * ((snew / sn) * s[i]) + tr / 3.f);
+    branch = RM_POWER;
+    k = snew / sn;
   }
-  st.branch = RM_IDENTITY;
-  m_copy(Ftr, st.Fe);
-  st.se[0] = st.s[0];
-  st.se[1] = st.s[1];
-  st.se[2] = st.s[2];
+  st.lo = 0.01f;
+  st.use_abs = 0;
+  st.branch = branch;
+  st.k = k;
+  st.se[0] = g[0];
+  st.se[1] = g[1];
+  st.se[2] = g[2];
+  if (branch == RM_POWER) m_udvt(st.U, g, st.V, st.Fe);
+  else m_copy(Ftr, st.Fe);
 }
 
 // sand_return_mapping (mpm_utils.py:237-274)
 EMPM_DEV void rm_sand(ConstitState& st, const float* Ftr, float mu, float lam, const MatParams& P) {
+  // locals per branch, state written once (as in rm_von_mises)
   float e[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) e[i] = logf(fmaxf(fabsf(st.s[i]), 1e-14f));
@@ -327,27 +335,30 @@ EMPM_DEV void rm_sand(ConstitState& st, const float* Ftr, float mu, float lam, c
   float eh[3] = {e[0] - tr / 3.f, e[1] - tr / 3.f, e[2] - tr / 3.f};
   const float ehn = sqrtf(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
   const float dg = ehn + (3.f * lam + 2.f * mu) / (2.f * mu) * tr * P.alpha;
+  float g[3] = {st.s[0], st.s[1], st.s[2]};
+  float k = 1.f;
+  int branch = RM_IDENTITY;
+  if (!(dg <= 0.f)) {  // the reference's test is delta_gamma <= 0 for the identity
+    if (tr > 0.f) {
+      branch = RM_ROTATION;
+      g[0] = g[1] = g[2] = 1.f;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) g[i] = expf(e[i] - eh[i] * (dg / ehn));
+      branch = RM_POWER;
+      k = 1.f - dg / ehn;
+    }
+  }
   st.lo = 1e-14f;
   st.use_abs = 1;
-  if (dg <= 0.f) {
-    st.branch = RM_IDENTITY;
-    m_copy(Ftr, st.Fe);
-    st.se[0] = st.s[0];
-    st.se[1] = st.s[1];
-    st.se[2] = st.s[2];
-    return;
-  }
-  if (tr > 0.f) {
-    st.branch = RM_ROTATION;
-    st.se[0] = st.se[1] = st.se[2] = 1.f;
-    m_mul_bt(st.U, st.V, st.Fe);
-    return;
-  }
-#pragma unroll
-  for (int i = 0; i < 3; ++i) st.se[i] = expf(e[i] - eh[i] * (dg / ehn));
-  m_udvt(st.U, st.se, st.V, st.Fe);
-  st.branch = RM_POWER;
-  st.k = 1.f - dg / ehn;
+  st.branch = branch;
+  st.k = k;
+  st.se[0] = g[0];
+  st.se[1] = g[1];
+  st.se[2] = g[2];
+  if (branch == RM_POWER) m_udvt(st.U, g, st.V, st.Fe);
+  else if (branch == RM_ROTATION) m_mul_bt(st.U, st.V, st.Fe);
+  else m_copy(Ftr, st.Fe);
 }
 
 // ------------------------- stresses (forward) -------------------------------

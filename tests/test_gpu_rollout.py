@@ -224,3 +224,42 @@ def test_particles_outside_grid_use_the_overflow_path(dev):
     ev = rel_err(s_out.v.cpu().numpy(), o.v)
     print(f"[outside grid] rel err x {ex:.2e} v {ev:.2e}", flush=True)
     assert ex < 1e-5 and ev < 1e-4
+
+
+def test_stopped_call_leaves_no_stale_bin_counts(dev):
+    """Within a rollout each substep's G2P does the counting step of the next
+    substep's binning.  A replay whose collider table turns invalid at
+    substep 1 stops right after such a count; the context's next rollout
+    must bin from fresh counts and match a rollout on a fresh context."""
+    import ctypes as C
+
+    from empm_amd._lib import Collider, EmpmError
+
+    sc = scenes.small_scene(n_particles=400, n_grid=16, material=0, seed=3)
+    print(f"[stopped call] N={sc.n}", flush=True)
+    model, s0, P, ocols, ostate = setup(sc, dev, v0=sc.params["v0"])
+    cols = [gpu_collider(c) for c in ocols]
+    ncol = len(cols)
+    ctx = rt.Context(sc.n_grid, sc.n, dev)
+    ro = rt.Rollout(ctx, model, sc.n, sort=False)
+    hist = (Collider * (2 * ncol))()
+    for k in range(2):
+        for i, c in enumerate(cols):
+            hist[k * ncol + i] = c
+    hist[ncol].kind = 9  # unknown kind at substep 1
+    times = (C.c_double * 2)(0.0, sc.dt)
+    states = [rt.ParticleState.empty(sc.n, dev) for _ in range(3)]
+    states[0].copy_(s0)
+    with pytest.raises(EmpmError, match="unknown kind"):
+        ro._replay(model.struct(), hist, ncol, times, C.c_double(sc.dt), 2, rt._states_array(states), None,
+                   rt.stream_handle(dev))
+    torch.cuda.synchronize()
+    steps = 20
+    a, _, _, _ = ro.forward(s0, [gpu_collider(c) for c in ocols], 0.0, sc.dt, steps)
+    ro_b = rt.Rollout(rt.Context(sc.n_grid, sc.n, dev), model, sc.n, sort=False)
+    b, _, _, _ = ro_b.forward(s0, [gpu_collider(c) for c in ocols], 0.0, sc.dt, steps)
+    torch.cuda.synchronize()
+    for k in ("x", "v", "F"):
+        err = rel_err(getattr(a, k).cpu().numpy(), getattr(b, k).cpu().numpy())
+        print(f"  {k}: rel err vs fresh context {err:.2e}", flush=True)
+        assert err < 1e-5
