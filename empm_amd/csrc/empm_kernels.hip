@@ -706,16 +706,15 @@ EMPM_DEV void block_scatter(const Batch& Bt, const Binning& B, float4* __restric
   }
 }
 
-// Sum of the tiles at node (ix, iy, iz).  Nodes whose owning block (i >> 1)
-// has no non-empty block among the candidates return at once.  The node
-// lies in the tiles of blocks b = ((i - 2) >> 1) + {0, 1} per axis (a tile
-// spans nodes 2b .. 2b + 3); all 8 occupancy flags and tile entries are
-// loaded before any is used, so the gather costs one memory latency.
+// Sum of the tiles at node (ix, iy, iz), a node of a near block (the node
+// passes visit no other).  The node lies in the tiles of blocks
+// b = ((i - 2) >> 1) + {0, 1} per axis (a tile spans nodes 2b .. 2b + 3);
+// all 8 occupancy flags and tile entries are loaded before any is used, so
+// the gather costs one memory latency.
 EMPM_DEV float4 gather_node(const Binning& B, const DevModel& M, const float4* __restrict__ tiles, int ix, int iy,
                             int iz) {
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   const int nb = M.nb;
-  if (!B.near[M.b0 + ((ix >> 1) * nb + (iy >> 1)) * nb + (iz >> 1)]) return acc;
   const int l0 = (ix - 2) >> 1, l1 = (iy - 2) >> 1, l2 = (iz - 2) >> 1;
   bool ok[8];
   int blk[8];
