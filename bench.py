@@ -45,8 +45,8 @@ def kernel_bytes(N, G):
         # reads (mv, m), writes v_out (12 B) per node
         "k_grid": 28 * G,
         # reads x, Fe (48 B), mu, lam, yield, vol (16 B) and v_out once per node; writes x, v, C, F_trial
-        # (96 B) and the next substep's Fe and S (72 B)
-        "k_g2p": 232 * N + 12 * G,
+        # (96 B), the next substep's Fe and S (72 B) and its binning's key and rank (8 B)
+        "k_g2p": 240 * N + 12 * G,
         # reads F_trial (36 B) and mu, lam, yield, vol (16 B); writes Fe and S (72 B)
         "k_stress": 124 * N,
         # reads x, Fe (48), the adjoint of the next state (96) and v_out; writes dL/dx partial (12), dL/dFe (36)
@@ -59,10 +59,10 @@ def kernel_bytes(N, G):
         # reads state + constants (116), partial adjoints (48), dL/d(mv, m) once per node;
         # writes the adjoint of the state (96) and read-modify-writes 5 parameter adjoints (40)
         "k_p2g_adj": 300 * N + 16 * G,
-        # the four binning kernels together: x (12 B) read, key and rank (8 B) written and read, perm (4 B)
-        # written per particle; per 2x2x2-cell block (G / 8): 8 counts read and re-zeroed, 8 offsets written
-        # and the occupancy, near and list entries (about 112 B, 14 B per node)
-        "k_bin": 32 * N + 14 * G,
+        # the binning kernels of a substep whose counting the previous G2P did: key and rank (8 B) read,
+        # perm (4 B) written per particle; per 2x2x2-cell block (G / 8): 8 counts read and re-zeroed,
+        # 8 offsets written and the occupancy, near and list entries (about 112 B, 14 B per node)
+        "k_bin": 12 * N + 14 * G,
     }
 
 
