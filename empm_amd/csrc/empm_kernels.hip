@@ -864,8 +864,10 @@ __global__ __launch_bounds__(kBlkWG) void k_p2g(Batch Bt, const float* __restric
 // node) pairs of the binning's near list, so the cost follows the occupied
 // part of the grid, not its size.
 constexpr int kNodePassGridCap = 1024;
-template <typename F>
-EMPM_DEV void for_near_nodes(const Batch& Bt, const Binning& B, F f) {
+// ONE: a batch of one simulation, whose table entry is then read with
+// scalar loads at once instead of per lane after the list entry
+template <bool ONE, typename F>
+EMPM_DEV void for_near_nodes_t(const Batch& Bt, const Binning& B, F f) {
   const size_t stride = (size_t)gridDim.x * blockDim.x, last = (size_t)B.nblk - 1;
   size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   // list entries are read ahead of the count test (the list has nblk slots)
@@ -875,13 +877,18 @@ EMPM_DEV void for_near_nodes(const Batch& Bt, const Binning& B, F f) {
     const int blk = blk_next, r = (int)(q & 7);
     const size_t qn = (q + stride) >> 3;
     blk_next = B.nlist[qn < last ? qn : last];
-    const DevModel& M = cand_of_block(Bt, blk);
+    const DevModel& M = ONE ? Bt.cand[0] : cand_of_block(Bt, blk);
     const int nb = M.nb, lb = blk - M.b0;
     const int ix = 2 * (lb / (nb * nb)) + (r >> 2), iy = 2 * ((lb / nb) % nb) + ((r >> 1) & 1),
               iz = 2 * (lb % nb) + (r & 1);
     if (ix >= M.n || iy >= M.n || iz >= M.n) continue;
     f(M, ix, iy, iz, (size_t)M.g0 + node_id(ix, iy, iz, M.n));
   }
+}
+template <typename F>
+EMPM_DEV void for_near_nodes(const Batch& Bt, const Binning& B, F f) {
+  if (Bt.ncand == 1) for_near_nodes_t<true>(Bt, B, f);
+  else for_near_nodes_t<false>(Bt, B, f);
 }
 
 __global__ __launch_bounds__(kBlock) void k_grid(Batch Bt, ColTable T, float time, float dt, Binning B,
