@@ -8,8 +8,8 @@ gradient w.r.t. (log E, nu, log yield).
 One step = one forward rollout recording its tape + loss + one reverse
 sweep over the recorded substeps.  value = N * 400 / step time.
 
-Prints one JSON line (see --help).  --gpus N > 1 exits: this build runs on
-one GPU.
+Prints one JSON line (see --help).  --gpus N > 1 exits: the library runs
+the MPM hot path on one GPU.
 """
 from __future__ import annotations
 
@@ -114,7 +114,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs the MPM hot path on one GPU only; --gpus must be 1 "
+        print("bench.py: the library runs the MPM hot path on one GPU only; --gpus must be 1 "
               "(multi-GPU sharding is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
