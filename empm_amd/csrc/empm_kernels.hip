@@ -180,6 +180,16 @@ EMPM_DEV void make_stencil(const float* xp, float inv_dx, Stencil& s) {
   }
 }
 
+// Whether the 3x3x3 stencils of all active lanes lie inside the n^3 grid
+// (almost always: positions are clamped two cells in from the bounds), so
+// that their node loads need no bounds tests: node (a, b, c) is then at
+// node_id(base) + a n^2 + b n + c.
+EMPM_DEV bool wave_stencils_inside(const Stencil& s, int n) {
+  const unsigned hi = (unsigned)(n - 3);
+  const bool in = (unsigned)s.base[0] <= hi && (unsigned)s.base[1] <= hi && (unsigned)s.base[2] <= hi;
+  return __ballot(!in) == 0;
+}
+
 // Weight lookups by a possibly run-time node index through selects, so the
 // stencil stays in registers whether or not the node loops are unrolled.
 EMPM_DEV float pick3(int i, const float* a) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
@@ -956,6 +966,9 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
   float zd[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) zd[c] = s.w[2][c] * ((float)c - s.f[2]);
+  // (the unmasked node loads of the adjoint kernels, for waves whose
+  // stencils are all inside the grid, let the compiler hoist all 27 loads
+  // here: 84 -> 129 VGPRs, 5 -> 3 waves, slower)
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -1072,6 +1085,12 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(Batch Bt, float dt, const fl
     dzd[c] = s.dw[2][c] * ((float)c - s.f[2]);
   }
   const float q[3] = {dC[2], dC[5], dC[8]};  // dC e_z
+  // node loads straight from the stencil's first node when every stencil
+  // of the wave is inside the grid, else bounds-tested (out-of-grid nodes
+  // count as zero)
+  const bool inside = wave_stencils_inside(s, M.n);
+  const size_t nn = (size_t)M.n * M.n;
+  const float4* __restrict__ vb = v_out + M.g0 + (inside ? node_id(s.base[0], s.base[1], s.base[2], M.n) : 0);
 #pragma unroll 1
   for (int a = 0; a < 3; ++a) {
     const float X = sw(s, 0, a), dX = sdw(s, 0, a), ddX = a == 1 ? -2.f : 1.f;
@@ -1083,10 +1102,16 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(Batch Bt, float dt, const fl
             A5[3] = {0.f, 0.f, 0.f};
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
-        const bool in = node_in(ix, iy, iz, M.n);
-        const float4 g4 = v_out[M.g0 + (in ? node_id(ix, iy, iz, M.n) : 0)];
-        const float vi[3] = {in ? g4.x : 0.f, in ? g4.y : 0.f, in ? g4.z : 0.f};
+        float vi[3];
+        if (inside) {
+          const float4 g4 = vb[a * nn + b * M.n + c];
+          vi[0] = g4.x, vi[1] = g4.y, vi[2] = g4.z;
+        } else {
+          const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
+          const bool in = node_in(ix, iy, iz, M.n);
+          const float4 g4 = v_out[M.g0 + (in ? node_id(ix, iy, iz, M.n) : 0)];
+          vi[0] = in ? g4.x : 0.f, vi[1] = in ? g4.y : 0.f, vi[2] = in ? g4.z : 0.f;
+        }
         const float Z = s.w[2][c], dZ = s.dw[2][c];
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
@@ -1266,9 +1291,20 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
     dzd[c] = s.dw[2][c] * ((float)c - s.f[2]);
   }
   const float Ct2[3] = {Ct[2], Ct[5], Ct[8]};
-  // the nodes of plane a (9 of them, b-major) are loaded one plane ahead:
-  // out-of-grid nodes read node 0 of the grid and count as zero
+  // the nodes of plane a (9 of them, b-major) are loaded one plane ahead;
+  // when a stencil of the wave leaves the grid, its out-of-grid nodes read
+  // node 0 of the grid and count as zero
+  const bool inside = wave_stencils_inside(s, M.n);
+  const size_t nn = (size_t)M.n * M.n;
+  const float4* __restrict__ gb =
+      gadj_in + M.g0 + (inside ? node_id(s.base[0], s.base[1], s.base[2], M.n) : 0);
   const auto load_plane = [&](int a, float4* pl) {
+    if (inside) {
+      const float4* __restrict__ pa = gb + a * nn;
+#pragma unroll
+      for (int j = 0; j < 9; ++j) pl[j] = pa[(j / 3) * M.n + j % 3];
+      return;
+    }
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
       const int ix = s.base[0] + a, iy = s.base[1] + j / 3, iz = s.base[2] + j % 3;

@@ -94,9 +94,24 @@ def _oracle_loss(d, material, w, P, cols):
 @pytest.mark.parametrize("mode", ["exact", "drop_grid_mass"])
 def test_substep_adjoint_directional(dev, material, mode):
     """<J^T w, u> vs central differences of the FP64 oracle."""
-    print(f"[substep adj] material={material} mode={mode}", flush=True)
+    _adjoint_directional(dev, material, mode)
+
+
+@pytest.mark.parametrize("material", [0, 7])
+def test_substep_adjoint_directional_stencils_leaving_grid(dev, material):
+    """As above with 12 particles whose stencils leave the grid (base -1 in
+    x, base n - 2 in z): the adjoint kernels test the bounds of every node
+    for the wave holding them and load the other waves' nodes unmasked."""
+    _adjoint_directional(dev, material, "exact", edge=True)
+
+
+def _adjoint_directional(dev, material, mode, edge=False):
+    print(f"[substep adj] material={material} mode={mode} edge={edge}", flush=True)
     Ns = 200
     d = random_state(Ns, N_GRID, seed=100 + material, material=material, deform=_deform(material))
+    if edge:
+        d["x"][:6, 0] = -0.7 / N_GRID  # x inv_dx - 0.5 = -1.2: base -1 (truncation)
+        d["x"][6:12, 2] = (N_GRID - 1.3) / N_GRID  # base n - 2: node n is outside
     rng = np.random.Generator(np.random.PCG64(7 + material))
     cols = [orc.surface_collider((0, 0, 0.3), (0, 0, 1))]
     model, P = _model(d, material, dev, visc=0.5 if material == 3 else 0.0, damping=0.98)
