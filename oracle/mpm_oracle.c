@@ -670,11 +670,18 @@ void oracle_advance_colliders(oracle_collider *cols, int ncol, double time, doub
 /* State s (x, v, C, F_trial) -> state s+1.  mu/lam/ys are updated in place */
 /* by material 5 as in the reference.  grid: n^3*4 scratch (mv.xyz, m).     */
 /* Fe_out / tau_out (optional) receive the return-mapped F and the stress. */
+/* m_fixed (optional, n^3): the grid mass the normalisation divides by and  */
+/* tests against 1e-15, in place of the mass P2G just accumulated (grid[.3] */
+/* still receives that mass).  Holding it at an unperturbed substep's value */
+/* gives the function whose derivative drops the grid-mass adjoint, as the  */
+/* reference Tape does (grid_m carries no grad: mpm_data_structure.py:      */
+/* 100-105 with warp_utils.py:116-120), so central differences of it check  */
+/* that gradient convention.                                                */
 /* ------------------------------------------------------------------ */
-int oracle_substep(const oracle_params *P, const oracle_collider *cols, int ncol, double time, double dt,
-                   const real *x, const real *v, const real *C, const real *Ftr, real *mu, real *lam, real *ys,
-                   const real *mass, const real *vol, const int *sel, real *x1, real *v1, real *C1, real *F1,
-                   real *grid, real *Fe_out, real *tau_out)
+static int substep_impl(const oracle_params *P, const oracle_collider *cols, int ncol, double time, double dt,
+                        const real *x, const real *v, const real *C, const real *Ftr, real *mu, real *lam, real *ys,
+                        const real *mass, const real *vol, const int *sel, real *x1, real *v1, real *C1, real *F1,
+                        real *grid, real *Fe_out, real *tau_out, const real *m_fixed)
 {
     const int N = P->n_particles, n = P->n_grid;
     const size_t G = (size_t)n * n * n;
@@ -761,8 +768,9 @@ int oracle_substep(const oracle_params *P, const oracle_collider *cols, int ncol
                 size_t id = ((size_t)ix * n + iy) * n + iz;
                 real *g = &grid[4 * id];
                 real vv[3] = {0, 0, 0};
-                if (g[3] > (real)1e-15) {
-                    real im = 1 / g[3];
+                const real gm = m_fixed ? m_fixed[id] : g[3];
+                if (gm > (real)1e-15) {
+                    real im = 1 / gm;
                     for (int i = 0; i < 3; ++i)
                         vv[i] = g[i] * im + rdt * (real)P->gravity[i];
                 }
@@ -830,6 +838,25 @@ int oracle_substep(const oracle_params *P, const oracle_collider *cols, int ncol
     return 0;
 }
 
+int oracle_substep(const oracle_params *P, const oracle_collider *cols, int ncol, double time, double dt,
+                   const real *x, const real *v, const real *C, const real *Ftr, real *mu, real *lam, real *ys,
+                   const real *mass, const real *vol, const int *sel, real *x1, real *v1, real *C1, real *F1,
+                   real *grid, real *Fe_out, real *tau_out)
+{
+    return substep_impl(P, cols, ncol, time, dt, x, v, C, Ftr, mu, lam, ys, mass, vol, sel, x1, v1, C1, F1, grid,
+                        Fe_out, tau_out, 0);
+}
+
+/* oracle_substep with the grid mass of the normalisation held at m_fixed. */
+int oracle_substep_frozen_mass(const oracle_params *P, const oracle_collider *cols, int ncol, double time, double dt,
+                               const real *x, const real *v, const real *C, const real *Ftr, real *mu, real *lam,
+                               real *ys, const real *mass, const real *vol, const int *sel, real *x1, real *v1,
+                               real *C1, real *F1, real *grid, const real *m_fixed)
+{
+    return substep_impl(P, cols, ncol, time, dt, x, v, C, Ftr, mu, lam, ys, mass, vol, sel, x1, v1, C1, F1, grid, 0,
+                        0, m_fixed);
+}
+
 /* The grid stage alone (steps 5-7 of the substep) on a given (mv, m) grid:
  * exported so the collider node sets can be checked directly. */
 void oracle_grid_update(const oracle_params *P, const oracle_collider *cols, int ncol, double time, double dt,
@@ -859,11 +886,15 @@ void oracle_grid_update(const oracle_params *P, const oracle_collider *cols, int
 }
 
 /* A rollout of n_steps substeps from state (x, v, C, F), in place, with the
- * per-substep host collider advance and time += dt (mpm_solver_diff.py:291).
- * Returns the final time. */
-double oracle_rollout(const oracle_params *P, oracle_collider *cols, int ncol, double t0, double dt, int n_steps,
-                      real *x, real *v, real *C, real *F, real *mu, real *lam, real *ys, const real *mass,
-                      const real *vol, const int *sel)
+ * per-substep host collider advance and time += dt (mpm_solver_diff.py:291),
+ * as the reference's host loops run it (interface.py:137-145,
+ * warp_mpm_wrapper.py:278-306).  Optional: m_hist_out (n_steps * n^3)
+ * receives each substep's P2G grid mass; m_hist_in (same shape) holds the
+ * grid mass of every substep's normalisation at the values given (see
+ * oracle_substep_frozen_mass).  Returns the final time (-1 on failure). */
+static double rollout_impl(const oracle_params *P, oracle_collider *cols, int ncol, double t0, double dt, int n_steps,
+                           real *x, real *v, real *C, real *F, real *mu, real *lam, real *ys, const real *mass,
+                           const real *vol, const int *sel, real *m_hist_out, const real *m_hist_in)
 {
     const int N = P->n_particles, n = P->n_grid;
     const size_t G = (size_t)n * n * n;
@@ -876,11 +907,14 @@ double oracle_rollout(const oracle_params *P, oracle_collider *cols, int ncol, d
         goto done;
     }
     for (int s = 0; s < n_steps; ++s) {
-        if (oracle_substep(P, cols, ncol, t, dt, x, v, C, F, mu, lam, ys, mass, vol, sel, x1, v1, C1, F1, grid, 0,
-                           0) != 0) {
+        if (substep_impl(P, cols, ncol, t, dt, x, v, C, F, mu, lam, ys, mass, vol, sel, x1, v1, C1, F1, grid, 0, 0,
+                         m_hist_in ? m_hist_in + (size_t)s * G : 0) != 0) {
             t = -1;
             goto done;
         }
+        if (m_hist_out)
+            for (size_t i = 0; i < G; ++i)
+                m_hist_out[(size_t)s * G + i] = grid[4 * i + 3];
         oracle_advance_colliders(cols, ncol, t, dt);
         memcpy(x, x1, sizeof(real) * 3 * (size_t)N);
         memcpy(v, v1, sizeof(real) * 3 * (size_t)N);
@@ -895,6 +929,22 @@ done:
     free(F1);
     free(grid);
     return t;
+}
+
+double oracle_rollout(const oracle_params *P, oracle_collider *cols, int ncol, double t0, double dt, int n_steps,
+                      real *x, real *v, real *C, real *F, real *mu, real *lam, real *ys, const real *mass,
+                      const real *vol, const int *sel)
+{
+    return rollout_impl(P, cols, ncol, t0, dt, n_steps, x, v, C, F, mu, lam, ys, mass, vol, sel, 0, 0);
+}
+
+double oracle_rollout_mass_hist(const oracle_params *P, oracle_collider *cols, int ncol, double t0, double dt,
+                                int n_steps, real *x, real *v, real *C, real *F, real *mu, real *lam, real *ys,
+                                const real *mass, const real *vol, const int *sel, real *m_hist_out,
+                                const real *m_hist_in)
+{
+    return rollout_impl(P, cols, ncol, t0, dt, n_steps, x, v, C, F, mu, lam, ys, mass, vol, sel, m_hist_out,
+                        m_hist_in);
 }
 
 int oracle_num_threads(void)

@@ -84,6 +84,12 @@ def load(precision: str = "f64"):
     lib.oracle_rollout.argtypes = [C.POINTER(OracleParams), C.POINTER(OracleCollider), C.c_int, C.c_double,
                                    C.c_double, C.c_int] + [vp] * 10
     lib.oracle_rollout.restype = C.c_double
+    lib.oracle_substep_frozen_mass.argtypes = [C.POINTER(OracleParams), C.POINTER(OracleCollider), C.c_int,
+                                               C.c_double, C.c_double] + [vp] * 15
+    lib.oracle_substep_frozen_mass.restype = C.c_int
+    lib.oracle_rollout_mass_hist.argtypes = [C.POINTER(OracleParams), C.POINTER(OracleCollider), C.c_int,
+                                             C.c_double, C.c_double, C.c_int] + [vp] * 12
+    lib.oracle_rollout_mass_hist.restype = C.c_double
     lib.oracle_grid_update.argtypes = [C.POINTER(OracleParams), C.POINTER(OracleCollider), C.c_int, C.c_double,
                                        C.c_double, vp, vp]
     lib.oracle_grid_update.restype = None
@@ -158,9 +164,11 @@ class OracleState:
                            self.sel, self.precision)
 
 
-def substep(P, cols, time, dt, s: OracleState, return_fe=False):
+def substep(P, cols, time, dt, s: OracleState, return_fe=False, m_fixed=None):
     """One substep; returns the next OracleState (mu/lam/ys updated in place
-    in `s` by material 5, then copied)."""
+    in `s` by material 5, then copied).  m_fixed (n^3 grid masses) holds the
+    normalisation's grid mass at those values: central differences of that
+    function give the gradient with the grid-mass adjoint dropped."""
     lib = load(s.precision)
     N, n = P.n_particles, P.n_grid
     d = _dt(s.precision)
@@ -172,9 +180,19 @@ def substep(P, cols, time, dt, s: OracleState, return_fe=False):
     Fe = np.zeros((N, 3, 3), d)
     tau = np.zeros((N, 3, 3), d)
     arr, nc = colliders_array(cols)
-    rc = lib.oracle_substep(C.byref(P), arr, nc, time, dt, _p(s.x), _p(s.v), _p(s.C), _p(s.F), _p(s.mu), _p(s.lam),
-                            _p(s.ys), _p(s.mass), _p(s.vol), _p(s.sel), _p(x1), _p(v1), _p(C1), _p(F1), _p(grid),
-                            _p(Fe), _p(tau))
+    if m_fixed is not None:
+        if return_fe:
+            raise ValueError("return_fe with m_fixed is not supported")
+        mf = np.ascontiguousarray(m_fixed, dtype=d).reshape(-1)
+        if mf.size != n * n * n:
+            raise ValueError(f"m_fixed has {mf.size} entries, the grid {n ** 3}")
+        rc = lib.oracle_substep_frozen_mass(C.byref(P), arr, nc, time, dt, _p(s.x), _p(s.v), _p(s.C), _p(s.F),
+                                            _p(s.mu), _p(s.lam), _p(s.ys), _p(s.mass), _p(s.vol), _p(s.sel), _p(x1),
+                                            _p(v1), _p(C1), _p(F1), _p(grid), _p(mf))
+    else:
+        rc = lib.oracle_substep(C.byref(P), arr, nc, time, dt, _p(s.x), _p(s.v), _p(s.C), _p(s.F), _p(s.mu),
+                                _p(s.lam), _p(s.ys), _p(s.mass), _p(s.vol), _p(s.sel), _p(x1), _p(v1), _p(C1),
+                                _p(F1), _p(grid), _p(Fe), _p(tau))
     if rc != 0:
         raise RuntimeError("oracle_substep failed")
     out = OracleState(x1, v1, C1, F1, s.mu, s.lam, s.ys, s.mass, s.vol, s.sel, s.precision)
@@ -183,16 +201,29 @@ def substep(P, cols, time, dt, s: OracleState, return_fe=False):
     return out
 
 
-def rollout(P, cols, t0, dt, n_steps, s: OracleState):
+def rollout(P, cols, t0, dt, n_steps, s: OracleState, m_hist_out=None, m_hist_in=None):
     """In-place rollout of n_steps; returns the final time.  `cols` is a
-    ctypes array (advanced in place) or a list."""
+    ctypes array (advanced in place) or a list.  m_hist_out (n_steps, n^3),
+    if given, receives every substep's grid mass; m_hist_in (same shape)
+    holds every substep's normalisation mass at those values."""
     lib = load(s.precision)
     if not isinstance(cols, C.Array):
         cols, nc = colliders_array(cols)
     else:
         nc = len(cols)
-    t = lib.oracle_rollout(C.byref(P), cols, nc, t0, dt, int(n_steps), _p(s.x), _p(s.v), _p(s.C), _p(s.F),
-                           _p(s.mu), _p(s.lam), _p(s.ys), _p(s.mass), _p(s.vol), _p(s.sel))
+    if m_hist_out is None and m_hist_in is None:
+        t = lib.oracle_rollout(C.byref(P), cols, nc, t0, dt, int(n_steps), _p(s.x), _p(s.v), _p(s.C), _p(s.F),
+                               _p(s.mu), _p(s.lam), _p(s.ys), _p(s.mass), _p(s.vol), _p(s.sel))
+    else:
+        G = P.n_grid ** 3
+        for h in (m_hist_out, m_hist_in):
+            if h is not None and (h.dtype != _dt(s.precision) or h.size != int(n_steps) * G
+                                  or not h.flags.c_contiguous):
+                raise ValueError(f"mass history must be a contiguous {_dt(s.precision).__name__} array of "
+                                 f"{n_steps} x {G}")
+        t = lib.oracle_rollout_mass_hist(C.byref(P), cols, nc, t0, dt, int(n_steps), _p(s.x), _p(s.v), _p(s.C),
+                                         _p(s.F), _p(s.mu), _p(s.lam), _p(s.ys), _p(s.mass), _p(s.vol), _p(s.sel),
+                                         _p(m_hist_out), _p(m_hist_in))
     if t < 0:
         raise RuntimeError("oracle_rollout failed")
     return t

@@ -52,6 +52,39 @@ def rel_err(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
 
 
+def state_errors(g, ref, x0=None, F0=None):
+    """Errors of a GPU state against a reference state (oracle).
+
+    g, ref: objects or dicts with x, v, C, F (any present).  Returns
+    - "x": max |dx| / max |x_ref| (positions, relative to their magnitude);
+    - "disp" (with x0): max |dx| / max |x_ref - x0|, the error relative to
+      how far the particles moved.  Positions near 0.5 m hide a displacement
+      error of several percent behind a 1e-4 position bar; this does not;
+    - "v", "C": max |d| / max |ref|;
+    - "F": max |dF| / max |F_ref| and, with F0, "F_strain": max |dF| /
+      max |F_ref - F0| (relative to the deformation itself)."""
+    get = (lambda o, k: o[k]) if isinstance(g, dict) else getattr
+    getr = (lambda o, k: o[k]) if isinstance(ref, dict) else getattr
+    out = {}
+    for k in ("x", "v", "C", "F"):
+        try:
+            a = np.asarray(get(g, k), np.float64)
+            b = np.asarray(getr(ref, k), np.float64)
+        except (KeyError, AttributeError):
+            continue
+        d = np.max(np.abs(a - b)) if a.size else 0.0
+        out[k] = float(d / max(np.max(np.abs(b)) if b.size else 0.0, 1e-30))
+        if k == "x" and x0 is not None:
+            out["disp"] = float(d / max(np.max(np.abs(b - np.asarray(x0, np.float64))), 1e-30))
+        if k == "F" and F0 is not None:
+            out["F_strain"] = float(d / max(np.max(np.abs(b - np.asarray(F0, np.float64))), 1e-30))
+    return out
+
+
+def fmt_errs(e):
+    return " ".join(f"{k} {v:.2e}" for k, v in e.items())
+
+
 def gpu_collider(c):
     """The empm_collider of an oracle collider (same fields, no mask)."""
     from empm_amd._lib import Collider

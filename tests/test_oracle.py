@@ -466,3 +466,170 @@ def test_float32_and_float64_oracles_agree_on_a_rollout():
         res[prec] = s
     assert np.abs(res["f32"].x - res["f64"].x).max() < 1e-5
     assert np.abs(res["f32"].F - res["f64"].F).max() < 1e-4
+
+
+# ---------------------------------------------------------------- branches on the caller's path
+def _nh_diag_tau(s, mu, lam):
+    """Kirchhoff stress of kirchoff_stress_neoHookean (mpm_utils.py:18-47) on a
+    diagonal F = diag(s): U = V = I, tau = diag(t_i s_i) with
+    t_i = mu J^(-2/3) (s_i^2 - mean s^2) + lam/2 (J^2 - 1)."""
+    J = np.prod(s)
+    b = s * s
+    t = mu * J ** (-2.0 / 3.0) * (b - b.mean()) + lam / 2 * (J * J - 1)
+    return np.diag(t * s)
+
+
+@pytest.mark.parametrize("scale", [0.98, 0.5, 1.0, 1.1])
+def test_grid_damping_closed_form(scale):
+    """add_damping_via_grid (mpm_utils.py:722-734) as the substep applies it
+    (mpm_solver_diff.py:244-251: only when scale < 1): after normalisation
+    and gravity, v_out = v - (1 - s) v = s (mv/m + dt g).  Fails for damping
+    before gravity, for a factor (1 - s) in place of s, or for a scale >= 1
+    that is applied anyway."""
+    n, dt = 16, 1e-3
+    g = np.array([0.5, 0.0, -9.8])
+    P = orc.make_params(1, n, gravity=tuple(g), grid_v_damping_scale=scale)
+    gf = np.array([P.gravity[0], P.gravity[1], P.gravity[2]])
+    v = orc.grid_update(P, [], 0.0, dt, _uniform_grid(n))
+    base = np.array([1.0, -2.0, 3.0]) + dt * gf
+    expect = scale * base if scale < 1 else base
+    assert np.abs(v - expect).max() < 1e-14
+    # empty nodes stay at zero (the mass threshold of :432)
+    empty = np.zeros((n, n, n, 4))
+    assert np.all(orc.grid_update(P, [], 0.0, dt, empty) == 0)
+
+
+def test_g2p_position_clamp_lands_on_the_bound():
+    """g2p_differentiable's clamp (mpm_utils.py:553-565): x' = clamp(x + dt v',
+    2 dx, lim - 2 dx) per axis.  A uniform velocity field without stress or
+    gravity gives v' = v exactly, so particles pushed past either bound land
+    on it and the others move by dt v.  Fails for bounds of dx or 3 dx, or
+    for a clamp applied before the velocity update."""
+    n, dt = 16, 1e-3
+    dx = 1.0 / n
+    v0 = np.array([5.0, -5.0, 0.1])
+    x = np.array([[1 - 2 * dx - 0.002, 0.5, 0.5],    # x crosses the upper bound
+                  [0.5, 2 * dx + 0.002, 0.5],         # y crosses the lower bound
+                  [0.5, 0.5, 0.5],                    # stays inside
+                  [1 - 2 * dx - 0.002, 2 * dx + 0.002, 0.5]])
+    N = len(x)
+    s = orc.OracleState(x, np.tile(v0, (N, 1)), np.zeros((N, 3, 3)), np.tile(np.eye(3), (N, 1, 1)),
+                        np.full(N, 3e3), np.full(N, 5e3), np.full(N, 1e9), np.full(N, 1e-4), np.full(N, 1e-7))
+    P = orc.make_params(N, n, material=4, gravity=(0, 0, 0))
+    o = orc.substep(P, [], 0.0, dt, s)
+    assert np.abs(o.v - v0).max() < 1e-12
+    lo, hi = 2 * np.float32(1.0 / n), 1.0 - 2 * np.float32(1.0 / n)
+    expect = np.clip(x + dt * v0, lo, hi)
+    assert np.abs(o.x - expect).max() < 1e-12
+    assert o.x[0, 0] == hi and o.x[1, 1] == lo and o.x[3, 0] == hi and o.x[3, 1] == lo
+    assert np.abs(o.x[2] - (x[2] + dt * v0)).max() < 1e-12
+
+
+@pytest.mark.parametrize("r", [0.0, 0.3, 1.0, -0.0005, -0.01])
+def test_rpic_blend_closed_form(r):
+    """The RPIC blend of p2g_apic_with_stress (mpm_utils.py:396-411):
+    C~ = (1 - r) C + r/2 (C - C^T); C~ = 0 when r < -0.001 (PIC).  One
+    particle at rest without stress scatters mv_i = w_i m C~ (x_i - x_p); the
+    quadratic B-spline's second moment sum_i w_i d_i d_i^T = dx^2/4 I then
+    recovers C~ = 4 / (m dx^2) sum_i mv_i d_i^T from the P2G grid.  r = 1
+    gives the skew part (C - C^T)/2; a transposed or sign-flipped blend fails."""
+    n = 16
+    dx = 1.0 / n
+    rng = np.random.Generator(np.random.PCG64(41))
+    A = rng.normal(size=(3, 3))
+    xp = np.array([[0.4713, 0.5291, 0.5077]])
+    m = 2.5e-4
+    s = orc.OracleState(xp, np.zeros((1, 3)), A[None], np.eye(3)[None], np.full(1, 3e3), np.full(1, 5e3),
+                        np.full(1, 1e9), np.full(1, m), np.full(1, 1e-7))
+    P = orc.make_params(1, n, material=4, gravity=(0, 0, 0), rpic_damping=r)
+    _, _, _, grid = orc.substep(P, [], 0.0, 1e-4, s, return_fe=True)
+    idx = np.argwhere(grid[..., 3] > 0)
+    assert len(idx) == 27
+    d = idx * P.dx - xp[0]
+    Ct = 4.0 / (m * dx * dx) * np.einsum("ki,kj->ij", grid[idx[:, 0], idx[:, 1], idx[:, 2], :3], d)
+    expect = np.zeros((3, 3)) if r < -0.001 else (1 - r) * A + r / 2 * (A - A.T)
+    assert np.abs(Ct - expect).max() < 1e-9 * max(1.0, np.abs(A).max())
+    if r == 1.0:
+        assert np.abs(Ct + Ct.T).max() < 1e-9
+
+
+@pytest.mark.parametrize("material", [1, 5])
+def test_hardening_update_closed_form(material):
+    """hardening == 1 (mpm_utils.py:124-127 for material 1, :178-181 after the
+    softening of material 5): Y' = Y (- softening |dgamma/n eps_hat|) +
+    2 mu' xi dgamma, dgamma = n - Y / (2 mu), n = |eps_hat| + 1e-6, eps the log
+    of the clamped singular values (numpy's SVD here).  The elastic F is the
+    one without hardening; fails for a missing factor 2, a dgamma/n in place
+    of dgamma, or a hardening that also moves F_e."""
+    mu, lam, ys, xi = 3e4, 5e4, 400.0, 0.35
+    F = rand_rot(RNG) @ np.diag([1.12, 0.93, 0.98]) @ rand_rot(RNG).T
+    P0 = P_(material, hardening=0.0, xi=xi, softening=0.1)
+    P1 = P_(material, hardening=1.0, xi=xi, softening=0.1)
+    Fe0, _, _, _, ys0 = orc.constitutive(material, F, mu, lam, ys, P0)
+    Fe1, _, mu1, lam1, ys1 = orc.constitutive(material, F, mu, lam, ys, P1)
+    e = np.log(np.maximum(np.linalg.svd(F, compute_uv=False), 0.01))
+    eh = e - e.mean()
+    n = np.linalg.norm(eh) + 1e-6
+    dg = n - ys / (2 * mu)
+    assert 2 * mu * np.linalg.norm(eh) > ys                      # the state yields
+    soft = 0.1 * np.linalg.norm(dg / n * eh) if material == 5 else 0.0
+    assert abs(ys0 - (ys - soft)) < 1e-9 * ys
+    assert abs(ys1 - (ys - soft + 2 * mu1 * xi * dg)) < 1e-9 * ys
+    assert mu1 == mu and lam1 == lam
+    assert np.abs(Fe1 - Fe0).max() < 1e-14
+
+
+def test_material7_is_the_von_mises_return_with_the_nh_stress():
+    """Id 7 (Neo-Hookean + von Mises, the workload's material; not a reference
+    id) composes von_mises_return_mapping (mpm_utils.py:87-130) with
+    kirchoff_stress_neoHookean (:18-47).  Below yield: F_e = F and tau is the
+    NH closed form on a diagonal F.  Above yield on a diagonal F: F_e =
+    diag(exp(eps - dgamma/n eps_hat)) and tau the NH closed form at F_e.  An
+    id 7 that dispatched to the StVK or FCR stress, or skipped the return
+    map, fails."""
+    mu, lam = 3e4, 5e4
+    s = np.array([1.06, 0.97, 0.99])
+    Fe, tau, *_ = orc.constitutive(7, np.diag(s), mu, lam, 1e9, P_(7))
+    assert np.abs(Fe - np.diag(s)).max() < 1e-15
+    ref = _nh_diag_tau(s, mu, lam)
+    assert np.abs(tau - ref).max() < 1e-9 * np.abs(ref).max()
+    ys = 300.0
+    Fe, tau, mu_o, lam_o, ys_o = orc.constitutive(7, np.diag(s), mu, lam, ys, P_(7))
+    e = np.log(s)
+    eh = e - e.mean()
+    n = np.linalg.norm(eh) + 1e-6
+    dg = n - ys / (2 * mu)
+    assert 2 * mu * np.linalg.norm(eh) > ys
+    se = np.exp(e - dg / n * eh)
+    assert np.abs(Fe - np.diag(se)).max() < 1e-12
+    ref = _nh_diag_tau(se, mu, lam)
+    assert np.abs(tau - ref).max() < 1e-9 * np.abs(ref).max()
+    assert (mu_o, lam_o, ys_o) == (mu, lam, ys)
+    # and the same F_e as id 1 on a general F
+    F = rand_rot(RNG) @ np.diag([1.1, 0.92, 1.0]) @ rand_rot(RNG).T
+    Fe7, *_ = orc.constitutive(7, F, mu, lam, ys, P_(7))
+    Fe1, *_ = orc.constitutive(1, F, mu, lam, ys, P_(1))
+    assert np.abs(Fe7 - Fe1).max() == 0
+
+
+def test_frozen_grid_mass_substep():
+    """oracle_substep_frozen_mass at the substep's own grid mass is the plain
+    substep (up to the order of the OpenMP atomics in P2G); at a scaled mass
+    the grid velocities (hence v') of a stress-free, gravity-free state scale
+    by its inverse."""
+    rng = np.random.Generator(np.random.PCG64(8))
+    N, n = 200, 16
+    s = _state(N, n, rng)
+    P = orc.make_params(N, n, material=4, gravity=(0, 0, 0))
+    o, _, _, grid = orc.substep(P, [], 0.0, 1e-4, s.copy(), return_fe=True)
+    o2 = orc.substep(P, [], 0.0, 1e-4, s.copy(), m_fixed=grid[..., 3].copy())
+    for k in ("x", "v", "C", "F"):
+        assert np.abs(getattr(o, k) - getattr(o2, k)).max() < 1e-13 * max(1.0, np.abs(getattr(o, k)).max())
+    o3 = orc.substep(P, [], 0.0, 1e-4, s.copy(), m_fixed=2.0 * grid[..., 3])
+    assert np.abs(o3.v - o.v / 2).max() < 1e-12 * np.abs(o.v).max()
+    # a rollout fed its own mass history reproduces itself
+    mh = np.zeros((5, n ** 3))
+    a, b = s.copy(), s.copy()
+    orc.rollout(P, [], 0.0, 1e-4, 5, a, m_hist_out=mh)
+    orc.rollout(P, [], 0.0, 1e-4, 5, b, m_hist_in=mh)
+    assert np.abs(a.x - b.x).max() < 1e-13 and np.abs(a.F - b.F).max() < 1e-13
