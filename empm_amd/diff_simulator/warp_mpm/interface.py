@@ -155,6 +155,7 @@ class MPMDifferentiableSimulationTrail(autograd.Function):
         ctx.mpm_solver, ctx.mpm_model, ctx.state, ctx.tape, ctx.ro = mpm_solver, mpm_model, state, tape, ro
         ctx.agg_E, ctx.agg_nu, ctx.record = agg_E, agg_nu, record
         ctx.n = state.n_particles
+        ctx.warmup = int(extra_no_grad_steps)
         ctx.save_for_backward(density_change_mask)
         return torch.stack(xs, dim=0)
 
@@ -180,12 +181,27 @@ class MPMDifferentiableSimulationTrail(autograd.Function):
         d0, grads = ctx.ro.backward(ctx.tape, d_final, seeds=seeds, flags=_flags(solver))
         solver.last_grads = grads
         E_grad, nu_grad = _param_grads(model, grads, ctx.agg_E, ctx.agg_nu, n)
-        velo_grad = d0.v.clone()
-        density_grad = grads.mass * ctx.state.particle_vol
-        if density_change_mask is not None:
-            density_grad = density_grad[density_change_mask.to(torch.bool)]
+        velo_grad, density_grad = _initial_state_grads(ctx, d0, grads, density_change_mask)
         return (None, None, None, None, None, None, None, velo_grad, E_grad, nu_grad, density_grad, None, None,
                 None, None, None)
+
+
+def _initial_state_grads(ctx, d0, grads, density_change_mask):
+    """init_velocity's and the density's gradients of the Trail and
+    Simulation variants.  The reference reads them from the caller's state
+    (mpm_state.particle_v.grad, .particle_density.grad: interface.py:201-204,
+    236-241, 421-424, 456-461).  With extra_no_grad_steps > 0 the tape starts
+    at the clone left by the warm-up steps (:112-117, 331-336), so nothing
+    reaches the caller's state and both gradients stay zero."""
+    if ctx.warmup > 0:
+        velo_grad = torch.zeros_like(d0.v)
+        density_grad = torch.zeros_like(grads.mass)
+    else:
+        velo_grad = d0.v.clone()
+        density_grad = grads.mass * ctx.state.particle_vol
+    if density_change_mask is not None:
+        density_grad = density_grad[density_change_mask.to(torch.bool)]
+    return velo_grad, density_grad
 
 
 def _state_from(template, ps: rt.ParticleState):
@@ -225,6 +241,7 @@ class MPMDifferentiableSimulation(autograd.Function):
                                           requires_grad, step_callback=False)
         ctx.mpm_solver, ctx.mpm_model, ctx.state, ctx.tape, ctx.ro = mpm_solver, mpm_model, state, tape, ro
         ctx.agg_E, ctx.agg_nu, ctx.n = agg_E, agg_nu, state.n_particles
+        ctx.warmup = int(extra_no_grad_steps)
         ctx.save_for_backward(density_change_mask)
         return final.x.clone()
 
@@ -237,10 +254,8 @@ class MPMDifferentiableSimulation(autograd.Function):
         d0, grads = ctx.ro.backward(ctx.tape, d_final, flags=_flags(solver))
         solver.last_grads = grads
         E_grad, nu_grad = _param_grads(model, grads, ctx.agg_E, ctx.agg_nu, n)
-        density_grad = grads.mass * ctx.state.particle_vol
-        if density_change_mask is not None:
-            density_grad = density_grad[density_change_mask.to(torch.bool)]
-        return (None, None, None, None, None, None, None, d0.v.clone(), E_grad, nu_grad, density_grad, None, None,
+        velo_grad, density_grad = _initial_state_grads(ctx, d0, grads, density_change_mask)
+        return (None, None, None, None, None, None, None, velo_grad, E_grad, nu_grad, density_grad, None, None,
                 None, None, None)
 
 
@@ -265,11 +280,17 @@ def _continue_forward(ctx, mpm_solver, mpm_state, mpm_model, substep_size, num_s
     return final
 
 
-def _continue_backward(ctx, out_pos_grad, out_velo_grad, out_F_grad, out_C_grad):
+def _continue_backward(ctx, out_pos_grad, out_velo_grad, out_F_grad, out_C_grad, seed_position_only=False):
+    """Shared backward of WCheckpoint and Clean.  WCheckpoint seeds the
+    cotangents of all four outputs (interface.py:621-720); the reference's
+    Clean seeds only the position (:907-927) and ignores out_velo_grad,
+    out_F_grad and out_C_grad, which seed_position_only reproduces."""
     solver, model, n = ctx.mpm_solver, ctx.mpm_model, ctx.n
     dev = out_pos_grad.device
     d = _zeros_like_state(n, dev)
     d.x.copy_(out_pos_grad)
+    if seed_position_only:
+        out_velo_grad = out_F_grad = out_C_grad = None
     if out_velo_grad is not None:
         d.v.copy_(out_velo_grad)
     if out_F_grad is not None:
@@ -321,4 +342,8 @@ class MPMDifferentiableSimulationClean(autograd.Function):
 
     @staticmethod
     def backward(ctx, out_pos_grad, out_velo_grad, out_F_grad, out_C_grad, out_cov_grad):
-        return _continue_backward(ctx, out_pos_grad, out_velo_grad, out_F_grad, out_C_grad)
+        # reference_compat: only the position seed, as the reference (interface.py:907-927);
+        # exact: the derivative of everything the caller differentiated
+        compat = _flags(ctx.mpm_solver) == EMPM_ADJ_DROP_GRID_MASS
+        return _continue_backward(ctx, out_pos_grad, out_velo_grad, out_F_grad, out_C_grad,
+                                  seed_position_only=compat)

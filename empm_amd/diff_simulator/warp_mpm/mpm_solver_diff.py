@@ -89,6 +89,11 @@ class MPMWARPDiff:
         # substep (no recompute); K > 1 keeps a state every K substeps and
         # replays each segment in the backward
         self.checkpoint_every = 1
+        # gradient convention of the autograd Functions: "reference_compat"
+        # reproduces the reference Tape's (grid-mass adjoint dropped, Trail
+        # frames seeded at the last state, Clean seeded by the position
+        # only); "exact" is the derivative of the discrete rollout
+        self.adjoint_mode = "reference_compat"
         self._ctx = None
         self._ctx_key = None
 

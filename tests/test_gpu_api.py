@@ -170,6 +170,117 @@ def test_trail_gradients_exact_mode_match_fd():
     assert abs(gnu - fdnu / N) < 2e-3 * abs(fdnu / N)
 
 
+def test_trail_reference_compat_matches_frozen_mass_fd():
+    """The default adjoint mode (reference_compat): the Trail sums its frames'
+    output gradients onto the last state (interface.py:176-197) and the grid
+    mass carries no adjoint (mpm_data_structure.py:100-105,
+    warp_utils.py:116-120).  With a loss sum_f <G_f, pos_f> of fixed G_f, the
+    gradient the reference's Tape returns is d/dE of <sum_f G_f, x_T(E)> for
+    the rollout whose every normalisation divides by the grid mass of the
+    unperturbed run: central differences of FP64 oracle rollouts fed that
+    mass history (bar 1e-3).  Scalar E enters x1e6 and comes back as the mean
+    of per-particle physical-E gradients (interface.py:68-70,207-232)."""
+    sc = scenes.small_scene(n_particles=250, n_grid=16, material=6, seed=2, E=3e3, dt=2e-4, center=(0.5, 0.5, 0.27),
+                            vel=(0.5, 0.0, -2.0), spin=1.0)
+    mpm = _trail_setup(sc)
+    assert mpm.mpm_solver.adjoint_mode == "reference_compat"
+    mpm.mpm_solver.checkpoint_every = 4
+    N, steps, every = sc.n, 30, 10
+    dev = "cuda:0"
+    x0 = torch.as_tensor(sc.x, device=dev)
+    v0 = torch.as_tensor(sc.params["v0"], device=dev)
+    G = torch.as_tensor(np.random.default_rng(5).standard_normal((3, N, 3)), dtype=torch.float32, device=dev)
+    rho = torch.full((N,), 1000.0, device=dev)
+    E = torch.tensor(0.003, device=dev, requires_grad=True)
+    nu = torch.tensor(0.3, device=dev, requires_grad=True)
+    mpm.mpm_solver.time = 0.0
+    pos = MPMDifferentiableSimulationTrail.apply(mpm.mpm_solver, mpm.mpm_state, mpm.mpm_model, every, sc.dt, steps,
+                                                x0, v0, E, nu, rho, None, None, dev, True, 0)
+    (pos * G).sum().backward()
+    gE, gnu = float(E.grad), float(nu.grad)
+    Gsum = G.sum(0).double().cpu().numpy()
+    P = orc.make_params(N, 16, material=6, gravity=(0, 0, -9.8))
+    cols = [orc.surface_collider(c["point"], c["normal"]) for c in sc.colliders]
+
+    def J(Ephys, nu_, m_hist_in=None, m_hist_out=None):
+        mu = np.full(N, Ephys / (2 * (1 + nu_)))
+        lam = np.full(N, Ephys * nu_ / ((1 + nu_) * (1 - 2 * nu_)))
+        o = orc.OracleState(sc.x, sc.params["v0"], np.zeros((N, 3, 3)), np.tile(np.eye(3), (N, 1, 1)), mu, lam,
+                            sc.yield_stress, sc.vol * sc.rho, sc.vol)
+        orc.rollout(P, cols, 0.0, sc.dt, steps, o, m_hist_in=m_hist_in, m_hist_out=m_hist_out)
+        return float(np.sum(Gsum * o.x))
+
+    Ep = 0.003e6
+    mh = np.zeros((steps, 16 ** 3))
+    J(Ep, 0.3, m_hist_out=mh)
+    h = 1e-4
+    fdE = (J(Ep * (1 + h), 0.3, mh) - J(Ep * (1 - h), 0.3, mh)) / (2 * h * Ep)
+    fdnu = (J(Ep, 0.3 + 1e-5, mh) - J(Ep, 0.3 - 1e-5, mh)) / 2e-5
+    exE = (J(Ep * (1 + h), 0.3) - J(Ep * (1 - h), 0.3)) / (2 * h * Ep)
+    print(f"[trail compat] E.grad {gE:.6e} vs frozen-mass FD/N {fdE / N:.6e} (exact FD/N {exE / N:.6e}); "
+          f"nu.grad {gnu:.6e} vs {fdnu / N:.6e}", flush=True)
+    assert abs(gE - fdE / N) < 1e-3 * abs(fdE / N)
+    assert abs(gnu - fdnu / N) < 1e-3 * abs(fdnu / N)
+
+
+def test_warmup_steps_leave_initial_velocity_and_density_gradients_zero():
+    """extra_no_grad_steps > 0: the reference's tape starts at the warm-up's
+    last clone, so init_velocity's and the density's gradients (read from the
+    caller's state, interface.py:201-204,236-241) stay zero, while E and nu
+    still get theirs."""
+    sc = scenes.small_scene(n_particles=200, n_grid=16, material=6, seed=7, E=3e3, center=(0.5, 0.5, 0.3),
+                            vel=(0.2, 0.0, -1.5))
+    mpm = _trail_setup(sc)
+    N, dev = sc.n, "cuda:0"
+    x0 = torch.as_tensor(sc.x, device=dev)
+    for fn, extra in ((MPMDifferentiableSimulationTrail, (5,)), (MPMDifferentiableSimulation, (1,))):
+        for warm in (0, 3):
+            v0 = torch.as_tensor(sc.params["v0"], device=dev).clone().requires_grad_()
+            E = torch.tensor(0.003 if fn is MPMDifferentiableSimulationTrail else 3e3, device=dev,
+                             requires_grad=True)
+            nu = torch.tensor(0.3, device=dev, requires_grad=True)
+            rho = torch.full((N,), 1000.0, device=dev, requires_grad=True)
+            mpm.mpm_solver.time = 0.0
+            out = fn.apply(mpm.mpm_solver, mpm.mpm_state, mpm.mpm_model, *extra, sc.dt, 10, x0, v0, E, nu, rho, None,
+                           None, dev, True, warm)
+            (out * out).sum().backward()
+            print(f"[warm-up] {fn.__name__} extra={warm}: |v0.grad| {float(v0.grad.abs().max()):.3e} "
+                  f"|rho.grad| {float(rho.grad.abs().max()):.3e} E.grad {float(E.grad):.3e}", flush=True)
+            assert torch.isfinite(E.grad) and float(E.grad) != 0.0
+            if warm:
+                assert float(v0.grad.abs().max()) == 0.0 and float(rho.grad.abs().max()) == 0.0
+            else:
+                assert float(v0.grad.abs().max()) > 0.0
+
+
+def test_clean_reference_compat_seeds_only_the_position():
+    """The reference's Clean backward seeds the position alone
+    (interface.py:907-927): a loss on the returned velocity has zero
+    gradient there in reference_compat mode, and its true gradient in exact
+    mode."""
+    sc = scenes.small_scene(n_particles=200, n_grid=16, material=0, seed=8, E=2e4, center=(0.5, 0.5, 0.3),
+                            vel=(0.2, 0.0, -1.5))
+    mpm = _trail_setup(sc, material="jelly")
+    N, dev = sc.n, "cuda:0"
+    x0 = torch.as_tensor(sc.x, device=dev)
+    v0 = torch.as_tensor(sc.params["v0"], device=dev)
+    F0 = torch.eye(3, device=dev).repeat(N, 1, 1)
+    C0 = torch.zeros((N, 3, 3), device=dev)
+    rho = torch.full((N,), 1000.0, device=dev)
+    res = {}
+    for mode in ("reference_compat", "exact"):
+        mpm.mpm_solver.adjoint_mode = mode
+        E = torch.full((N,), 2e4, device=dev, requires_grad=True)
+        nu = torch.full((N,), 0.3, device=dev)
+        mpm.mpm_solver.time = 0.0
+        outs = MPMDifferentiableSimulationClean.apply(mpm.mpm_solver, mpm.mpm_state, mpm.mpm_model, sc.dt, 10, x0,
+                                                      v0, F0, C0, E, nu, rho, None, dev, True, 0)
+        (outs[1] * outs[1]).sum().backward()
+        res[mode] = float(E.grad.abs().max())
+    print(f"[clean] max |E.grad| of a velocity loss: {res}", flush=True)
+    assert res["reference_compat"] == 0.0 and res["exact"] > 0.0
+
+
 def test_function_variants_shapes_and_consistency():
     sc = scenes.small_scene(n_particles=200, n_grid=16, material=0, seed=5)
     mpm = _trail_setup(sc, material="jelly")

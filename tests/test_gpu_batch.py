@@ -129,3 +129,87 @@ def test_batch_backward_rejects_material_5(dev):
                                             tape=True)
     with pytest.raises(EmpmError):
         ro.backward(tape, rt.ParticleState.empty(bm.n, dev))
+
+
+EDGE_SIMS = [(0, 16, 150, 31), (7, 24, 190, 32), (6, 12, 120, 33)]
+
+
+def _edge_sim(dev, material, n, N, seed):
+    """Random particles of one simulation with stencils across the grid's
+    faces: base -1 in x (x = -0.7 dx, base = trunc(-1.2)) and base n - 2 in z
+    (node n is outside)."""
+    from tests.helpers import random_state
+
+    d = random_state(N, n, seed=seed, material=material, deform=0.1 if material == 7 else 0.05)
+    d["x"][:5, 0] = np.float32(-0.7 / n)
+    d["x"][5:10, 2] = np.float32((n - 1.3) / n)
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev).clone()
+    P = orc.make_params(N, n, material=material, gravity=(0.0, 0.0, -9.8))
+    m = rt.ModelArrays(material=material, n_grid=n, grid_lim=1.0, dx=P.dx, inv_dx=P.inv_dx, mu=t(d["mu"]),
+                       lam=t(d["lam"]), yield_stress=t(d["ys"]), mass=t(d["mass"]), vol=t(d["vol"]),
+                       gravity=(0.0, 0.0, -9.8), alpha=P.alpha)
+    s = rt.ParticleState(t(d["x"]), t(d["v"]), t(d["C"]), t(d["F"]))
+    return d, P, m, s
+
+
+def test_batch_adjoint_with_edge_stencils_matches_alone_and_fd(dev):
+    """Simulations on different grids (16^3, 24^3, 12^3) in one batch, each
+    with stencils across its grid's faces: the adjoint kernels pick the
+    bounds-tested node loads per wave, from the wave's own simulation (its
+    grid size and node offset).  One substep's adjoint in the batch must
+    equal each simulation's alone (1e-4 on the state cotangents, 1e-3 on
+    per-particle mu gradients) and central differences of the FP64
+    oracle in a random direction (1e-3)."""
+    from tests.helpers import oracle_state
+
+    col = orc.surface_collider((0.0, 0.0, 0.3), (0.0, 0.0, 1.0))
+    dt = 1e-4
+    built = [_edge_sim(dev, *a) for a in EDGE_SIMS]
+    models = [b[2] for b in built]
+    bm = rt.BatchModel.concat(models)
+    ctx = rt.Context(max(m.n_grid for m in models), bm.n, dev, max_cand=len(models))
+    ro = rt.Rollout(ctx, bm, bm.n)
+    s0 = cat_states([b[3] for b in built])
+    _, _, _, tape = ro.forward(s0, [gpu_collider(col)], 0.0, dt, 1, tape=True)
+    rng = np.random.Generator(np.random.PCG64(77))
+    w = {k: rng.normal(0, 1, (bm.n,) + sh) for k, sh in (("x", (3,)), ("v", (3,)), ("C", (3, 3)), ("F", (3, 3)))}
+    tt = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=dev)
+    d_out = rt.ParticleState(tt(w["x"]), tt(w["v"]), tt(w["C"]), tt(w["F"]))
+    d0, g = ro.backward(tape, d_out, flags=0)
+    torch.cuda.synchronize()
+    for c, (d, P, m, s) in enumerate(built):
+        a, b = bm.offsets[c], bm.offsets[c + 1]
+        ctx1 = rt.Context(m.n_grid, b - a, dev)
+        r1 = rt.Rollout(ctx1, m, b - a)
+        _, _, _, tp1 = r1.forward(s, [gpu_collider(col)], 0.0, dt, 1, tape=True)
+        d1 = rt.ParticleState(*(getattr(d_out, f)[a:b].clone() for f in ("x", "v", "C", "F")))
+        e0, g1 = r1.backward(tp1, d1, flags=0)
+        torch.cuda.synchronize()
+        errs = {f: rel_err(getattr(d0, f)[a:b].cpu().numpy(), getattr(e0, f).cpu().numpy())
+                for f in ("x", "v", "C", "F")}
+        errs["mu"] = rel_err(g.mu[a:b].cpu().numpy(), g1.mu.cpu().numpy())
+        # directional derivative of <w, step(s)> against the FP64 oracle
+        wc = {k: v[a:b] for k, v in w.items()}
+        u = {k: rng.normal(0, 1, d[k].shape) for k in ("x", "v", "mu")}
+        sc_ = {"x": 1e-3 / m.n_grid, "v": 1e-2, "mu": 1e-3 * d["mu"][:, None]}
+        an = (np.sum(d0.x[a:b].double().cpu().numpy() * u["x"] * sc_["x"])
+              + np.sum(d0.v[a:b].double().cpu().numpy() * u["v"] * sc_["v"])
+              + np.sum(g.mu[a:b].double().cpu().numpy() * u["mu"] * sc_["mu"][:, 0]))
+        eps = 1e-4
+
+        def J(sign):
+            dd = {k: np.asarray(v, np.float64).copy() for k, v in d.items()}
+            dd["x"] = dd["x"] + sign * eps * u["x"] * sc_["x"]
+            dd["v"] = dd["v"] + sign * eps * u["v"] * sc_["v"]
+            dd["mu"] = dd["mu"] + sign * eps * u["mu"] * sc_["mu"][:, 0]
+            o = orc.substep(P, [col], 0.0, dt, oracle_state(dd, "f64"))
+            return sum(float(np.sum(wc[k] * getattr(o, k))) for k in ("x", "v", "C", "F"))
+
+        fd = (J(1) - J(-1)) / (2 * eps)
+        rel = abs(an - fd) / abs(fd)
+        print(f"  [edge batch] simulation {c} n={m.n_grid} material {m.material}: batch vs alone "
+              + " ".join(f"{k} {v:.1e}" for k, v in errs.items()) + f" | adjoint {an:+.6e} fd {fd:+.6e} "
+              f"rel {rel:.1e}", flush=True)
+        # per-particle mu gradients carry the order of the float atomics (see the module docstring)
+        assert max(errs[f] for f in ("x", "v", "C", "F")) < 1e-4 and errs["mu"] < 1e-3, errs
+        assert rel < 1e-3, (an, fd, rel)

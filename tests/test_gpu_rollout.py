@@ -15,7 +15,7 @@ import torch
 from empm_amd import runtime as rt
 from empm_amd import scenes
 from oracle import oracle as orc
-from tests.helpers import gpu_collider, rel_err
+from tests.helpers import fmt_errs, gpu_collider, rel_err, state_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -56,10 +56,10 @@ def test_cube_workload_full_size_forward(dev):
     for prec in ("f32", "f64"):
         o = ostate(prec)
         orc.rollout(P, list(ocols), 0.0, sc.dt, sc.n_steps, o)
-        res[prec] = {k: rel_err(g[k], getattr(o, k)) for k in g}
-    print(f"  rel err vs f32 oracle {res['f32']}\n  rel err vs f64 oracle {res['f64']}", flush=True)
+        res[prec] = state_errors(g, o, x0=sc.x, F0=np.tile(np.eye(3), (sc.n, 1, 1)))
+    print(f"  vs f32 oracle: {fmt_errs(res['f32'])}\n  vs f64 oracle: {fmt_errs(res['f64'])}", flush=True)
     assert abs(t1 - sc.n_steps * sc.dt) < 1e-12
-    assert res["f32"]["x"] < 1e-4 and res["f32"]["F"] < 1e-4
+    assert res["f32"]["x"] < 1e-4 and res["f32"]["F"] < 1e-4 and res["f32"]["disp"] < 1e-3
     assert res["f64"]["x"] < 1e-4 and res["f64"]["F"] < 1e-4
 
 
@@ -75,10 +75,10 @@ def test_small_rollout_every_material(dev, material):
     torch.cuda.synchronize()
     o = ostate("f32")
     orc.rollout(P, list(ocols), 0.0, sc.dt, steps, o)
-    ex = rel_err(final.x.cpu().numpy(), o.x)
-    eF = rel_err(final.F.cpu().numpy(), o.F)
-    print(f"  rel err x {ex:.2e} F {eF:.2e}", flush=True)
-    assert ex < 1e-4 and eF < 1e-4
+    e = state_errors({"x": final.x.cpu().numpy(), "v": final.v.cpu().numpy(), "C": final.C.cpu().numpy(),
+                      "F": final.F.cpu().numpy()}, o, x0=sc.x)
+    print(f"  {fmt_errs(e)}", flush=True)
+    assert e["x"] < 1e-4 and e["F"] < 1e-4 and e["disp"] < 1e-3, e
 
 
 def _loss_grad_gpu(sc, dev, target, prm, steps, ckpt, flags=0):
