@@ -38,6 +38,7 @@ EXPORTED_SYMBOLS = (
     "empm_mass_from_density",
     "empm_nn_sqdist",
     "empm_nn_sqdist_workspace",
+    "empm_nn_sqdist_noalloc",
     "empm_nn_sqdist_segments",
     "empm_nn_sqdist_segments_workspace",
     "empm_substep_fwd",
@@ -202,6 +203,8 @@ def _declare(lib):
     lib.empm_mass_from_density.argtypes = [C.c_int, vp, vp, vp, vp]
     lib.empm_nn_sqdist.restype = C.c_int
     lib.empm_nn_sqdist.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, C.c_size_t, vp]
+    lib.empm_nn_sqdist_noalloc.restype = C.c_int
+    lib.empm_nn_sqdist_noalloc.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp]
     lib.empm_nn_sqdist_workspace.restype = C.c_size_t
     lib.empm_nn_sqdist_workspace.argtypes = [C.c_int, C.c_int]
     lib.empm_nn_sqdist_segments.restype = C.c_int

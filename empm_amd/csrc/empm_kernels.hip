@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -1811,6 +1812,15 @@ __global__ __launch_bounds__(kBlock) void k_nn_sqdist_seg(const int* __restrict_
   }
 }
 
+// Scratch of empm_nn_sqdist_noalloc: the per-split partial results (a float
+// distance and an int index per (split, query) pair), in a device array that
+// the library owns for the life of the process, so that the call allocates
+// nothing.  nn_splits makes s = ceil(2048 / ceil(n / 256)) splits at most, so
+// s * n < 2048 * 256 + n <= 2^20 pairs whenever s > 1 (s > 1 needs
+// n <= 2047 * 256): 8 MiB holds every split plan.
+constexpr size_t kNnScratchPairs = (size_t)1 << 20;
+static __device__ __align__(256) float scratch[2 * kNnScratchPairs];
+
 // Splits of B for n points of A: enough blocks for 8 per CU, at least 512
 // points of B per split.
 int nn_splits(int n, int m, int n_seg = 1) {
@@ -2749,6 +2759,59 @@ int empm_nn_sqdist(int n, const float* a, int m, const float* b, float* dist, in
     hipLaunchKernelGGL(k_nn_sqdist, dim3(nblocks(n), s), dim3(kBlock), 0, st, n, a, m, b, chunk, pd, pi);
     hipLaunchKernelGGL(k_nn_reduce, dim3(nblocks(n)), dim3(kBlock), 0, st, n, s, pd, pi, dist, idx);
   }
+  HIP_TRY(hipGetLastError());
+  return EMPM_OK;
+}
+
+// The device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy of the array at its own address; the
+// array's name is a host-side stand-in that must not reach a kernel).
+static int nn_scratch_ptr(float** out) {
+  static std::mutex mu;
+  static void* ptrs[256] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_err(EMPM_E_HIP, "empm_nn_sqdist_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= 256) return set_err(EMPM_E_INVALID_ARG, "empm_nn_sqdist_noalloc: device %d", dev);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_err(EMPM_E_HIP, "empm_nn_sqdist_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                     hipGetErrorString(e));
+    ptrs[dev] = p;
+  }
+  *out = static_cast<float*>(ptrs[dev]);
+  return EMPM_OK;
+}
+
+int empm_nn_sqdist_noalloc(int n, const float* a, int m, const float* b, float* dist, int* idx, void* stream) {
+  if (n < 0 || m < 0 || (n > 0 && (m < 1 || !a || !b || !dist || !idx)))
+    return set_err(EMPM_E_INVALID_ARG, "empm_nn_sqdist_noalloc: bad arguments (n=%d, m=%d)", n, m);
+  if (n == 0) return EMPM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int s = nn_splits(n, m);
+  if (s == 1) {
+    hipLaunchKernelGGL(k_nn_sqdist, dim3(nblocks(n), 1), dim3(kBlock), 0, st, n, a, m, b, m, dist, idx);
+    HIP_TRY(hipGetLastError());
+    return EMPM_OK;
+  }
+  if ((size_t)s * (size_t)n > kNnScratchPairs)
+    return set_err(EMPM_E_INVALID_ARG,
+                   "empm_nn_sqdist_noalloc: %d splits of %d queries need %zu bytes of scratch, more than its %zu "
+                   "(use empm_nn_sqdist with a workspace)",
+                   s, n, (size_t)s * (size_t)n * 8, kNnScratchPairs * 8);
+  float* base = nullptr;
+  int rc = nn_scratch_ptr(&base);
+  if (rc) return rc;
+  // every (split, query) entry is written by k_nn_sqdist before k_nn_reduce
+  // reads it, so nothing needs clearing between calls
+  float* pd = base;
+  int* pi = reinterpret_cast<int*>(base + (size_t)s * n);
+  const int chunk = (m + s - 1) / s;
+  hipLaunchKernelGGL(k_nn_sqdist, dim3(nblocks(n), s), dim3(kBlock), 0, st, n, a, m, b, chunk, pd, pi);
+  hipLaunchKernelGGL(k_nn_reduce, dim3(nblocks(n)), dim3(kBlock), 0, st, n, s, pd, pi, dist, idx);
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }

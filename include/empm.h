@@ -217,6 +217,18 @@ int empm_nn_sqdist(int n, const float *a, int m, const float *b, float *dist, in
 /* Workspace bytes empm_nn_sqdist uses for n points against m (0: none). */
 size_t empm_nn_sqdist_workspace(int n, int m);
 
+/* empm_nn_sqdist that allocates nothing and takes no workspace: the split
+ * search's partial results live in a library-owned device array of 8 MiB
+ * per GPU (2^20 (split, query) pairs; held until the process ends), which
+ * holds every split plan the search makes, so the result equals
+ * empm_nn_sqdist's with a workspace.  No counterpart in the reference: it is
+ * for callers that cannot allocate (HIP graph capture of the training loss,
+ * serving loops, harnesses that forbid hipMalloc).  Runs on the current
+ * device; `stream` must belong to it.  Calls that overlap on one GPU share
+ * that array and race: order them on one stream.  Returns EMPM_E_INVALID_ARG
+ * for bad arguments or a split plan that would exceed the array. */
+int empm_nn_sqdist_noalloc(int n, const float *a, int m, const float *b, float *dist, int *idx, void *stream);
+
 /* empm_nn_sqdist over n_seg segments in one launch (the frames of the
  * training loss, trainer_warp_mpm.py:743-752): segment g pairs the rows
  * [a_off[g], a_off[g+1]) of a with the rows [b_off[g], b_off[g+1]) of b.
