@@ -338,8 +338,8 @@ def main():
 def bench_population(args, dev):
     """Batched vs sequential evaluation of a 32-candidate CMA-ES population
     (50k-150k particles each, cube / rod / ellipsoid, own 64^3 grids, 300
-    substeps, dt 1e-4): forward-only fitness (sand / plasticine) and
-    fitness + per-candidate gradients (sand / Neo-Hookean + von Mises).
+    substeps, dt 1e-4), sand and plasticine alternating: forward-only
+    fitness and fitness + per-candidate gradients.
     Throughput = sum of particles * 300 / wall time of one evaluation of the
     whole population; the sequential leg runs the same candidates as plain
     rollouts one after another on the same GPU."""
@@ -360,8 +360,8 @@ def bench_population(args, dev):
     out = {"workload": "CMA-ES population: 32 candidates x 50k-150k particles (cube/rod/ellipsoid), own 64^3 grids, "
                        "300 substeps, dt 1e-4"}
     for leg, diff in (("fwd", False), ("fwd_bwd", True)):
-        sc = scenes.population_c3(differentiable=diff)
-        nom = scenes.population_c3(differentiable=diff, nominal=True)
+        sc = scenes.population_c3()
+        nom = scenes.population_c3(nominal=True)
         ev = pop.PopulationEvaluator(sc, dev)
         ev.set_targets([pop.Candidate.of_scene(s) for s in nom])
         cands = [pop.Candidate.of_scene(s) for s in sc]

@@ -12,7 +12,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libempm.so")
 
-EMPM_ABI_VERSION = 4
+EMPM_ABI_VERSION = 5
 EMPM_MAX_COLLIDERS = 16
 
 EMPM_OK = 0
@@ -112,7 +112,7 @@ class Model(C.Structure):
 
 class Particles(C.Structure):
     _fields_ = [("x", C.c_void_p), ("v", C.c_void_p), ("C", C.c_void_p), ("F", C.c_void_p),
-                ("Fe", C.c_void_p)]
+                ("Fe", C.c_void_p), ("mstate", C.c_void_p)]
 
 
 class Batch(C.Structure):

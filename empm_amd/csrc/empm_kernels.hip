@@ -803,12 +803,39 @@ EMPM_DEV void count_bin(bool active, int key, int p, int* __restrict__ count, in
 // return map writes Fe (and, for material 5 / hardening, mu, lam, yield
 // stress in place); S = -dt V tau inv_dx is the stress part of the P2G
 // payload.
+// Whether a simulation's return map rewrites its yield stress, mu and lam
+// (material 5's softening, hardening == 1 of the von Mises maps).
+template <int MAT>
+EMPM_DEV bool mutates_params(const DevModel& M) {
+  return MAT == 5 || ((MAT == 1 || MAT == 7) && M.mp.hardening == 1.f);
+}
+
+// A material-state record (empm_particles.mstate): yield stress, mu and lam,
+// NA floats each (NA = the arena's particle count).
 template <int MAT>
 EMPM_DEV void particle_stress(const DevModel& M, float dt, int p, const float* Ftr, float* Fe_out,
-                              float* __restrict__ S_out, int allow_write) {
-  // material 5 and hardening rewrite the model arrays in place (forward only)
-  const bool write_params = allow_write && (MAT == 5 || M.mp.hardening == 1.f);
-  float mu = M.mu[p], lam = M.lam[p], ys = M.ys[p];
+                              float* __restrict__ S_out, int allow_write, int NA, const float* mst_in,
+                              float* mst_rec) {
+  // material 5 and hardening rewrite the model arrays in place (forward only);
+  // their values at this substep's start come from mst_in when given (a
+  // replay's first substep) and are recorded into mst_rec when given
+  const bool mut = mutates_params<MAT>(M);
+  const bool write_params = allow_write && mut;
+  float mu, lam, ys;
+  if (mut && mst_in) {
+    ys = mst_in[p];
+    mu = mst_in[(size_t)NA + p];
+    lam = mst_in[2 * (size_t)NA + p];
+  } else {
+    mu = M.mu[p];
+    lam = M.lam[p];
+    ys = M.ys[p];
+  }
+  if (mut && mst_rec) {
+    mst_rec[p] = ys;
+    mst_rec[(size_t)NA + p] = mu;
+    mst_rec[2 * (size_t)NA + p] = lam;
+  }
   MatParams matp = M.mp;
   matp.material = MAT;  // compile-time material: one code path, constitutive state stays in registers
   ConstitState st;
@@ -832,12 +859,13 @@ EMPM_DEV void particle_stress(const DevModel& M, float dt, int p, const float* F
 template <int MAT>
 __global__ __launch_bounds__(kBlock) void k_stress(Batch Bt, float dt,
                                                    const float* __restrict__ F, float* __restrict__ Fe_out,
-                                                   float* __restrict__ S_out, int write_params) {
+                                                   float* __restrict__ S_out, int write_params,
+                                                   const float* __restrict__ mst_in, float* __restrict__ mst_rec) {
   per_particle(Bt, MAT, [&](const DevModel& M, const int p) {
   if (M.sel && M.sel[p] != 0) return;
   float Ftr[9];
   load9(F, p, Ftr);
-  particle_stress<MAT>(M, dt, p, Ftr, Fe_out, S_out, write_params);
+  particle_stress<MAT>(M, dt, p, Ftr, Fe_out, S_out, write_params, Bt.N, mst_in, mst_rec);
   });
 }
 
@@ -935,7 +963,8 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
                                                 float* __restrict__ x1, float* __restrict__ v1,
                                                 float* __restrict__ C1, float* __restrict__ F1, int next_stress,
                                                 float* Fe_next, float* __restrict__ S_next, int write_params,
-                                                int write_F, int* __restrict__ count, int2* __restrict__ key_rank) {
+                                                int write_F, int* __restrict__ count, int2* __restrict__ key_rank,
+                                                float* __restrict__ mst_rec_next) {
   // with count set, the counting step of the next substep's binning is done
   // here on the new positions (k_bin_count's work), last: counting in the
   // middle keeps the gather's state live across the stress stage
@@ -1024,7 +1053,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
   load9(Fe, p, Fp);
   m_mul(IF, Fp, Fn);
   if (write_F) store9(F1, p, Fn);
-  if (next_stress) particle_stress<MAT>(M, dt, p, Fn, Fe_next, S_next, write_params);
+  if (next_stress) particle_stress<MAT>(M, dt, p, Fn, Fe_next, S_next, write_params, Bt.N, nullptr, mst_rec_next);
   if (count) count_bin(true, bin_key(xo, M), p, count, key_rank);
   });
 }
@@ -1242,7 +1271,11 @@ __global__ __launch_bounds__(kBlock) void k_grid_adj(Batch Bt, ColTable T, float
   });
 }
 
-template <int MAT>
+// MUT: the simulations of this material may rewrite (yield stress, mu, lam)
+// in their return map (material 5, hardening): those are then state, read
+// from the substep's record `mst` (or the model arrays when it is null), and
+// d_mu, d_lam, d_ys carry their cotangents backwards through the update.
+template <int MAT, bool MUT>
 __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const float* __restrict__ x,
                                                     const float* __restrict__ v, const float* __restrict__ C,
                                                     const float* __restrict__ F, const float4* __restrict__ gadj_in,
@@ -1251,7 +1284,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
                                                     float* __restrict__ dCo, float* __restrict__ dFo,
                                                     float* __restrict__ dmu, float* __restrict__ dlam,
                                                     float* __restrict__ dys, float* __restrict__ dmass,
-                                                    float* __restrict__ dvol) {
+                                                    float* __restrict__ dvol, const float* __restrict__ mst) {
   per_particle(Bt, MAT, [&](const DevModel& M, const int p) {
   if (M.sel && M.sel[p] != 0) {
     float Z[9];
@@ -1380,7 +1413,16 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
   // which does not need them, so that their state is not live during it)
   float Ftr[9];
   load9(F, p, Ftr);
-  float mu = M.mu[p], lam = M.lam[p], ys = M.ys[p];
+  float mu, lam, ys;
+  if (MUT && mst) {
+    ys = mst[p];
+    mu = mst[(size_t)Bt.N + p];
+    lam = mst[2 * (size_t)Bt.N + p];
+  } else {
+    mu = M.mu[p];
+    lam = M.lam[p];
+    ys = M.ys[p];
+  }
   const float mu0 = mu, lam0 = lam, ys0 = ys;
   MatParams matp = M.mp;
   matp.material = MAT;
@@ -1442,6 +1484,31 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
   load9(Febar_in, p, Feb);
   m_zero(Ftrb);
   float mub = 0.f, lamb = 0.f, ysb = 0.f;
+  if constexpr (MUT) {
+    if (mutates_params<MAT>(M)) {
+      // the stress reads the updated (mu, lam); the cotangents of the updated
+      // (yield stress, mu, lam) are d_ys, d_mu, d_lam on entry, and on return
+      // those of the values this substep started from
+      float mub1 = dmu ? dmu[p] : 0.f, lamb1 = dlam ? dlam[p] : 0.f;
+      const float ysb1 = dys ? dys[p] : 0.f;
+      stress_adj(st, MAT, mu, lam, taur, Feb, mub1, lamb1);
+      return_map_adj(st, Ftr, mu0, lam0, ys0, matp, dt, Feb, Ftrb, mub, lamb, ysb);
+      vm_state_adj(st, mu0, ys0, mu, matp, MAT == 5, ysb1, mub1, lamb1, Ftrb, ysb, mub, lamb);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        dx[3 * p + i] = xo[i];
+        dv[3 * p + i] = vb[i];
+      }
+      store9(dCo, p, Cb);
+      store9(dFo, p, Ftrb);
+      if (dmu) dmu[p] = mub;
+      if (dlam) dlam[p] = lamb;
+      if (dys) dys[p] = ysb;
+      if (dmass) dmass[p] += mb;
+      if (dvol) dvol[p] += Vb;
+      return;
+    }
+  }
   stress_adj(st, MAT, mu0, lam0, taur, Feb, mub, lamb);
   return_map_adj(st, Ftr, mu0, lam0, ys0, matp, dt, Feb, Ftrb, mub, lamb, ysb);
 #pragma unroll
@@ -1932,6 +1999,7 @@ struct HostBatch {
   long long G = 0;
   int nwaves = 0;
   bool has_material[8] = {};
+  bool mutates[8] = {};     // a simulation of this material rewrites (yield stress, mu, lam)
   std::vector<int2> mlist;  // Batch::mlist, material-major
   int moff[9] = {};
   int mwaves[8] = {};
@@ -2027,11 +2095,15 @@ int build_batch(const empm_ctx* ctx, const empm_batch* b, HostBatch& hb) {
 // the total, and the per-material lists.
 void index_waves(HostBatch& hb) {
   int w = 0;
-  for (int m = 0; m < 8; ++m) hb.has_material[m] = false;
+  for (int m = 0; m < 8; ++m) hb.has_material[m] = hb.mutates[m] = false;
   for (DevModel& d : hb.cand) {
     d.w0 = w;
     w += (d.N + kWaveSize - 1) / kWaveSize;
-    if (d.N > 0) hb.has_material[d.material] = true;
+    if (d.N > 0) {
+      hb.has_material[d.material] = true;
+      if (d.material == 5 || ((d.material == 1 || d.material == 7) && d.mp.hardening == 1.f))
+        hb.mutates[d.material] = true;
+    }
   }
   hb.nwaves = w;
   hb.mlist.clear();
@@ -2138,6 +2210,32 @@ int make_table(const empm_collider* cols, int ncol, ColTable& T) {
                       (hb).dev, __VA_ARGS__);                                                   \
     }
 
+// k_p2g_adj<mat, mutating> over the particle waves of each material present;
+// the mutating variant only for materials 1, 5 and 7 when one of their
+// simulations rewrites its parameters
+#define EMPM_LAUNCH_P2G_ADJ(hb, stream, ...)                                                     \
+  for (int mat_ = 0; mat_ < 8; ++mat_)                                                          \
+    if ((hb).has_material[mat_]) {                                                              \
+      const dim3 g_(material_grid(hb, mat_)), b_(kBlock);                                       \
+      const bool mut_ = (hb).mutates[mat_];                                                     \
+      switch (mat_) {                                                                           \
+        case 0: hipLaunchKernelGGL((k_p2g_adj<0, false>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__); break; \
+        case 1:                                                                                 \
+          if (mut_) hipLaunchKernelGGL((k_p2g_adj<1, true>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__);     \
+          else hipLaunchKernelGGL((k_p2g_adj<1, false>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__);         \
+          break;                                                                                \
+        case 2: hipLaunchKernelGGL((k_p2g_adj<2, false>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__); break; \
+        case 3: hipLaunchKernelGGL((k_p2g_adj<3, false>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__); break; \
+        case 4: hipLaunchKernelGGL((k_p2g_adj<4, false>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__); break; \
+        case 5: hipLaunchKernelGGL((k_p2g_adj<5, true>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__); break;  \
+        case 6: hipLaunchKernelGGL((k_p2g_adj<6, false>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__); break; \
+        default:                                                                                \
+          if (mut_) hipLaunchKernelGGL((k_p2g_adj<7, true>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__);     \
+          else hipLaunchKernelGGL((k_p2g_adj<7, false>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__);         \
+          break;                                                                                \
+      }                                                                                         \
+    }
+
 size_t pad4(size_t k) { return (k + 3) & ~(size_t)3; }
 
 // Layout of one binning record: [perm N | offs 8 nblk + 1 | act 2 + nblk |
@@ -2209,7 +2307,8 @@ int do_bin(empm_ctx* ctx, const HostBatch& hb, const float* x, const BinBuf& out
 // when given).  `first`: run the stress stage here (else the previous
 // substep's G2P did); `last`: no stress stage for a next substep.
 int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time, float dt, const empm_particles* in,
-               const empm_particles* out, const empm_grid_rec* rec, bool first, bool last, hipStream_t st) {
+               const empm_particles* out, const empm_grid_rec* rec, bool first, bool last, hipStream_t st,
+               bool replay = false) {
   float* Fe = in->Fe ? in->Fe : ctx->Fe;
   float* Fe_next = out->Fe ? out->Fe : ctx->Fe;
   float4* keep = rec ? reinterpret_cast<float4*>(rec->mv_m) : nullptr;
@@ -2220,7 +2319,8 @@ int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
   const Binning B = binning(bb, hb.nblk);
   if (hb.N > 0 && first) {
     ProfScope ps(ctx, EMPM_KERNEL_STRESS, st);
-    EMPM_LAUNCH_RANGES(hb, k_stress, st, dt, in->F, Fe, ctx->S, 1);
+    // a replay starts from the material state recorded with its first state
+    EMPM_LAUNCH_RANGES(hb, k_stress, st, dt, in->F, Fe, ctx->S, 1, replay ? in->mstate : nullptr, in->mstate);
   }
   if (hb.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
@@ -2240,7 +2340,7 @@ int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
     // a next substep also gets its binning's counting step from here
     EMPM_LAUNCH_RANGES(hb, k_g2p, st, dt, in->x, Fe, v_out, out->x, out->v, out->C, out->F, last ? 0 : 1, Fe_next,
                        ctx->S, 1, (last || rec) ? 1 : 0, last ? nullptr : ctx->count,
-                       last ? nullptr : ctx->key_rank);
+                       last ? nullptr : ctx->key_rank, last ? nullptr : out->mstate);
     if (!last) ctx->counts_pending = true;
   }
   HIP_TRY(hipGetLastError());
@@ -2267,7 +2367,7 @@ int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
   if (!recorded) {
     if (hb.N > 0) {
       ProfScope ps(ctx, EMPM_KERNEL_STRESS, st);
-      EMPM_LAUNCH_RANGES(hb, k_stress, st, dt, in->F, Fe, ctx->S, 0);
+      EMPM_LAUNCH_RANGES(hb, k_stress, st, dt, in->F, Fe, ctx->S, 0, in->mstate, nullptr);
     }
     if (hb.N > 0) {
       ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
@@ -2299,23 +2399,29 @@ int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
   }
   if (hb.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G_ADJ, st);
-    EMPM_LAUNCH_RANGES(hb, k_p2g_adj, st, dt, in->x, in->v, in->C, in->F, ctx->gadj_in, ctx->xbar, ctx->Febar,
-                       d_in->x, d_in->v, d_in->C, d_in->F, d_mu, d_lam, d_ys, d_mass, d_vol);
+    EMPM_LAUNCH_P2G_ADJ(hb, st, dt, in->x, in->v, in->C, in->F, ctx->gadj_in, ctx->xbar, ctx->Febar, d_in->x,
+                        d_in->v, d_in->C, d_in->F, d_mu, d_lam, d_ys, d_mass, d_vol, in->mstate);
   }
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }
 
-int check_adj_supported(const HostBatch& hb) {
-  for (size_t c = 0; c < hb.cand.size(); ++c) {
-    const DevModel& d = hb.cand[c];
-    if (d.material == 5)
-      return set_err(EMPM_E_UNSUPPORTED,
-                     "adjoint of material 5 (plasticine) is not supported: it rewrites yield stress, mu and lam "
-                     "inside the substep");
-    if (d.mp.hardening == 1.f && (d.material == 1 || d.material == 7))
-      return set_err(EMPM_E_UNSUPPORTED, "adjoint with hardening == 1 is not supported (yield stress is state)");
-  }
+bool any_mutates(const HostBatch& hb) {
+  for (int m = 0; m < 8; ++m)
+    if (hb.mutates[m]) return true;
+  return false;
+}
+
+// Materials that rewrite (yield stress, mu, lam) need each substep's starting
+// values (states[k].mstate): the model arrays only hold the latest ones.
+int check_mstate(const HostBatch& hb, const empm_particles* states, int n, const char* what) {
+  if (!any_mutates(hb) || hb.N == 0) return EMPM_OK;
+  for (int k = 0; k < n; ++k)
+    if (!states[k].mstate)
+      return set_err(EMPM_E_INVALID_ARG,
+                     "%s: states[%d].mstate is null, but material 5 / hardening == 1 rewrite yield stress, mu and "
+                     "lam inside the substep: each substep's starting values are needed",
+                     what, k);
   return EMPM_OK;
 }
 
@@ -2417,13 +2523,14 @@ int rollout_fwd_replay(empm_ctx* ctx, const empm_batch* batch, const empm_collid
   for (int k = 0; k < n_steps; ++k)
     if (hb.N > 0 && states[k].x == states[k + 1].x)
       return set_err(EMPM_E_INVALID_ARG, "states[%d] and states[%d] alias", k, k + 1);
+  if (n_steps > 0 && (rc = check_mstate(hb, states, 1, "empm_rollout_fwd_replay"))) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
   if ((rc = upload_batch(ctx, hb, st))) return rc;
   ColTable T;
   for (int k = 0; k < n_steps; ++k) {
     if ((rc = make_table(&cols_hist[(size_t)k * ncol], ncol, T))) return rc;
     if ((rc = launch_fwd(ctx, hb, T, (float)times_hist[k], (float)dt, &states[k], &states[k + 1],
-                         grids ? &grids[k] : nullptr, k == 0, k == n_steps - 1, st)))
+                         grids ? &grids[k] : nullptr, k == 0, k == n_steps - 1, st, true)))
       return rc;
   }
   return EMPM_OK;
@@ -2436,11 +2543,11 @@ int rollout_adj(empm_ctx* ctx, const empm_batch* batch, const empm_collider* col
   HostBatch hb;
   int rc = build_batch(ctx, batch, hb);
   if (rc) return rc;
-  if ((rc = check_adj_supported(hb))) return rc;
   if (!states || !adj || !times_hist || n_steps < 0 || (ncol > 0 && !cols_hist))
     return set_err(EMPM_E_INVALID_ARG, "empm_rollout_adj: bad arguments");
   for (int k = 0; k < n_steps; ++k)
     if ((rc = check_particles(&states[k], hb.N, "states[k]"))) return rc;
+  if ((rc = check_mstate(hb, states, n_steps, "empm_rollout_adj"))) return rc;
   if ((rc = check_particles(&adj[0], hb.N, "adj[0]")) || (rc = check_particles(&adj[1], hb.N, "adj[1]"))) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
   if ((rc = upload_batch(ctx, hb, st))) return rc;
@@ -2840,7 +2947,6 @@ int empm_substep_adj(empm_ctx* ctx, const empm_model* model, const empm_collider
   hipStream_t st = (hipStream_t)stream;
   int rc = build_batch(ctx, &b, hb);
   if (rc) return rc;
-  if ((rc = check_adj_supported(hb))) return rc;
   if ((rc = check_particles(in, hb.N, "in")) || (rc = check_particles(d_out, hb.N, "d_out")) ||
       (rc = check_particles(d_in, hb.N, "d_in")))
     return rc;

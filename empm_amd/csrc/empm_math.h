@@ -224,6 +224,7 @@ struct ConstitState {
   float k;       // exponent of the power map
   float lo;      // clamp threshold of sbar (0.01, or 1e-14 for sand on |s|)
   int use_abs;   // sand uses |s|
+  float ys_soft; // von Mises: the yield stress after softening, before hardening (decides mu, lam = 0)
   bool have_svd;
 };
 
@@ -250,7 +251,7 @@ EMPM_DEV void rm_von_mises(ConstitState& st, const float* Ftr, float& mu, float&
   const float c0 = tau[0] - st3, c1 = tau[1] - st3, c2 = tau[2] - st3;
   const float cn = sqrtf(c0 * c0 + c1 * c1 + c2 * c2);
   float g[3] = {st.s[0], st.s[1], st.s[2]};
-  float mu_n = mu, lam_n = lam, ys_n = ys, k = 1.f;
+  float mu_n = mu, lam_n = lam, ys_n = ys, ys_soft = ys, k = 1.f;
   int branch = RM_IDENTITY;
   if (cn > ys) {
     if (damage && ys <= 0.f) {
@@ -262,6 +263,7 @@ EMPM_DEV void rm_von_mises(ConstitState& st, const float* Ftr, float& mu, float&
       if (damage) {
         const float q0 = (dg / ehn) * eh[0], q1 = (dg / ehn) * eh[1], q2 = (dg / ehn) * eh[2];
         ys_n = ys - P.softening * sqrtf(q0 * q0 + q1 * q1 + q2 * q2);
+        ys_soft = ys_n;
         if (ys_n <= 0.f) {
           mu_n = 0.f;
           lam_n = 0.f;
@@ -278,6 +280,7 @@ EMPM_DEV void rm_von_mises(ConstitState& st, const float* Ftr, float& mu, float&
   st.use_abs = 0;
   st.branch = branch;
   st.k = k;
+  st.ys_soft = ys_soft;
   st.se[0] = g[0];
   st.se[1] = g[1];
   st.se[2] = g[2];
@@ -774,6 +777,70 @@ EMPM_DEV void return_map_adj(const ConstitState& st, const float* Ftr, float mu,
     }
   }
   iso_adj(st.U, st.V, s, g, Dd, Jg, Febar, Ftrbar);
+}
+
+// Adjoint of the constitutive state update of the von Mises return maps on
+// their yielding branch: material 5's softening (mpm_utils.py:162-167:
+// Y' = Y - softening |(dgamma / n) ehat|, and mu' = lam' = 0 once Y' <= 0)
+// and hardening (:124-127, :178-181: Y'' = Y' + 2 mu' xi dgamma), with
+// n = |ehat| + 1e-6, dgamma = n - Y / (2 mu), ehat the deviator of
+// log max(s, 0.01).  Other branches leave (Y, mu, lam) unchanged.  Given the
+// cotangents of the updated values (ysb1, mub1, lamb1), accumulates those of
+// the values before the update (ysb0, mub0, lamb0) and of F_trial (through
+// its singular values: d h(s) / dF = U diag(dh/ds) V^T).
+EMPM_DEV void vm_state_adj(const ConstitState& st, float mu0, float ys0, float mu1, const MatParams& P,
+                           bool damage, float ysb1, float mub1, float lamb1, float* Ftrbar, float& ysb0,
+                           float& mub0, float& lamb0) {
+  if (st.branch != RM_POWER) {
+    ysb0 += ysb1;
+    mub0 += mub1;
+    lamb0 += lamb1;
+    return;
+  }
+  float e[3], l[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    e[i] = logf(fmaxf(st.s[i], 0.01f));
+    l[i] = st.s[i] > 0.01f ? 1.f / st.s[i] : 0.f;
+  }
+  const float m3 = (e[0] + e[1] + e[2]) / 3.f;
+  const float eh[3] = {e[0] - m3, e[1] - m3, e[2] - m3};
+  const float L = sqrtf(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
+  const float n = L + 1e-6f;
+  const float dg = n - ys0 / (2.f * mu0);
+  float Yb = ysb1;      // cotangent of Y' (after softening)
+  float mupb = mub1;    // cotangent of mu'
+  float dgb = 0.f, Lb = 0.f;
+  if (P.hardening == 1.f) {
+    mupb += ysb1 * 2.f * P.xi * dg;
+    dgb += ysb1 * 2.f * mu1 * P.xi;
+  }
+  ysb0 += Yb;
+  if (damage) {
+    // Y' = Y - softening dgamma L / n;  d(L/n)/dL = 1e-6 / n^2
+    dgb -= Yb * P.softening * L / n;
+    Lb -= Yb * P.softening * dg * (1e-6f / (n * n));
+    if (st.ys_soft > 0.f) {  // mu' = mu, lam' = lam; else both are 0 whatever they were
+      mub0 += mupb;
+      lamb0 += lamb1;
+    }
+  } else {
+    mub0 += mupb;
+    lamb0 += lamb1;
+  }
+  // dgamma = L + 1e-6 - Y / (2 mu)
+  Lb += dgb;
+  ysb0 -= dgb / (2.f * mu0);
+  mub0 += dgb * ys0 / (2.f * mu0 * mu0);
+  // L = |ehat|: dL/ds_m = (ehat_m / L) d log sbar_m / ds_m
+  const float iL = L > 0.f ? 1.f / L : 0.f;
+  const float sb[3] = {Lb * eh[0] * iL * l[0], Lb * eh[1] * iL * l[1], Lb * eh[2] * iL * l[2]};
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      Ftrbar[r * 3 + c] += st.U[r * 3 + 0] * sb[0] * st.V[c * 3 + 0] + st.U[r * 3 + 1] * sb[1] * st.V[c * 3 + 1] +
+                           st.U[r * 3 + 2] * sb[2] * st.V[c * 3 + 2];
 }
 
 }  // namespace empm

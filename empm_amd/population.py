@@ -171,6 +171,7 @@ class PopulationEvaluator:
         batched reverse sweep of the summed fitness (candidates do not
         interact, so each candidate's gradient is that of its own fitness)."""
         bm, E, nu = self.model(cands)
+        ys0 = bm.yield_stress.clone()  # plasticine rewrites it during the rollout
         ro = rt.Rollout(self.ctx, bm, self.n)
         final, _, _, tape = ro.forward(self.initial_state(), list(self.cols), 0.0, self.dt, self.n_steps,
                                        tape=True, ckpt_every=ckpt_every)
@@ -183,7 +184,7 @@ class PopulationEvaluator:
         _, g = ro.backward(tape, d, flags=_lib.EMPM_ADJ_EXACT)
         dE, dnu = torch.zeros_like(E), torch.zeros_like(E)
         rt.mu_lam_adj(E, nu, g.mu, g.lam, dE, dnu)
-        grad = torch.stack([self._seg_sum(dE * E), self._seg_sum(dnu), self._seg_sum(g.ys * bm.yield_stress)], 1)
+        grad = torch.stack([self._seg_sum(dE * E), self._seg_sum(dnu), self._seg_sum(g.ys * ys0)], 1)
         return fit, grad
 
     # ------------------------------------------------------------ one at a time
@@ -212,6 +213,7 @@ class PopulationEvaluator:
         """fitness_and_grad() as one plain rollout + reverse sweep per
         candidate, one after another."""
         bm, E, nu = self.model(cands)
+        ys0 = bm.yield_stress.clone()
         fits, grads = [], []
         for c in range(self.n_cand):
             ro, s0, a, b = self._solo(c, bm)
@@ -226,5 +228,5 @@ class PopulationEvaluator:
             dE, dnu = torch.zeros(b - a, device=self.device), torch.zeros(b - a, device=self.device)
             rt.mu_lam_adj(E[a:b], nu[a:b], g.mu, g.lam, dE, dnu)
             grads.append(torch.stack([(dE * E[a:b]).double().sum(), dnu.double().sum(),
-                                      (g.ys * bm.yield_stress[a:b]).double().sum()]))
+                                      (g.ys * ys0[a:b]).double().sum()]))
         return torch.stack(fits), torch.stack(grads)

@@ -76,12 +76,15 @@ class Context:
 
 
 class ParticleState:
-    """x (N,3), v (N,3), C (N,3,3), F (N,3,3) = F_trial, optional Fe (N,3,3)."""
+    """x (N,3), v (N,3), C (N,3,3), F (N,3,3) = F_trial, optional Fe (N,3,3)
+    and optional mstate (3, N): the yield stress, mu and lam the next substep
+    starts from, state for the materials that rewrite them (material 5,
+    hardening == 1)."""
 
-    __slots__ = ("x", "v", "C", "F", "Fe")
+    __slots__ = ("x", "v", "C", "F", "Fe", "mstate")
 
-    def __init__(self, x, v, C_, F, Fe=None):
-        self.x, self.v, self.C, self.F, self.Fe = x, v, C_, F, Fe
+    def __init__(self, x, v, C_, F, Fe=None, mstate=None):
+        self.x, self.v, self.C, self.F, self.Fe, self.mstate = x, v, C_, F, Fe, mstate
 
     @staticmethod
     def empty(n, device, with_fe=False):
@@ -99,14 +102,18 @@ class ParticleState:
         self.v.copy_(o.v)
         self.C.copy_(o.C)
         self.F.copy_(o.F)
+        if self.mstate is not None and o.mstate is not None:
+            self.mstate.copy_(o.mstate)
         return self
 
     def clone(self):
-        return ParticleState(self.x.clone(), self.v.clone(), self.C.clone(), self.F.clone())
+        return ParticleState(self.x.clone(), self.v.clone(), self.C.clone(), self.F.clone(),
+                             mstate=self.mstate.clone() if self.mstate is not None else None)
 
     def desc(self) -> Particles:
         return Particles(x=_ptr(self.x), v=_ptr(self.v), C=_ptr(self.C), F=_ptr(self.F),
-                         Fe=_ptr(self.Fe) if self.Fe is not None else None)
+                         Fe=_ptr(self.Fe) if self.Fe is not None else None,
+                         mstate=_ptr(self.mstate) if self.mstate is not None else None)
 
 
 @dataclass
@@ -132,6 +139,22 @@ class ModelArrays:
     plastic_viscosity: float = 0.0
     softening: float = 0.1
 
+    @property
+    def mutates(self) -> bool:
+        """Whether the return map rewrites yield stress, mu and lam (material
+        5's softening, hardening == 1 of the von Mises maps)."""
+        return _mutates(self)
+
+    def material_state(self) -> torch.Tensor:
+        """(3, N) copy of (yield stress, mu, lam): the current material state."""
+        return torch.stack([self.yield_stress, self.mu, self.lam]).contiguous()
+
+    def with_private_params(self) -> "ModelArrays":
+        """The same model with its own copies of mu, lam and yield stress."""
+        import dataclasses
+        return dataclasses.replace(self, mu=self.mu.clone(), lam=self.lam.clone(),
+                                   yield_stress=self.yield_stress.clone())
+
     def struct(self) -> Model:
         m = Model()
         m.n_particles = int(self.mu.shape[0])
@@ -156,6 +179,10 @@ class ModelArrays:
         m.vol = _ptr(self.vol)
         m.selection = _ptr(self.selection) if self.selection is not None else None
         return m
+
+
+def _mutates(m) -> bool:
+    return int(m.material) == 5 or (int(m.material) in (1, 7) and float(m.hardening) == 1.0)
 
 
 class BatchModel:
@@ -218,6 +245,17 @@ class BatchModel:
         b.mass, b.vol = _ptr(self.mass), _ptr(self.vol)
         b.selection = _ptr(self.selection) if self.selection is not None else None
         return b
+
+    @property
+    def mutates(self) -> bool:
+        return any(_mutates(m) for m in self.sims)
+
+    def material_state(self) -> torch.Tensor:
+        return torch.stack([self.yield_stress, self.mu, self.lam]).contiguous()
+
+    def with_private_params(self) -> "BatchModel":
+        return BatchModel(self.sims, self.counts, self.mu.clone(), self.lam.clone(), self.yield_stress.clone(),
+                          self.mass, self.vol, self.selection)
 
     def sizes(self):
         """(total grid nodes, ints of one binning record)."""
@@ -320,35 +358,40 @@ class Records:
     ((m v, m) and v_out, float4 per node) and their particle binnings, one
     tensor per field."""
 
-    def __init__(self, n: int, n_particles: int, n_nodes: int, bin_ints: int, device):
+    def __init__(self, n: int, n_particles: int, n_nodes: int, bin_ints: int, device, mstate: bool = False):
         N, G = int(n_particles), int(n_nodes)
         e = lambda *shape: torch.empty(shape, dtype=F32, device=device)
         self.n = int(n)
         self.x, self.v = e(n + 1, N, 3), e(n + 1, N, 3)
         self.C, self.F, self.Fe = e(n + 1, N, 3, 3), e(n + 1, N, 3, 3), e(n + 1, N, 3, 3)
+        self.mstate = e(n + 1, 3, N) if mstate else None
         self.grid = e(max(n, 1), 2, G, 4)
         self.bins = torch.empty((max(n, 1), int(bin_ints)), dtype=torch.int32, device=device)
 
     def state(self, k: int) -> ParticleState:
-        return ParticleState(self.x[k], self.v[k], self.C[k], self.F[k], self.Fe[k])
+        return ParticleState(self.x[k], self.v[k], self.C[k], self.F[k], self.Fe[k],
+                             self.mstate[k] if self.mstate is not None else None)
 
     def record(self, k: int):
         """(grid, bins) of substep k, the entry of a grid-record list."""
         return (self.grid[k], self.bins[k])
 
 
-def tape_bytes(n_particles: int, n_nodes: int, bin_ints: int, n_steps: int, ckpt_every: int) -> int:
+def tape_bytes(n_particles: int, n_nodes: int, bin_ints: int, n_steps: int, ckpt_every: int,
+               mutates: bool = False) -> int:
     """Device bytes a reverse-mode rollout holds for its tape: every
     substep's record (state with Fe 132 B/particle, grid 32 B/node, binning)
     when ckpt_every == 1, else a 96 B/particle checkpoint every ckpt_every
     substeps plus one segment of records; plus the reverse sweep's
-    cotangents and parameter gradients."""
+    cotangents and parameter gradients.  Materials that rewrite (yield
+    stress, mu, lam) add 12 B/particle to every record and checkpoint."""
     N, G, T, K = int(n_particles), int(n_nodes), int(n_steps), max(1, int(ckpt_every))
-    rec = lambda k: (k + 1) * N * 132 + k * (32 * G + 4 * int(bin_ints))
+    ms = 12 if mutates else 0
+    rec = lambda k: (k + 1) * N * (132 + ms) + k * (32 * G + 4 * int(bin_ints))
     sweep = 2 * 96 * N + 20 * N
     if K == 1:
         return rec(T) + sweep
-    return (-(-T // K)) * 96 * N + rec(min(K, T)) + sweep
+    return (-(-T // K)) * (96 + ms) * N + rec(min(K, T)) + sweep
 
 
 def _states_array(states: list[ParticleState]):
@@ -460,7 +503,7 @@ class Rollout:
         final, t1, xs, tp = self._forward_core(pm, s_sorted, cols, time, dt, n_steps, record, tape, ckpt_every,
                                                cols_out)
         sims = pm.sims if self.batch else [pm]
-        if any(s.material == 5 or s.hardening == 1.0 for s in sims):  # in-place updates go back to the caller
+        if any(_mutates(s) for s in sims):  # in-place updates go back to the caller
             for a, b in ((self.model.mu, pm.mu), (self.model.lam, pm.lam), (self.model.yield_stress, pm.yield_stress)):
                 a.index_copy_(0, perm, b)
         if tp is not None:
@@ -479,9 +522,12 @@ class Rollout:
             tp.cols_hist = (Collider * max(1, n_steps * ncol))()
             tp.times_hist = (C.c_double * max(1, n_steps))()
         stream = stream_handle(self.ctx.device)
+        mut = tape and model.mutates
         if tape and tp.ckpt_every == 1:
-            # recorded mode: every substep writes its own output slot, Fe and grid
-            R = Records(n_steps, self.n, self._nodes, self._bin_ints, self.ctx.device)
+            # recorded mode: every substep writes its own output slot, Fe and
+            # grid (and the material state it starts from, for materials that
+            # rewrite it)
+            R = Records(n_steps, self.n, self._nodes, self._bin_ints, self.ctx.device, mstate=mut)
             R.state(0).copy_(state)
             states = [R.state(k) for k in range(n_steps + 1)]
             grids = [R.record(k) for k in range(n_steps)]
@@ -503,6 +549,8 @@ class Rollout:
         out_x = []
         if tape:
             tp.checkpoints[0] = state.clone()
+            if mut:
+                tp.checkpoints[0].mstate = model.material_state()
         for stop in stops:
             n = stop - k
             if n <= 0:
@@ -521,6 +569,8 @@ class Rollout:
                 out_x.append(cur.x.clone())
             if tape and k < n_steps and k % tp.ckpt_every == 0:
                 tp.checkpoints[k] = cur.clone()
+                if mut:
+                    tp.checkpoints[k].mstate = model.material_state()
         if cols_out is not None:
             cols_out[:] = [arr[i] for i in range(ncol)]
         return cur, t.value, out_x, tp
@@ -588,7 +638,12 @@ class Rollout:
             reverse(0, tp.n_steps, tp.rec_states, tp.rec_grids)
         else:
             K = tp.ckpt_every
-            seg = Records(min(K, tp.n_steps), n, self._nodes, self._bin_ints, dev)
+            mut = model.mutates
+            if mut:
+                # replays rewrite the model's parameter arrays: give them their own
+                model = model.with_private_params()
+                m = model.struct()
+            seg = Records(min(K, tp.n_steps), n, self._nodes, self._bin_ints, dev, mstate=mut)
             seg_starts = sorted(tp.checkpoints.keys())
             k_hi = tp.n_steps
             for s0 in reversed(seg_starts):

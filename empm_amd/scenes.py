@@ -189,16 +189,16 @@ def shape_points(shape, n, rng, center):
     return c + (rng.random((n, 3)) - 0.5) * ext, float(np.prod(ext)), ext[2] / 2
 
 
-def population_c3(n_cand=32, seed=4, n_grid=64, n_steps=300, dt=1e-4, differentiable=False, nominal=False,
+def population_c3(n_cand=32, seed=4, n_grid=64, n_steps=300, dt=1e-4, second_material=5, nominal=False,
                   drop_height=0.3, n_range=(50_000, 150_000)):
     """A CMA-ES population: n_cand candidates, each an object of N ~ U{50k..150k}
     particles shaped as a cube (side 0.2), a rod (0.4 x 0.06 x 0.06) or an
     ellipsoid (radii 0.2, 0.1, 0.15), drawn from the seed, on its own 64^3
     grid with 3-cell sticky walls.  Materials alternate Drucker-Prager sand
     (id 2) and von Mises plasticine (id 5, yield stress 5000, softening 0.1,
-    the reference's gallery preset, warp_mpm_wrapper.py:43-47); with
-    differentiable=True the second material is Neo-Hookean + von Mises (id 7,
-    yield stress 5000), whose adjoint exists.  Parameters are log-normal
+    the reference's gallery preset, warp_mpm_wrapper.py:43-47), or another
+    second material (second_material, e.g. 7: Neo-Hookean + von Mises with
+    yield stress 5000).  Parameters are log-normal
     around the nominal E = 1e5, nu = 0.3, friction angle 30 degrees:
     E = 1e5 exp(0.3 z0), nu = min(0.3 exp(0.1 z1), 0.489),
     angle = 30 exp(0.1 z2); nominal=True gives the same objects with the
@@ -213,7 +213,7 @@ def population_c3(n_cand=32, seed=4, n_grid=64, n_steps=300, dt=1e-4, differenti
         N = int(rng.integers(n_range[0], n_range[1] + 1))
         shape = C3_SHAPES[int(rng.integers(len(C3_SHAPES)))]
         z = rng.standard_normal(3)
-        material = (2, 7 if differentiable else 5)[c % 2]
+        material = (2, int(second_material))[c % 2]
         E = 1e5 * np.exp(0.3 * z[0])
         nu = min(0.3 * np.exp(0.1 * z[1]), 0.489)
         phi = 30.0 * np.exp(0.1 * z[2])

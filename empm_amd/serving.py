@@ -255,6 +255,7 @@ class _Dev:
     state: rt.ParticleState
     ckpt: dict
     target: torch.Tensor | None = None
+    ys0: torch.Tensor | None = None  # yield stress at admission (the gradient is w.r.t. its log)
     cot: rt.ParticleState | None = None
     grads: rt.ParamGrads | None = None
     loss: float | None = None
@@ -286,9 +287,6 @@ class ServingEngine:
     def submit(self, req: RolloutRequest, arrival: float | None = None) -> int:
         if req.want_grad and req.target is None:
             raise ValueError("a gradient request needs a target")
-        if req.want_grad and req.material == 5:
-            raise ValueError("the adjoint of material 5 (plasticine) is not supported: it rewrites its own "
-                             "parameters inside the substep")
         rid = self._next
         self._next += 1
         s = self.sched.submit(rid, req, time.perf_counter() if arrival is None else arrival)
@@ -324,7 +322,16 @@ class ServingEngine:
         s.dev = _Dev(perm=perm, model=model, E=E, nu=nu, state=state, ckpt={},
                      target=g(t(r.target)) if r.want_grad else None)
         if r.want_grad:
-            s.dev.ckpt[0] = state
+            s.dev.ys0 = model.yield_stress.clone()
+            s.dev.ckpt[0] = self._checkpoint(state, model)
+
+    @staticmethod
+    def _checkpoint(state: rt.ParticleState, model: rt.ModelArrays) -> rt.ParticleState:
+        """A segment start of a gradient request: its state and, for a
+        material that rewrites (yield stress, mu, lam) in its return map,
+        their values there (the segment's re-run starts from them)."""
+        return rt.ParticleState(state.x, state.v, state.C, state.F,
+                                mstate=model.material_state() if model.mutates else None)
 
     # ------------------------------------------------------------ ticks
     def _batch(self, slots: list[Slot]):
@@ -348,7 +355,7 @@ class ServingEngine:
         for s, st, mu, lam, ys in zip(slots, self._split(final, counts), torch.split(bm.mu, counts),
                                       torch.split(bm.lam, counts), torch.split(bm.yield_stress, counts)):
             s.dev.state = st
-            if s.req.material == 5:  # plasticine rewrites its parameters: they are state
+            if s.dev.model.mutates:  # plasticine rewrites its parameters: they are state
                 s.dev.model.mu.copy_(mu)
                 s.dev.model.lam.copy_(lam)
                 s.dev.model.yield_stress.copy_(ys)
@@ -362,21 +369,37 @@ class ServingEngine:
                     d.cot.x.copy_(diff * (2.0 / s.req.n))
                     d.grads = rt.ParamGrads.zeros(s.req.n, self.device)
             elif s.req.want_grad:
-                s.dev.ckpt[s.k] = st
+                s.dev.ckpt[s.k] = self._checkpoint(st, s.dev.model)
 
     def _backward(self, w: Work, slots: list[Slot]):
-        bm = self._batch(slots)
+        # the segment is re-run from its start: a material that rewrites its
+        # parameters starts from their values there (the checkpoint's mstate)
+        import dataclasses
+
+        models = []
+        for s in slots:
+            ck = s.dev.ckpt[s.starts[-1]]
+            m = s.dev.model
+            if ck.mstate is not None:
+                m = dataclasses.replace(m, yield_stress=ck.mstate[0], mu=ck.mstate[1], lam=ck.mstate[2])
+            models.append(m)
+        bm = rt.BatchModel.concat(models)
         counts = [s.req.n for s in slots]
         ro = rt.Rollout(self.ctx, bm, bm.n, sort=False)
         start = self._cat([s.dev.ckpt[s.starts[-1]] for s in slots])
         _, _, _, tape = ro.forward(start, list(self.cols), 0.0, self.dt, w.length, tape=True, ckpt_every=1)
-        d0, g = ro.backward(tape, self._cat([s.dev.cot for s in slots]), flags=_lib.EMPM_ADJ_EXACT)
+        # the parameter cotangents run through the segments: for materials
+        # that rewrite (yield stress, mu, lam) they are those of the state at
+        # the segment's end on entry and at its start on return; for the
+        # others the segment adds its share
+        g_in = rt.ParamGrads(*(torch.cat([getattr(s.dev.grads, f) for s in slots]).contiguous()
+                               for f in ("mu", "lam", "ys", "mass", "vol")))
+        d0, g = ro.backward(tape, self._cat([s.dev.cot for s in slots]), flags=_lib.EMPM_ADJ_EXACT, grads=g_in)
         gparts = [torch.split(t, counts) for t in (g.mu, g.lam, g.ys, g.mass, g.vol)]
         for i, (s, cot) in enumerate(zip(slots, self._split(d0, counts))):
             d = s.dev
             d.cot = cot
-            for acc, part in zip((d.grads.mu, d.grads.lam, d.grads.ys, d.grads.mass, d.grads.vol), gparts):
-                acc.add_(part[i])
+            d.grads = rt.ParamGrads(*(part[i].clone() for part in gparts))
             d.ckpt.pop(s.starts[-1], None)
             self.sched.backward_done(s.rid)
 
@@ -391,7 +414,7 @@ class ServingEngine:
             rt.mu_lam_adj(d.E, d.nu, d.grads.mu, d.grads.lam, dE, dnu)
             res.loss = float(d.loss)
             res.grad = torch.stack([(dE * d.E).double().sum(), dnu.double().sum(),
-                                    (d.grads.ys * d.model.yield_stress).double().sum()])
+                                    (d.grads.ys * d.ys0).double().sum()])
         return res
 
     def step(self) -> list[int]:

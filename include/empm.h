@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define EMPM_ABI_VERSION 4
+#define EMPM_ABI_VERSION 5
 #define EMPM_MAX_COLLIDERS 16
 
 enum {
@@ -97,6 +97,16 @@ typedef struct {
     float *C;  /* N*9 */
     float *F;  /* N*9, F_trial */
     float *Fe; /* N*9 or NULL */
+    /* 3*N or NULL: the yield stress, mu and lam (N floats each, in that
+     * order) that the substep reading this state starts from.  Used only for
+     * simulations whose return map rewrites them (material 5's softening,
+     * mpm_utils.py:162-167; hardening == 1, :124-127,178-181), for which they
+     * are state: a forward substep records its starting values here before
+     * updating the model arrays, a replay starts from states[0].mstate, and
+     * an adjoint substep reads them (the model arrays hold only the latest
+     * values).  The reference keeps them in its model arrays only, which its
+     * Tape cannot rewind. */
+    float *mstate;
 } empm_particles;
 
 /* The grid of one forward substep, recorded for reverse mode so that the
@@ -257,8 +267,11 @@ int empm_substep_fwd(empm_ctx *ctx, const empm_model *model, const empm_collider
  * d_mu, d_lam, d_ys, d_mass, d_vol (any may be NULL).  The substep's grid is
  * recomputed internally.  Replaces the Warp Tape replay of one substep's
  * launches (interface.py:197, warp_utils.py:93-127).  flags: EMPM_ADJ_*.
- * Materials 5 and hardening==1 mutate model arrays inside the substep and
- * are rejected with EMPM_E_UNSUPPORTED. */
+ * For material 5 and hardening == 1, whose return map rewrites (yield stress,
+ * mu, lam), those are state: d_ys, d_mu, d_lam hold on entry the cotangents
+ * of the updated values and on return those of the values the substep
+ * started from (for other materials this is the same as accumulating); the
+ * starting values are in->mstate, or the model arrays when it is NULL. */
 int empm_substep_adj(empm_ctx *ctx, const empm_model *model, const empm_collider *cols, int ncol, float time,
                      float dt, const empm_particles *in, const empm_particles *d_out, const empm_particles *d_in,
                      float *d_mu, float *d_lam, float *d_ys, float *d_mass, float *d_vol, int flags, void *stream);
@@ -273,7 +286,9 @@ int empm_substep_adj(empm_ctx *ctx, const empm_model *model, const empm_collider
  * collider table and time each substep saw are recorded for the adjoint.
  * If grids is non-NULL (host, n_steps entries) substep k writes its grid
  * into grids[k]; with states[k].Fe non-NULL as well, the substep is fully
- * recorded and empm_rollout_adj differentiates it without recompute. */
+ * recorded and empm_rollout_adj differentiates it without recompute.
+ * states[k].mstate, when non-NULL, receives the material state substep k
+ * starts from (material 5 / hardening; see empm_particles). */
 int empm_rollout_fwd(empm_ctx *ctx, const empm_model *model, empm_collider *cols, int ncol, double *time, double dt,
                      int n_steps, const empm_particles *states, const empm_grid_rec *grids, empm_collider *cols_hist,
                      double *times_hist, void *stream);
@@ -281,7 +296,9 @@ int empm_rollout_fwd(empm_ctx *ctx, const empm_model *model, empm_collider *cols
 /* Forward replay of n_steps substeps with the recorded per-substep collider
  * tables and times of empm_rollout_fwd (no collider advance): the recompute
  * step of checkpointed reverse mode (the reference keeps every state alive
- * instead, interface.py:137-145).  grids: as for empm_rollout_fwd. */
+ * instead, interface.py:137-145).  grids: as for empm_rollout_fwd.  For
+ * material 5 / hardening the replay starts from states[0].mstate (required)
+ * and rewrites the model arrays as the forward does. */
 int empm_rollout_fwd_replay(empm_ctx *ctx, const empm_model *model, const empm_collider *cols_hist, int ncol,
                             const double *times_hist, double dt, int n_steps, const empm_particles *states,
                             const empm_grid_rec *grids, void *stream);
@@ -292,7 +309,10 @@ int empm_rollout_fwd_replay(empm_ctx *ctx, const empm_model *model, const empm_c
  * adj[n_steps % 2] holds dL/d(state 0).  Parameter adjoints accumulate.
  * If grids is non-NULL (host, n_steps entries, as written by the forward)
  * and states[k].Fe is non-NULL, substep k uses the recorded grid and Fe;
- * otherwise its P2G and boundary pass are recomputed. */
+ * otherwise its P2G and boundary pass are recomputed.  For material 5 /
+ * hardening every states[k].mstate is required and d_ys, d_mu, d_lam carry
+ * the cotangents of the material state (see empm_substep_adj): on return
+ * they hold dL/d(values before substep 0). */
 int empm_rollout_adj(empm_ctx *ctx, const empm_model *model, const empm_collider *cols_hist, int ncol,
                      const double *times_hist, double dt, int n_steps, const empm_particles *states,
                      const empm_grid_rec *grids, const empm_particles *adj, float *d_mu, float *d_lam, float *d_ys,

@@ -19,10 +19,11 @@ from tests.helpers import rel_err
 pytestmark = pytest.mark.gpu
 
 
-def _wrapper(sc, material="jelly", requires_grad=False, frame_dt=None, colliders=True):
+def _wrapper(sc, material="jelly", requires_grad=False, frame_dt=None, colliders=True, params=None):
     mpm = WarpMPMWrapper(grid_size=sc.n_grid, grid_lim=1.0, frame_dt=frame_dt or sc.dt * 10, sim_dt=sc.dt,
                          device="cuda:0", requires_grad=requires_grad)
     mpm.material_params = dict(material_gallery[material])
+    mpm.material_params.update(params or {})
     mpm.add_particles(sc.x, sc.params.get("v0"), sc.vol.astype(np.float32), sc.E.astype(np.float32),
                       sc.nu.astype(np.float32), sc.rho.astype(np.float32))
     if colliders:
@@ -312,24 +313,58 @@ def test_function_variants_shapes_and_consistency():
     assert outs_c[4].shape == (N, 6)
 
 
-def test_material_5_forward_rewrites_yield_and_backward_is_rejected():
-    sc = scenes.small_scene(n_particles=200, n_grid=16, material=5, seed=6, E=3e4)
-    mpm = _wrapper(sc, material="plasticine", requires_grad=True)
+@pytest.mark.parametrize("ckpt", [1, 3])
+def test_material_5_forward_rewrites_yield_and_backward_differentiates_it(ckpt):
+    """Plasticine rewrites the model's yield stress during the forward
+    (persisting across rollouts, App.-B-8 behaviour of the reference: only
+    mu and lam are recomputed per rollout); the backward differentiates
+    through those updates without touching the model's arrays again, and its
+    E gradient matches central differences of FP64 oracle rollouts that start
+    from the same softened yield stress."""
+    sc = scenes.small_scene(n_particles=200, n_grid=16, material=5, seed=6, E=3e4, center=(0.5, 0.5, 0.3),
+                            vel=(0.3, 0.0, -2.0))
+    mpm = _wrapper(sc, material="plasticine", requires_grad=True, params={"yield_stress": 50.0})
+    mpm.mpm_solver.adjoint_mode = "exact"
+    mpm.mpm_solver.checkpoint_every = ckpt
     ys0 = mpm.mpm_model.yield_stress.clone()
     mpm.advance_to(2)
-    assert torch.all(mpm.mpm_model.yield_stress <= ys0)
-    from empm_amd._lib import EmpmError
+    assert torch.all(mpm.mpm_model.yield_stress <= ys0) and torch.any(mpm.mpm_model.yield_stress < ys0)
+    N, dev, steps = sc.n, "cuda:0", 9
+    ys_start = mpm.mpm_model.yield_stress.double().cpu().numpy().copy()
+    x0 = torch.as_tensor(sc.x, device=dev)
+    v0 = torch.as_tensor(sc.params["v0"], device=dev)
+    E = torch.tensor(0.03, device=dev, requires_grad=True)
+    nu = torch.tensor(0.3, device=dev)
+    mpm.mpm_solver.time = 0.0
+    pos = MPMDifferentiableSimulation.apply(mpm.mpm_solver, mpm.mpm_state, mpm.mpm_model, 1, sc.dt, steps, x0, v0,
+                                            E * 1e6, nu, torch.full((N,), 1000.0, device=dev), None, None, dev,
+                                            True, 0)
+    ys_after_fwd = mpm.mpm_model.yield_stress.clone()
+    assert float((ys_after_fwd - torch.as_tensor(ys_start, device=dev, dtype=torch.float32)).abs().max()) > 0
+    # smooth weights (height plus spread about the centre): a random-sign
+    # weighting would make the gradient a cancelling sum at FP32 round-off
+    Gn = np.zeros((N, 3))
+    Gn[:, 2] = 1.0
+    Gn += 2.0 * (sc.x - sc.x.mean(0))
+    G = torch.as_tensor(Gn, dtype=torch.float32, device=dev)
+    (pos * G).sum().backward()
+    assert torch.equal(mpm.mpm_model.yield_stress, ys_after_fwd)
+    gE = float(E.grad) / 1e6  # d/dE_phys of the mean-aggregated scalar gradient
 
-    N = sc.n
-    E = torch.tensor(0.03, device="cuda:0", requires_grad=True)
-    nu = torch.tensor(0.3, device="cuda:0")
-    pos = MPMDifferentiableSimulationTrail.apply(mpm.mpm_solver, mpm.mpm_state, mpm.mpm_model, 1, sc.dt, 3,
-                                                torch.as_tensor(sc.x, device="cuda:0"),
-                                                torch.zeros((N, 3), device="cuda:0"), E, nu,
-                                                torch.full((N,), 1000.0, device="cuda:0"), None, None, "cuda:0", True,
-                                                0)
-    with pytest.raises(EmpmError):
-        pos.sum().backward()
+    def J(Ephys):
+        mu = np.full(N, Ephys / (2 * 1.3))
+        lam = np.full(N, Ephys * 0.3 / (1.3 * 0.4))
+        P = orc.make_params(N, 16, material=5, gravity=(0, 0, -9.8), softening=0.1)
+        cols = [orc.surface_collider(c["point"], c["normal"]) for c in sc.colliders]
+        o = orc.OracleState(sc.x, sc.params["v0"], np.zeros((N, 3, 3)), np.tile(np.eye(3), (N, 1, 1)), mu, lam,
+                            ys_start, sc.vol * sc.rho, sc.vol)
+        orc.rollout(P, cols, 0.0, sc.dt, steps, o)
+        return float(np.sum(G.double().cpu().numpy() * o.x))
+
+    h = 1e-4
+    fd = (J(3e4 * (1 + h)) - J(3e4 * (1 - h))) / (2 * h * 3e4) / N
+    print(f"[plasticine backward, ckpt {ckpt}] E.grad {gE:.6e} vs FD/N {fd:.6e}", flush=True)
+    assert abs(gE - fd) < 1e-3 * abs(fd)
 
 
 def test_errors_mirror_reference():

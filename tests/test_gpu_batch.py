@@ -90,7 +90,7 @@ def test_batch_forward_matches_each_simulation_alone(dev):
 @pytest.mark.parametrize("ckpt", [1, 6])
 def test_batch_gradients_match_each_simulation_alone(dev, ckpt):
     steps = 18
-    spec = [a for a in SIMS if a[0] != 5]
+    spec = list(SIMS)  # material 5's (yield stress, mu, lam) are state of the tape
     built = [sim(dev, *a) for a in spec]
     models = [b[0] for b in built]
     states = [b[1] for b in built]
@@ -123,12 +123,22 @@ def test_batch_gradients_match_each_simulation_alone(dev, ckpt):
         assert all(v < bar_sum for v in sums.values()), sums
 
 
-def test_batch_backward_rejects_material_5(dev):
+def test_batch_backward_needs_the_material_state_of_material_5(dev):
+    """Material 5 rewrites (yield stress, mu, lam) in every yielding substep:
+    a reverse sweep over states without their recorded material state is
+    refused rather than differentiated from the latest values."""
+    import ctypes as C
+
     built = [sim(dev, *a) for a in SIMS[:1] + SIMS[-1:]]
     ro, bm, (final, _, _, tape) = run_batch(dev, [b[0] for b in built], [b[1] for b in built], built[0][2], 4,
                                             tape=True)
-    with pytest.raises(EmpmError):
-        ro.backward(tape, rt.ParticleState.empty(bm.n, dev))
+    states = [rt.ParticleState(s.x, s.v, s.C, s.F, s.Fe) for s in tape.rec_states]  # mstate dropped
+    adj = [rt.ParticleState.empty(bm.n, dev), rt.ParticleState.empty(bm.n, dev)]
+    g = rt.ParamGrads.zeros(bm.n, dev)
+    with pytest.raises(EmpmError, match="mstate"):
+        ro._adj(tape.model.struct(), tape.cols_hist, tape.ncol, tape.times_hist, C.c_double(tape.dt), 4,
+                rt._states_array(states), rt._grid_array(tape.rec_grids), rt._states_array(adj), *g.ptrs(), 0,
+                rt.stream_handle(dev))
 
 
 EDGE_SIMS = [(0, 16, 150, 31), (7, 24, 190, 32), (6, 12, 120, 33)]

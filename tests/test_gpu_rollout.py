@@ -20,7 +20,7 @@ from tests.helpers import fmt_errs, gpu_collider, rel_err, state_errors
 pytestmark = pytest.mark.gpu
 
 
-def setup(sc, dev, E=None, nu=None, ys=None, v0=None):
+def setup(sc, dev, E=None, nu=None, ys=None, v0=None, hardening=0.0, xi=0.0):
     N = sc.n
     E = sc.E.astype(np.float64) if E is None else np.full(N, E)
     nu = sc.nu.astype(np.float64) if nu is None else np.full(N, nu)
@@ -29,11 +29,12 @@ def setup(sc, dev, E=None, nu=None, ys=None, v0=None):
     lam = (E * nu / ((1 + nu) * (1 - 2 * nu))).astype(np.float32)
     mass = (sc.vol * sc.rho).astype(np.float32)
     v0 = np.zeros((N, 3), np.float32) if v0 is None else v0.astype(np.float32)
-    P = orc.make_params(N, sc.n_grid, material=sc.material, gravity=sc.gravity, friction_angle=sc.friction_angle)
+    P = orc.make_params(N, sc.n_grid, material=sc.material, gravity=sc.gravity, friction_angle=sc.friction_angle,
+                        hardening=hardening, xi=xi)
     t = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=dev).clone()
     model = rt.ModelArrays(material=sc.material, n_grid=sc.n_grid, grid_lim=1.0, dx=P.dx, inv_dx=P.inv_dx,
                            mu=t(mu), lam=t(lam), yield_stress=t(ys), mass=t(mass), vol=t(sc.vol),
-                           gravity=sc.gravity, alpha=P.alpha)
+                           gravity=sc.gravity, alpha=P.alpha, hardening=hardening, xi=xi)
     s0 = rt.ParticleState(t(sc.x), t(v0), torch.zeros((N, 3, 3), device=dev),
                           torch.eye(3, device=dev).repeat(N, 1, 1).contiguous())
     ocols = [orc.surface_collider(c["point"], c["normal"]) for c in sc.colliders]
@@ -81,9 +82,10 @@ def test_small_rollout_every_material(dev, material):
     assert e["x"] < 1e-4 and e["F"] < 1e-4 and e["disp"] < 1e-3, e
 
 
-def _loss_grad_gpu(sc, dev, target, prm, steps, ckpt, flags=0):
+def _loss_grad_gpu(sc, dev, target, prm, steps, ckpt, flags=0, hardening=0.0, xi=0.0):
     N = sc.n
-    model, s0, P, ocols, _ = setup(sc, dev, E=prm["E"], nu=prm["nu"], ys=prm["ys"], v0=sc.params["v0"])
+    model, s0, P, ocols, _ = setup(sc, dev, E=prm["E"], nu=prm["nu"], ys=prm["ys"], v0=sc.params["v0"],
+                                   hardening=hardening, xi=xi)
     ctx = rt.Context(sc.n_grid, N, dev)
     ro = rt.Rollout(ctx, model, N)
     final, _, _, tape = ro.forward(s0, [gpu_collider(c) for c in ocols], 0.0, sc.dt, steps, tape=True,
@@ -103,12 +105,12 @@ def _loss_grad_gpu(sc, dev, target, prm, steps, ckpt, flags=0):
     return loss, g
 
 
-def _loss_oracle(sc, target, prm, steps):
+def _loss_oracle(sc, target, prm, steps, hardening=0.0, xi=0.0):
     N = sc.n
     E, nu, ys = prm["E"], prm["nu"], prm["ys"]
     mu = np.full(N, E / (2 * (1 + nu)))
     lam = np.full(N, E * nu / ((1 + nu) * (1 - 2 * nu)))
-    P = orc.make_params(N, sc.n_grid, material=sc.material, gravity=sc.gravity)
+    P = orc.make_params(N, sc.n_grid, material=sc.material, gravity=sc.gravity, hardening=hardening, xi=xi)
     ocols = [orc.surface_collider(c["point"], c["normal"]) for c in sc.colliders]
     o = orc.OracleState(sc.x, sc.params["v0"], np.zeros((N, 3, 3)), np.tile(np.eye(3), (N, 1, 1)), mu, lam,
                         np.full(N, ys), sc.vol * sc.rho, sc.vol, precision="f64")
@@ -116,9 +118,14 @@ def _loss_oracle(sc, target, prm, steps):
     return float(np.mean(np.sum((o.x - target) ** 2, axis=1)))
 
 
-@pytest.mark.parametrize("material", [7, 0, 6, 1])
-def test_parameter_gradients_vs_central_differences(dev, material):
+@pytest.mark.parametrize("material,hardening", [(7, 0.0), (0, 0.0), (6, 0.0), (1, 0.0), (5, 0.0), (1, 1.0),
+                                                (7, 1.0), (5, 1.0)])
+def test_parameter_gradients_vs_central_differences(dev, material, hardening):
+    """Material 5 (plasticine) and hardening == 1 rewrite (yield stress, mu,
+    lam) every yielding substep: the rollout's gradient w.r.t. the initial
+    values runs back through those updates."""
     steps = 40
+    xi = 0.3 if hardening else 0.0
     # a soft body hitting the sticky ground: deforms (and yields) within the
     # horizon, so the loss is far above FP32 round-off
     sc = scenes.small_scene(n_particles=300, n_grid=16, material=material, seed=3, E=3e3, ys=60.0, dt=2e-4,
@@ -128,7 +135,7 @@ def test_parameter_gradients_vs_central_differences(dev, material):
     target = sc.x.astype(np.float64).copy()
     # target rollout with different parameters (FP64 oracle)
     N = sc.n
-    P = orc.make_params(N, sc.n_grid, material=material, gravity=sc.gravity)
+    P = orc.make_params(N, sc.n_grid, material=material, gravity=sc.gravity, hardening=hardening, xi=xi)
     ocols = [orc.surface_collider(c["point"], c["normal"]) for c in sc.colliders]
     mu_t = np.full(N, tprm["E"] / (2 * (1 + tprm["nu"])))
     lam_t = np.full(N, tprm["E"] * tprm["nu"] / ((1 + tprm["nu"]) * (1 - 2 * tprm["nu"])))
@@ -139,30 +146,35 @@ def test_parameter_gradients_vs_central_differences(dev, material):
     # ulps of a position; a seeded offset keeps the residual x - target far
     # above FP32 round-off so the comparison tests the adjoint, not rounding
     target = o.x + 0.01 * np.random.default_rng(11).standard_normal((N, 3))
-    print(f"[grad vs FD] material={material} N={N} steps={steps}", flush=True)
-    loss, g = _loss_grad_gpu(sc, dev, target, base, steps, ckpt=7)
+    print(f"[grad vs FD] material={material} hardening={hardening} N={N} steps={steps}", flush=True)
+    loss, g = _loss_grad_gpu(sc, dev, target, base, steps, ckpt=7, hardening=hardening, xi=xi)
     fd = []
+    lo = lambda prm: _loss_oracle(sc, target, prm, steps, hardening=hardening, xi=xi)
     for i, (key, h) in enumerate((("E", 1e-4), ("nu", 1e-5), ("ys", 1e-4))):
         pp, pm = dict(base), dict(base)
         if key == "nu":
             pp[key] += h
             pm[key] -= h
-            fd.append((_loss_oracle(sc, target, pp, steps) - _loss_oracle(sc, target, pm, steps)) / (2 * h))
+            fd.append((lo(pp) - lo(pm)) / (2 * h))
         else:
             pp[key] *= np.exp(h)
             pm[key] *= np.exp(-h)
-            fd.append((_loss_oracle(sc, target, pp, steps) - _loss_oracle(sc, target, pm, steps)) / (2 * h))
+            fd.append((lo(pp) - lo(pm)) / (2 * h))
     fd = np.array(fd)
     print(f"  loss {loss:.6e}\n  adjoint d/d(logE, nu, logY) {g}\n  central FD              {fd}", flush=True)
     for i in range(3):
-        if i == 2 and material not in (1, 7):
+        if i == 2 and material not in (1, 5, 7):
             assert abs(g[i]) < 1e-12 + 1e-6 * np.abs(g).max()
             continue
         assert abs(g[i] - fd[i]) <= 1e-3 * abs(fd[i]) + 1e-9 * np.abs(fd).max(), (i, g[i], fd[i])
 
 
-def test_checkpoint_interval_does_not_change_gradients(dev):
-    sc = scenes.small_scene(n_particles=300, n_grid=16, material=7, seed=4, E=2e4, ys=150.0)
+@pytest.mark.parametrize("material", [7, 5])
+def test_checkpoint_interval_does_not_change_gradients(dev, material):
+    """Recorded (every substep) vs checkpointed reverse sweeps; for material
+    5 the checkpoints also carry the rewritten (yield stress, mu, lam) that
+    each replayed segment starts from."""
+    sc = scenes.small_scene(n_particles=300, n_grid=16, material=material, seed=4, E=2e4, ys=150.0)
     target = sc.x + 0.01
     prm = {"E": 2e4, "nu": 0.3, "ys": 150.0}
     l1, g1 = _loss_grad_gpu(sc, dev, target, prm, 24, ckpt=24)
