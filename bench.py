@@ -109,8 +109,8 @@ def main():
     ap.add_argument("--pop-batched-only", action="store_true",
                     help="population workload: skip the sequential legs (for profiling the batched kernels)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="first run a plain device copy of this many MiB (3 times): the known-size kernel that "
-                         "calibrates FETCH_SIZE / WRITE_SIZE counter readings")
+                    help="first run a plain device copy of this many MiB (3 times) and a read-only sum of it "
+                         "(3 times): the known-size kernels that calibrate FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -132,6 +132,12 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log(f"[bench] copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written per launch")
+        # a read-only probe of the same size (a sum: reads it, writes one
+        # float per workgroup), which pins the FETCH_SIZE scale on its own
+        for _ in range(3):
+            src.sum()
+        torch.cuda.synchronize()
+        log(f"[bench] read probe: {args.copy_probe_mib} MiB read per launch (sum)")
         del src, dst
     if args.only == "population":
         print(json.dumps({"population": bench_population(args, dev)}), flush=True)
