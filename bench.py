@@ -266,6 +266,34 @@ def main():
         f"{dom}: {l.value} launches, "
         f"avg {avg_ms * 1000:.1f} us, {achieved:.0f} GB/s; loss={float(loss):.6e} grad={g.tolist()}")
 
+    # the same step with every rollout call replayed as a HIP graph (no
+    # per-kernel events: a graph's launches are not timed one by one)
+    gctx = rt.Context(n, N, dev).set_graphs(True)
+    gro = rt.Rollout(gctx, model, N)
+
+    def gstep():
+        final, _, _, tape = gro.forward(s0, cols(), 0.0, dt, T, tape=True, ckpt_every=CKPT_EVERY)
+        diff = final.x - target
+        d = rt.ParticleState.empty(N, dev)
+        d.x.copy_(diff * (2.0 / N))
+        _, grads = gro.backward(tape, d, flags=_lib.EMPM_ADJ_EXACT)
+        return grads
+
+    for _ in range(max(2, args.warmup)):
+        gstep()
+    torch.cuda.synchronize()
+    g0 = time.perf_counter()
+    for _ in range(args.steps):
+        gstep()
+    torch.cuda.synchronize()
+    graph_ms = (time.perf_counter() - g0) * 1000.0 / args.steps
+    gcaps, greps = gctx.graph_stats()
+    log(f"[bench] fwd+bwd with graphs: {graph_ms:.2f} ms/step ({N * T / (graph_ms / 1000.0):.4e} p-substeps/s), "
+        f"{gcaps} captures / {greps} replays")
+    graph_leg = {"ms_per_step": graph_ms, "value": N * T / (graph_ms / 1000.0), "captures": gcaps, "replays": greps}
+    del gro, gctx
+    torch.cuda.empty_cache()
+
     # forward-only throughput (reported beside the headline)
     fwd_only()
     torch.cuda.synchronize()
@@ -319,6 +347,7 @@ def main():
                    "n_particles": N, "n_grid": n, "substeps": T, "ckpt_every": CKPT_EVERY},
         "fwd_value": fwd_value,
         "fwd_ms_per_rollout": fwd_ms,
+        "fwd_bwd_graph": graph_leg,
         "hbm_roofline_fraction_fwd_bwd": (B_fb * T / (ms_per_step / 1000.0)) / 1e9 / HBM_PEAK_GBS,
         "hbm_roofline_fraction_fwd": (B_f * T / (fwd_ms / 1000.0)) / 1e9 / HBM_PEAK_GBS,
         "grad_logE_nu_logY": [float(v) for v in g.tolist()],
@@ -534,9 +563,32 @@ def bench_c1(args, dev):
         torch.cuda.synchronize()
         ms.append(e0.elapsed_time(e1))
     med = sorted(ms)[len(ms) // 2]
+    # the same rollouts replayed as a HIP graph (one launch per call instead
+    # of 8 per substep): the launch-bound regime of a small scene
+    gctx = rt.Context(n, N, dev).set_graphs(True)
+    gro = rt.Rollout(gctx, model, N)
+    for _ in range(2):
+        gro.forward(s0, cols(), 0.0, dt, T)
+    torch.cuda.synchronize()
+    gms = []
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        gf, _, _, _ = gro.forward(s0, cols(), 0.0, dt, T)
+        e1.record()
+        torch.cuda.synchronize()
+        gms.append(e0.elapsed_time(e1))
+    gmed = sorted(gms)[len(gms) // 2]
+    ef, _, _, _ = ro.forward(s0, cols(), 0.0, dt, T)
+    gerr = float((gf.x - ef.x).abs().max() / ef.x.abs().max())
+    caps, reps = gctx.graph_stats()
+    log(f"[bench] C1 graph: {gmed:.2f} ms per rollout (eager {med:.2f}), {caps} captures / {reps} replays, "
+        f"rel diff x {gerr:.1e}")
     out = {"workload": f"cube 32^3, {N} particles (8 per covered cell), FCR, 3-cell sticky walls, dt 1e-4, "
                        f"{T} forward substeps", "value": N * T / (med * 1e-3), "unit": "particle-substeps/s (forward)",
-           "ms_per_rollout_median": med}
+           "ms_per_rollout_median": med, "us_per_substep": med * 1e3 / T,
+           "graph": {"value": N * T / (gmed * 1e-3), "ms_per_rollout_median": gmed, "us_per_substep": gmed * 1e3 / T,
+                     "captures": caps, "replays": reps, "rel_diff_x_vs_eager": gerr}}
     if not args.no_cpu_baseline:
         from oracle import oracle as orc
 

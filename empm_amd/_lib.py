@@ -22,6 +22,8 @@ EMPM_E_OOM = -3
 EMPM_E_UNSUPPORTED_MATERIAL = -4
 EMPM_E_NONFINITE = -5
 EMPM_E_UNSUPPORTED = -6
+EMPM_STATUS_NONFINITE = 1
+EMPM_STATUS_CFL = 2
 
 EMPM_ADJ_EXACT = 0
 EMPM_ADJ_DROP_GRID_MASS = 1
@@ -39,6 +41,9 @@ EXPORTED_SYMBOLS = (
     "empm_nn_sqdist",
     "empm_nn_sqdist_workspace",
     "empm_nn_sqdist_noalloc",
+    "empm_read_status",
+    "empm_set_graphs",
+    "empm_graph_stats",
     "empm_nn_sqdist_segments",
     "empm_nn_sqdist_segments_workspace",
     "empm_substep_fwd",
@@ -203,6 +208,12 @@ def _declare(lib):
     lib.empm_mass_from_density.argtypes = [C.c_int, vp, vp, vp, vp]
     lib.empm_nn_sqdist.restype = C.c_int
     lib.empm_nn_sqdist.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp, C.c_size_t, vp]
+    lib.empm_set_graphs.restype = C.c_int
+    lib.empm_set_graphs.argtypes = [vp, C.c_int]
+    lib.empm_graph_stats.restype = C.c_int
+    lib.empm_graph_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    lib.empm_read_status.restype = C.c_int
+    lib.empm_read_status.argtypes = [vp, vp, C.c_int, C.c_int, vp]
     lib.empm_nn_sqdist_noalloc.restype = C.c_int
     lib.empm_nn_sqdist_noalloc.argtypes = [C.c_int, vp, C.c_int, vp, vp, vp, vp]
     lib.empm_nn_sqdist_workspace.restype = C.c_size_t

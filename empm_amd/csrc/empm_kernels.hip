@@ -149,9 +149,15 @@ EMPM_DEV void per_particle(const Batch& Bt, int mat, F body) {
   body(M, M.p0 + k);
 }
 
+// One substep's schedule entry: its time and collider table.  The node
+// passes read it from device memory (the context's schedule buffer, uploaded
+// once per call), so a captured launch sequence replays with new times and
+// collider positions.
 struct ColTable {
-  empm_collider c[EMPM_MAX_COLLIDERS];
+  float time;
   int n;
+  int pad0, pad1;
+  empm_collider c[EMPM_MAX_COLLIDERS];
 };
 
 // Quadratic B-spline stencil of one particle (mpm_utils.py:365-383).  The
@@ -341,8 +347,9 @@ EMPM_DEV void collide(const empm_collider& c, int ix, int iy, int iz, int n, flo
 
 // grid_normalization_and_gravity (mpm_utils.py:427-438) + add_damping_via_grid
 // (mpm_utils.py:722-734) + colliders, for one node.
-EMPM_DEV void grid_node(const DevModel& M, const ColTable& T, float time, float dt, int ix, int iy, int iz,
-                        const float4 g, float* v, float* fac) {
+EMPM_DEV void grid_node(const DevModel& M, const ColTable& T, float dt, int ix, int iy, int iz, const float4 g,
+                        float* v, float* fac) {
+  const float time = T.time;
   v[0] = v[1] = v[2] = 0.f;
   fac[0] = fac[1] = fac[2] = 1.f;
   if (g.w > 1e-15f) {
@@ -392,6 +399,8 @@ constexpr int kHeavyChunks = 8;   // blocks with more chunks than this are scatt
 constexpr int kDynamicRounds = 4; // more blocks per scatter workgroup than this: handed out dynamically
 constexpr int kWave = 64;
 constexpr int kAdjPayload = 4;    // floats per particle of the G2P-adjoint scatter payload (dL/dv', pad)
+constexpr int kSchedCap = 1024;   // substeps per schedule upload (longer calls run in chunks)
+constexpr size_t kGraphCache = 8; // instantiated graphs kept per context (least recently used evicted)
 
 // Particle binning of the current segment: perm[k] = k-th particle in
 // block-major order, offs[b * 8 + c] = first position of cell c of block b
@@ -930,7 +939,7 @@ EMPM_DEV void for_near_nodes(const Batch& Bt, const Binning& B, F f) {
   else for_near_nodes_t<false>(Bt, B, f);
 }
 
-__global__ __launch_bounds__(kBlock) void k_grid(Batch Bt, ColTable T, float time, float dt, Binning B,
+__global__ __launch_bounds__(kBlock) void k_grid(Batch Bt, const ColTable* __restrict__ T, float dt, Binning B,
                                                  const float4* __restrict__ tiles, Overflow ov,
                                                  float4* __restrict__ keep, float4* __restrict__ v_out) {
   for_near_nodes(Bt, B, [&](const DevModel& M, int ix, int iy, int iz, size_t id) {
@@ -947,7 +956,7 @@ __global__ __launch_bounds__(kBlock) void k_grid(Batch Bt, ColTable T, float tim
   }
   if (keep) keep[id] = g;
   float v[3], fac[3];
-  grid_node(M, T, time, dt, ix, iy, iz, g, v, fac);
+  grid_node(M, *T, dt, ix, iy, iz, g, v, fac);
   v_out[id] = make_float4(v[0], v[1], v[2], g.w);
   });
 }
@@ -964,7 +973,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
                                                 float* __restrict__ C1, float* __restrict__ F1, int next_stress,
                                                 float* Fe_next, float* __restrict__ S_next, int write_params,
                                                 int write_F, int* __restrict__ count, int2* __restrict__ key_rank,
-                                                float* __restrict__ mst_rec_next) {
+                                                float* __restrict__ mst_rec_next, int* __restrict__ status) {
   // with count set, the counting step of the next substep's binning is done
   // here on the new positions (k_bin_count's work), last: counting in the
   // middle keeps the gather's state live across the stress stage
@@ -1053,6 +1062,18 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
   load9(Fe, p, Fp);
   m_mul(IF, Fp, Fn);
   if (write_F) store9(F1, p, Fn);
+  // failure flags of the simulation (read once per rollout by the host):
+  // a non-finite new state, or a particle faster than a cell per substep
+  {
+    float acc = xo[0] + xo[1] + xo[2] + nv[0] + nv[1] + nv[2];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) acc += Fn[i];
+    const bool nonfinite = !isfinite(acc);
+    const float cfl = dt * fmaxf(fmaxf(fabsf(nv[0]), fabsf(nv[1])), fabsf(nv[2]));
+    const bool fast = cfl > M.dx;
+    if (nonfinite || fast)
+      atomicOr(&status[&M - Bt.cand], (nonfinite ? EMPM_STATUS_NONFINITE : 0) | (fast ? EMPM_STATUS_CFL : 0));
+  }
   if (next_stress) particle_stress<MAT>(M, dt, p, Fn, Fe_next, S_next, write_params, Bt.N, nullptr, mst_rec_next);
   if (count) count_bin(true, bin_key(xo, M), p, count, key_rank);
   });
@@ -1240,7 +1261,7 @@ __global__ __launch_bounds__(kBlkWG) void k_g2p_adj_scatter(Batch Bt, float dt, 
 // node's forward (normalisation, damping, colliders) to get dv_out/dv per
 // component, and write dL/d(mv, m).  The grid-mass term is dropped when
 // `drop_mass` is set (reference Tape behaviour).
-__global__ __launch_bounds__(kBlock) void k_grid_adj(Batch Bt, ColTable T, float time, float dt, Binning B,
+__global__ __launch_bounds__(kBlock) void k_grid_adj(Batch Bt, const ColTable* __restrict__ T, float dt, Binning B,
                                                      const float4* __restrict__ tiles, Overflow ov,
                                                      const float4* __restrict__ keep, float4* __restrict__ gadj_in,
                                                      int drop_mass) {
@@ -1257,7 +1278,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_adj(Batch Bt, ColTable T, float
   }
   const float4 g = keep[id];
   float v[3], fac[3];
-  grid_node(M, T, time, dt, ix, iy, iz, g, v, fac);
+  grid_node(M, *T, dt, ix, iy, iz, g, v, fac);
   float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
   if (g.w > 1e-15f) {
     const float im = 1.f / g.w;
@@ -1945,6 +1966,30 @@ struct empm_ctx {
   DevModel* cand_pinned = nullptr;  // its pinned staging copy
   std::vector<DevModel> cand_last;  // what cand_dev holds
   hipEvent_t cand_upload = nullptr; // completion of the last table upload
+  int* status = nullptr;            // max_cand: per-simulation failure bits (EMPM_STATUS_*), set by k_g2p
+  int* status_host = nullptr;       // its pinned read-back copy
+  // per-substep schedule (time, collider table) of the current call, read by
+  // the node passes; uploaded from a two-slot pinned ring
+  ColTable* sched = nullptr;        // kSchedCap entries
+  ColTable* sched_host[2] = {nullptr, nullptr};
+  hipEvent_t sched_ev[2] = {nullptr, nullptr};
+  int sched_slot = 0;
+  hipStream_t last_stream = nullptr;  // stream of the previous call (calls on another stream are ordered after it)
+  bool have_last_stream = false;
+  hipEvent_t order_ev = nullptr;
+  // HIP graph replay of whole calls (empm_set_graphs)
+  bool graphs = false;
+  hipStream_t gstream = nullptr;    // capture / replay stream (default priority)
+  hipEvent_t gev_in = nullptr, gev_out = nullptr;
+  struct GraphEntry {
+    std::vector<unsigned long long> key;
+    hipGraphExec_t exec = nullptr;
+    bool pending_after = false;
+    unsigned long long used = 0;
+  };
+  std::vector<GraphEntry> gcache;
+  unsigned long long gclock = 0;
+  long long graph_captures = 0, graph_replays = 0;
   KernelProfile* prof = nullptr;
 };
 
@@ -2306,7 +2351,7 @@ int do_bin(empm_ctx* ctx, const HostBatch& hb, const float* x, const BinBuf& out
 // One forward substep in -> out, binning `in` first (into the record's bins
 // when given).  `first`: run the stress stage here (else the previous
 // substep's G2P did); `last`: no stress stage for a next substep.
-int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time, float dt, const empm_particles* in,
+int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable* T, float dt, const empm_particles* in,
                const empm_particles* out, const empm_grid_rec* rec, bool first, bool last, hipStream_t st,
                bool replay = false) {
   float* Fe = in->Fe ? in->Fe : ctx->Fe;
@@ -2329,7 +2374,7 @@ int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
-    hipLaunchKernelGGL(k_grid, node_grid(hb.nblk), dim3(kBlock), 0, st, hb.dev, T, time, dt, B, ctx->tiles,
+    hipLaunchKernelGGL(k_grid, node_grid(hb.nblk), dim3(kBlock), 0, st, hb.dev, T, dt, B, ctx->tiles,
                        overflow(ctx), keep, v_out);
   }
   if (hb.N > 0) {
@@ -2340,7 +2385,7 @@ int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
     // a next substep also gets its binning's counting step from here
     EMPM_LAUNCH_RANGES(hb, k_g2p, st, dt, in->x, Fe, v_out, out->x, out->v, out->C, out->F, last ? 0 : 1, Fe_next,
                        ctx->S, 1, (last || rec) ? 1 : 0, last ? nullptr : ctx->count,
-                       last ? nullptr : ctx->key_rank, last ? nullptr : out->mstate);
+                       last ? nullptr : ctx->key_rank, last ? nullptr : out->mstate, ctx->status);
     if (!last) ctx->counts_pending = true;
   }
   HIP_TRY(hipGetLastError());
@@ -2350,7 +2395,7 @@ int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
 // Adjoint of one substep.  With a recorded grid and Fe (rec->mv_m, v_out and
 // in->Fe) the forward stress, P2G and boundary pass are not recomputed; with
 // recorded bins (rec->bins) the particles are not re-binned.
-int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time, float dt, const empm_particles* in,
+int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable* T, float dt, const empm_particles* in,
                const empm_grid_rec* rec, const empm_particles* d_out, const empm_particles* d_in, float* d_mu,
                float* d_lam, float* d_ys, float* d_mass, float* d_vol, int flags, hipStream_t st) {
   const bool recorded = rec && rec->mv_m && rec->v_out && in->Fe;
@@ -2376,7 +2421,7 @@ int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
     }
     {
       ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
-      hipLaunchKernelGGL(k_grid, node_grid(hb.nblk), dim3(kBlock), 0, st, hb.dev, T, time, dt, B, ctx->tiles,
+      hipLaunchKernelGGL(k_grid, node_grid(hb.nblk), dim3(kBlock), 0, st, hb.dev, T, dt, B, ctx->tiles,
                          overflow(ctx), keep, v_out);
     }
   }
@@ -2394,7 +2439,7 @@ int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable& T, float time
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID_ADJ, st);
-    hipLaunchKernelGGL(k_grid_adj, node_grid(hb.nblk), dim3(kBlock), 0, st, hb.dev, T, time, dt, B, ctx->tiles,
+    hipLaunchKernelGGL(k_grid_adj, node_grid(hb.nblk), dim3(kBlock), 0, st, hb.dev, T, dt, B, ctx->tiles,
                        overflow(ctx), keep, ctx->gadj_in, (flags & EMPM_ADJ_DROP_GRID_MASS) ? 1 : 0);
   }
   if (hb.N > 0) {
@@ -2461,6 +2506,15 @@ int alloc_ctx(empm_ctx* c) {
   e = e == hipSuccess ? hipMalloc((void**)&c->cand_dev, cand_bytes) : e;
   e = e == hipSuccess ? hipHostMalloc((void**)&c->cand_pinned, cand_bytes) : e;
   e = e == hipSuccess ? hipEventCreateWithFlags(&c->cand_upload, hipEventDisableTiming) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->sched, kSchedCap * sizeof(ColTable)) : e;
+  for (int i = 0; i < 2; ++i) {
+    e = e == hipSuccess ? hipHostMalloc((void**)&c->sched_host[i], kSchedCap * sizeof(ColTable)) : e;
+    e = e == hipSuccess ? hipEventCreateWithFlags(&c->sched_ev[i], hipEventDisableTiming) : e;
+  }
+  e = e == hipSuccess ? hipEventCreateWithFlags(&c->order_ev, hipEventDisableTiming) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->status, (size_t)c->max_cand * sizeof(int)) : e;
+  e = e == hipSuccess ? hipHostMalloc((void**)&c->status_host, (size_t)c->max_cand * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMemset(c->status, 0, (size_t)c->max_cand * sizeof(int)) : e;
   e = e == hipSuccess ? hipMemset(c->count, 0, nblk * 8 * sizeof(int)) : e;
   e = e == hipSuccess ? hipMemset(c->ovf, 0, G * sizeof(float4)) : e;
   e = e == hipSuccess ? hipMemset(c->ovf_flag, 0, sizeof(int)) : e;
@@ -2480,7 +2534,176 @@ int alloc_ctx(empm_ctx* c) {
   return EMPM_OK;
 }
 
+// A call on a stream other than the previous call's is ordered after that
+// call (the context's scratch and schedule buffers are shared by its calls).
+int order_after_last(empm_ctx* ctx, hipStream_t st) {
+  if (ctx->have_last_stream && ctx->last_stream != st) {
+    HIP_TRY(hipEventRecord(ctx->order_ev, ctx->last_stream));
+    HIP_TRY(hipStreamWaitEvent(st, ctx->order_ev, 0));
+  }
+  ctx->last_stream = st;
+  ctx->have_last_stream = true;
+  return EMPM_OK;
+}
+
+// The pinned staging slot for the next schedule upload (waits until the
+// upload that last used it has been copied).
+ColTable* sched_stage(empm_ctx* ctx) {
+  (void)hipEventSynchronize(ctx->sched_ev[ctx->sched_slot]);
+  return ctx->sched_host[ctx->sched_slot];
+}
+
+// Copy n staged entries to the device schedule, stream-ordered on st.
+int sched_commit(empm_ctx* ctx, int n, hipStream_t st) {
+  const int slot = ctx->sched_slot;
+  HIP_TRY(hipMemcpyAsync(ctx->sched, ctx->sched_host[slot], (size_t)n * sizeof(ColTable), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipEventRecord(ctx->sched_ev[slot], st));
+  ctx->sched_slot = slot ^ 1;
+  return EMPM_OK;
+}
+
+// What a captured launch sequence bakes in besides the per-substep
+// pointers: the batch's launch shapes (read at run time from the device
+// table are the simulations' scalars and arrays, so they are not part of it).
+void key_batch(std::vector<unsigned long long>& k, const HostBatch& hb) {
+  const Batch& b = hb.dev;
+  k.push_back((unsigned long long)b.ncand);
+  k.push_back((unsigned long long)b.N);
+  k.push_back((unsigned long long)b.nblk);
+  k.push_back((unsigned long long)b.nwaves);
+  for (int m = 0; m <= 8; ++m) k.push_back((unsigned long long)b.moff[m]);
+  unsigned long long bits = 0;
+  for (int m = 0; m < 8; ++m) {
+    k.push_back((unsigned long long)b.mwaves[m]);
+    bits |= (hb.has_material[m] ? 1ull : 0ull) << m;
+    bits |= (hb.mutates[m] ? 1ull : 0ull) << (8 + m);
+  }
+  k.push_back(bits);
+}
+void key_ptr(std::vector<unsigned long long>& k, const void* p) { k.push_back((unsigned long long)(uintptr_t)p); }
+void key_state(std::vector<unsigned long long>& k, const empm_particles& s) {
+  key_ptr(k, s.x), key_ptr(k, s.v), key_ptr(k, s.C), key_ptr(k, s.F), key_ptr(k, s.Fe), key_ptr(k, s.mstate);
+}
+void key_grid(std::vector<unsigned long long>& k, const empm_grid_rec* g) {
+  if (!g) {
+    k.push_back(0);
+    return;
+  }
+  key_ptr(k, g->mv_m), key_ptr(k, g->v_out), key_ptr(k, g->bins);
+}
+unsigned long long fbits(double d) {
+  unsigned long long u;
+  memcpy(&u, &d, sizeof(u));
+  return u;
+}
+
+// Run one call's launches: eagerly on st, or (graph mode) as an instantiated
+// HIP graph cached under `key` and replayed on the context's own stream,
+// ordered after st's earlier work and before its later work by events.  The
+// graph's kernels read the schedule buffer and the simulation table at run
+// time, so replays with new times, collider positions and parameters need no
+// re-capture; what the launch sequence bakes in (pointers, counts, flags) is
+// in the key.
+template <typename Body>
+int run_call(empm_ctx* ctx, hipStream_t st, std::vector<unsigned long long>& key, Body&& body) {
+  if (!ctx->graphs || (ctx->prof && ctx->prof->mask)) return body(st);
+  key.push_back(ctx->counts_pending ? 1 : 0);
+  empm_ctx::GraphEntry* e = nullptr;
+  for (auto& g : ctx->gcache)
+    if (g.key == key) e = &g;
+  if (e) {
+    ctx->counts_pending = e->pending_after;
+    ++ctx->graph_replays;
+  } else {
+    const bool pending_before = ctx->counts_pending;
+    HIP_TRY(hipStreamBeginCapture(ctx->gstream, hipStreamCaptureModeThreadLocal));
+    const int rc = body(ctx->gstream);
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(ctx->gstream, &graph);
+    if (rc || ce != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      ctx->counts_pending = pending_before;
+      return rc ? rc : set_err(EMPM_E_HIP, "graph capture: %s", hipGetErrorString(ce));
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      ctx->counts_pending = pending_before;
+      return set_err(EMPM_E_HIP, "graph instantiate: %s", hipGetErrorString(ie));
+    }
+    if (ctx->gcache.size() >= kGraphCache) {
+      size_t lru = 0;
+      for (size_t i = 1; i < ctx->gcache.size(); ++i)
+        if (ctx->gcache[i].used < ctx->gcache[lru].used) lru = i;
+      HIP_TRY(hipStreamSynchronize(ctx->gstream));
+      (void)hipGraphExecDestroy(ctx->gcache[lru].exec);
+      ctx->gcache.erase(ctx->gcache.begin() + (long)lru);
+    }
+    empm_ctx::GraphEntry ne;
+    ne.key = key;
+    ne.exec = exec;
+    ne.pending_after = ctx->counts_pending;
+    ctx->gcache.push_back(std::move(ne));
+    e = &ctx->gcache.back();
+    ++ctx->graph_captures;
+  }
+  e->used = ++ctx->gclock;
+  HIP_TRY(hipEventRecord(ctx->gev_in, st));
+  HIP_TRY(hipStreamWaitEvent(ctx->gstream, ctx->gev_in, 0));
+  HIP_TRY(hipGraphLaunch(e->exec, ctx->gstream));
+  HIP_TRY(hipEventRecord(ctx->gev_out, ctx->gstream));
+  HIP_TRY(hipStreamWaitEvent(st, ctx->gev_out, 0));
+  return EMPM_OK;
+}
+
 // Forward rollout of a batch over n_steps substeps (see empm_rollout_fwd).
+// With cols (and time) the colliders are advanced on the host per substep
+// and the table / time each substep saw optionally recorded; with
+// cols_hist / times_hist given instead (a replay) they are replayed.
+int forward_calls(empm_ctx* ctx, const HostBatch& hb, empm_collider* cols, int ncol, double* time,
+                  const empm_collider* hist_in, const double* times_in, double dt, int n_steps,
+                  const empm_particles* states, const empm_grid_rec* grids, empm_collider* cols_hist,
+                  double* times_hist, bool replay, hipStream_t st) {
+  for (int k0 = 0; k0 < n_steps; k0 += kSchedCap) {
+    const int n = std::min(kSchedCap, n_steps - k0);
+    ColTable* stage = sched_stage(ctx);
+    for (int j = 0; j < n; ++j) {
+      const int k = k0 + j;
+      int rc;
+      if (replay) {
+        if ((rc = make_table(&hist_in[(size_t)k * ncol], ncol, stage[j]))) return rc;
+        stage[j].time = (float)times_in[k];
+      } else {
+        if ((rc = make_table(cols, ncol, stage[j]))) return rc;
+        stage[j].time = (float)*time;
+        if (cols_hist) memcpy(&cols_hist[(size_t)k * ncol], cols, sizeof(empm_collider) * ncol);
+        if (times_hist) times_hist[k] = *time;
+        advance_colliders(cols, ncol, *time, dt);
+        *time = *time + dt;
+      }
+    }
+    int rc = sched_commit(ctx, n, st);
+    if (rc) return rc;
+    std::vector<unsigned long long> key = {replay ? 2ull : 1ull, (unsigned long long)k0, (unsigned long long)n,
+                                           (unsigned long long)n_steps, fbits(dt)};
+    key_batch(key, hb);
+    for (int k = k0; k <= k0 + n; ++k) key_state(key, states[k]);
+    for (int k = k0; k < k0 + n; ++k) key_grid(key, grids ? &grids[k] : nullptr);
+    rc = run_call(ctx, st, key, [&](hipStream_t cs) {
+      for (int j = 0; j < n; ++j) {
+        const int k = k0 + j;
+        const int r = launch_fwd(ctx, hb, ctx->sched + j, (float)dt, &states[k], &states[k + 1],
+                                 grids ? &grids[k] : nullptr, k == 0, k == n_steps - 1, cs, replay);
+        if (r) return r;
+      }
+      return (int)EMPM_OK;
+    });
+    if (rc) return rc;
+  }
+  return EMPM_OK;
+}
+
 int rollout_fwd(empm_ctx* ctx, const empm_batch* batch, empm_collider* cols, int ncol, double* time, double dt,
                 int n_steps, const empm_particles* states, const empm_grid_rec* grids, empm_collider* cols_hist,
                 double* times_hist, hipStream_t st) {
@@ -2496,18 +2719,10 @@ int rollout_fwd(empm_ctx* ctx, const empm_batch* batch, empm_collider* cols, int
   ColTable T;
   if ((rc = make_table(cols, ncol, T))) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
+  if ((rc = order_after_last(ctx, st))) return rc;
   if ((rc = upload_batch(ctx, hb, st))) return rc;
-  for (int k = 0; k < n_steps; ++k) {
-    make_table(cols, ncol, T);
-    if (cols_hist) memcpy(&cols_hist[(size_t)k * ncol], cols, sizeof(empm_collider) * ncol);
-    if (times_hist) times_hist[k] = *time;
-    if ((rc = launch_fwd(ctx, hb, T, (float)*time, (float)dt, &states[k], &states[k + 1], grids ? &grids[k] : nullptr,
-                         k == 0, k == n_steps - 1, st)))
-      return rc;
-    advance_colliders(cols, ncol, *time, dt);
-    *time = *time + dt;
-  }
-  return EMPM_OK;
+  return forward_calls(ctx, hb, cols, ncol, time, nullptr, nullptr, dt, n_steps, states, grids, cols_hist, times_hist,
+                       false, st);
 }
 
 int rollout_fwd_replay(empm_ctx* ctx, const empm_batch* batch, const empm_collider* cols_hist, int ncol,
@@ -2525,15 +2740,10 @@ int rollout_fwd_replay(empm_ctx* ctx, const empm_batch* batch, const empm_collid
       return set_err(EMPM_E_INVALID_ARG, "states[%d] and states[%d] alias", k, k + 1);
   if (n_steps > 0 && (rc = check_mstate(hb, states, 1, "empm_rollout_fwd_replay"))) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
+  if ((rc = order_after_last(ctx, st))) return rc;
   if ((rc = upload_batch(ctx, hb, st))) return rc;
-  ColTable T;
-  for (int k = 0; k < n_steps; ++k) {
-    if ((rc = make_table(&cols_hist[(size_t)k * ncol], ncol, T))) return rc;
-    if ((rc = launch_fwd(ctx, hb, T, (float)times_hist[k], (float)dt, &states[k], &states[k + 1],
-                         grids ? &grids[k] : nullptr, k == 0, k == n_steps - 1, st, true)))
-      return rc;
-  }
-  return EMPM_OK;
+  return forward_calls(ctx, hb, nullptr, ncol, nullptr, cols_hist, times_hist, dt, n_steps, states, grids, nullptr,
+                       nullptr, true, st);
 }
 
 int rollout_adj(empm_ctx* ctx, const empm_batch* batch, const empm_collider* cols_hist, int ncol,
@@ -2550,16 +2760,42 @@ int rollout_adj(empm_ctx* ctx, const empm_batch* batch, const empm_collider* col
   if ((rc = check_mstate(hb, states, n_steps, "empm_rollout_adj"))) return rc;
   if ((rc = check_particles(&adj[0], hb.N, "adj[0]")) || (rc = check_particles(&adj[1], hb.N, "adj[1]"))) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
+  if ((rc = order_after_last(ctx, st))) return rc;
   if ((rc = upload_batch(ctx, hb, st))) return rc;
-  ColTable T;
-  for (int j = 0; j < n_steps; ++j) {
-    const int k = n_steps - 1 - j;
-    if ((rc = make_table(&cols_hist[(size_t)k * ncol], ncol, T))) return rc;
-    const empm_particles* dout = &adj[j % 2];
-    const empm_particles* din = &adj[(j + 1) % 2];
-    if ((rc = launch_adj(ctx, hb, T, (float)times_hist[k], (float)dt, &states[k], grids ? &grids[k] : nullptr, dout,
-                         din, d_mu, d_lam, d_ys, d_mass, d_vol, flags, st)))
-      return rc;
+  // reverse order: sweep j = 0.. handles substep k = n_steps - 1 - j
+  for (int j0 = 0; j0 < n_steps; j0 += kSchedCap) {
+    const int n = std::min(kSchedCap, n_steps - j0);
+    ColTable* stage = sched_stage(ctx);
+    for (int i = 0; i < n; ++i) {
+      const int k = n_steps - 1 - (j0 + i);
+      if ((rc = make_table(&cols_hist[(size_t)k * ncol], ncol, stage[i]))) return rc;
+      stage[i].time = (float)times_hist[k];
+    }
+    if ((rc = sched_commit(ctx, n, st))) return rc;
+    std::vector<unsigned long long> key = {3ull, (unsigned long long)j0, (unsigned long long)n,
+                                           (unsigned long long)n_steps, fbits(dt), (unsigned long long)flags};
+    key_batch(key, hb);
+    key_state(key, adj[0]);
+    key_state(key, adj[1]);
+    for (const float* q : {d_mu, d_lam, d_ys, d_mass, d_vol}) key_ptr(key, q);
+    for (int i = 0; i < n; ++i) {
+      const int k = n_steps - 1 - (j0 + i);
+      key_state(key, states[k]);
+      key_grid(key, grids ? &grids[k] : nullptr);
+    }
+    rc = run_call(ctx, st, key, [&](hipStream_t cs) {
+      for (int i = 0; i < n; ++i) {
+        const int j = j0 + i;
+        const int k = n_steps - 1 - j;
+        const empm_particles* dout = &adj[j % 2];
+        const empm_particles* din = &adj[(j + 1) % 2];
+        const int r = launch_adj(ctx, hb, ctx->sched + i, (float)dt, &states[k], grids ? &grids[k] : nullptr, dout,
+                                 din, d_mu, d_lam, d_ys, d_mass, d_vol, flags, cs);
+        if (r) return r;
+      }
+      return (int)EMPM_OK;
+    });
+    if (rc) return rc;
   }
   return EMPM_OK;
 }
@@ -2664,7 +2900,69 @@ void empm_destroy(empm_ctx* c) {
   (void)hipFree(c->cand_dev);
   if (c->cand_pinned) (void)hipHostFree(c->cand_pinned);
   if (c->cand_upload) (void)hipEventDestroy(c->cand_upload);
+  (void)hipFree(c->status);
+  if (c->status_host) (void)hipHostFree(c->status_host);
+  (void)hipFree(c->sched);
+  for (int i = 0; i < 2; ++i) {
+    if (c->sched_host[i]) (void)hipHostFree(c->sched_host[i]);
+    if (c->sched_ev[i]) (void)hipEventDestroy(c->sched_ev[i]);
+  }
+  if (c->order_ev) (void)hipEventDestroy(c->order_ev);
+  for (auto& g : c->gcache) (void)hipGraphExecDestroy(g.exec);
+  if (c->gstream) (void)hipStreamDestroy(c->gstream);
+  if (c->gev_in) (void)hipEventDestroy(c->gev_in);
+  if (c->gev_out) (void)hipEventDestroy(c->gev_out);
   delete c;
+}
+
+int empm_set_graphs(empm_ctx* ctx, int enable) {
+  if (!ctx) return set_err(EMPM_E_INVALID_ARG, "empm_set_graphs: null context");
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (enable && !ctx->gstream) {
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->gstream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&ctx->gev_in, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&ctx->gev_out, hipEventDisableTiming));
+  }
+  if (!enable && !ctx->gcache.empty()) {
+    HIP_TRY(hipStreamSynchronize(ctx->gstream));
+    for (auto& g : ctx->gcache) (void)hipGraphExecDestroy(g.exec);
+    ctx->gcache.clear();
+  }
+  ctx->graphs = enable != 0;
+  return EMPM_OK;
+}
+
+int empm_graph_stats(empm_ctx* ctx, long long* captures, long long* replays) {
+  if (!ctx) return set_err(EMPM_E_INVALID_ARG, "empm_graph_stats: null context");
+  if (captures) *captures = ctx->graph_captures;
+  if (replays) *replays = ctx->graph_replays;
+  return EMPM_OK;
+}
+
+int empm_read_status(empm_ctx* ctx, int* status, int n, int clear, void* stream) {
+  if (!ctx || n < 0 || (n > 0 && !status))
+    return set_err(EMPM_E_INVALID_ARG, "empm_read_status: bad arguments");
+  if (n > ctx->max_cand) n = ctx->max_cand;
+  hipStream_t st = (hipStream_t)stream;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipMemcpyAsync(ctx->status_host, ctx->status, (size_t)ctx->max_cand * sizeof(int), hipMemcpyDeviceToHost,
+                         st));
+  if (clear)  // a fill kernel, not a memset node, so that callers may capture this call
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblocks((size_t)ctx->max_cand)), dim3(kBlock), 0, st, ctx->status,
+                       (size_t)ctx->max_cand, 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  int any = 0, first = -1;
+  for (int c = 0; c < ctx->max_cand; ++c) {
+    if (c < n) status[c] = ctx->status_host[c];
+    any |= ctx->status_host[c];
+    if (first < 0 && (ctx->status_host[c] & EMPM_STATUS_NONFINITE)) first = c;
+  }
+  if (any & EMPM_STATUS_NONFINITE)
+    return set_err(EMPM_E_NONFINITE,
+                   "non-finite particle state (x, v or F_trial) produced by simulation %d since the last status read",
+                   first);
+  return EMPM_OK;
 }
 
 int empm_sort_by_cell(empm_ctx* ctx, int n_particles, const float* x, float inv_dx, int* perm, void* stream) {
@@ -2929,13 +3227,18 @@ int empm_substep_fwd(empm_ctx* ctx, const empm_model* model, const empm_collider
   const empm_batch b = batch_of(model);
   HostBatch hb;
   hipStream_t st = (hipStream_t)stream;
-  int rc = prepare(ctx, &b, hb, st);
+  int rc = build_batch(ctx, &b, hb);
   if (rc) return rc;
   if ((rc = check_particles(in, hb.N, "in")) || (rc = check_particles(out, hb.N, "out"))) return rc;
   if (hb.N > 0 && (in->x == out->x || in->F == out->F)) return set_err(EMPM_E_INVALID_ARG, "in and out states alias");
-  ColTable T;
-  if ((rc = make_table(cols, ncol, T))) return rc;
-  return launch_fwd(ctx, hb, T, time, dt, in, out, nullptr, true, true, st);
+  HIP_TRY(hipSetDevice(ctx->device));
+  if ((rc = order_after_last(ctx, st))) return rc;
+  if ((rc = upload_batch(ctx, hb, st))) return rc;
+  ColTable* T = sched_stage(ctx);
+  if ((rc = make_table(cols, ncol, *T))) return rc;
+  T->time = time;
+  if ((rc = sched_commit(ctx, 1, st))) return rc;
+  return launch_fwd(ctx, hb, ctx->sched, dt, in, out, nullptr, true, true, st);
 }
 
 int empm_substep_adj(empm_ctx* ctx, const empm_model* model, const empm_collider* cols, int ncol, float time, float dt,
@@ -2950,11 +3253,14 @@ int empm_substep_adj(empm_ctx* ctx, const empm_model* model, const empm_collider
   if ((rc = check_particles(in, hb.N, "in")) || (rc = check_particles(d_out, hb.N, "d_out")) ||
       (rc = check_particles(d_in, hb.N, "d_in")))
     return rc;
-  ColTable T;
-  if ((rc = make_table(cols, ncol, T))) return rc;
   HIP_TRY(hipSetDevice(ctx->device));
+  if ((rc = order_after_last(ctx, st))) return rc;
   if ((rc = upload_batch(ctx, hb, st))) return rc;
-  return launch_adj(ctx, hb, T, time, dt, in, nullptr, d_out, d_in, d_mu, d_lam, d_ys, d_mass, d_vol, flags, st);
+  ColTable* T = sched_stage(ctx);
+  if ((rc = make_table(cols, ncol, *T))) return rc;
+  T->time = time;
+  if ((rc = sched_commit(ctx, 1, st))) return rc;
+  return launch_adj(ctx, hb, ctx->sched, dt, in, nullptr, d_out, d_in, d_mu, d_lam, d_ys, d_mass, d_vol, flags, st);
 }
 
 int empm_rollout_fwd(empm_ctx* ctx, const empm_model* model, empm_collider* cols, int ncol, double* time, double dt,

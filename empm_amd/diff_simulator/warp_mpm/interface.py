@@ -62,8 +62,9 @@ def _run_forward(solver, state, model, dt, n_steps, record, want_tape, step_call
     if cb is None:
         cols = solver.collider_table()
         cols_out = []
+        # the user-facing API reads the device's failure flags once per call
         final, t1, xs, tape = ro.forward(s0, cols, solver.time, dt, n_steps, record=record, tape=want_tape,
-                                         ckpt_every=K, cols_out=cols_out)
+                                         ckpt_every=K, cols_out=cols_out, check=True)
         solver.sync_colliders_from(cols_out)
         solver.time = t1
         return final, xs, tape, ro
@@ -78,7 +79,8 @@ def _run_forward(solver, state, model, dt, n_steps, record, want_tape, step_call
         cb(k)
         cols = solver.collider_table()
         cols_out = []
-        nxt, t1, _, tp = ro.forward(cur, cols, solver.time, dt, 1, tape=want_tape, ckpt_every=1, cols_out=cols_out)
+        nxt, t1, _, tp = ro.forward(cur, cols, solver.time, dt, 1, tape=want_tape, ckpt_every=1, cols_out=cols_out,
+                                    check=(k == n_steps - 1))
         solver.sync_colliders_from(cols_out)
         solver.time = t1
         if want_tape:

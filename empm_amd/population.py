@@ -101,6 +101,7 @@ class PopulationEvaluator:
         self.ctx = rt.Context(max(sc.n_grid for sc in scenes), self.n, self.device, max_cand=self.n_cand)
         self._solo_ctx = None
         self.targets = targets
+        self.failed: list[int] = []
 
     # ------------------------------------------------------------ arena set-up
     def _per_particle(self, vals):
@@ -145,11 +146,27 @@ class PopulationEvaluator:
 
     # ------------------------------------------------------------ batched
     def rollout(self, cands: list[Candidate]) -> rt.ParticleState:
-        """Final state of every candidate's rollout (arena order)."""
+        """Final state of every candidate's rollout (arena order); the
+        candidates whose rollout produced a non-finite state are in
+        self.failed afterwards."""
         bm, _, _ = self.model(cands)
         ro = rt.Rollout(self.ctx, bm, self.n)
-        final, _, _, _ = ro.forward(self.initial_state(), list(self.cols), 0.0, self.dt, self.n_steps)
+        final, _, _, _ = ro.forward(self.initial_state(), list(self.cols), 0.0, self.dt, self.n_steps, check=False)
+        self._read_failures()
         return final
+
+    def _read_failures(self):
+        """Per-candidate failure flags of the last batched forward (one read):
+        a candidate whose parameters drive its rollout to a non-finite state
+        gets an infinite fitness instead of failing the whole population."""
+        _, st = self.ctx.read_status()
+        self.failed = [c for c in range(self.n_cand) if st[c] & _lib.EMPM_STATUS_NONFINITE]
+
+    def _mask_failed(self, fit: torch.Tensor) -> torch.Tensor:
+        if self.failed:
+            fit = fit.clone()
+            fit[torch.as_tensor(self.failed, device=fit.device)] = float("inf")
+        return fit
 
     def set_targets(self, cands: list[Candidate]):
         """Target final positions: the rollout of these candidates."""
@@ -158,7 +175,7 @@ class PopulationEvaluator:
     def fitness(self, cands: list[Candidate]) -> torch.Tensor:
         """Per-candidate mean over its particles of |x_T - x*_T|^2 (float64)."""
         d = self.rollout(cands).x - self.targets
-        return self._seg_sum((d * d).sum(1)) / self.counts_t
+        return self._mask_failed(self._seg_sum((d * d).sum(1)) / self.counts_t)
 
     def tape_bytes(self, ckpt_every: int) -> int:
         bm, _, _ = self.model([Candidate(1e5, 0.3, 1e3)] * self.n_cand)
@@ -174,9 +191,10 @@ class PopulationEvaluator:
         ys0 = bm.yield_stress.clone()  # plasticine rewrites it during the rollout
         ro = rt.Rollout(self.ctx, bm, self.n)
         final, _, _, tape = ro.forward(self.initial_state(), list(self.cols), 0.0, self.dt, self.n_steps,
-                                       tape=True, ckpt_every=ckpt_every)
+                                       tape=True, ckpt_every=ckpt_every, check=False)
+        self._read_failures()
         diff = final.x - self.targets
-        fit = self._seg_sum((diff * diff).sum(1)) / self.counts_t
+        fit = self._mask_failed(self._seg_sum((diff * diff).sum(1)) / self.counts_t)
         d = rt.ParticleState.empty(self.n, self.device)
         d.zero_()
         scale = (2.0 / self.counts_t.float())[self.cand_id]

@@ -63,6 +63,38 @@ class Context:
     def handle(self):
         return self._h
 
+    def set_graphs(self, enable: bool = True):
+        """Replay every rollout call of this context as a cached HIP graph."""
+        check(self.lib.empm_set_graphs(self._h, int(bool(enable))))
+        return self
+
+    def graph_stats(self):
+        """(captures, replays) of this context's graph cache."""
+        a, b = C.c_longlong(), C.c_longlong()
+        check(self.lib.empm_graph_stats(self._h, C.byref(a), C.byref(b)))
+        return int(a.value), int(b.value)
+
+    def read_status(self, clear: bool = True):
+        """(return code, per-simulation EMPM_STATUS_* bits) since the last read:
+        one synchronisation of the device's current stream."""
+        n = int(self.max_cand or 1)
+        arr = (C.c_int * n)()
+        rc = self.lib.empm_read_status(self._h, arr, n, int(bool(clear)), stream_handle(self.device))
+        return rc, list(arr)
+
+    def check_status(self):
+        """Raise EmpmError (EMPM_E_NONFINITE) if a forward substep since the
+        last check produced a non-finite state; warn on a CFL violation."""
+        import warnings
+
+        rc, st = self.read_status()
+        fast = [c for c, b in enumerate(st) if b & _lib.EMPM_STATUS_CFL]
+        if fast:
+            warnings.warn(f"particles moved more than a cell in one substep (simulations {fast}): the time step is "
+                          f"too large for the velocities", RuntimeWarning, stacklevel=3)
+        check(rc)
+        return st
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             self.lib.empm_destroy(self._h)
@@ -488,18 +520,35 @@ class Rollout:
         return ParticleState(f(perm, s.x), f(perm, s.v), f(perm, s.C), f(perm, s.F))
 
     def forward(self, state: ParticleState, cols: list[Collider], time: float, dt: float, n_steps: int,
-                record=(), tape: bool = False, ckpt_every: int = 1, cols_out: list | None = None):
+                record=(), tape: bool = False, ckpt_every: int = 1, cols_out: list | None = None,
+                check: bool = False):
         """Returns (final_state, final_time, recorded_x list, tape or None).
         `cols` (ctypes Collider structs) are advanced in place like the
         reference's modify_bc; `cols_out` receives the advanced table.  With
         sort=True the rollout runs on a cell-ordered copy of the particles
-        and every output is returned in the caller's order."""
+        and every output is returned in the caller's order.  With check, the
+        device's failure flags are read once at the end (one
+        synchronisation): a non-finite state raises EmpmError
+        (EMPM_E_NONFINITE), a CFL violation warns.  Without it nothing
+        synchronises; the flags stay set for the next read."""
+        out = self._forward_sorted(state, cols, time, dt, n_steps, record, tape, ckpt_every, cols_out)
+        if check:
+            self.ctx.check_status()
+        return out
+
+    def _forward_sorted(self, state, cols, time, dt, n_steps, record, tape, ckpt_every, cols_out):
         if not self.sort or self.n == 0:
             return self._forward_core(self.model, state, cols, time, dt, n_steps, record, tape, ckpt_every,
                                       cols_out)
         perm = self.cell_order(state.x)
         pm = self._permuted_model(perm)
-        s_sorted = self._perm_state(perm, state)
+        # the cell-ordered copy lives in a buffer of the rollout's own, so that
+        # repeated calls present the same pointers (graph replay)
+        if getattr(self, "_sorted", None) is None:
+            self._sorted = ParticleState.empty(self.n, self.ctx.device)
+        s_sorted = self._sorted
+        for f in ("x", "v", "C", "F"):
+            torch.index_select(getattr(state, f), 0, perm, out=getattr(s_sorted, f))
         final, t1, xs, tp = self._forward_core(pm, s_sorted, cols, time, dt, n_steps, record, tape, ckpt_every,
                                                cols_out)
         sims = pm.sims if self.batch else [pm]
@@ -660,6 +709,8 @@ class Rollout:
                              stream)
                 reverse(s0, k_hi, states[:L], grids)
                 k_hi = s0
+            # the replays repeat the forward's failure flags: clear them
+            self.ctx.read_status()
         if 0 in seeds:
             add_seed(0)
         return adj[0], grads

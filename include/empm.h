@@ -39,6 +39,10 @@ enum {
     EMPM_E_UNSUPPORTED = -6
 };
 
+/* Failure bits of one simulation (empm_read_status) */
+#define EMPM_STATUS_NONFINITE 1 /* a substep produced a non-finite x, v or F_trial */
+#define EMPM_STATUS_CFL 2       /* a particle moved more than a cell in one substep (|v| dt > dx) */
+
 /* Adjoint flags */
 #define EMPM_ADJ_EXACT 0
 /* Drop the grid-mass adjoint, as the reference Tape does (grid_m has no
@@ -200,6 +204,34 @@ int empm_create_batch(empm_ctx **out, const empm_batch_create_desc *desc);
 
 /* Free a context (synchronises its device first). */
 void empm_destroy(empm_ctx *ctx);
+
+/* Replay each call of empm_rollout_fwd / _fwd_replay / _adj (and their
+ * batch variants) as a HIP graph (enable != 0) instead of launching its
+ * kernels one by one.  A call's launch sequence is captured once, cached in
+ * the context (the 8 most recently used) under everything it bakes in (the
+ * state, record and gradient pointers, substep counts, flags and launch
+ * shapes) and replayed with one launch on a stream of the context's own,
+ * ordered after the caller's stream's earlier work and before its later
+ * work.  Times and collider tables are read from a device schedule uploaded
+ * per call, and model parameters from the device simulation table, so
+ * replays with new values need no new capture.  Results are those of the
+ * eager launches.  Disabled while per-kernel profiling is on.  No
+ * counterpart in the reference, whose host loop launches every Warp kernel
+ * of every substep (mpm_solver_diff.py:165-291). */
+int empm_set_graphs(empm_ctx *ctx, int enable);
+
+/* Number of graph captures and replays of the context so far. */
+int empm_graph_stats(empm_ctx *ctx, long long *captures, long long *replays);
+
+/* Failure flags of the context's simulations, set on the device by every
+ * forward substep since the last read (a per-rollout check that costs one
+ * read, not one synchronisation per substep): status[c] (host, n entries)
+ * receives simulation c's EMPM_STATUS_* bits; clear != 0 resets them.
+ * Synchronises `stream`.  Returns EMPM_E_NONFINITE when any simulation has
+ * EMPM_STATUS_NONFINITE.  No counterpart in the reference, which has no
+ * failure detection: a Neo-Hookean particle with J <= 0 turns NaN
+ * (mpm_utils.py:38) and the rollout continues silently. */
+int empm_read_status(empm_ctx *ctx, int *status, int n, int clear, void *stream);
 
 /* mu, lam from E, nu: compute_mu_lam_from_E_nu (mpm_utils.py:277-283). */
 int empm_mu_lam(int n, const float *E, const float *nu, float *mu, float *lam, void *stream);
