@@ -294,6 +294,30 @@ def main():
     del gro, gctx
     torch.cuda.empty_cache()
 
+    # the checkpoint-recompute policy SURVEY's B_fb is defined on: a state
+    # every 50 substeps, each segment replayed (recorded) before its reverse
+    # sweep
+    def kstep():
+        final, _, _, tape = ro.forward(s0, cols(), 0.0, dt, T, tape=True, ckpt_every=B_FB_K)
+        diff = final.x - target
+        d = rt.ParticleState.empty(N, dev)
+        d.x.copy_(diff * (2.0 / N))
+        _, grads = ro.backward(tape, d, flags=_lib.EMPM_ADJ_EXACT)
+        return grads
+
+    kstep()
+    torch.cuda.synchronize()
+    k0 = time.perf_counter()
+    for _ in range(args.steps):
+        kstep()
+    torch.cuda.synchronize()
+    k50_ms = (time.perf_counter() - k0) * 1000.0 / args.steps
+    k50_leg = {"ckpt_every": B_FB_K, "ms_per_step": k50_ms, "value": N * T / (k50_ms / 1000.0),
+               "tape_gb": rt.tape_bytes(N, G, ro._bin_ints, T, B_FB_K) / 1e9,
+               "tape_gb_recorded": rt.tape_bytes(N, G, ro._bin_ints, T, 1) / 1e9}
+    log(f"[bench] fwd+bwd checkpoint every {B_FB_K}: {k50_ms:.2f} ms/step ({k50_leg['value']:.4e} p-substeps/s), "
+        f"tape {k50_leg['tape_gb']:.2f} GB (recorded: {k50_leg['tape_gb_recorded']:.1f} GB)")
+
     # forward-only throughput (reported beside the headline)
     fwd_only()
     torch.cuda.synchronize()
@@ -348,7 +372,12 @@ def main():
         "fwd_value": fwd_value,
         "fwd_ms_per_rollout": fwd_ms,
         "fwd_bwd_graph": graph_leg,
+        "fwd_bwd_ckpt50": k50_leg,
+        # both against B_fb (checkpoint-recompute every 50 substeps): the
+        # headline's recorded tape moves other bytes (no recompute, the
+        # records instead), so its fraction is of the same unit of work
         "hbm_roofline_fraction_fwd_bwd": (B_fb * T / (ms_per_step / 1000.0)) / 1e9 / HBM_PEAK_GBS,
+        "hbm_roofline_fraction_fwd_bwd_ckpt50": (B_fb * T / (k50_ms / 1000.0)) / 1e9 / HBM_PEAK_GBS,
         "hbm_roofline_fraction_fwd": (B_f * T / (fwd_ms / 1000.0)) / 1e9 / HBM_PEAK_GBS,
         "grad_logE_nu_logY": [float(v) for v in g.tolist()],
         "loss": float(loss),
