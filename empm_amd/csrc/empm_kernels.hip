@@ -51,7 +51,7 @@ int set_err(int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return set_err(EMPM_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 64;  // per-particle launches: one wave per workgroup spreads the waves evenly over the SIMDs
 constexpr int kWaveSize = 64;
 
 // One simulation of a batch (a single rollout is a batch of one).  Its
@@ -1346,8 +1346,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
     dzd[c] = s.dw[2][c] * ((float)c - s.f[2]);
   }
   const float Ct2[3] = {Ct[2], Ct[5], Ct[8]};
-  // the nodes of plane a (9 of them, b-major) are loaded one plane ahead;
-  // when a stencil of the wave leaves the grid, its out-of-grid nodes read
+  // the nodes of plane a (9 of them, b-major) are loaded together; when a
+  // stencil of the wave leaves the grid, its out-of-grid nodes read
   // node 0 of the grid and count as zero
   const bool inside = wave_stencils_inside(s, M.n);
   const size_t nn = (size_t)M.n * M.n;
@@ -1368,12 +1368,12 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
       pl[j] = in ? ga : make_float4(0.f, 0.f, 0.f, 0.f);
     }
   };
-  float4 cur[9];
-  load_plane(0, cur);
+  // one plane in flight at a time (prefetching the next plane costs 42
+  // VGPRs and measured slower)
 #pragma unroll 1
   for (int a = 0; a < 3; ++a) {
-    float4 nxt[9];
-    if (a < 2) load_plane(a + 1, nxt);
+    float4 cur[9];
+    load_plane(a, cur);
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
       // column sums over c: Z a, Z d_z a, dZ a, dZ d_z a, ddZ a, Z b, dZ b
@@ -1427,8 +1427,6 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
       fA[1] += XdY * e1;
       fA[2] += XY * e3;
     }
-#pragma unroll
-    for (int j = 0; j < 9; ++j) cur[j] = nxt[j];
   }
   // the substep's return map and stress, recomputed (after the node loop,
   // which does not need them, so that their state is not live during it)
