@@ -203,6 +203,12 @@ def main():
 
     model, Et, nut = model_for(sc.params)
     ro = rt.Rollout(ctx, model, N)
+    # launch parameters picked by the autotuner on this workload (every
+    # candidate timed on 40 substeps of forward + reverse sweep; set-up, not
+    # timed)
+    tune = ro.autotune(s0, cols(), dt, n_steps=40, backward=True, cache={}, force=True)
+    log(f"[bench] autotune: {tune['tuning']} after {tune['timings']} timings; 40 substeps fwd+bwd "
+        f"{tune['seconds'] * 1e3:.2f} ms, default {tune['default_seconds'] * 1e3:.2f} ms")
 
     def step():
         final, _, _, tape = ro.forward(s0, cols(), 0.0, dt, T, tape=True, ckpt_every=CKPT_EVERY)
@@ -268,7 +274,7 @@ def main():
 
     # the same step with every rollout call replayed as a HIP graph (no
     # per-kernel events: a graph's launches are not timed one by one)
-    gctx = rt.Context(n, N, dev).set_graphs(True)
+    gctx = rt.Context(n, N, dev).set_graphs(True).set_tuning(tune["tuning"])
     gro = rt.Rollout(gctx, model, N)
 
     def gstep():
@@ -373,6 +379,8 @@ def main():
         "fwd_ms_per_rollout": fwd_ms,
         "fwd_bwd_graph": graph_leg,
         "fwd_bwd_ckpt50": k50_leg,
+        "tuning": {"chosen": tune["tuning"], "timings": tune["timings"], "seconds_40_substeps": tune["seconds"],
+                   "default_seconds_40_substeps": tune["default_seconds"]},
         # both against B_fb (checkpoint-recompute every 50 substeps): the
         # headline's recorded tape moves other bytes (no recompute, the
         # records instead), so its fraction is of the same unit of work
@@ -431,6 +439,16 @@ def bench_population(args, dev):
         cands = [pop.Candidate.of_scene(s) for s in sc]
         work = ev.n * ev.n_steps
         info = {"materials": sorted(set(s.material for s in sc)), "n_particles": ev.n, "n_cand": ev.n_cand}
+        # launch parameters tuned on 20 substeps of each leg's own workload
+        # (the batched evaluation, and the one-at-a-time rollouts it is
+        # compared with); set-up, not timed
+        tb_tune = ev.autotune(cands, backward=diff, n_steps=20, cache={}, force=True)
+        info["tuning_batched"] = tb_tune["tuning"]
+        if not args.pop_batched_only:
+            info["tuning_sequential"] = ev.autotune(cands, backward=diff, n_steps=20, sequential=True, cache={},
+                                                    force=True)["tuning"]
+        log(f"[bench] population {leg} autotune: batched {info['tuning_batched']}, "
+            f"one-at-a-time {info.get('tuning_sequential')}")
         if not diff:
             tb, fb = timed(lambda: ev.fitness(cands), args.pop_steps)
             ts, fs = (None, None) if args.pop_batched_only else timed(lambda: ev.fitness_sequential(cands),
@@ -580,6 +598,9 @@ def bench_c1(args, dev):
     s0 = rt.ParticleState(t(sc.x), torch.zeros((N, 3), device=dev), torch.zeros((N, 3, 3), device=dev),
                           torch.eye(3, device=dev).repeat(N, 1, 1).contiguous())
     ro = rt.Rollout(rt.Context(n, N, dev), model, N)
+    tune = ro.autotune(s0, cols(), dt, n_steps=T, backward=False, cache={}, force=True)
+    log(f"[bench] C1 autotune: {tune['tuning']}; {tune['seconds'] * 1e3:.2f} ms per rollout, default "
+        f"{tune['default_seconds'] * 1e3:.2f} ms")
     for _ in range(2):
         ro.forward(s0, cols(), 0.0, dt, T)
     torch.cuda.synchronize()
@@ -594,7 +615,7 @@ def bench_c1(args, dev):
     med = sorted(ms)[len(ms) // 2]
     # the same rollouts replayed as a HIP graph (one launch per call instead
     # of 8 per substep): the launch-bound regime of a small scene
-    gctx = rt.Context(n, N, dev).set_graphs(True)
+    gctx = rt.Context(n, N, dev).set_graphs(True).set_tuning(tune["tuning"])
     gro = rt.Rollout(gctx, model, N)
     for _ in range(2):
         gro.forward(s0, cols(), 0.0, dt, T)
@@ -617,7 +638,9 @@ def bench_c1(args, dev):
                        f"{T} forward substeps", "value": N * T / (med * 1e-3), "unit": "particle-substeps/s (forward)",
            "ms_per_rollout_median": med, "us_per_substep": med * 1e3 / T,
            "graph": {"value": N * T / (gmed * 1e-3), "ms_per_rollout_median": gmed, "us_per_substep": gmed * 1e3 / T,
-                     "captures": caps, "replays": reps, "rel_diff_x_vs_eager": gerr}}
+                     "captures": caps, "replays": reps, "rel_diff_x_vs_eager": gerr},
+           "tuning": {"chosen": tune["tuning"], "ms_per_rollout": tune["seconds"] * 1e3,
+                      "default_ms_per_rollout": tune["default_seconds"] * 1e3}}
     if not args.no_cpu_baseline:
         from oracle import oracle as orc
 

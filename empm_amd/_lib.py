@@ -44,6 +44,8 @@ EXPORTED_SYMBOLS = (
     "empm_read_status",
     "empm_set_graphs",
     "empm_graph_stats",
+    "empm_set_tuning",
+    "empm_get_tuning",
     "empm_nn_sqdist_segments",
     "empm_nn_sqdist_segments_workspace",
     "empm_substep_fwd",
@@ -147,6 +149,22 @@ class GridRec(C.Structure):
     _fields_ = [("mv_m", C.c_void_p), ("v_out", C.c_void_p), ("bins", C.c_void_p)]
 
 
+class Tuning(C.Structure):
+    """empm_tuning: launch parameters of the binned kernels."""
+    _fields_ = [("scatter_wg", C.c_int), ("scatter_per_cu", C.c_int), ("heavy_chunks", C.c_int),
+                ("dynamic_rounds", C.c_int), ("node_grid_cap", C.c_int)]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+    @classmethod
+    def of(cls, d):
+        t = cls()
+        for f, _ in cls._fields_:
+            setattr(t, f, int(d[f]))
+        return t
+
+
 class Collider(C.Structure):
     _fields_ = [
         ("kind", C.c_int),
@@ -212,6 +230,10 @@ def _declare(lib):
     lib.empm_set_graphs.argtypes = [vp, C.c_int]
     lib.empm_graph_stats.restype = C.c_int
     lib.empm_graph_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    lib.empm_set_tuning.restype = C.c_int
+    lib.empm_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    lib.empm_get_tuning.restype = C.c_int
+    lib.empm_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     lib.empm_read_status.restype = C.c_int
     lib.empm_read_status.argtypes = [vp, vp, C.c_int, C.c_int, vp]
     lib.empm_nn_sqdist_noalloc.restype = C.c_int

@@ -393,10 +393,10 @@ constexpr int kBlkCells = 2;                                    // cells per blo
 constexpr int kTileMargin = 0;                                  // binning is exact: no drift margin
 constexpr int kTileN = kBlkCells + 2 + 2 * kTileMargin;         // 4 tile nodes per edge
 constexpr int kTileNodes = kTileN * kTileN * kTileN;            // 64
-constexpr int kBlkWG = 256;       // workgroup size of the block-owned scatter kernels
+constexpr int kBlkWG = 256;       // default workgroup size of the block-owned scatter kernels (tunable: 128, 256, 512)
 constexpr int kBlkGridCap = 1024; // workgroups of a block-owned scatter launch when the occupancy query fails
-constexpr int kHeavyChunks = 8;   // blocks with more chunks than this are scattered first
-constexpr int kDynamicRounds = 4; // more blocks per scatter workgroup than this: handed out dynamically
+constexpr int kHeavyChunks = 8;   // default: blocks with more chunks than this are scattered first
+constexpr int kDynamicRounds = 4; // default: more blocks per scatter workgroup than this are handed out dynamically
 constexpr int kWave = 64;
 constexpr int kAdjPayload = 4;    // floats per particle of the G2P-adjoint scatter payload (dL/dv', pad)
 constexpr int kSchedCap = 1024;   // substeps per schedule upload (longer calls run in chunks)
@@ -415,7 +415,7 @@ struct Overflow {
 struct Binning {
   const int* perm;
   const int* offs;
-  // the non-empty blocks: act[0] heavy ones (more than kHeavyChunks chunks)
+  // the non-empty blocks: act[0] heavy ones (more than the tuning's heavy_chunks chunks)
   // at act[2 ..], act[1] light ones at act[2 + nblk - 1 ...] downwards; the
   // scatter takes the heavy ones first
   const int* act;
@@ -424,6 +424,7 @@ struct Binning {
   const int* near;     // per block: some non-empty block's tile covers a node this block owns
   const int* nlist;    // ids of the near blocks, in no particular order
   const int* nnear;    // their count (device scalar)
+  int dyn_rounds;      // more active blocks per scatter workgroup than this: handed out by a work counter
 };
 
 // ---- cross-lane exchanges (wave64) ----------------------------------------
@@ -641,7 +642,7 @@ EMPM_DEV void block_scatter(const Batch& Bt, const Binning& B, float4* __restric
   const int wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
   const int last = B.nblk - 1;
   // List entry w goes to workgroup w (read ahead of the counts, as if
-  // heavy).  With up to kDynamicRounds entries per workgroup the rest are
+  // heavy).  With up to B.dyn_rounds entries per workgroup the rest are
   // taken in strides of gridDim.x, the next id read at the top of each
   // round.  With more, workgroups take entries gridDim.x + (work counter)
   // one at a time: thread 0 requests the next entry and resolves its block
@@ -650,7 +651,7 @@ EMPM_DEV void block_scatter(const Batch& Bt, const Binning& B, float4* __restric
   int ai = blockIdx.x;
   const int first = B.act[2 + (ai < last ? ai : last)];
   const int nh = B.act[0], nact = nh + B.act[1];
-  const bool dynamic = nact > kDynamicRounds * (int)gridDim.x;
+  const bool dynamic = nact > B.dyn_rounds * (int)gridDim.x;
   const auto entry = [&](int k) { return k < nh ? B.act[2 + k] : B.act[2 + last - (k - nh)]; };
   int blk = ai < nh ? first : (ai < nact ? entry(ai) : 0);
   for (int it = 0; ai < nact; ++it) {
@@ -880,7 +881,8 @@ __global__ __launch_bounds__(kBlock) void k_stress(Batch Bt, float dt,
 
 // p2g_apic_with_stress (mpm_utils.py:358-423): APIC momentum, mass and the
 // stress impulse into the block tiles.
-__global__ __launch_bounds__(kBlkWG) void k_p2g(Batch Bt, const float* __restrict__ x, const float* __restrict__ v,
+template <int WG>
+__global__ __launch_bounds__(WG) void k_p2g(Batch Bt, const float* __restrict__ x, const float* __restrict__ v,
                                                 const float* __restrict__ C, const float* __restrict__ S, Binning B,
                                                 float4* __restrict__ tiles, Overflow ov, int* __restrict__ work) {
   block_scatter<4>(Bt, B, tiles, ov.grid, ov.flag, work, [&](int p, float* xp, ScatterPayload& P, const DevModel& M) {
@@ -1230,7 +1232,8 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(Batch Bt, float dt, const fl
 // Per particle: A = dL/dv' (clamp mask applied, from k_g2p_adj), and from
 // the state's cotangents B = 4 inv_dx dL/dC' and S = dt inv_dx dL/dF' Fe^T
 // (F' = (I + dt grad v) Fe).
-__global__ __launch_bounds__(kBlkWG) void k_g2p_adj_scatter(Batch Bt, float dt, const float* __restrict__ x,
+template <int WG>
+__global__ __launch_bounds__(WG) void k_g2p_adj_scatter(Batch Bt, float dt, const float* __restrict__ x,
                                                             const float* __restrict__ payload,
                                                             const float* __restrict__ dC1,
                                                             const float* __restrict__ dF1,
@@ -1653,7 +1656,8 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_scan(int nblk, const int* __
 __global__ __launch_bounds__(kBinScanWG) void k_bin_finish(Batch Bt, int* __restrict__ count,
                                                            const int* __restrict__ blkoff,
                                                            const int* __restrict__ wsum, int* __restrict__ offs,
-                                                           int* __restrict__ act, int* __restrict__ occ) {
+                                                           int* __restrict__ act, int* __restrict__ occ,
+                                                           int heavy_chunks) {
   __shared__ int red[kBinScanWG / 64];
   const int nblk = Bt.nblk;
   const int b = blockIdx.x * kBinScanWG + threadIdx.x;
@@ -1700,7 +1704,7 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_finish(Batch Bt, int* __rest
   int nch = 0;
 #pragma unroll
   for (int k = 0; k < 8; ++k) nch += (c[k] + kWave - 1) / kWave;
-  const bool heavy = nch > kHeavyChunks;
+  const bool heavy = nch > heavy_chunks;
   const unsigned long long hmask = __ballot(heavy), lmask = __ballot(!heavy);
   int hbase = 0, lbase = 0;
   if (__lane_id() == leader) {
@@ -1959,7 +1963,9 @@ struct empm_ctx {
   int* blkoff = nullptr;     // max_blocks: workgroup-local exclusive block offsets
   int* wsum = nullptr;       // block-scan workgroup totals
   int* work = nullptr;       // 2: work counters of the k_p2g and k_g2p_adj_scatter launches
-  unsigned scatter_wgs[2] = {kBlkGridCap, kBlkGridCap};  // their launch widths: workgroups resident at once
+  empm_tuning tune = {kBlkWG, 0, kHeavyChunks, kDynamicRounds, kNodePassGridCap};
+  int cus = 0;               // compute units of the device
+  int scatter_res[3][2] = {};  // scatter workgroups resident per CU: [128, 256, 512 threads][k_p2g, k_g2p_adj_scatter]
   DevModel* cand_dev = nullptr;     // device table of the simulations of the current call
   DevModel* cand_pinned = nullptr;  // its pinned staging copy
   std::vector<DevModel> cand_last;  // what cand_dev holds
@@ -2301,18 +2307,45 @@ BinBuf bin_buf(int* rec, int nblk, int n_particles) {
   return b;
 }
 
-Binning binning(const BinBuf& b, int nblk) {
-  return Binning{b.perm, b.offs, b.act, nblk, b.occ, b.near, b.nlist + 1, b.nlist};
+Binning binning(const empm_ctx* ctx, const BinBuf& b, int nblk) {
+  return Binning{b.perm, b.offs, b.act, nblk, b.occ, b.near, b.nlist + 1, b.nlist, ctx->tune.dynamic_rounds};
 }
+
+int wg_variant(int wg) { return wg == 128 ? 0 : (wg == 512 ? 2 : 1); }
+
+// Launch width of a block-owned scatter (idx 0: k_p2g, 1: k_g2p_adj_scatter)
+// with the tuned workgroup size: the workgroups resident at once, or the
+// tuning's count per CU when that is lower.
+unsigned scatter_grid(const empm_ctx* ctx, int idx) {
+  int per_cu = ctx->scatter_res[wg_variant(ctx->tune.scatter_wg)][idx];
+  if (ctx->tune.scatter_per_cu > 0 && (per_cu <= 0 || ctx->tune.scatter_per_cu < per_cu))
+    per_cu = ctx->tune.scatter_per_cu;
+  const long long w = (long long)per_cu * ctx->cus;
+  return w > 0 ? (unsigned)w : (unsigned)kBlkGridCap;
+}
+
+// KERNEL<tuned workgroup size> of the block-owned scatters
+#define EMPM_LAUNCH_SCATTER(ctx, idx, KERNEL, stream, ...)                                                    \
+  switch ((ctx)->tune.scatter_wg) {                                                                          \
+    case 128:                                                                                                \
+      hipLaunchKernelGGL(KERNEL<128>, dim3(scatter_grid(ctx, idx)), dim3(128), 0, stream, __VA_ARGS__);     \
+      break;                                                                                                 \
+    case 512:                                                                                                \
+      hipLaunchKernelGGL(KERNEL<512>, dim3(scatter_grid(ctx, idx)), dim3(512), 0, stream, __VA_ARGS__);     \
+      break;                                                                                                 \
+    default:                                                                                                 \
+      hipLaunchKernelGGL(KERNEL<256>, dim3(scatter_grid(ctx, idx)), dim3(256), 0, stream, __VA_ARGS__);     \
+      break;                                                                                                 \
+  }
 
 Overflow overflow(const empm_ctx* ctx) { return Overflow{ctx->ovf, ctx->ovf_flag}; }
 
 unsigned nblocks(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 
-dim3 node_grid(int nblk) {
-  const size_t wgs = ((size_t)nblk * 8 + kBlock - 1) / kBlock;
-  return dim3((unsigned)(wgs < (size_t)kNodePassGridCap ? (wgs > 0 ? wgs : 1) : kNodePassGridCap));
+dim3 node_grid(const empm_ctx* ctx, int nblk) {
+  const size_t wgs = ((size_t)nblk * 8 + kBlock - 1) / kBlock, cap = (size_t)ctx->tune.node_grid_cap;
+  return dim3((unsigned)(wgs < cap ? (wgs > 0 ? wgs : 1) : cap));
 }
 
 // Bin the particles of a state (positions x) into `out`.
@@ -2339,7 +2372,7 @@ int do_bin(empm_ctx* ctx, const HostBatch& hb, const float* x, const BinBuf& out
   hipLaunchKernelGGL(k_bin_scan, dim3(wgs), dim3(kBinScanWG), 0, st, nblk, ctx->count, ctx->blkoff, ctx->wsum,
                      out.act, out.nlist, ctx->ovf_flag, ctx->work);
   hipLaunchKernelGGL(k_bin_finish, dim3(wgs), dim3(kBinScanWG), 0, st, hb.dev, ctx->count, ctx->blkoff, ctx->wsum,
-                     out.offs, out.act, out.occ);
+                     out.offs, out.act, out.occ, ctx->tune.heavy_chunks);
   hipLaunchKernelGGL(k_bin_place, dim3(nblocks(hb.N > nblk ? hb.N : nblk)), dim3(kBlock), 0, st, hb.dev,
                      ctx->key_rank, out.offs, out.perm, out.occ, out.near, out.nlist);
   HIP_TRY(hipGetLastError());
@@ -2359,7 +2392,7 @@ int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable* T, float dt, 
   const BinBuf bb = bin_buf(rec && rec->bins ? rec->bins : ctx->bins, hb.nblk, hb.N);
   int rc = do_bin(ctx, hb, in->x, bb, st, !first);
   if (rc) return rc;
-  const Binning B = binning(bb, hb.nblk);
+  const Binning B = binning(ctx, bb, hb.nblk);
   if (hb.N > 0 && first) {
     ProfScope ps(ctx, EMPM_KERNEL_STRESS, st);
     // a replay starts from the material state recorded with its first state
@@ -2367,12 +2400,12 @@ int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable* T, float dt, 
   }
   if (hb.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
-    hipLaunchKernelGGL(k_p2g, dim3(ctx->scatter_wgs[0]), dim3(kBlkWG), 0, st, hb.dev, in->x, in->v, in->C, ctx->S,
-                       B, ctx->tiles, overflow(ctx), ctx->work);
+    EMPM_LAUNCH_SCATTER(ctx, 0, k_p2g, st, hb.dev, in->x, in->v, in->C, ctx->S, B, ctx->tiles, overflow(ctx),
+                        ctx->work);
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
-    hipLaunchKernelGGL(k_grid, node_grid(hb.nblk), dim3(kBlock), 0, st, hb.dev, T, dt, B, ctx->tiles,
+    hipLaunchKernelGGL(k_grid, node_grid(ctx, hb.nblk), dim3(kBlock), 0, st, hb.dev, T, dt, B, ctx->tiles,
                        overflow(ctx), keep, v_out);
   }
   if (hb.N > 0) {
@@ -2406,7 +2439,7 @@ int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable* T, float dt, 
     int rc = do_bin(ctx, hb, in->x, bb, st);
     if (rc) return rc;
   }
-  const Binning B = binning(bb, hb.nblk);
+  const Binning B = binning(ctx, bb, hb.nblk);
   if (!recorded) {
     if (hb.N > 0) {
       ProfScope ps(ctx, EMPM_KERNEL_STRESS, st);
@@ -2414,12 +2447,12 @@ int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable* T, float dt, 
     }
     if (hb.N > 0) {
       ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
-      hipLaunchKernelGGL(k_p2g, dim3(ctx->scatter_wgs[0]), dim3(kBlkWG), 0, st, hb.dev, in->x, in->v, in->C,
-                         ctx->S, B, ctx->tiles, overflow(ctx), ctx->work);
+      EMPM_LAUNCH_SCATTER(ctx, 0, k_p2g, st, hb.dev, in->x, in->v, in->C, ctx->S, B, ctx->tiles, overflow(ctx),
+                          ctx->work);
     }
     {
       ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
-      hipLaunchKernelGGL(k_grid, node_grid(hb.nblk), dim3(kBlock), 0, st, hb.dev, T, dt, B, ctx->tiles,
+      hipLaunchKernelGGL(k_grid, node_grid(ctx, hb.nblk), dim3(kBlock), 0, st, hb.dev, T, dt, B, ctx->tiles,
                          overflow(ctx), keep, v_out);
     }
   }
@@ -2431,13 +2464,13 @@ int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable* T, float dt, 
     }
     {
       ProfScope ps(ctx, EMPM_KERNEL_G2P_ADJ_SCATTER, st);
-      hipLaunchKernelGGL(k_g2p_adj_scatter, dim3(ctx->scatter_wgs[1]), dim3(kBlkWG), 0, st, hb.dev, dt, in->x,
-                         ctx->payload, d_out->C, d_out->F, Fe, B, ctx->tiles, overflow(ctx), ctx->work + 1);
+      EMPM_LAUNCH_SCATTER(ctx, 1, k_g2p_adj_scatter, st, hb.dev, dt, in->x, ctx->payload, d_out->C, d_out->F, Fe, B,
+                          ctx->tiles, overflow(ctx), ctx->work + 1);
     }
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID_ADJ, st);
-    hipLaunchKernelGGL(k_grid_adj, node_grid(hb.nblk), dim3(kBlock), 0, st, hb.dev, T, dt, B, ctx->tiles,
+    hipLaunchKernelGGL(k_grid_adj, node_grid(ctx, hb.nblk), dim3(kBlock), 0, st, hb.dev, T, dt, B, ctx->tiles,
                        overflow(ctx), keep, ctx->gadj_in, (flags & EMPM_ADJ_DROP_GRID_MASS) ? 1 : 0);
   }
   if (hb.N > 0) {
@@ -2518,15 +2551,19 @@ int alloc_ctx(empm_ctx* c) {
   e = e == hipSuccess ? hipMemset(c->ovf_flag, 0, sizeof(int)) : e;
   e = e == hipSuccess ? hipDeviceSynchronize() : e;
   // the block-owned scatters are launched as wide as the device holds them
-  int cus = 0;
-  e = e == hipSuccess ? hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) : e;
-  const void* scatter_kernels[2] = {reinterpret_cast<const void*>(k_p2g),
-                                    reinterpret_cast<const void*>(k_g2p_adj_scatter)};
-  for (int i = 0; i < 2 && e == hipSuccess; ++i) {
-    int per_cu = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, scatter_kernels[i], kBlkWG, 0);
-    if (e == hipSuccess && per_cu > 0 && cus > 0) c->scatter_wgs[i] = (unsigned)(per_cu * cus);
-  }
+  // (per workgroup size of the tuning)
+  e = e == hipSuccess ? hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, c->device) : e;
+  const void* scatter_kernels[3][2] = {
+      {reinterpret_cast<const void*>(k_p2g<128>), reinterpret_cast<const void*>(k_g2p_adj_scatter<128>)},
+      {reinterpret_cast<const void*>(k_p2g<256>), reinterpret_cast<const void*>(k_g2p_adj_scatter<256>)},
+      {reinterpret_cast<const void*>(k_p2g<512>), reinterpret_cast<const void*>(k_g2p_adj_scatter<512>)}};
+  const int wgs[3] = {128, 256, 512};
+  for (int v = 0; v < 3; ++v)
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+      int per_cu = 0;
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, scatter_kernels[v][i], wgs[v], 0);
+      if (e == hipSuccess) c->scatter_res[v][i] = per_cu;
+    }
   if (e != hipSuccess)
     return set_err(e == hipErrorOutOfMemory ? EMPM_E_OOM : EMPM_E_HIP, "empm_create: %s", hipGetErrorString(e));
   return EMPM_OK;
@@ -2927,6 +2964,34 @@ int empm_set_graphs(empm_ctx* ctx, int enable) {
     ctx->gcache.clear();
   }
   ctx->graphs = enable != 0;
+  return EMPM_OK;
+}
+
+int empm_set_tuning(empm_ctx* ctx, const empm_tuning* t) {
+  if (!ctx || !t) return set_err(EMPM_E_INVALID_ARG, "empm_set_tuning: null argument");
+  if (t->scatter_wg != 128 && t->scatter_wg != 256 && t->scatter_wg != 512)
+    return set_err(EMPM_E_INVALID_ARG, "empm_set_tuning: scatter_wg=%d (128, 256 or 512)", t->scatter_wg);
+  if (t->scatter_per_cu < 0 || t->scatter_per_cu > 32)
+    return set_err(EMPM_E_INVALID_ARG, "empm_set_tuning: scatter_per_cu=%d outside [0, 32]", t->scatter_per_cu);
+  if (t->heavy_chunks < 0) return set_err(EMPM_E_INVALID_ARG, "empm_set_tuning: heavy_chunks=%d < 0", t->heavy_chunks);
+  if (t->dynamic_rounds < 1 || t->dynamic_rounds > 1024)
+    return set_err(EMPM_E_INVALID_ARG, "empm_set_tuning: dynamic_rounds=%d outside [1, 1024]", t->dynamic_rounds);
+  if (t->node_grid_cap < 1 || t->node_grid_cap > 65536)
+    return set_err(EMPM_E_INVALID_ARG, "empm_set_tuning: node_grid_cap=%d outside [1, 65536]", t->node_grid_cap);
+  if (!ctx->gcache.empty()) {
+    // the cached graphs bake the old launch shapes
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->gstream));
+    for (auto& g : ctx->gcache) (void)hipGraphExecDestroy(g.exec);
+    ctx->gcache.clear();
+  }
+  ctx->tune = *t;
+  return EMPM_OK;
+}
+
+int empm_get_tuning(empm_ctx* ctx, empm_tuning* t) {
+  if (!ctx || !t) return set_err(EMPM_E_INVALID_ARG, "empm_get_tuning: null argument");
+  *t = ctx->tune;
   return EMPM_OK;
 }
 

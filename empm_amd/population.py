@@ -205,6 +205,55 @@ class PopulationEvaluator:
         grad = torch.stack([self._seg_sum(dE * E), self._seg_sum(dnu), self._seg_sum(g.ys * ys0)], 1)
         return fit, grad
 
+    # ------------------------------------------------------------ tuning
+    def autotune(self, cands: list[Candidate], backward: bool = False, n_steps: int = 20, sequential: bool = False,
+                 cache: dict | None = None, force: bool = False, **kw) -> dict:
+        """Tune the launch parameters (empm_amd.autotune) of the batched
+        evaluation (fitness, or with backward fitness_and_grad) on n_steps
+        substeps, or with `sequential` those of the one-at-a-time context on
+        the first four candidates; the targets must be set.  The result is
+        cached per workload signature and left set on the context."""
+        from . import autotune as at
+
+        if self.targets is None:
+            raise ValueError("autotune: set_targets first")
+        saved = self.n_steps
+        self.n_steps = int(n_steps)
+        try:
+            grids = [sc.n_grid for sc in self.scenes]
+            mats = [sc.material for sc in self.scenes]
+            if sequential:
+                sub = list(range(min(4, self.n_cand)))
+                bm, _, _ = self.model(cands)
+                self._solo(0, bm)  # creates the one-at-a-time context
+                ctx = self._solo_ctx
+
+                def run():
+                    for c in sub:
+                        ro, s0, a, b = self._solo(c, bm)
+                        final, _, _, tape = ro.forward(s0, list(self.cols), 0.0, self.dt, self.n_steps,
+                                                       tape=backward)
+                        if backward:
+                            d = rt.ParticleState.empty(b - a, self.device)
+                            d.zero_()
+                            d.x.copy_(final.x - self.targets[a:b])
+                            ro.backward(tape, d, flags=_lib.EMPM_ADJ_EXACT)
+
+                key = at.workload_key(self.device, grids, max(self.counts), mats, 1, backward) + "|one-at-a-time"
+            else:
+                ctx = self.ctx
+
+                def run():
+                    if backward:
+                        self.fitness_and_grad(cands)
+                    else:
+                        self.fitness(cands)
+
+                key = at.workload_key(self.device, grids, self.n, mats, self.n_cand, backward)
+            return at.tune(ctx, run, key=key, cache=cache, force=force, **kw)
+        finally:
+            self.n_steps = saved
+
     # ------------------------------------------------------------ one at a time
     def _solo(self, c: int, bm: rt.BatchModel):
         if self._solo_ctx is None:

@@ -68,6 +68,17 @@ class Context:
         check(self.lib.empm_set_graphs(self._h, int(bool(enable))))
         return self
 
+    def set_tuning(self, tuning: dict):
+        """Launch parameters of the binned kernels (empm_tuning fields; see
+        empm_amd.autotune)."""
+        check(self.lib.empm_set_tuning(self._h, C.byref(_lib.Tuning.of(tuning))))
+        return self
+
+    def get_tuning(self) -> dict:
+        t = _lib.Tuning()
+        check(self.lib.empm_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
     def graph_stats(self):
         """(captures, replays) of this context's graph cache."""
         a, b = C.c_longlong(), C.c_longlong()
@@ -518,6 +529,31 @@ class Rollout:
     def _perm_state(self, perm, s: ParticleState, inverse=False):
         f = self._scatter if inverse else self._gather
         return ParticleState(f(perm, s.x), f(perm, s.v), f(perm, s.C), f(perm, s.F))
+
+    def autotune(self, state: ParticleState, cols: list[Collider], dt: float, n_steps: int = 40,
+                 backward: bool = True, cache: dict | None = None, force: bool = False, **kw) -> dict:
+        """Tune this rollout's context (empm_amd.autotune) on n_steps substeps
+        of this workload from `state` (forward, and with `backward` the
+        reverse sweep of a recorded tape); the result is cached per workload
+        signature and left set on the context."""
+        from . import autotune as at
+
+        dev = self.ctx.device
+        m = self.model
+        sims = m.sims if isinstance(m, BatchModel) else [m]
+        key = at.workload_key(dev, [s.n_grid for s in sims], self.n, [s.material for s in sims], len(sims),
+                              backward)
+        cols = list(cols)
+
+        def run():
+            final, _, _, tape = self.forward(state, list(cols), 0.0, dt, n_steps, tape=backward, ckpt_every=1)
+            if backward:
+                d = ParticleState.empty(self.n, dev)
+                d.zero_()
+                d.x.fill_(1e-3)
+                self.backward(tape, d, flags=0)
+
+        return at.tune(self.ctx, run, key=key, cache=cache, force=force, **kw)
 
     def forward(self, state: ParticleState, cols: list[Collider], time: float, dt: float, n_steps: int,
                 record=(), tape: bool = False, ckpt_every: int = 1, cols_out: list | None = None,

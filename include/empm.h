@@ -223,6 +223,27 @@ int empm_set_graphs(empm_ctx *ctx, int enable);
 /* Number of graph captures and replays of the context so far. */
 int empm_graph_stats(empm_ctx *ctx, long long *captures, long long *replays);
 
+/* Launch parameters of a context's binned kernels, picked per workload by
+ * the autotuner (empm_amd.autotune) or set by hand.  Results do not depend
+ * on them beyond the order of float atomic sums.  No counterpart in the
+ * reference (Warp picks its own launch shapes, mpm_solver_diff.py:165-291). */
+typedef struct {
+    int scatter_wg;      /* threads per workgroup of the block-owned scatters (the waves sharing one block's
+                          * LDS tile): 128, 256 or 512; default 256 */
+    int scatter_per_cu;  /* scatter workgroups launched per CU, capped by occupancy (0: as many as fit) */
+    int heavy_chunks;    /* blocks of more than this many 64-particle chunks are scattered first (0: every
+                          * block counts as heavy, so the list keeps binning order); default 8 */
+    int dynamic_rounds;  /* with more active blocks than this per scatter workgroup, blocks are handed out
+                          * by a work counter instead of in fixed strides; default 4 */
+    int node_grid_cap;   /* workgroups of a grid node pass (they stride over the near nodes); default 1024 */
+} empm_tuning;
+
+/* Set (validated; EMPM_E_INVALID_ARG names the bad field) or read the
+ * context's tuning.  Setting it drops the context's cached graphs, whose
+ * launch shapes it changes. */
+int empm_set_tuning(empm_ctx *ctx, const empm_tuning *tuning);
+int empm_get_tuning(empm_ctx *ctx, empm_tuning *tuning);
+
 /* Failure flags of the context's simulations, set on the device by every
  * forward substep since the last read (a per-rollout check that costs one
  * read, not one synchronisation per substep): status[c] (host, n entries)
