@@ -24,7 +24,16 @@ device and advances them in ticks:
 So requests that arrive while others are on the GPU join the next tick and
 share its launches with them; a request's result does not depend on what
 it was batched with (the batched kernels give each simulation its own
-particle range, grid and blocks).  The reference has no serving path: its
+particle range, grid and blocks).
+
+Ticks are asynchronous: a tick's launches are queued and the host goes on
+to plan the next one while the device runs them.  Each tick records an
+event; the requests that finished in it are timestamped (and their loss and
+failure flag read back, through pinned host copies queued before the
+event) when that event completes.  At most `max_inflight` ticks are queued
+ahead of the device, so a request that arrives is planned within that many
+ticks.  A request whose state turned non-finite (an inverted element, say)
+is reported as failed in its result; the others are unaffected.  The reference has no serving path: its
 rollouts run one at a time through WarpMPMWrapper.advance_to
 (phys_sim/diff_simulator/warp_mpm/warp_mpm_wrapper.py:278-306).
 
@@ -91,6 +100,7 @@ class RolloutResult:
     arrival: float = 0.0
     admitted: float = 0.0
     finished: float = 0.0
+    failed: bool = False              # the final state holds non-finite values
 
     @property
     def latency(self) -> float:
@@ -263,13 +273,17 @@ class _Dev:
 
 class ServingEngine:
     """Executes Scheduler plans on one device.  `submit` enqueues a request
-    (arrival time defaults to now), `step` runs one tick (admission, then
-    every planned call, then a synchronisation that timestamps the requests
-    that finished) and returns the ids finished in it."""
+    (arrival time defaults to now), `step` queues one tick (admission, then
+    every planned call) and returns the ids of the requests it finishes;
+    their results appear in `results` when the device has run the tick
+    (`drain` waits for every queued tick)."""
 
     def __init__(self, device="cuda:0", max_particles=2_000_000, max_cand=32, max_grid=128, dt=1e-4, chunk=50,
-                 hbm_budget_bytes: int | None = None, gravity=(0.0, 0.0, -9.8)):
+                 hbm_budget_bytes: int | None = None, gravity=(0.0, 0.0, -9.8), max_inflight: int = 2):
         self.device = torch.device(device)
+        self.max_inflight = max(1, int(max_inflight))
+        # queued ticks: (event, [(rid, result, pinned loss, pinned finite flag)])
+        self._inflight: collections.deque = collections.deque()
         self.dt = float(dt)
         self.gravity = tuple(gravity)
         self.ctx = rt.Context(max_grid, max_particles, self.device, max_cand=max_cand)
@@ -298,7 +312,13 @@ class ServingEngine:
 
     def _admit_device(self, s: Slot):
         r, dev = s.req, self.device
-        t = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=dev)
+
+        def t(a):
+            # through pinned memory: the upload is queued behind the ticks in
+            # flight instead of waiting for them
+            h = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).pin_memory()
+            return h.to(dev, non_blocking=True)
+
         x = t(r.x)
         n = r.n
         # cell order of the initial positions: neighbouring particles share blocks
@@ -403,22 +423,53 @@ class ServingEngine:
             d.ckpt.pop(s.starts[-1], None)
             self.sched.backward_done(s.rid)
 
-    def _result(self, s: Slot, now: float) -> RolloutResult:
+    def _result(self, s: Slot):
+        """The result of a finished request (device work queued; the loss and
+        the failure flag come back through pinned host copies)."""
         d = s.dev
         inv = torch.empty_like(d.perm)
         inv[d.perm] = torch.arange(d.perm.numel(), device=self.device)
         res = RolloutResult(x=d.state.x.index_select(0, inv), v=d.state.v.index_select(0, inv),
-                            arrival=s.arrival, admitted=s.admitted, finished=now)
+                            arrival=s.arrival, admitted=s.admitted)
+        finite = torch.empty((), dtype=torch.bool).pin_memory()
+        finite.copy_(torch.isfinite(d.state.x).all() & torch.isfinite(d.state.v).all(), non_blocking=True)
+        loss = None
         if s.req.want_grad:
             dE, dnu = torch.zeros_like(d.E), torch.zeros_like(d.E)
             rt.mu_lam_adj(d.E, d.nu, d.grads.mu, d.grads.lam, dE, dnu)
-            res.loss = float(d.loss)
+            loss = torch.empty((), dtype=torch.float64).pin_memory()
+            loss.copy_(d.loss, non_blocking=True)
             res.grad = torch.stack([(dE * d.E).double().sum(), dnu.double().sum(),
                                     (d.grads.ys * d.ys0).double().sum()])
-        return res
+        return res, loss, finite
+
+    def _poll(self, block: bool = False):
+        """Complete the queued ticks whose events have passed, oldest first
+        (with block, wait for the oldest): timestamp their finished requests
+        and read back their losses and failure flags."""
+        while self._inflight:
+            ev, items = self._inflight[0]
+            if not ev.query():
+                if not block:
+                    return
+                ev.synchronize()
+            block = False
+            now = time.perf_counter()
+            for rid, res, loss, finite in items:
+                res.finished = now
+                res.failed = not bool(finite)
+                if loss is not None:
+                    res.loss = float(loss)
+                self.results[rid] = res
+            self._inflight.popleft()
 
     def step(self) -> list[int]:
-        """One tick; returns the ids of the requests finished in it."""
+        """One tick: admission, then every planned call, queued on the
+        device; returns the ids of the requests whose last work it queued
+        (their results appear in `results` once the device has run it)."""
+        self._poll()
+        while len(self._inflight) >= self.max_inflight:
+            self._poll(block=True)
         now = time.perf_counter()
         for s in self.sched.admit(now):
             self._admit_device(s)
@@ -427,23 +478,30 @@ class ServingEngine:
             slots = [self.sched.slots[r] for r in w.rids]
             (self._forward if w.kind == FORWARD else self._backward)(w, slots)
             self.calls += 1
-        torch.cuda.synchronize(self.device)
-        now = time.perf_counter()
-        done = []
+        items = []
         for s in self.sched.slots.values():
             if s.phase == DONE and s.dev is not None:
-                self.results[s.rid] = self._result(s, now)
+                items.append((s.rid, *self._result(s)))
                 s.dev = None
-                done.append(s.rid)
+        ev = torch.cuda.Event()
+        ev.record()
+        self._inflight.append((ev, items))
         self.ticks += 1
-        return done
+        return [it[0] for it in items]
+
+    def drain(self):
+        """Wait for every queued tick and complete its results."""
+        while self._inflight:
+            self._poll(block=True)
 
     def run_until_idle(self) -> dict[int, RolloutResult]:
         while not self.sched.idle():
             self.step()
+        self.drain()
         return self.results
 
     def close(self):
+        self.drain()
         self.ctx.close()
 
 
@@ -464,6 +522,9 @@ def serve_poisson(engine: ServingEngine, requests: list[RolloutRequest], qps: fl
             off, req = pending.popleft()
             ids.append(engine.submit(req, arrival=t0 + off))
         if engine.sched.idle():
+            # complete what is queued first, so that finish times are not
+            # taken after the sleep
+            engine.drain()
             if pending:
                 time.sleep(max(0.0, pending[0][0] - (time.perf_counter() - t0)))
             continue
@@ -471,10 +532,12 @@ def serve_poisson(engine: ServingEngine, requests: list[RolloutRequest], qps: fl
         if log and engine.ticks % 20 == 0:
             log(f"[serve] t={time.perf_counter() - t0:.1f}s tick {engine.ticks}: {len(engine.sched.active())} active, "
                 f"{len(engine.sched.queue)} queued, {len(engine.results)} done")
+    engine.drain()
     wall = time.perf_counter() - t0
     lat = np.array([engine.results[i].latency for i in ids])
     work = sum(r.n * r.n_steps for r in requests)
     return {"requests": len(ids), "qps": qps, "wall_s": wall, "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)), "mean_latency_s": float(lat.mean()),
             "particle_substeps_per_s": work / wall, "ticks": engine.ticks, "calls": engine.calls,
-            "grad_requests": sum(1 for r in requests if r.want_grad)}
+            "grad_requests": sum(1 for r in requests if r.want_grad),
+            "failed_requests": sum(1 for i in ids if engine.results[i].failed)}

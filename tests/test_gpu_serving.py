@@ -147,3 +147,39 @@ def test_plasticine_gradient_request_matches_the_plain_rollout(dev):
     print(f"  plasticine served vs plain: loss {el:.1e} grad {eg:.1e} (grad {grad.tolist()})", flush=True)
     assert float(model.yield_stress.min()) < 300.0  # it did soften
     assert el < 1e-4 and eg < 1e-3
+
+
+def test_a_failing_request_is_flagged_and_the_others_are_unaffected(dev):
+    """nu = 0.5 makes lambda infinite: the Neo-Hookean stress of that request
+    is NaN from its first substep (inf * (J^2 - 1) at J = 1), and its
+    velocities stay NaN; the requests batched with it keep their results."""
+    good = [(7, 16, 900, 30, False, 21), (0, 24, 1200, 25, True, 22)]
+    solo = [alone(dev, s) for s in good]
+    e = engine(dev)
+    ids = [e.submit(make_request(*s)) for s in good]
+    bad = make_request(6, 16, 800, 30, False, 23)
+    bad.nu = 0.5
+    bad_id = e.submit(bad)
+    res = e.run_until_idle()
+    print(f"[serving failure] failed flags {[res[i].failed for i in ids + [bad_id]]}", flush=True)
+    assert res[bad_id].failed and not torch.isfinite(res[bad_id].v).all()
+    for spec, rid, ref in zip(good, ids, solo):
+        assert not res[rid].failed
+        compare(spec, res[rid], ref)
+    e.close()
+
+
+def test_ticks_are_queued_ahead_of_the_device(dev):
+    """step() returns before the device has run the tick (at most
+    max_inflight ticks are queued), and the results appear after drain."""
+    e = engine(dev)
+    ids = [e.submit(make_request(*s)) for s in MIX[:4]]
+    e.step()
+    assert len(e._inflight) >= 1
+    while not e.sched.idle():
+        e.step()
+        assert len(e._inflight) <= e.max_inflight
+    e.drain()
+    assert not e._inflight and all(i in e.results for i in ids)
+    assert all(e.results[i].finished >= e.results[i].arrival for i in ids)
+    e.close()
