@@ -13,9 +13,15 @@ the backward replays each segment before reversing it.
 Adjoint mode (``mpm_solver.adjoint_mode``):
   "reference_compat" (default) reproduces the reference Tape's gradient
       conventions: the grid-mass adjoint is dropped
-      (mpm_data_structure.py:100-105 + warp_utils.py:116-120) and the Trail
+      (mpm_data_structure.py:100-105 + warp_utils.py:116-120), the Trail
       sums its per-frame output gradients onto the last state
-      (interface.py:176-197);
+      (interface.py:176-197), and the collider masks of the reverse sweep
+      are those of the collider parameters as they stand when backward runs:
+      the reference's Tape records each grid_postprocess launch with the
+      solver's collider_params object itself (mpm_solver_diff.py:265-278),
+      whose point modify_bc moves on after the launch (:860-866), so its
+      replay reads the parameters of the object's last state (times are
+      recorded by value per launch);
   "exact" is the derivative of the discrete rollout (what finite
       differences measure): grid-mass adjoint kept, each frame's gradient
       injected at the substep it was taken from.
@@ -29,11 +35,28 @@ import torch.autograd as autograd
 from torch import Tensor
 
 from ... import runtime as rt
-from ..._lib import EMPM_ADJ_DROP_GRID_MASS, EMPM_ADJ_EXACT
+from ..._lib import EMPM_ADJ_DROP_GRID_MASS, EMPM_ADJ_EXACT, Collider
 
 
 def _flags(solver):
     return EMPM_ADJ_EXACT if getattr(solver, "adjoint_mode", "reference_compat") == "exact" else EMPM_ADJ_DROP_GRID_MASS
+
+
+def _compat_colliders(ctx):
+    """reference_compat: the reverse sweep masks every substep with the
+    collider table as it stands now (see the module docstring); the replays
+    of a checkpointed tape keep the table each substep ran with."""
+    solver, tp = ctx.mpm_solver, ctx.tape
+    if _flags(solver) != EMPM_ADJ_DROP_GRID_MASS or tp is None or tp.ncol == 0:
+        return
+    cur = solver.collider_table()
+    if len(cur) != tp.ncol:
+        return
+    arr = (Collider * (tp.n_steps * tp.ncol))()
+    for k in range(tp.n_steps):
+        for c in range(tp.ncol):
+            arr[k * tp.ncol + c] = cur[c]
+    tp.adj_cols_hist = arr
 
 
 def _set_E_nu(solver, model, E, nu, scale_scalar_E):
@@ -180,6 +203,7 @@ class MPMDifferentiableSimulationTrail(autograd.Function):
                 else:
                     sd = seeds.setdefault(k, _zeros_like_state(n, dev))
                     sd.x.add_(g)
+        _compat_colliders(ctx)
         d0, grads = ctx.ro.backward(ctx.tape, d_final, seeds=seeds, flags=_flags(solver))
         solver.last_grads = grads
         E_grad, nu_grad = _param_grads(model, grads, ctx.agg_E, ctx.agg_nu, n)
@@ -253,6 +277,7 @@ class MPMDifferentiableSimulation(autograd.Function):
         density_change_mask = ctx.saved_tensors[0]
         d_final = _zeros_like_state(n, out_pos_grad.device)
         d_final.x.copy_(out_pos_grad)
+        _compat_colliders(ctx)
         d0, grads = ctx.ro.backward(ctx.tape, d_final, flags=_flags(solver))
         solver.last_grads = grads
         E_grad, nu_grad = _param_grads(model, grads, ctx.agg_E, ctx.agg_nu, n)
@@ -299,6 +324,10 @@ def _continue_backward(ctx, out_pos_grad, out_velo_grad, out_F_grad, out_C_grad,
         d.F.copy_(out_F_grad.reshape(n, 3, 3))
     if out_C_grad is not None:
         d.C.copy_(out_C_grad.reshape(n, 3, 3))
+    if seed_position_only:
+        # Clean's Tape is recorded in its forward (interface.py:853-872), so
+        # its reverse sweep reads the colliders' last state as well
+        _compat_colliders(ctx)
     d0, grads = ctx.ro.backward(ctx.tape, d, flags=_flags(solver))
     solver.last_grads = grads
     E_grad, nu_grad = _param_grads(model, grads, ctx.agg_E, ctx.agg_nu, n)

@@ -388,6 +388,9 @@ class RolloutTape:
     ncol: int
     checkpoints: dict = field(default_factory=dict)  # step -> ParticleState
     cols_hist: object = None  # (Collider * (n_steps*ncol))
+    # the collider table the reverse sweep masks with, when it is not the
+    # one each substep ran with (None: cols_hist); replays always use cols_hist
+    adj_cols_hist: object = None
     times_hist: object = None  # (c_double * n_steps)
     perm: object = None  # cell order used internally (None: caller order)
     model: object = None  # ModelArrays in the internal order
@@ -711,7 +714,8 @@ class Rollout:
                 cnt = k - j
                 sub = _states_array(states[j - s0:k - s0])
                 garr = _grid_array(grids[j - s0:k - s0])
-                chj = C.cast(C.byref(tp.cols_hist, j * ncol * C.sizeof(Collider)), C.POINTER(Collider))
+                adj_cols = tp.adj_cols_hist if tp.adj_cols_hist is not None else tp.cols_hist
+                chj = C.cast(C.byref(adj_cols, j * ncol * C.sizeof(Collider)), C.POINTER(Collider))
                 thj = C.cast(C.byref(tp.times_hist, j * C.sizeof(C.c_double)), C.POINTER(C.c_double))
                 self._adj(m, chj, ncol, thj, C.c_double(tp.dt), cnt, sub, garr, _states_array(adj), *grads.ptrs(),
                           int(flags), stream)

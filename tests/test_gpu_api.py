@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 import torch
 
+from empm_amd import runtime as rt
 from empm_amd import scenes
 from empm_amd.diff_simulator import WarpMPMWrapper
 from empm_amd.diff_simulator.warp_mpm.interface import (MPMDifferentiableSimulation,
@@ -379,3 +380,70 @@ def test_errors_mirror_reference():
     w = WarpMPMWrapper(grid_size=16)
     with pytest.raises(AssertionError):
         w.add_particles(np.array([[0.5, 0.5, 1.5]]))
+
+
+def test_reference_compat_masks_the_reverse_sweep_with_the_colliders_last_state():
+    """The reference's Tape records each collider launch with the solver's
+    collider_params object (mpm_solver_diff.py:265-278) and modify_bc moves
+    that object's point on after the launch (:860-866): the replay reads the
+    object's last state.  In reference_compat mode the mirror's Trail,
+    Simulation and Clean sweep with the table as it stands at backward time
+    (tape.adj_cols_hist), while a checkpointed tape's replays keep each
+    substep's own table.  Checks: the swept table is the solver's current one
+    at every substep (exact mode: none); with a cuboid that moves through
+    the body the gradient differs from the sweep masked by each substep's
+    own cuboid; and it is the same with a recorded tape and with
+    checkpoint-and-replay (K = 4).  Parity unpinned: what the reference's
+    Tape returns here is read from its code, not run."""
+    import types
+
+    from empm_amd import _lib
+    from empm_amd.diff_simulator.warp_mpm import interface as itf
+
+    sc = scenes.small_scene(n_particles=600, n_grid=16, material=6, seed=4, E=3e3, dt=2e-4, center=(0.5, 0.5, 0.3),
+                            vel=(0.0, 0.0, -1.0), spin=0.0)
+    mpm = _trail_setup(sc)
+    solver = mpm.mpm_solver
+    # a cuboid pushing through the body along x over the whole rollout
+    solver.set_velocity_on_cuboid((0.38, 0.5, 0.3), (0.05, 0.2, 0.2), (2.0, 0.0, 0.0))
+    arrays = solver.model_arrays(mpm.mpm_model, mpm.mpm_state)
+    ctx = solver.context(arrays.n_grid, sc.n, "cuda:0")
+    st = mpm.mpm_state
+    s0 = rt.ParticleState(st.particle_x.clone(), st.particle_v.clone(), torch.zeros((sc.n, 3, 3), device="cuda:0"),
+                          torch.eye(3, device="cuda:0").repeat(sc.n, 1, 1).contiguous())
+    steps = 24
+    grads = {}
+    for K in (1, 4):
+        solver.time = 0.0
+        solver.collider_params[-1].point = (0.38, 0.5, 0.3)
+        ro = rt.Rollout(ctx, arrays, sc.n)
+        cols0 = solver.collider_table()
+        cols_out = []
+        final, _, _, tape = ro.forward(s0, cols0, 0.0, sc.dt, steps, tape=True, ckpt_every=K, cols_out=cols_out)
+        solver.sync_colliders_from(cols_out)
+        assert abs(solver.collider_params[-1].point[0] - (0.38 + 2.0 * sc.dt * steps)) < 1e-5
+        for mode in ("exact", "reference_compat"):
+            solver.adjoint_mode = mode
+            fake = types.SimpleNamespace(mpm_solver=solver, tape=tape)
+            tape.adj_cols_hist = None
+            itf._compat_colliders(fake)
+            if mode == "exact":
+                assert tape.adj_cols_hist is None
+                continue
+            cur = solver.collider_table()
+            for k in (0, steps // 2, steps - 1):
+                for c in range(tape.ncol):
+                    got = tape.adj_cols_hist[k * tape.ncol + c]
+                    assert tuple(got.point) == tuple(cur[c].point) and got.kind == cur[c].kind
+        d = rt.ParticleState.empty(sc.n, "cuda:0")
+        d.zero_()
+        d.x.copy_(torch.sin(final.x * 11.0))
+        _, g_last = ro.backward(tape, d, flags=_lib.EMPM_ADJ_DROP_GRID_MASS)
+        tape.adj_cols_hist = None
+        _, g_own = ro.backward(tape, d, flags=_lib.EMPM_ADJ_DROP_GRID_MASS)
+        grads[K] = [float(t.double().sum()) for t in (g_last.mu, g_last.lam)]
+        own = [float(t.double().sum()) for t in (g_own.mu, g_own.lam)]
+        print(f"[compat colliders K={K}] last-state masks {grads[K]} own masks {own}", flush=True)
+        assert abs(grads[K][0] - own[0]) > 1e-3 * abs(own[0])
+    for a, b in zip(grads[1], grads[4]):
+        assert abs(a - b) <= 1e-4 * abs(a)
