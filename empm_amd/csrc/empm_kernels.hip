@@ -3,7 +3,8 @@
 // include/empm.h.
 //
 // Per substep the forward runs:
-//   k_bin_*  counting sort of the particles by stencil cell (4 launches);
+//   k_bin_*  counting sort of the particles by stencil cell (two launches
+//            after the previous G2P did the counting);
 //   k_p2g    per particle block: APIC scatter of momentum, mass and the
 //            stress impulse into block-owned LDS tiles;
 //   k_grid   per node: gather of the covering tiles, normalise, gravity,
@@ -403,8 +404,10 @@ constexpr int kSchedCap = 1024;   // substeps per schedule upload (longer calls 
 constexpr size_t kGraphCache = 8; // instantiated graphs kept per context (least recently used evicted)
 
 // Particle binning of the current segment: perm[k] = k-th particle in
-// block-major order, offs[b * 8 + c] = first position of cell c of block b
-// (offs has nb^3 * 8 + 1 entries).
+// block order, offs[b * 9 + c] = first position of cell c of block b and
+// offs[b * 9 + 8] its end (entries of empty blocks are not written).  The
+// blocks' ranges are in block order (empm_sort_by_cell), or in the order
+// their workgroups allocated them (the per-substep binning).
 // Dense grid for contributions of particles outside their block's tile (only
 // particles outside the grid), and a flag raised when it holds anything.
 struct Overflow {
@@ -678,7 +681,7 @@ EMPM_DEV void block_scatter(const Batch& Bt, const Binning& B, float4* __restric
     __syncthreads();
     int cs[9];
 #pragma unroll
-    for (int c = 0; c <= 8; ++c) cs[c] = B.offs[blk * 8 + c];
+    for (int c = 0; c <= 8; ++c) cs[c] = B.offs[blk * 9 + c];
     int nch = 0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) nch += (cs[c + 1] - cs[c] + kWave - 1) / kWave;
@@ -1089,14 +1092,13 @@ __global__ __launch_bounds__(kBlock) void k_g2p(Batch Bt, float dt, const float*
 // S = dt inv_dx dL/dF' Fe^T): k_g2p_adj stores A, the only part that needs
 // the gather (the clamp mask), and the scatter forms B and S from the
 // state's cotangents itself.
-__global__ __launch_bounds__(kBlock) void k_g2p_adj(Batch Bt, float dt, const float* __restrict__ x,
-                                                    const float* __restrict__ Fe, const float4* __restrict__ v_out,
-                                                    const float* __restrict__ dx1, const float* __restrict__ dv1,
-                                                    const float* __restrict__ dC1, const float* __restrict__ dF1,
-                                                    float* __restrict__ xbar, float* __restrict__ Febar,
-                                                    float* __restrict__ payload, int* __restrict__ scatter_work) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *scatter_work = 0;  // for the k_g2p_adj_scatter launch that follows
-  per_particle(Bt, -1, [&](const DevModel& M, const int p) {
+// Particle p's part: the cotangents of its next state (dx1, dv1, dC1, dF1)
+// come in registers, so the P2G adjoint of the later substep can run this
+// for the same particle right after its own work (k_p2g_adj<.., FUSE>).
+EMPM_DEV void g2p_adj_particle(const DevModel& M, float dt, int p, const float* __restrict__ x,
+                               const float* __restrict__ Fe, const float4* __restrict__ v_out, const float* dx1,
+                               const float* dv1, const float* dC, const float* dF, float* __restrict__ xbar,
+                               float* __restrict__ Febar, float* __restrict__ payload) {
   if (M.sel && M.sel[p] != 0) {
     xbar[3 * p] = xbar[3 * p + 1] = xbar[3 * p + 2] = 0.f;
     float Z[9];
@@ -1107,10 +1109,8 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(Batch Bt, float dt, const fl
   const float inv_dx = M.inv_dx;
   const float c4 = inv_dx * 4.f;
   const float xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
-  float dF[9], Fp[9], dC[9], Gb[9];
-  load9(dF1, p, dF);
+  float Fp[9], Gb[9];
   load9(Fe, p, Fp);
-  load9(dC1, p, dC);
   // F1 = (I + dt G) Fe:  Gbar = dt dF Fe^T ; Febar = (I + dt G)^T dF
   m_mul_bt(dF, Fp, Gb);
 #pragma unroll
@@ -1212,9 +1212,9 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(Batch Bt, float dt, const fl
     const float xn = xp[k] + dt * nv[k];
     // wp.clamp passes the adjoint to x unless x < lo or x > hi
     const float msk = (xn < amin || xn > amax) ? 0.f : 1.f;
-    const float g = dx1[3 * p + k] * msk;
+    const float g = dx1[k] * msk;
     xb[k] = g;
-    nvb[k] = dv1[3 * p + k] + dt * g;
+    nvb[k] = dv1[k] + dt * g;
   }
   float IF[9], Feb[9];
 #pragma unroll
@@ -1225,6 +1225,20 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(Batch Bt, float dt, const fl
   for (int l = 0; l < 3; ++l)
     xbar[3 * p + l] = xb[l] + inv_dx * fb[l] + (nvb[0] * nG[l] + nvb[1] * nG[3 + l] + nvb[2] * nG[6 + l]);
   reinterpret_cast<float4*>(payload)[p] = make_float4(nvb[0], nvb[1], nvb[2], 0.f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_g2p_adj(Batch Bt, float dt, const float* __restrict__ x,
+                                                    const float* __restrict__ Fe, const float4* __restrict__ v_out,
+                                                    const float* __restrict__ dx1, const float* __restrict__ dv1,
+                                                    const float* __restrict__ dC1, const float* __restrict__ dF1,
+                                                    float* __restrict__ xbar, float* __restrict__ Febar,
+                                                    float* __restrict__ payload, int* __restrict__ scatter_work) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *scatter_work = 0;  // for the k_g2p_adj_scatter launch that follows
+  // the cotangents are read where the body uses them (the position and
+  // velocity ones only after the node loop)
+  per_particle(Bt, -1, [&](const DevModel& M, const int p) {
+    g2p_adj_particle(M, dt, p, x, Fe, v_out, dx1 + 3 * (size_t)p, dv1 + 3 * (size_t)p, dC1 + 9 * (size_t)p,
+                     dF1 + 9 * (size_t)p, xbar, Febar, payload);
   });
 }
 
@@ -1299,24 +1313,48 @@ __global__ __launch_bounds__(kBlock) void k_grid_adj(Batch Bt, const ColTable* _
 // in their return map (material 5, hardening): those are then state, read
 // from the substep's record `mst` (or the model arrays when it is null), and
 // d_mu, d_lam, d_ys carry their cotangents backwards through the update.
-template <int MAT, bool MUT>
+// The G2P adjoint of the substep before, for the same particle, when the
+// reverse sweep fuses the two (k_p2g_adj<.., FUSE = true>): that substep's
+// recorded input positions, Fe and grid velocity, and where its partial
+// adjoints go (the xbar / Febar this launch reads are overwritten per
+// particle after they are read).
+struct PrevG2PAdj {
+  const float* x;
+  const float* Fe;
+  const float4* v_out;
+  float* xbar;
+  float* Febar;
+  float* payload;
+  int* scatter_work;
+};
+
+// With FUSE, the cotangents of the position and velocity of this substep's
+// input state go straight into the previous substep's G2P adjoint in
+// registers and are not stored (the sweep reads them nowhere else); those of
+// C and F_trial are stored for that substep's scatter.
+template <int MAT, bool MUT, bool FUSE>
 __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const float* __restrict__ x,
                                                     const float* __restrict__ v, const float* __restrict__ C,
                                                     const float* __restrict__ F, const float4* __restrict__ gadj_in,
-                                                    const float* __restrict__ xbar, const float* __restrict__ Febar_in,
+                                                    const float* xbar, const float* Febar_in,
                                                     float* __restrict__ dx, float* __restrict__ dv,
                                                     float* __restrict__ dCo, float* __restrict__ dFo,
                                                     float* __restrict__ dmu, float* __restrict__ dlam,
                                                     float* __restrict__ dys, float* __restrict__ dmass,
-                                                    float* __restrict__ dvol, const float* __restrict__ mst) {
+                                                    float* __restrict__ dvol, const float* __restrict__ mst,
+                                                    PrevG2PAdj prev) {
+  if (FUSE && blockIdx.x == 0 && threadIdx.x == 0) *prev.scatter_work = 0;  // for the scatter that follows
   per_particle(Bt, MAT, [&](const DevModel& M, const int p) {
   if (M.sel && M.sel[p] != 0) {
     float Z[9];
     m_zero(Z);
-    dx[3 * p] = dx[3 * p + 1] = dx[3 * p + 2] = 0.f;
-    dv[3 * p] = dv[3 * p + 1] = dv[3 * p + 2] = 0.f;
+    if (!FUSE) {
+      dx[3 * p] = dx[3 * p + 1] = dx[3 * p + 2] = 0.f;
+      dv[3 * p] = dv[3 * p + 1] = dv[3 * p + 2] = 0.f;
+    }
     store9(dCo, p, Z);
     store9(dFo, p, Z);
+    if (FUSE) g2p_adj_particle(M, dt, p, prev.x, prev.Fe, prev.v_out, Z, Z, Z, Z, prev.xbar, prev.Febar, prev.payload);
     return;
   }
   const float xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
@@ -1516,10 +1554,12 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
       stress_adj(st, MAT, mu, lam, taur, Feb, mub1, lamb1);
       return_map_adj(st, Ftr, mu0, lam0, ys0, matp, dt, Feb, Ftrb, mub, lamb, ysb);
       vm_state_adj(st, mu0, ys0, mu, matp, MAT == 5, ysb1, mub1, lamb1, Ftrb, ysb, mub, lamb);
+      if (!FUSE) {
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        dx[3 * p + i] = xo[i];
-        dv[3 * p + i] = vb[i];
+        for (int i = 0; i < 3; ++i) {
+          dx[3 * p + i] = xo[i];
+          dv[3 * p + i] = vb[i];
+        }
       }
       store9(dCo, p, Cb);
       store9(dFo, p, Ftrb);
@@ -1528,15 +1568,20 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
       if (dys) dys[p] = ysb;
       if (dmass) dmass[p] += mb;
       if (dvol) dvol[p] += Vb;
+      if (FUSE)
+        g2p_adj_particle(M, dt, p, prev.x, prev.Fe, prev.v_out, xo, vb, Cb, Ftrb, prev.xbar, prev.Febar,
+                         prev.payload);
       return;
     }
   }
   stress_adj(st, MAT, mu0, lam0, taur, Feb, mub, lamb);
   return_map_adj(st, Ftr, mu0, lam0, ys0, matp, dt, Feb, Ftrb, mub, lamb, ysb);
+  if (!FUSE) {
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    dx[3 * p + i] = xo[i];
-    dv[3 * p + i] = vb[i];
+    for (int i = 0; i < 3; ++i) {
+      dx[3 * p + i] = xo[i];
+      dv[3 * p + i] = vb[i];
+    }
   }
   store9(dCo, p, Cb);
   store9(dFo, p, Ftrb);
@@ -1545,33 +1590,35 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
   if (dys) dys[p] += ysb;
   if (dmass) dmass[p] += mb;
   if (dvol) dvol[p] += Vb;
+  if (FUSE)
+    g2p_adj_particle(M, dt, p, prev.x, prev.Fe, prev.v_out, xo, vb, Cb, Ftrb, prev.xbar, prev.Febar, prev.payload);
   });
 }
 
 // ---------------------------------------------------------------------------
 // Binning, every substep: counting sort of the particles by the cell their
-// stencil starts at.  Bins are the 8 cells of a 2x2x2-cell block, in
-// block-major, cell-minor order.
+// stencil starts at.  Bins are the 8 cells of a 2x2x2-cell block; a block's
+// particles are contiguous, cell-minor.
 //   k_bin_count   per particle: bin key and rank within the bin (same-bin
 //                 lanes of a wave share one atomic); counts are zero on entry
-//   k_bin_scan    one thread per block, kBinScanWG blocks per workgroup:
-//                 block totals and their workgroup-local exclusive scan;
-//                 clears the active and near counts and the scatter's
-//                 work counter
-//   k_bin_finish  one thread per block: global block offset (the sum of the
-//                 earlier workgroups' totals, reduced by the workgroup),
-//                 per-bin offsets, occupancy, active list; re-zeroes the
-//                 counts for the next binning
+//                 (within a rollout the previous substep's G2P does this)
+//   k_bin_alloc   one thread per block, kBinScanWG blocks per workgroup:
+//                 block totals, their workgroup-local scan and one atomic
+//                 per workgroup for its particle range; per-bin offsets,
+//                 occupancy, active list; re-zeroes the counts
 //   k_bin_place   per particle: perm[offs[key] + rank] = p; per block: the
 //                 near flag (a gather of the 8 occupancy flags whose tiles
 //                 reach the block) and the near list
+// empm_sort_by_cell needs the blocks' ranges in block order and runs
+// k_bin_scan + k_bin_finish (a grid-wide prefix over the workgroups)
+// instead of k_bin_alloc.
 // ---------------------------------------------------------------------------
 constexpr int kBinScanWG = 1024;
 
 // What the scatter and grid kernels read of one binning (device arrays).
 struct BinBuf {
   int* perm;  // N
-  int* offs;  // nb^3 * 8 + 1
+  int* offs;  // nb^3 * 9
   int* act;   // 2 + nb^3: heavy and light counts, then the non-empty block ids
   int* occ;   // nb^3
   int* near;  // nb^3
@@ -1653,6 +1700,47 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_scan(int nblk, const int* __
   }
 }
 
+// Block b (non-empty, with per-cell counts c) joins the active list: heavy
+// blocks (more than heavy_chunks chunks) from act[2] upwards, light ones
+// from act[2 + nblk - 1] downwards; the lists' lengths are counted in
+// *nheavy and *nlight, one atomic per wave and list.
+EMPM_DEV void append_active(int b, int nblk, const int* c, int heavy_chunks, int* __restrict__ act, int* nheavy,
+                            int* nlight) {
+  const unsigned long long lt = (1ull << __lane_id()) - 1ull;
+  const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+  int nch = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) nch += (c[k] + kWave - 1) / kWave;
+  const bool heavy = nch > heavy_chunks;
+  const unsigned long long hmask = __ballot(heavy), lmask = __ballot(!heavy);
+  int hbase = 0, lbase = 0;
+  if (__lane_id() == leader) {
+    if (hmask) hbase = atomicAdd(nheavy, __popcll(hmask));
+    if (lmask) lbase = atomicAdd(nlight, __popcll(lmask));
+  }
+  hbase = __shfl(hbase, leader);
+  lbase = __shfl(lbase, leader);
+  if (heavy)
+    act[2 + hbase + __popcll(hmask & lt)] = b;
+  else
+    act[2 + nblk - 1 - (lbase + __popcll(lmask & lt))] = b;
+}
+
+// Per-bin offsets of block b (non-empty, counts c, first position o), its
+// end, and its counts re-zeroed for the next binning.
+EMPM_DEV void write_block_offsets(int b, int o, const int* c, int* __restrict__ offs, int* __restrict__ count) {
+  int* ob = offs + (size_t)b * 9;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    ob[k] = o;
+    o += c[k];
+  }
+  ob[8] = o;
+  int4* c4 = reinterpret_cast<int4*>(count + (size_t)b * 8);
+  c4[0] = make_int4(0, 0, 0, 0);
+  c4[1] = make_int4(0, 0, 0, 0);
+}
+
 __global__ __launch_bounds__(kBinScanWG) void k_bin_finish(Batch Bt, int* __restrict__ count,
                                                            const int* __restrict__ blkoff,
                                                            const int* __restrict__ wsum, int* __restrict__ offs,
@@ -1678,55 +1766,67 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_finish(Batch Bt, int* __rest
   const int e = blkoff[b];
   const bool full = e >= 0;
   occ[b] = full ? 1 : 0;
-  if (!full) {
-    if (b == nblk - 1) offs[(size_t)nblk * 8] = prefix - 1 - e;
-    return;
-  }
-  int4* c4 = reinterpret_cast<int4*>(count + (size_t)b * 8);
+  if (!full) return;
+  const int4* c4 = reinterpret_cast<const int4*>(count + (size_t)b * 8);
   const int4 lo = c4[0], hi = c4[1];
   const int c[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-  int o = prefix + e;
-  int run[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    run[k] = o;
-    o += c[k];
+  write_block_offsets(b, prefix + e, c, offs, count);
+  append_active(b, nblk, c, heavy_chunks, act, &act[0], &act[1]);
+}
+
+// Offsets of the per-substep binning in one launch (k_bin_scan +
+// k_bin_finish without the grid-wide prefix): each workgroup scans its
+// blocks' totals and takes its particle range with one atomic on ctr[0], so
+// the blocks' ranges are in the order the workgroups got there.  The active
+// list is counted in ctr[1], ctr[2]; k_bin_place copies those counts into
+// act[0], act[1] and re-zeroes ctr[0..2].  Also clears the near-list count,
+// the overflow flag and the k_p2g work counter.
+__global__ __launch_bounds__(kBinScanWG) void k_bin_alloc(Batch Bt, int* __restrict__ count, int* __restrict__ offs,
+                                                          int* __restrict__ act, int* __restrict__ occ,
+                                                          int* __restrict__ nlist, int* __restrict__ ovf_flag,
+                                                          int* __restrict__ p2g_work, int* __restrict__ ctr,
+                                                          int heavy_chunks) {
+  __shared__ int smem[kBinScanWG / 64 + 1];
+  __shared__ int wbase;
+  const int nblk = Bt.nblk;
+  const int b = blockIdx.x * kBinScanWG + threadIdx.x;
+  int c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (b < nblk) {
+    const int4* c4 = reinterpret_cast<const int4*>(count + (size_t)b * 8);
+    const int4 lo = c4[0], hi = c4[1];
+    c[0] = lo.x, c[1] = lo.y, c[2] = lo.z, c[3] = lo.w, c[4] = hi.x, c[5] = hi.y, c[6] = hi.z, c[7] = hi.w;
   }
-  int4* o4 = reinterpret_cast<int4*>(offs + (size_t)b * 8);
-  o4[0] = make_int4(run[0], run[1], run[2], run[3]);
-  o4[1] = make_int4(run[4], run[5], run[6], run[7]);
-  offs[(size_t)(b + 1) * 8] = o;  // this block's end (the next block writes the same value if non-empty)
-  c4[0] = make_int4(0, 0, 0, 0);
-  c4[1] = make_int4(0, 0, 0, 0);
-  // list appends are aggregated per wave (one atomic per list)
-  const unsigned long long lt = (1ull << __lane_id()) - 1ull;
-  const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
-  int nch = 0;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) nch += (c[k] + kWave - 1) / kWave;
-  const bool heavy = nch > heavy_chunks;
-  const unsigned long long hmask = __ballot(heavy), lmask = __ballot(!heavy);
-  int hbase = 0, lbase = 0;
-  if (__lane_id() == leader) {
-    if (hmask) hbase = atomicAdd(&act[0], __popcll(hmask));
-    if (lmask) lbase = atomicAdd(&act[1], __popcll(lmask));
+  const int tot = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7];
+  int wtot;
+  const int ex = wg_exclusive_scan(tot, smem, wtot);
+  if (threadIdx.x == 0) wbase = wtot > 0 ? atomicAdd(&ctr[0], wtot) : 0;
+  if (b == 0) {
+    nlist[0] = 0;
+    *ovf_flag = 0;
+    *p2g_work = 0;  // for the k_p2g launch that follows
   }
-  hbase = __shfl(hbase, leader);
-  lbase = __shfl(lbase, leader);
-  if (heavy)
-    act[2 + hbase + __popcll(hmask & lt)] = b;
-  else
-    act[2 + nblk - 1 - (lbase + __popcll(lmask & lt))] = b;
+  __syncthreads();
+  if (b >= nblk) return;
+  occ[b] = tot > 0 ? 1 : 0;
+  if (tot == 0) return;
+  write_block_offsets(b, wbase + ex, c, offs, count);
+  append_active(b, nblk, c, heavy_chunks, act, &ctr[1], &ctr[2]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_bin_place(Batch Bt, const int2* __restrict__ key_rank,
                                                       const int* __restrict__ offs, int* __restrict__ perm,
                                                       const int* __restrict__ occ, int* __restrict__ near,
-                                                      int* __restrict__ nlist) {
+                                                      int* __restrict__ nlist, int* __restrict__ act,
+                                                      int* __restrict__ ctr) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ctr && i == 0) {  // after k_bin_alloc: the active-list counts into the record, counters re-zeroed
+    act[0] = ctr[1];
+    act[1] = ctr[2];
+    ctr[0] = ctr[1] = ctr[2] = 0;
+  }
   if (i < Bt.N) {
     const int2 kr = key_rank[i];
-    perm[offs[kr.x] + kr.y] = i;
+    perm[offs[(kr.x >> 3) * 9 + (kr.x & 7)] + kr.y] = i;
   }
   // block i is near when a non-empty block of its simulation has it in its
   // tile (blocks i - {0,1}^3 per axis); near blocks are appended to the
@@ -1962,6 +2062,7 @@ struct empm_ctx {
   int2* key_rank = nullptr;  // N: bin key and rank of each particle
   int* blkoff = nullptr;     // max_blocks: workgroup-local exclusive block offsets
   int* wsum = nullptr;       // block-scan workgroup totals
+  int* bin_ctr = nullptr;    // 3: k_bin_alloc's particle-range and active-list counters, zero between binnings
   int* work = nullptr;       // 2: work counters of the k_p2g and k_g2p_adj_scatter launches
   empm_tuning tune = {kBlkWG, 0, kHeavyChunks, kDynamicRounds, kNodePassGridCap};
   int cus = 0;               // compute units of the device
@@ -2259,40 +2360,42 @@ int make_table(const empm_collider* cols, int ncol, ColTable& T) {
                       (hb).dev, __VA_ARGS__);                                                   \
     }
 
-// k_p2g_adj<mat, mutating> over the particle waves of each material present;
-// the mutating variant only for materials 1, 5 and 7 when one of their
-// simulations rewrites its parameters
-#define EMPM_LAUNCH_P2G_ADJ(hb, stream, ...)                                                     \
-  for (int mat_ = 0; mat_ < 8; ++mat_)                                                          \
-    if ((hb).has_material[mat_]) {                                                              \
-      const dim3 g_(material_grid(hb, mat_)), b_(kBlock);                                       \
-      const bool mut_ = (hb).mutates[mat_];                                                     \
-      switch (mat_) {                                                                           \
-        case 0: hipLaunchKernelGGL((k_p2g_adj<0, false>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__); break; \
-        case 1:                                                                                 \
-          if (mut_) hipLaunchKernelGGL((k_p2g_adj<1, true>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__);     \
-          else hipLaunchKernelGGL((k_p2g_adj<1, false>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__);         \
-          break;                                                                                \
-        case 2: hipLaunchKernelGGL((k_p2g_adj<2, false>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__); break; \
-        case 3: hipLaunchKernelGGL((k_p2g_adj<3, false>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__); break; \
-        case 4: hipLaunchKernelGGL((k_p2g_adj<4, false>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__); break; \
-        case 5: hipLaunchKernelGGL((k_p2g_adj<5, true>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__); break;  \
-        case 6: hipLaunchKernelGGL((k_p2g_adj<6, false>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__); break; \
-        default:                                                                                \
-          if (mut_) hipLaunchKernelGGL((k_p2g_adj<7, true>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__);     \
-          else hipLaunchKernelGGL((k_p2g_adj<7, false>), g_, b_, 0, stream, (hb).dev, __VA_ARGS__);         \
-          break;                                                                                \
-      }                                                                                         \
+// k_p2g_adj<mat, mutating, FUSE> over the particle waves of each material
+// present; the mutating variant only for materials 1, 5 and 7 when one of
+// their simulations rewrites its parameters
+template <bool FUSE, typename... A>
+void launch_p2g_adj(const HostBatch& hb, hipStream_t stream, A... args) {
+  for (int mat = 0; mat < 8; ++mat) {
+    if (!hb.has_material[mat]) continue;
+    const dim3 g(material_grid(hb, mat)), b(kBlock);
+    const bool mut = hb.mutates[mat];
+    switch (mat) {
+      case 0: hipLaunchKernelGGL((k_p2g_adj<0, false, FUSE>), g, b, 0, stream, hb.dev, args...); break;
+      case 1:
+        if (mut) hipLaunchKernelGGL((k_p2g_adj<1, true, FUSE>), g, b, 0, stream, hb.dev, args...);
+        else hipLaunchKernelGGL((k_p2g_adj<1, false, FUSE>), g, b, 0, stream, hb.dev, args...);
+        break;
+      case 2: hipLaunchKernelGGL((k_p2g_adj<2, false, FUSE>), g, b, 0, stream, hb.dev, args...); break;
+      case 3: hipLaunchKernelGGL((k_p2g_adj<3, false, FUSE>), g, b, 0, stream, hb.dev, args...); break;
+      case 4: hipLaunchKernelGGL((k_p2g_adj<4, false, FUSE>), g, b, 0, stream, hb.dev, args...); break;
+      case 5: hipLaunchKernelGGL((k_p2g_adj<5, true, FUSE>), g, b, 0, stream, hb.dev, args...); break;
+      case 6: hipLaunchKernelGGL((k_p2g_adj<6, false, FUSE>), g, b, 0, stream, hb.dev, args...); break;
+      default:
+        if (mut) hipLaunchKernelGGL((k_p2g_adj<7, true, FUSE>), g, b, 0, stream, hb.dev, args...);
+        else hipLaunchKernelGGL((k_p2g_adj<7, false, FUSE>), g, b, 0, stream, hb.dev, args...);
+        break;
     }
+  }
+}
 
 size_t pad4(size_t k) { return (k + 3) & ~(size_t)3; }
 
-// Layout of one binning record: [perm N | offs 8 nblk + 1 | act 2 + nblk |
+// Layout of one binning record: [perm N | offs 9 nblk | act 2 + nblk |
 // occ nblk | near nblk | nlist 1 + nblk], each section padded to a multiple
 // of 4 ints so that the int4 accesses of the binning kernels stay 16-byte
 // aligned.
 size_t bin_record_ints(long long nblk, long long n_particles) {
-  return pad4((size_t)n_particles) + pad4((size_t)nblk * 8 + 1) + pad4(2 + (size_t)nblk) +
+  return pad4((size_t)n_particles) + pad4((size_t)nblk * 9) + pad4(2 + (size_t)nblk) +
          pad4(1 + (size_t)nblk) + 2 * pad4((size_t)nblk);
 }
 
@@ -2300,7 +2403,7 @@ BinBuf bin_buf(int* rec, int nblk, int n_particles) {
   BinBuf b;
   b.perm = rec;
   b.offs = b.perm + pad4((size_t)n_particles);
-  b.act = b.offs + pad4((size_t)nblk * 8 + 1);
+  b.act = b.offs + pad4((size_t)nblk * 9);
   b.occ = b.act + pad4(2 + (size_t)nblk);
   b.near = b.occ + pad4((size_t)nblk);
   b.nlist = b.near + pad4((size_t)nblk);
@@ -2352,9 +2455,12 @@ dim3 node_grid(const empm_ctx* ctx, int nblk) {
 // With `counted`, the counting step was done by the previous substep's G2P
 // on these positions (ctx->counts_pending); else it runs here, after
 // clearing the counts of a G2P count whose binning never ran (a call that
-// stopped between the two).
+// stopped between the two).  `ordered` lays the blocks' particle ranges out
+// in block order (k_bin_scan + k_bin_finish: the simulations of a batch
+// keep their arena ranges); else they are allocated in one launch
+// (k_bin_alloc), in no particular block order.
 int do_bin(empm_ctx* ctx, const HostBatch& hb, const float* x, const BinBuf& out, hipStream_t st,
-           bool counted = false) {
+           bool counted = false, bool ordered = false) {
   ProfScope ps(ctx, EMPM_KERNEL_BIN, st);
   const int nblk = hb.nblk;
   const unsigned wgs = (unsigned)((nblk + kBinScanWG - 1) / kBinScanWG);
@@ -2369,12 +2475,18 @@ int do_bin(empm_ctx* ctx, const HostBatch& hb, const float* x, const BinBuf& out
                          ctx->key_rank);
   }
   ctx->counts_pending = false;
-  hipLaunchKernelGGL(k_bin_scan, dim3(wgs), dim3(kBinScanWG), 0, st, nblk, ctx->count, ctx->blkoff, ctx->wsum,
-                     out.act, out.nlist, ctx->ovf_flag, ctx->work);
-  hipLaunchKernelGGL(k_bin_finish, dim3(wgs), dim3(kBinScanWG), 0, st, hb.dev, ctx->count, ctx->blkoff, ctx->wsum,
-                     out.offs, out.act, out.occ, ctx->tune.heavy_chunks);
+  if (ordered) {
+    hipLaunchKernelGGL(k_bin_scan, dim3(wgs), dim3(kBinScanWG), 0, st, nblk, ctx->count, ctx->blkoff, ctx->wsum,
+                       out.act, out.nlist, ctx->ovf_flag, ctx->work);
+    hipLaunchKernelGGL(k_bin_finish, dim3(wgs), dim3(kBinScanWG), 0, st, hb.dev, ctx->count, ctx->blkoff,
+                       ctx->wsum, out.offs, out.act, out.occ, ctx->tune.heavy_chunks);
+  } else {
+    hipLaunchKernelGGL(k_bin_alloc, dim3(wgs), dim3(kBinScanWG), 0, st, hb.dev, ctx->count, out.offs, out.act,
+                       out.occ, out.nlist, ctx->ovf_flag, ctx->work, ctx->bin_ctr, ctx->tune.heavy_chunks);
+  }
   hipLaunchKernelGGL(k_bin_place, dim3(nblocks(hb.N > nblk ? hb.N : nblk)), dim3(kBlock), 0, st, hb.dev,
-                     ctx->key_rank, out.offs, out.perm, out.occ, out.near, out.nlist);
+                     ctx->key_rank, out.offs, out.perm, out.occ, out.near, out.nlist, out.act,
+                     ordered ? nullptr : ctx->bin_ctr);
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }
@@ -2426,10 +2538,20 @@ int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable* T, float dt, 
 // Adjoint of one substep.  With a recorded grid and Fe (rec->mv_m, v_out and
 // in->Fe) the forward stress, P2G and boundary pass are not recomputed; with
 // recorded bins (rec->bins) the particles are not re-binned.
+// Substep k of a reverse sweep may be fused with substep k - 1's G2P adjoint
+// (`prev` non-null: that substep's recorded state and grid), and its own G2P
+// adjoint may already have been done that way by substep k + 1
+// (`g2p_adj_done`).
+bool fully_recorded(const empm_particles* in, const empm_grid_rec* rec) {
+  return rec && rec->mv_m && rec->v_out && in->Fe;
+}
+
 int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable* T, float dt, const empm_particles* in,
                const empm_grid_rec* rec, const empm_particles* d_out, const empm_particles* d_in, float* d_mu,
-               float* d_lam, float* d_ys, float* d_mass, float* d_vol, int flags, hipStream_t st) {
-  const bool recorded = rec && rec->mv_m && rec->v_out && in->Fe;
+               float* d_lam, float* d_ys, float* d_mass, float* d_vol, int flags, hipStream_t st,
+               const empm_particles* prev = nullptr, const empm_grid_rec* prev_rec = nullptr,
+               bool g2p_adj_done = false) {
+  const bool recorded = fully_recorded(in, rec);
   const bool have_bins = recorded && rec->bins;
   float* Fe = recorded ? in->Fe : ctx->Fe;
   float4* keep = recorded ? reinterpret_cast<float4*>(rec->mv_m) : ctx->keep;
@@ -2457,7 +2579,7 @@ int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable* T, float dt, 
     }
   }
   if (hb.N > 0) {
-    {
+    if (!(g2p_adj_done && recorded)) {
       ProfScope ps(ctx, EMPM_KERNEL_G2P_ADJ, st);
       hipLaunchKernelGGL(k_g2p_adj, dim3(particle_grid(hb)), dim3(kBlock), 0, st, hb.dev, dt, in->x, Fe, v_out,
                          d_out->x, d_out->v, d_out->C, d_out->F, ctx->xbar, ctx->Febar, ctx->payload, ctx->work + 1);
@@ -2475,8 +2597,19 @@ int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable* T, float dt, 
   }
   if (hb.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G_ADJ, st);
-    EMPM_LAUNCH_P2G_ADJ(hb, st, dt, in->x, in->v, in->C, in->F, ctx->gadj_in, ctx->xbar, ctx->Febar, d_in->x,
-                        d_in->v, d_in->C, d_in->F, d_mu, d_lam, d_ys, d_mass, d_vol, in->mstate);
+    if (prev && fully_recorded(prev, prev_rec)) {
+      const PrevG2PAdj pg{prev->x, prev->Fe, reinterpret_cast<const float4*>(prev_rec->v_out), ctx->xbar,
+                          ctx->Febar, ctx->payload, ctx->work + 1};
+      launch_p2g_adj<true>(hb, st, dt, (const float*)in->x, (const float*)in->v, (const float*)in->C,
+                           (const float*)in->F, (const float4*)ctx->gadj_in, (const float*)ctx->xbar,
+                           (const float*)ctx->Febar, d_in->x, d_in->v, d_in->C, d_in->F, d_mu, d_lam, d_ys, d_mass,
+                           d_vol, (const float*)in->mstate, pg);
+    } else {
+      launch_p2g_adj<false>(hb, st, dt, (const float*)in->x, (const float*)in->v, (const float*)in->C,
+                            (const float*)in->F, (const float4*)ctx->gadj_in, (const float*)ctx->xbar,
+                            (const float*)ctx->Febar, d_in->x, d_in->v, d_in->C, d_in->F, d_mu, d_lam, d_ys, d_mass,
+                            d_vol, (const float*)in->mstate, PrevG2PAdj{});
+    }
   }
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
@@ -2532,6 +2665,8 @@ int alloc_ctx(empm_ctx* c) {
   e = e == hipSuccess ? hipMalloc((void**)&c->key_rank, np * sizeof(int2)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->blkoff, nblk * sizeof(int)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->wsum, nwsum * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMalloc((void**)&c->bin_ctr, 4 * sizeof(int)) : e;
+  e = e == hipSuccess ? hipMemset(c->bin_ctr, 0, 4 * sizeof(int)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->work, 2 * sizeof(int)) : e;
   const size_t cand_bytes = (size_t)c->max_cand * (sizeof(DevModel) + sizeof(int2));  // table + material lists
   e = e == hipSuccess ? hipMalloc((void**)&c->cand_dev, cand_bytes) : e;
@@ -2819,14 +2954,21 @@ int rollout_adj(empm_ctx* ctx, const empm_batch* batch, const empm_collider* col
       key_grid(key, grids ? &grids[k] : nullptr);
     }
     rc = run_call(ctx, st, key, [&](hipStream_t cs) {
+      // within a schedule chunk, substep k's P2G adjoint also runs substep
+      // k - 1's G2P adjoint when both are fully recorded
+      bool done = false;
       for (int i = 0; i < n; ++i) {
         const int j = j0 + i;
         const int k = n_steps - 1 - j;
         const empm_particles* dout = &adj[j % 2];
         const empm_particles* din = &adj[(j + 1) % 2];
+        const bool fuse = i + 1 < n && grids && fully_recorded(&states[k], &grids[k]) &&
+                          fully_recorded(&states[k - 1], &grids[k - 1]);
         const int r = launch_adj(ctx, hb, ctx->sched + i, (float)dt, &states[k], grids ? &grids[k] : nullptr, dout,
-                                 din, d_mu, d_lam, d_ys, d_mass, d_vol, flags, cs);
+                                 din, d_mu, d_lam, d_ys, d_mass, d_vol, flags, cs, fuse ? &states[k - 1] : nullptr,
+                                 fuse ? &grids[k - 1] : nullptr, done);
         if (r) return r;
+        done = fuse;
       }
       return (int)EMPM_OK;
     });
@@ -2931,6 +3073,7 @@ void empm_destroy(empm_ctx* c) {
   (void)hipFree(c->key_rank);
   (void)hipFree(c->blkoff);
   (void)hipFree(c->wsum);
+  (void)hipFree(c->bin_ctr);
   (void)hipFree(c->work);
   (void)hipFree(c->cand_dev);
   if (c->cand_pinned) (void)hipHostFree(c->cand_pinned);
@@ -3060,7 +3203,7 @@ int empm_sort_by_cell(empm_ctx* ctx, int n_particles, const float* x, float inv_
   int rc = upload_batch(ctx, hb, st);
   if (rc) return rc;
   const BinBuf bb = bin_buf(ctx->bins, hb.nblk, n_particles);
-  if ((rc = do_bin(ctx, hb, x, bb, st))) return rc;
+  if ((rc = do_bin(ctx, hb, x, bb, st, false, true))) return rc;
   HIP_TRY(hipMemcpyAsync(perm, bb.perm, (size_t)n_particles * sizeof(int), hipMemcpyDeviceToDevice, st));
   return EMPM_OK;
 }
@@ -3073,7 +3216,7 @@ int empm_batch_sort_by_cell(empm_ctx* ctx, const empm_batch* batch, const float*
   if (hb.N == 0) return EMPM_OK;
   if (!x || !perm) return set_err(EMPM_E_INVALID_ARG, "empm_batch_sort_by_cell: null array");
   const BinBuf bb = bin_buf(ctx->bins, hb.nblk, hb.N);
-  if ((rc = do_bin(ctx, hb, x, bb, st))) return rc;
+  if ((rc = do_bin(ctx, hb, x, bb, st, false, true))) return rc;
   HIP_TRY(hipMemcpyAsync(perm, bb.perm, (size_t)hb.N * sizeof(int), hipMemcpyDeviceToDevice, st));
   return EMPM_OK;
 }
