@@ -36,9 +36,19 @@ CKPT_EVERY = 1
 B_FB_K = 50
 
 
-def kernel_bytes(N, G):
+def kernel_bytes(N, G, T=1):
     """Algorithmic HBM bytes per launch of each kernel (FP32): the particle
-    and node fields that kernel must read or write, each once."""
+    and node fields that kernel must read or write, each once.  T: substeps
+    of the reverse sweep whose launches are averaged (all but its last run
+    the fused P2G + G2P adjoint)."""
+    # P2G adjoint alone: reads state + constants (116), partial adjoints (48) and dL/d(mv, m) once per node;
+    # writes the adjoint of the state (96) and read-modify-writes 5 parameter adjoints (40)
+    p2g_adj = 300 * N + 16 * G
+    # fused with the G2P adjoint of the substep before: the adjoints of x and v (24 B) stay in registers
+    # instead of being written, and that G2P adjoint reads x, Fe (48) and v_out once per node and writes
+    # dL/dx partial (12), dL/dFe (36) and dL/dv' (16); the adjoint of the next state comes in registers
+    fused = (p2g_adj - 24 * N) + (112 * N + 12 * G)
+    T = max(1, int(T))
     return {
         # reads x, v, C (60 B), the stress payload S (36 B) and mass (4 B); writes (mv, m) once per node
         "k_p2g": 100 * N + 16 * G,
@@ -49,20 +59,19 @@ def kernel_bytes(N, G):
         "k_g2p": 240 * N + 12 * G,
         # reads F_trial (36 B) and mu, lam, yield, vol (16 B); writes Fe and S (72 B)
         "k_stress": 124 * N,
-        # reads x, Fe (48), the adjoint of the next state (96) and v_out; writes dL/dx partial (12), dL/dFe (36)
-        # and dL/dv' for the scatter (16)
+        # (the first substep of a sweep only) reads x, Fe (48), the adjoint of the next state (96) and v_out;
+        # writes dL/dx partial (12), dL/dFe (36) and dL/dv' for the scatter (16)
         "k_g2p_adj": 208 * N + 12 * G,
         # reads x, dL/dv' (28 B), dL/dC', dL/dF' and Fe (108 B); writes dL/dv_out once per node (12 B)
         "k_g2p_adj_scatter": 136 * N + 12 * G,
         # reads (mv, m) and dL/dv_out, writes dL/d(mv, m)
         "k_grid_adj": 44 * G,
-        # reads state + constants (116), partial adjoints (48), dL/d(mv, m) once per node;
-        # writes the adjoint of the state (96) and read-modify-writes 5 parameter adjoints (40)
-        "k_p2g_adj": 300 * N + 16 * G,
+        # T - 1 fused launches and one alone per sweep
+        "k_p2g_adj": (fused * (T - 1) + p2g_adj) / T,
         # the binning kernels of a substep whose counting the previous G2P did: key and rank (8 B) read,
         # perm (4 B) written per particle; per 2x2x2-cell block (G / 8): 8 counts read and re-zeroed,
-        # 8 offsets written and the occupancy, near and list entries (about 112 B, 14 B per node)
-        "k_bin": 12 * N + 14 * G,
+        # 9 offsets written and the occupancy, near and list entries (about 116 B, 14.5 B per node)
+        "k_bin": 12 * N + 14.5 * G,
     }
 
 
@@ -239,7 +248,7 @@ def main():
         per_kernel[name] = (l.value, ms.value)
     lib.empm_profile_enable(ctx.handle, 0)
     _lib.check(lib.empm_profile_reset(ctx.handle))
-    known = kernel_bytes(N, G)
+    known = kernel_bytes(N, G, T)
     dom = max((k for k in per_kernel if k in known), key=lambda k: per_kernel[k][1])
     log("[bench] per-kernel (launches, ms) in one step: " +
         ", ".join(f"{k}={v[0]}/{v[1]:.2f}" for k, v in per_kernel.items()) + f" -> dominant {dom}")
@@ -266,7 +275,7 @@ def main():
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     value = N * T / (ms_per_step / 1000.0)
     avg_ms = ms.value / max(l.value, 1)
-    kb = kernel_bytes(N, G)[dom]
+    kb = kernel_bytes(N, G, T)[dom]
     achieved = kb / (avg_ms / 1000.0) / 1e9
     log(f"[bench] fwd+bwd: {ms_per_step:.2f} ms/step (median {ms_median:.2f}), {value:.4e} p-substeps/s; "
         f"{dom}: {l.value} launches, "

@@ -379,16 +379,20 @@ EMPM_DEV void grid_node(const DevModel& M, const ColTable& T, float dt, int ix, 
 // particles into a 4x4x4-node LDS tile (the block's stencil nodes), then
 // writes the whole tile to its slot with plain stores.  The grid pass
 // gathers, for each node, the <= 8 tiles that cover it.  No global atomics
-// are issued except for particles outside the grid (binned to an edge cell),
-// which add into a dense overflow grid and raise a flag the grid pass reads.
+// are issued.  Every in-grid node of a particle's stencil lies in its bin
+// block's tile: the bin's cell is the stencil base clamped to the grid, so a
+// base inside the grid puts the whole stencil in the tile, a base below 0
+// leaves only nodes 0 and 1 in the grid (tile nodes 0..3 of block 0), and a
+// base at n or above leaves none.  Stencil nodes outside the tile are thus
+// outside the grid and are dropped.
 //
 // The waves of a workgroup take the block's particles in chunks of <= 64
 // that never straddle a cell, so all lanes of a chunk share one stencil
 // base.  Their per-node contributions are summed across the wave by a
 // transposed butterfly (v_permlane32_swap, v_permlane16_swap, then DPP
 // row/quad exchanges), leaving one LDS atomic per (node, component) per
-// chunk.  A particle whose base differs from its bin's (outside the grid)
-// adds its own contributions lane by lane.
+// chunk.  A particle whose base differs from its bin's (one outside the
+// grid, in a first substep) adds its own contributions lane by lane.
 // ---------------------------------------------------------------------------
 constexpr int kBlkCells = 2;                                    // cells per block edge
 constexpr int kTileMargin = 0;                                  // binning is exact: no drift margin
@@ -408,13 +412,6 @@ constexpr size_t kGraphCache = 8; // instantiated graphs kept per context (least
 // offs[b * 9 + 8] its end (entries of empty blocks are not written).  The
 // blocks' ranges are in block order (empm_sort_by_cell), or in the order
 // their workgroups allocated them (the per-substep binning).
-// Dense grid for contributions of particles outside their block's tile (only
-// particles outside the grid), and a flag raised when it holds anything.
-struct Overflow {
-  float4* grid;  // n^3, zero between uses
-  int* flag;     // cleared by the binning
-};
-
 struct Binning {
   const int* perm;
   const int* offs;
@@ -595,31 +592,23 @@ EMPM_DEV void group_scatter(float (*tile)[kTileNodes], const int* t0, bool in_gr
   }
 }
 
-// One lane's own 27-node scatter: into the tile if the stencil lies inside
-// it, else into the overflow grid with global atomics.
+// One lane's own 27-node scatter into the tile (o: the tile's origin), for a
+// particle whose stencil does not start at its bin's cell: nodes outside the
+// tile are outside the grid and are dropped.
 template <int NC>
-EMPM_DEV void lane_scatter(float (*tile)[kTileNodes], const int* o, float4* __restrict__ ovf, int* ovf_flag, int n,
-                           long long g0, const ScatterPayload& P, const Stencil& s) {
-  const bool in_tile = (unsigned)(s.base[0] - o[0]) <= (unsigned)(kTileN - 3) &&
-                       (unsigned)(s.base[1] - o[1]) <= (unsigned)(kTileN - 3) &&
-                       (unsigned)(s.base[2] - o[2]) <= (unsigned)(kTileN - 3);
+EMPM_DEV void lane_scatter(float (*tile)[kTileNodes], const int* o, const ScatterPayload& P, const Stencil& s) {
 #pragma unroll 1
   for (int j = 0; j < 27; ++j) {
     const int a = j / 9, b = (j / 3) % 3, c = j % 3;
+    const int tx = s.base[0] + a - o[0], ty = s.base[1] + b - o[1], tz = s.base[2] + c - o[2];
+    if ((unsigned)tx >= (unsigned)kTileN || (unsigned)ty >= (unsigned)kTileN || (unsigned)tz >= (unsigned)kTileN)
+      continue;
     float out[4];
     node_contrib<NC>(P, s, a, b, c, sw(s, 0, a), sw(s, 1, b), sw(s, 2, c), sdw(s, 0, a), sdw(s, 1, b), sdw(s, 2, c),
                      out);
-    const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
-    if (in_tile) {
-      const int t = ((ix - o[0]) * kTileN + (iy - o[1])) * kTileN + (iz - o[2]);
+    const int t = (tx * kTileN + ty) * kTileN + tz;
 #pragma unroll
-      for (int k = 0; k < NC; ++k) atomicAdd(&tile[k][t], out[k]);
-    } else if (node_in(ix, iy, iz, n)) {
-      float* g = reinterpret_cast<float*>(&ovf[g0 + node_id(ix, iy, iz, n)]);
-#pragma unroll
-      for (int k = 0; k < NC; ++k) atomicAdd(g + k, out[k]);
-      *ovf_flag = 1;
-    }
+    for (int k = 0; k < NC; ++k) atomicAdd(&tile[k][t], out[k]);
   }
 }
 
@@ -637,8 +626,8 @@ EMPM_DEV void tile_origin(int blk, int nb, int* o) {
 // fit on the device at once, so no workgroup waits for another to finish
 // before it starts.
 template <int NC, typename Load>
-EMPM_DEV void block_scatter(const Batch& Bt, const Binning& B, float4* __restrict__ tiles, float4* __restrict__ ovf,
-                            int* ovf_flag, int* __restrict__ work, Load load) {
+EMPM_DEV void block_scatter(const Batch& Bt, const Binning& B, float4* __restrict__ tiles, int* __restrict__ work,
+                            Load load) {
   __shared__ float tile[NC][kTileNodes];
   __shared__ int next[2][2];
   const int lane = __lane_id();
@@ -674,7 +663,6 @@ EMPM_DEV void block_scatter(const Batch& Bt, const Binning& B, float4* __restric
     const DevModel& M = Bt.cand[__builtin_amdgcn_readfirstlane(ci)];
     const int* __restrict__ sel = M.sel;
     const float inv_dx = M.inv_dx;
-    const int n = M.n;
     int o[3];
     tile_origin(blk - M.b0, M.nb, o);
     for (int i = threadIdx.x; i < NC * kTileNodes; i += blockDim.x) (&tile[0][0])[i] = 0.f;
@@ -707,7 +695,7 @@ EMPM_DEV void block_scatter(const Batch& Bt, const Binning& B, float4* __restric
       make_stencil(xp, inv_dx, s);
       const bool home = active && s.base[0] == o[0] + t0[0] && s.base[1] == o[1] + t0[1] && s.base[2] == o[2] + t0[2];
       if (__ballot(home)) group_scatter<NC>(tile, t0, home, P, s, lane);
-      if (active && !home) lane_scatter<NC>(tile, o, ovf, ovf_flag, n, M.g0, P, s);
+      if (active && !home) lane_scatter<NC>(tile, o, P, s);
     }
     int fetched_blk = 0;
     if (dynamic && threadIdx.x == 0 && fetched < nact) fetched_blk = entry(fetched);
@@ -887,8 +875,8 @@ __global__ __launch_bounds__(kBlock) void k_stress(Batch Bt, float dt,
 template <int WG>
 __global__ __launch_bounds__(WG) void k_p2g(Batch Bt, const float* __restrict__ x, const float* __restrict__ v,
                                                 const float* __restrict__ C, const float* __restrict__ S, Binning B,
-                                                float4* __restrict__ tiles, Overflow ov, int* __restrict__ work) {
-  block_scatter<4>(Bt, B, tiles, ov.grid, ov.flag, work, [&](int p, float* xp, ScatterPayload& P, const DevModel& M) {
+                                                float4* __restrict__ tiles, int* __restrict__ work) {
+  block_scatter<4>(Bt, B, tiles, work, [&](int p, float* xp, ScatterPayload& P, const DevModel& M) {
     const float dx = M.dx;
 #pragma unroll
     for (int d = 0; d < 3; ++d) xp[d] = x[3 * p + d];
@@ -905,10 +893,9 @@ __global__ __launch_bounds__(WG) void k_p2g(Batch Bt, const float* __restrict__ 
   });
 }
 
-// Per node: gather (mv, m) from the tiles and the overflow grid, then
-// grid_normalization_and_gravity, damping and the colliders.  The overflow
-// grid is re-zeroed for the next scatter; (mv, m) is kept when `keep` is set
-// (the adjoint needs it).
+// Per node: gather (mv, m) from the tiles, then
+// grid_normalization_and_gravity, damping and the colliders; (mv, m) is kept
+// when `keep` is set (the adjoint needs it).
 // The node passes visit only the nodes a scatter can have touched: the 2x2x2
 // nodes owned by each block (nodes 2o .. 2o + 1 per axis) whose `near` flag
 // is set.  Other nodes keep stale values, and no later kernel reads them:
@@ -945,20 +932,10 @@ EMPM_DEV void for_near_nodes(const Batch& Bt, const Binning& B, F f) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_grid(Batch Bt, const ColTable* __restrict__ T, float dt, Binning B,
-                                                 const float4* __restrict__ tiles, Overflow ov,
+                                                 const float4* __restrict__ tiles,
                                                  float4* __restrict__ keep, float4* __restrict__ v_out) {
   for_near_nodes(Bt, B, [&](const DevModel& M, int ix, int iy, int iz, size_t id) {
-  float4 g = gather_node(B, M, tiles, ix, iy, iz);
-  if (*ov.flag) {
-    const float4 o = ov.grid[id];
-    if (o.x != 0.f || o.y != 0.f || o.z != 0.f || o.w != 0.f) {
-      g.x += o.x;
-      g.y += o.y;
-      g.z += o.z;
-      g.w += o.w;
-      ov.grid[id] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-  }
+  const float4 g = gather_node(B, M, tiles, ix, iy, iz);
   if (keep) keep[id] = g;
   float v[3], fac[3];
   grid_node(M, *T, dt, ix, iy, iz, g, v, fac);
@@ -1252,9 +1229,9 @@ __global__ __launch_bounds__(WG) void k_g2p_adj_scatter(Batch Bt, float dt, cons
                                                             const float* __restrict__ dC1,
                                                             const float* __restrict__ dF1,
                                                             const float* __restrict__ Fe, Binning B,
-                                                            float4* __restrict__ tiles, Overflow ov,
+                                                            float4* __restrict__ tiles,
                                                             int* __restrict__ work) {
-  block_scatter<3>(Bt, B, tiles, ov.grid, ov.flag, work, [&](int p, float* xp, ScatterPayload& P, const DevModel& M) {
+  block_scatter<3>(Bt, B, tiles, work, [&](int p, float* xp, ScatterPayload& P, const DevModel& M) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) xp[d] = x[3 * p + d];
     const float4 a = reinterpret_cast<const float4*>(payload)[p];
@@ -1274,25 +1251,16 @@ __global__ __launch_bounds__(WG) void k_g2p_adj_scatter(Batch Bt, float dt, cons
   });
 }
 
-// Adjoint of the grid stage: gather dL/dv_out (tiles + overflow), replay the
+// Adjoint of the grid stage: gather dL/dv_out from the tiles, replay the
 // node's forward (normalisation, damping, colliders) to get dv_out/dv per
 // component, and write dL/d(mv, m).  The grid-mass term is dropped when
 // `drop_mass` is set (reference Tape behaviour).
 __global__ __launch_bounds__(kBlock) void k_grid_adj(Batch Bt, const ColTable* __restrict__ T, float dt, Binning B,
-                                                     const float4* __restrict__ tiles, Overflow ov,
+                                                     const float4* __restrict__ tiles,
                                                      const float4* __restrict__ keep, float4* __restrict__ gadj_in,
                                                      int drop_mass) {
   for_near_nodes(Bt, B, [&](const DevModel& M, int ix, int iy, int iz, size_t id) {
-  float4 b = gather_node(B, M, tiles, ix, iy, iz);
-  if (*ov.flag) {
-    const float4 ob = ov.grid[id];
-    if (ob.x != 0.f || ob.y != 0.f || ob.z != 0.f) {
-      b.x += ob.x;
-      b.y += ob.y;
-      b.z += ob.z;
-      ov.grid[id] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-  }
+  const float4 b = gather_node(B, M, tiles, ix, iy, iz);
   const float4 g = keep[id];
   float v[3], fac[3];
   grid_node(M, *T, dt, ix, iy, iz, g, v, fac);
@@ -1683,7 +1651,6 @@ EMPM_DEV int block_count(const int* __restrict__ count, int b) {
 __global__ __launch_bounds__(kBinScanWG) void k_bin_scan(int nblk, const int* __restrict__ count,
                                                          int* __restrict__ blkoff, int* __restrict__ wsum,
                                                          int* __restrict__ act, int* __restrict__ nlist,
-                                                         int* __restrict__ ovf_flag,
                                                          int* __restrict__ p2g_work) {
   __shared__ int smem[kBinScanWG / 64 + 1];
   const int b = blockIdx.x * kBinScanWG + threadIdx.x;
@@ -1695,7 +1662,6 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_scan(int nblk, const int* __
   if (b == 0) {
     act[0] = act[1] = 0;
     nlist[0] = 0;
-    *ovf_flag = 0;
     *p2g_work = 0;  // for the k_p2g launch that follows
   }
 }
@@ -1779,12 +1745,12 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_finish(Batch Bt, int* __rest
 // blocks' totals and takes its particle range with one atomic on ctr[0], so
 // the blocks' ranges are in the order the workgroups got there.  The active
 // list is counted in ctr[1], ctr[2]; k_bin_place copies those counts into
-// act[0], act[1] and re-zeroes ctr[0..2].  Also clears the near-list count,
-// the overflow flag and the k_p2g work counter.
+// act[0], act[1] and re-zeroes ctr[0..2].  Also clears the near-list count
+// and the k_p2g work counter.
 __global__ __launch_bounds__(kBinScanWG) void k_bin_alloc(Batch Bt, int* __restrict__ count, int* __restrict__ offs,
                                                           int* __restrict__ act, int* __restrict__ occ,
-                                                          int* __restrict__ nlist, int* __restrict__ ovf_flag,
-                                                          int* __restrict__ p2g_work, int* __restrict__ ctr,
+                                                          int* __restrict__ nlist, int* __restrict__ p2g_work,
+                                                          int* __restrict__ ctr,
                                                           int heavy_chunks) {
   __shared__ int smem[kBinScanWG / 64 + 1];
   __shared__ int wbase;
@@ -1802,7 +1768,6 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_alloc(Batch Bt, int* __restr
   if (threadIdx.x == 0) wbase = wtot > 0 ? atomicAdd(&ctr[0], wtot) : 0;
   if (b == 0) {
     nlist[0] = 0;
-    *ovf_flag = 0;
     *p2g_work = 0;  // for the k_p2g launch that follows
   }
   __syncthreads();
@@ -2046,8 +2011,6 @@ struct empm_ctx {
   long long max_nodes = 0;   // node arena capacity
   long long max_blocks = 0;  // block arena capacity
   float4* tiles = nullptr;   // max_blocks tiles of kTileNodes float4 (block-owned scatter)
-  float4* ovf = nullptr;     // node arena: overflow grid for particles outside their tile (kept zero)
-  int* ovf_flag = nullptr;   // 1: overflow grid may be non-zero
   float4* keep = nullptr;    // node arena: (mv, m) of the substep being differentiated
   float4* v_out = nullptr;   // node arena
   float4* gadj_in = nullptr; // node arena: dL/d(mv, m)
@@ -2441,7 +2404,6 @@ unsigned scatter_grid(const empm_ctx* ctx, int idx) {
       break;                                                                                                 \
   }
 
-Overflow overflow(const empm_ctx* ctx) { return Overflow{ctx->ovf, ctx->ovf_flag}; }
 
 unsigned nblocks(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
@@ -2477,12 +2439,12 @@ int do_bin(empm_ctx* ctx, const HostBatch& hb, const float* x, const BinBuf& out
   ctx->counts_pending = false;
   if (ordered) {
     hipLaunchKernelGGL(k_bin_scan, dim3(wgs), dim3(kBinScanWG), 0, st, nblk, ctx->count, ctx->blkoff, ctx->wsum,
-                       out.act, out.nlist, ctx->ovf_flag, ctx->work);
+                       out.act, out.nlist, ctx->work);
     hipLaunchKernelGGL(k_bin_finish, dim3(wgs), dim3(kBinScanWG), 0, st, hb.dev, ctx->count, ctx->blkoff,
                        ctx->wsum, out.offs, out.act, out.occ, ctx->tune.heavy_chunks);
   } else {
     hipLaunchKernelGGL(k_bin_alloc, dim3(wgs), dim3(kBinScanWG), 0, st, hb.dev, ctx->count, out.offs, out.act,
-                       out.occ, out.nlist, ctx->ovf_flag, ctx->work, ctx->bin_ctr, ctx->tune.heavy_chunks);
+                       out.occ, out.nlist, ctx->work, ctx->bin_ctr, ctx->tune.heavy_chunks);
   }
   hipLaunchKernelGGL(k_bin_place, dim3(nblocks(hb.N > nblk ? hb.N : nblk)), dim3(kBlock), 0, st, hb.dev,
                      ctx->key_rank, out.offs, out.perm, out.occ, out.near, out.nlist, out.act,
@@ -2512,13 +2474,12 @@ int launch_fwd(empm_ctx* ctx, const HostBatch& hb, const ColTable* T, float dt, 
   }
   if (hb.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
-    EMPM_LAUNCH_SCATTER(ctx, 0, k_p2g, st, hb.dev, in->x, in->v, in->C, ctx->S, B, ctx->tiles, overflow(ctx),
-                        ctx->work);
+    EMPM_LAUNCH_SCATTER(ctx, 0, k_p2g, st, hb.dev, in->x, in->v, in->C, ctx->S, B, ctx->tiles, ctx->work);
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
-    hipLaunchKernelGGL(k_grid, node_grid(ctx, hb.nblk), dim3(kBlock), 0, st, hb.dev, T, dt, B, ctx->tiles,
-                       overflow(ctx), keep, v_out);
+    hipLaunchKernelGGL(k_grid, node_grid(ctx, hb.nblk), dim3(kBlock), 0, st, hb.dev, T, dt, B, ctx->tiles, keep,
+                       v_out);
   }
   if (hb.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_G2P, st);
@@ -2569,13 +2530,12 @@ int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable* T, float dt, 
     }
     if (hb.N > 0) {
       ProfScope ps(ctx, EMPM_KERNEL_P2G, st);
-      EMPM_LAUNCH_SCATTER(ctx, 0, k_p2g, st, hb.dev, in->x, in->v, in->C, ctx->S, B, ctx->tiles, overflow(ctx),
-                          ctx->work);
+      EMPM_LAUNCH_SCATTER(ctx, 0, k_p2g, st, hb.dev, in->x, in->v, in->C, ctx->S, B, ctx->tiles, ctx->work);
     }
     {
       ProfScope ps(ctx, EMPM_KERNEL_GRID, st);
-      hipLaunchKernelGGL(k_grid, node_grid(ctx, hb.nblk), dim3(kBlock), 0, st, hb.dev, T, dt, B, ctx->tiles,
-                         overflow(ctx), keep, v_out);
+      hipLaunchKernelGGL(k_grid, node_grid(ctx, hb.nblk), dim3(kBlock), 0, st, hb.dev, T, dt, B, ctx->tiles, keep,
+                         v_out);
     }
   }
   if (hb.N > 0) {
@@ -2587,13 +2547,13 @@ int launch_adj(empm_ctx* ctx, const HostBatch& hb, const ColTable* T, float dt, 
     {
       ProfScope ps(ctx, EMPM_KERNEL_G2P_ADJ_SCATTER, st);
       EMPM_LAUNCH_SCATTER(ctx, 1, k_g2p_adj_scatter, st, hb.dev, dt, in->x, ctx->payload, d_out->C, d_out->F, Fe, B,
-                          ctx->tiles, overflow(ctx), ctx->work + 1);
+                          ctx->tiles, ctx->work + 1);
     }
   }
   {
     ProfScope ps(ctx, EMPM_KERNEL_GRID_ADJ, st);
-    hipLaunchKernelGGL(k_grid_adj, node_grid(ctx, hb.nblk), dim3(kBlock), 0, st, hb.dev, T, dt, B, ctx->tiles,
-                       overflow(ctx), keep, ctx->gadj_in, (flags & EMPM_ADJ_DROP_GRID_MASS) ? 1 : 0);
+    hipLaunchKernelGGL(k_grid_adj, node_grid(ctx, hb.nblk), dim3(kBlock), 0, st, hb.dev, T, dt, B, ctx->tiles, keep,
+                       ctx->gadj_in, (flags & EMPM_ADJ_DROP_GRID_MASS) ? 1 : 0);
   }
   if (hb.N > 0) {
     ProfScope ps(ctx, EMPM_KERNEL_P2G_ADJ, st);
@@ -2650,8 +2610,6 @@ int alloc_ctx(empm_ctx* c) {
   const size_t nwsum = (nblk + kBinScanWG - 1) / kBinScanWG;
   hipError_t e = hipSuccess;
   e = e == hipSuccess ? hipMalloc((void**)&c->tiles, nblk * kTileNodes * sizeof(float4)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->ovf, G * sizeof(float4)) : e;
-  e = e == hipSuccess ? hipMalloc((void**)&c->ovf_flag, sizeof(int)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->keep, G * sizeof(float4)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->v_out, G * sizeof(float4)) : e;
   e = e == hipSuccess ? hipMalloc((void**)&c->gadj_in, G * sizeof(float4)) : e;
@@ -2682,8 +2640,6 @@ int alloc_ctx(empm_ctx* c) {
   e = e == hipSuccess ? hipHostMalloc((void**)&c->status_host, (size_t)c->max_cand * sizeof(int)) : e;
   e = e == hipSuccess ? hipMemset(c->status, 0, (size_t)c->max_cand * sizeof(int)) : e;
   e = e == hipSuccess ? hipMemset(c->count, 0, nblk * 8 * sizeof(int)) : e;
-  e = e == hipSuccess ? hipMemset(c->ovf, 0, G * sizeof(float4)) : e;
-  e = e == hipSuccess ? hipMemset(c->ovf_flag, 0, sizeof(int)) : e;
   e = e == hipSuccess ? hipDeviceSynchronize() : e;
   // the block-owned scatters are launched as wide as the device holds them
   // (per workgroup size of the tuning)
@@ -3058,8 +3014,6 @@ void empm_destroy(empm_ctx* c) {
     delete c->prof;
   }
   (void)hipFree(c->tiles);
-  (void)hipFree(c->ovf);
-  (void)hipFree(c->ovf_flag);
   (void)hipFree(c->keep);
   (void)hipFree(c->v_out);
   (void)hipFree(c->gadj_in);

@@ -210,11 +210,11 @@ def test_fast_particles_match_oracle(dev):
     assert torch.isfinite(d0.v).all() and torch.isfinite(g.mu).all()
 
 
-def test_particles_outside_grid_use_the_overflow_path(dev):
+def test_particles_outside_grid_scatter_their_in_grid_nodes(dev):
     """A substep whose input has particles outside [0, 1)^3 (binned to an edge
-    cell, so their stencil leaves the block tile) scatters them through the
-    overflow grid; the substep must match the oracle's, which clips nothing
-    on input either."""
+    cell, so their stencil starts outside the block tile) scatters them lane
+    by lane into the tile, dropping the stencil nodes outside the grid; the
+    substep must match the oracle's, which clips nothing on input either."""
     from tests.helpers import oracle_state, random_state
 
     N, n = 256, 16
