@@ -12,7 +12,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libempm.so")
 
-EMPM_ABI_VERSION = 5
+EMPM_ABI_VERSION = 6
 EMPM_MAX_COLLIDERS = 16
 
 EMPM_OK = 0
@@ -50,6 +50,7 @@ EXPORTED_SYMBOLS = (
     "empm_nn_sqdist_segments_workspace",
     "empm_substep_fwd",
     "empm_substep_adj",
+    "empm_restart_deformation",
     "empm_rollout_fwd",
     "empm_rollout_fwd_replay",
     "empm_rollout_adj",
@@ -253,6 +254,8 @@ def _declare(lib):
     lib.empm_lbs_bones.argtypes = [C.c_int, vp, vp, C.c_int, vp, C.c_int, vp, vp, vp]
     lib.empm_lbs_points.restype = C.c_int
     lib.empm_lbs_points.argtypes = [C.c_int, vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, vp, vp, vp, vp, vp]
+    lib.empm_restart_deformation.restype = C.c_int
+    lib.empm_restart_deformation.argtypes = [vp, P(Model), vp, vp, vp, vp]
     lib.empm_substep_fwd.restype = C.c_int
     lib.empm_substep_fwd.argtypes = [vp, P(Model), P(Collider), C.c_int, C.c_float, C.c_float,
                                      P(Particles), P(Particles), vp]

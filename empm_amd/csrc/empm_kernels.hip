@@ -893,6 +893,63 @@ __global__ __launch_bounds__(WG) void k_p2g(Batch Bt, const float* __restrict__ 
   });
 }
 
+// set_F_C_p2g (mpm_utils.py:843-876) of restart_and_compute_F_C: the mass
+// and the mass-weighted displacement target - x into the block tiles (no
+// APIC or stress term).
+template <int WG>
+__global__ __launch_bounds__(WG) void k_p2g_disp(Batch Bt, const float* __restrict__ x,
+                                                 const float* __restrict__ target, Binning B,
+                                                 float4* __restrict__ tiles, int* __restrict__ work) {
+  block_scatter<4>(Bt, B, tiles, work, [&](int p, float* xp, ScatterPayload& P, const DevModel& M) {
+    const float mp = M.mass[p];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      xp[d] = x[3 * p + d];
+      P.A[d] = mp * (target[3 * p + d] - xp[d]);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) P.B[k] = P.S[k] = 0.f;
+    P.m = mp;
+  });
+}
+
+// set_F_C_g2p (mpm_utils.py:879-919): F_trial = I + sum_i v_i (x) grad w_i
+// over the normalised displacement grid (its C is not stored, as the
+// reference; cov is not updated, update_cov_with_F being off).  Particles
+// with selection != 0 keep theirs.
+__global__ __launch_bounds__(kBlock) void k_restart_F(Batch Bt, const float* __restrict__ x,
+                                                      const float4* __restrict__ v_out, float* __restrict__ F) {
+  per_particle(Bt, -1, [&](const DevModel& M, const int p) {
+    if (M.sel && M.sel[p] != 0) return;
+    const float xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
+    Stencil s;
+    make_stencil(xp, M.inv_dx, s);
+    float nG[9];
+    m_zero(nG);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const int ix = s.base[0] + a, iy = s.base[1] + b, iz = s.base[2] + c;
+          const bool in = node_in(ix, iy, iz, M.n);
+          const float4 g = v_out[M.g0 + (in ? node_id(ix, iy, iz, M.n) : 0)];
+          const float gv[3] = {in ? g.x : 0.f, in ? g.y : 0.f, in ? g.z : 0.f};
+          const float gw[3] = {s.dw[0][a] * s.w[1][b] * s.w[2][c], s.w[0][a] * s.dw[1][b] * s.w[2][c],
+                               s.w[0][a] * s.w[1][b] * s.dw[2][c]};
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) nG[i * 3 + j] += gv[i] * gw[j];
+        }
+    float Fn[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Fn[k] = ((k % 4 == 0) ? 1.f : 0.f) + nG[k] * M.inv_dx;
+    store9(F, p, Fn);
+  });
+}
+
 // Per node: gather (mv, m) from the tiles, then
 // grid_normalization_and_gravity, damping and the colliders; (mv, m) is kept
 // when `keep` is set (the adjoint needs it).
@@ -3421,6 +3478,38 @@ int empm_substep_adj(empm_ctx* ctx, const empm_model* model, const empm_collider
   T->time = time;
   if ((rc = sched_commit(ctx, 1, st))) return rc;
   return launch_adj(ctx, hb, ctx->sched, dt, in, nullptr, d_out, d_in, d_mu, d_lam, d_ys, d_mass, d_vol, flags, st);
+}
+
+int empm_restart_deformation(empm_ctx* ctx, const empm_model* model, const float* x, const float* target, float* F_trial,
+                   void* stream) {
+  if (!ctx || !model) return set_err(EMPM_E_INVALID_ARG, "empm_restart_deformation: null context or model");
+  if (model->n_particles > 0 && (!x || !target || !F_trial))
+    return set_err(EMPM_E_INVALID_ARG, "empm_restart_deformation: null x, target or F_trial");
+  // the normalisation alone: no damping, no colliders, dt = 0 (no gravity)
+  empm_model m = *model;
+  m.grid_v_damping_scale = 1.1f;
+  const empm_batch b = batch_of(&m);
+  hipStream_t st = (hipStream_t)stream;
+  HostBatch hb;
+  int rc = build_batch(ctx, &b, hb);
+  if (rc) return rc;
+  if (hb.N == 0) return EMPM_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if ((rc = order_after_last(ctx, st))) return rc;
+  if ((rc = upload_batch(ctx, hb, st))) return rc;
+  ColTable* stage = sched_stage(ctx);
+  if ((rc = make_table(nullptr, 0, stage[0]))) return rc;
+  stage[0].time = 0.f;
+  if ((rc = sched_commit(ctx, 1, st))) return rc;
+  const BinBuf bb = bin_buf(ctx->bins, hb.nblk, hb.N);
+  if ((rc = do_bin(ctx, hb, x, bb, st))) return rc;
+  const Binning B = binning(ctx, bb, hb.nblk);
+  EMPM_LAUNCH_SCATTER(ctx, 0, k_p2g_disp, st, hb.dev, x, target, B, ctx->tiles, ctx->work);
+  hipLaunchKernelGGL(k_grid, node_grid(ctx, hb.nblk), dim3(kBlock), 0, st, hb.dev, ctx->sched, 0.f, B, ctx->tiles,
+                     nullptr, ctx->v_out);
+  hipLaunchKernelGGL(k_restart_F, dim3(particle_grid(hb)), dim3(kBlock), 0, st, hb.dev, x, ctx->v_out, F_trial);
+  HIP_TRY(hipGetLastError());
+  return EMPM_OK;
 }
 
 int empm_rollout_fwd(empm_ctx* ctx, const empm_model* model, empm_collider* cols, int ncol, double* time, double dt,

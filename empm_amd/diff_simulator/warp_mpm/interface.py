@@ -38,6 +38,18 @@ from ... import runtime as rt
 from ..._lib import EMPM_ADJ_DROP_GRID_MASS, EMPM_ADJ_EXACT, Collider
 
 
+def _reset_or_restart(mpm_solver, mpm_state, mpm_model, init_pos, init_velocity, static_pos, device,
+                      requires_grad):
+    """interface.py:47-67,281-301: reset the state at init_pos, or, with
+    static_pos, reset it at static_pos and restart it towards init_pos
+    (restart_and_compute_F_C: F_trial from the displacement, x = init_pos)."""
+    if static_pos is None:
+        mpm_state.reset_state(init_pos.clone(), None, init_velocity, device=device, requires_grad=requires_grad)
+        return
+    mpm_state.reset_state(static_pos.clone(), None, init_velocity, device=device, requires_grad=requires_grad)
+    mpm_solver.restart_and_compute_F_C(mpm_model, mpm_state, init_pos.clone().detach().contiguous(), device)
+
+
 def _flags(solver):
     return EMPM_ADJ_EXACT if getattr(solver, "adjoint_mode", "reference_compat") == "exact" else EMPM_ADJ_DROP_GRID_MASS
 
@@ -159,9 +171,8 @@ class MPMDifferentiableSimulationTrail(autograd.Function):
                 particle_density: Optional[Tensor] = None, density_change_mask: Optional[Tensor] = None,
                 static_pos: Optional[Tensor] = None, device: str = "cuda:0", requires_grad: bool = True,
                 extra_no_grad_steps: int = 0):
-        if static_pos is not None:
-            mpm_solver.restart_and_compute_F_C(mpm_model, mpm_state, static_pos, device)
-        mpm_state.reset_state(init_pos.clone(), None, init_velocity, device=device, requires_grad=requires_grad)
+        _reset_or_restart(mpm_solver, mpm_state, mpm_model, init_pos, init_velocity, static_pos, device,
+                          requires_grad)
         agg_E, agg_nu = _set_E_nu(mpm_solver, mpm_model, E, nu, scale_scalar_E=True)
         mpm_state.reset_density(tensor_density=particle_density, selection_mask=density_change_mask, device=device,
                                 requires_grad=requires_grad)
@@ -248,9 +259,8 @@ class MPMDifferentiableSimulation(autograd.Function):
                 particle_density: Optional[Tensor] = None, density_change_mask: Optional[Tensor] = None,
                 static_pos: Optional[Tensor] = None, device: str = "cuda:0", requires_grad: bool = True,
                 extra_no_grad_steps: int = 0):
-        if static_pos is not None:
-            mpm_solver.restart_and_compute_F_C(mpm_model, mpm_state, static_pos, device)
-        mpm_state.reset_state(init_pos.clone(), None, init_velocity, device=device, requires_grad=requires_grad)
+        _reset_or_restart(mpm_solver, mpm_state, mpm_model, init_pos, init_velocity, static_pos, device,
+                          requires_grad)
         agg_E, agg_nu = _set_E_nu(mpm_solver, mpm_model, E, nu, scale_scalar_E=False)
         mpm_state.reset_density(tensor_density=particle_density, selection_mask=density_change_mask, device=device,
                                 requires_grad=requires_grad)

@@ -348,6 +348,17 @@ class MPMWARPDiff:
     def set_particle_mover(self, *args, **kwargs):
         raise NotImplementedError("set_particle_mover is not supported by the HIP path")
 
-    # mpm_solver_diff.py:1214-1262 (used only with static_pos, which callers never pass)
+    # mpm_solver_diff.py:1214-1262 (reached through static_pos, which the
+    # reference's own callers leave at None)
     def restart_and_compute_F_C(self, mpm_model, mpm_state, target_pos, device):
-        raise NotImplementedError("restart_and_compute_F_C (static_pos) is not supported by the HIP path")
+        """F_trial of every selected particle from the displacement to
+        target_pos (set_F_C_p2g, normalisation with dt = 0, set_F_C_g2p:
+        mpm_utils.py:843-919), then particle_x = target_pos for all particles
+        (set_vec3_to_vec3).  C is left as it is, as in the reference."""
+        n = mpm_state.n_particles
+        x = mpm_state.particle_x
+        ctx = self.context(mpm_model.n_grid, n, x.device)
+        model = self.model_arrays(mpm_model, mpm_state)
+        target = torch.as_tensor(target_pos).to(device=x.device, dtype=torch.float32).reshape(n, 3).contiguous()
+        rt.restart_F(ctx, model, x.contiguous(), target, mpm_state.particle_F_trial)
+        x.copy_(target)

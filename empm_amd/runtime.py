@@ -347,6 +347,20 @@ def substep_fwd(ctx: Context, model: ModelArrays, cols: list[Collider], time: fl
                                    C.byref(s_in.desc()), C.byref(s_out.desc()), stream_handle(ctx.device)))
 
 
+def restart_F(ctx: Context, model: ModelArrays, x: torch.Tensor, target: torch.Tensor, F_trial: torch.Tensor):
+    """restart_and_compute_F_C's deformation gradient (empm_restart_deformation): F_trial
+    (N, 3, 3) of the particles with selection == 0 becomes I + the grid
+    gradient of the mass-weighted displacement target - x.  Positions are
+    not written."""
+    n = int(model.mu.shape[0])
+    for name, t, shape in (("x", x, (n, 3)), ("target", target, (n, 3)), ("F_trial", F_trial, (n, 3, 3))):
+        if tuple(t.shape) != shape or t.dtype != F32 or not t.is_contiguous():
+            raise RuntimeError(f"restart_F: {name} must be a contiguous float32 tensor of shape {shape}")
+    m = model.struct()
+    check(ctx.lib.empm_restart_deformation(ctx.handle, C.byref(m), _ptr(x), _ptr(target), _ptr(F_trial),
+                                 stream_handle(ctx.device)))
+
+
 def substep_adj(ctx: Context, model: ModelArrays, cols: list[Collider], time: float, dt: float,
                 s_in: ParticleState, d_out: ParticleState, d_in: ParticleState, grads: "ParamGrads", flags=0):
     arr, n = collider_array(cols)

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define EMPM_ABI_VERSION 5
+#define EMPM_ABI_VERSION 6
 #define EMPM_MAX_COLLIDERS 16
 
 enum {
@@ -328,6 +328,18 @@ int empm_substep_fwd(empm_ctx *ctx, const empm_model *model, const empm_collider
 int empm_substep_adj(empm_ctx *ctx, const empm_model *model, const empm_collider *cols, int ncol, float time,
                      float dt, const empm_particles *in, const empm_particles *d_out, const empm_particles *d_in,
                      float *d_mu, float *d_lam, float *d_ys, float *d_mass, float *d_vol, int flags, void *stream);
+
+/* restart_and_compute_F_C (mpm_solver_diff.py:1214-1262) on the positions
+ * x (N*3) and target positions target (N*3): zero_grid, set_F_C_p2g
+ * (mpm_utils.py:843-876: mass and mass-weighted displacement target - x
+ * into the grid), grid_normalization_and_gravity with dt = 0
+ * (mpm_utils.py:427-438: no gravity, and no damping or colliders) and
+ * set_F_C_g2p (mpm_utils.py:879-919): F_trial[p] = I + sum_i v_i (x) grad w_ip
+ * (N*9, written for particles with selection == 0 only; the new C is not
+ * stored, as in the reference).  Positions are not written: the reference
+ * then sets x = target (set_vec3_to_vec3, mpm_solver_diff.py:1255-1261). */
+int empm_restart_deformation(empm_ctx *ctx, const empm_model *model, const float *x, const float *target, float *F_trial,
+                   void *stream);
 
 /* n_steps substeps in one call: states[k] -> states[k+1] for k < n_steps
  * (states: host array of n_steps+1 descriptors; consecutive entries must not

@@ -847,6 +847,80 @@ int oracle_substep(const oracle_params *P, const oracle_collider *cols, int ncol
                         Fe_out, tau_out, 0);
 }
 
+/* restart_and_compute_F_C (mpm_solver_diff.py:1214-1262) without its last
+ * step (x = target, set_vec3_to_vec3): zero_grid; set_F_C_p2g
+ * (mpm_utils.py:843-876), the mass and mass-weighted displacement
+ * target - x of every selected particle into the grid;
+ * grid_normalization_and_gravity with dt = 0 (mpm_utils.py:427-438);
+ * set_F_C_g2p (mpm_utils.py:879-919), F_trial = I + sum_i v_i (x) dweight_i
+ * for every selected particle (F_out of other particles is not written). */
+int oracle_restart_F(const oracle_params *P, const real *x, const real *target, const real *mass, const int *sel,
+                     real *F_out)
+{
+    const int N = P->n_particles, n = P->n_grid;
+    const size_t G = (size_t)n * n * n;
+    const real inv_dx = (real)P->inv_dx;
+    real *grid = (real *)calloc(4 * G, sizeof(real));
+    if (!grid)
+        return -1;
+    for (int p = 0; p < N; ++p) {
+        if (sel && sel[p] != 0)
+            continue;
+        int base[3];
+        real fx[3], w[3][3], dw[3][3];
+        bspline(&x[3 * p], inv_dx, base, fx, w, dw);
+        const real disp[3] = {target[3 * p] - x[3 * p], target[3 * p + 1] - x[3 * p + 1],
+                              target[3 * p + 2] - x[3 * p + 2]};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int c = 0; c < 3; ++c) {
+                    const int ix = base[0] + a, iy = base[1] + b, iz = base[2] + c;
+                    if (ix < 0 || iy < 0 || iz < 0 || ix >= n || iy >= n || iz >= n)
+                        continue;
+                    const real wt = w[0][a] * w[1][b] * w[2][c];
+                    real *g = &grid[4 * (((size_t)ix * n + iy) * n + iz)];
+                    for (int i = 0; i < 3; ++i)
+                        g[i] += wt * mass[p] * disp[i];
+                    g[3] += wt * mass[p];
+                }
+    }
+    for (size_t id = 0; id < G; ++id) {
+        real *g = &grid[4 * id];
+        if (g[3] > (real)1e-15) {
+            const real im = 1 / g[3];
+            for (int i = 0; i < 3; ++i)
+                g[i] = g[i] * im; /* + dt * gravity with dt = 0 */
+        } else {
+            g[0] = g[1] = g[2] = 0;
+        }
+    }
+    for (int p = 0; p < N; ++p) {
+        if (sel && sel[p] != 0)
+            continue;
+        int base[3];
+        real fx[3], w[3][3], dw[3][3];
+        bspline(&x[3 * p], inv_dx, base, fx, w, dw);
+        real nF[9] = {0};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int c = 0; c < 3; ++c) {
+                    const int ix = base[0] + a, iy = base[1] + b, iz = base[2] + c;
+                    if (ix < 0 || iy < 0 || iz < 0 || ix >= n || iy >= n || iz >= n)
+                        continue;
+                    const real *gv = &grid[4 * (((size_t)ix * n + iy) * n + iz)];
+                    const real dwt[3] = {dw[0][a] * w[1][b] * w[2][c] * inv_dx, w[0][a] * dw[1][b] * w[2][c] * inv_dx,
+                                         w[0][a] * w[1][b] * dw[2][c] * inv_dx};
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j)
+                            nF[i * 3 + j] += gv[i] * dwt[j];
+                }
+        for (int i = 0; i < 9; ++i)
+            F_out[9 * p + i] = (i % 4 == 0 ? 1 : 0) + nF[i];
+    }
+    free(grid);
+    return 0;
+}
+
 /* oracle_substep with the grid mass of the normalisation held at m_fixed. */
 int oracle_substep_frozen_mass(const oracle_params *P, const oracle_collider *cols, int ncol, double time, double dt,
                                const real *x, const real *v, const real *C, const real *Ftr, real *mu, real *lam,

@@ -95,6 +95,8 @@ def load(precision: str = "f64"):
     lib.oracle_grid_update.restype = None
     lib.oracle_advance_colliders.argtypes = [C.POINTER(OracleCollider), C.c_int, C.c_double, C.c_double]
     lib.oracle_advance_colliders.restype = None
+    lib.oracle_restart_F.argtypes = [C.POINTER(OracleParams), vp, vp, vp, vp, vp]
+    lib.oracle_restart_F.restype = C.c_int
     lib.oracle_num_threads.restype = C.c_int
     lib.oracle_set_num_threads.argtypes = [C.c_int]
     lib.oracle_set_num_threads.restype = None
@@ -198,6 +200,23 @@ def substep(P, cols, time, dt, s: OracleState, return_fe=False, m_fixed=None):
     out = OracleState(x1, v1, C1, F1, s.mu, s.lam, s.ys, s.mass, s.vol, s.sel, s.precision)
     if return_fe:
         return out, Fe, tau, grid.reshape(n, n, n, 4)
+    return out
+
+
+def restart_F(P, x, target, mass, F, sel=None, precision="f64"):
+    """restart_and_compute_F_C's deformation gradient (mpm_solver_diff.py:
+    1214-1262, mpm_utils.py:843-919): returns a copy of F (N, 3, 3) with
+    every selected particle's F_trial = I + the grid gradient of the
+    mass-weighted displacement target - x (oracle_restart_F)."""
+    lib = load(precision)
+    d = _dt(precision)
+    x = np.ascontiguousarray(x, dtype=d)
+    target = np.ascontiguousarray(target, dtype=d)
+    mass = np.ascontiguousarray(mass, dtype=d)
+    out = np.ascontiguousarray(F, dtype=d).copy()
+    sl = None if sel is None else np.ascontiguousarray(sel, dtype=np.int32)
+    if lib.oracle_restart_F(C.byref(P), _p(x), _p(target), _p(mass), _p(sl), _p(out)) != 0:
+        raise RuntimeError("oracle_restart_F failed")
     return out
 
 

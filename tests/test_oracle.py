@@ -633,3 +633,68 @@ def test_frozen_grid_mass_substep():
     orc.rollout(P, [], 0.0, 1e-4, 5, a, m_hist_out=mh)
     orc.rollout(P, [], 0.0, 1e-4, 5, b, m_hist_in=mh)
     assert np.abs(a.x - b.x).max() < 1e-13 and np.abs(a.F - b.F).max() < 1e-13
+
+
+# ---------------------------------------------------------------- restart_and_compute_F_C
+def _lattice(n, k0, k1):
+    """Particles on a regular lattice of spacing dx/2 at (k + 1/2) dx/2: the
+    lattice is symmetric about every grid node, so a node whose B-spline
+    support lies inside it averages the lattice positions to its own."""
+    dx = 1.0 / n
+    c = (np.arange(k0, k1) + 0.5) * dx / 2
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1), c
+
+
+def test_restart_F_translation_gives_identity():
+    # a uniform displacement u makes every touched node's velocity u, and
+    # sum_i u (x) grad w_i = u (x) sum_i grad w_i = 0: F_trial = I exactly
+    # (a mistake taking w for grad w, or the displacement unweighted, fails)
+    n, N = 16, 400
+    x = RNG.uniform(0.25, 0.75, (N, 3))
+    u = np.array([0.013, -0.021, 0.007])
+    P = orc.make_params(N, n)
+    F = orc.restart_F(P, x, x + u, RNG.uniform(1e-4, 1e-3, N), np.tile(np.eye(3), (N, 1, 1)))
+    # (not exactly I: a node reached only by a stencil's far edge can hold
+    # less than the 1e-15 mass threshold and keep v = 0, as in the
+    # reference; its weight gradient is then tiny as well)
+    assert np.abs(F - np.eye(3)).max() < 1e-8
+
+
+def test_restart_F_linear_field_interior_closed_form():
+    # displacement A x + b on the symmetric lattice: interior nodes take
+    # v_i = A x_i + b, and sum_i x_i (x) grad w_i = I gives F_trial = I + A
+    # exactly for particles whose stencil nodes are all interior; a missing
+    # mass normalisation scales A by the node mass and fails
+    n = 16
+    dx = 1.0 / n
+    x, c = _lattice(n, 4, 28)
+    N = len(x)
+    A = np.array([[0.01, 0.02, -0.005], [0.003, -0.01, 0.004], [-0.02, 0.006, 0.015]])
+    b = np.array([0.001, -0.002, 0.0005])
+    P = orc.make_params(N, n)
+    F = orc.restart_F(P, x, x + x @ A.T + b, np.full(N, 1e-3), np.tile(np.eye(3), (N, 1, 1)))
+    inner = np.all((x > c.min() + 3 * dx) & (x < c.max() - 3 * dx), axis=1)
+    assert inner.sum() >= 500
+    assert np.abs(F[inner] - (np.eye(3) + A)).max() < 1e-12
+    # at the lattice's faces the average is one-sided: F differs from I + A there
+    assert np.abs(F[~inner] - (np.eye(3) + A)).max() > 1e-3
+
+
+def test_restart_F_leaves_unselected_particles():
+    # only particle_selection == 0 is restarted (mpm_utils.py:847,884), and
+    # the unselected ones neither deposit mass nor read the grid
+    n, N = 16, 300
+    x = RNG.uniform(0.3, 0.7, (N, 3))
+    target = x + RNG.normal(0, 0.01, (N, 3))
+    F0 = np.tile(np.eye(3), (N, 1, 1)) + RNG.normal(0, 0.1, (N, 3, 3))
+    sel = np.zeros(N, np.int32)
+    sel[::3] = 1
+    P = orc.make_params(N, n)
+    mass = RNG.uniform(1e-4, 1e-3, N)
+    F = orc.restart_F(P, x, target, mass, F0, sel)
+    assert np.array_equal(F[sel == 1], F0[sel == 1])
+    keep = sel == 0
+    Pk = orc.make_params(int(keep.sum()), n)
+    Fk = orc.restart_F(Pk, x[keep], target[keep], mass[keep], F0[keep])
+    assert np.abs(F[keep] - Fk).max() < 1e-14
