@@ -160,6 +160,36 @@ struct ColTable {
   int pad0, pad1;
   empm_collider c[EMPM_MAX_COLLIDERS];
 };
+static_assert(sizeof(ColTable) % 16 == 0, "ColTable is copied in 16-byte pieces");
+
+// The node passes copy the substep's table into LDS once per workgroup of
+// kBlock threads, its loads in flight together with the pass's first list
+// loads, before their node loop: evaluating the colliders straight from
+// device memory chains one scalar load per collider field per node.
+constexpr int kTablePieces = (int)(sizeof(ColTable) / 16);
+constexpr int kTableRounds = (kTablePieces + 63) / 64;
+struct TableCopy {
+  int4 v[kTableRounds];
+};
+EMPM_DEV TableCopy table_load(const ColTable* __restrict__ T) {
+  TableCopy t;
+  const int4* src = reinterpret_cast<const int4*>(T);
+#pragma unroll
+  for (int r = 0; r < kTableRounds; ++r) {
+    const int i = (int)threadIdx.x + 64 * r;
+    t.v[r] = i < kTablePieces ? src[i] : make_int4(0, 0, 0, 0);
+  }
+  return t;
+}
+EMPM_DEV void table_store(const TableCopy& t, ColTable& Ts) {  // every thread of the workgroup calls this
+  int4* dst = reinterpret_cast<int4*>(&Ts);
+#pragma unroll
+  for (int r = 0; r < kTableRounds; ++r) {
+    const int i = (int)threadIdx.x + 64 * r;
+    if (i < kTablePieces) dst[i] = t.v[r];
+  }
+  __syncthreads();
+}
 
 // Quadratic B-spline stencil of one particle (mpm_utils.py:365-383).  The
 // reference's w = wp.mat33(vec, vec, vec) takes the vectors as columns, so
@@ -964,12 +994,16 @@ constexpr int kNodePassGridCap = 1024;
 // ONE: a batch of one simulation, whose table entry is then read with
 // scalar loads at once instead of per lane after the list entry
 template <bool ONE, typename F>
-EMPM_DEV void for_near_nodes_t(const Batch& Bt, const Binning& B, F f) {
+EMPM_DEV void for_near_nodes_t(const Batch& Bt, const Binning& B, const ColTable* __restrict__ T, ColTable& Ts,
+                               F f) {
   const size_t stride = (size_t)gridDim.x * blockDim.x, last = (size_t)B.nblk - 1;
   size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  // list entries are read ahead of the count test (the list has nblk slots)
+  // list entries are read ahead of the count test (the list has nblk slots),
+  // together with the collider table
+  const TableCopy tc = table_load(T);
   int blk_next = B.nlist[(q >> 3) < last ? (q >> 3) : last];
   const size_t npairs = (size_t)(*B.nnear) * 8;
+  table_store(tc, Ts);
   for (; q < npairs; q += stride) {
     const int blk = blk_next, r = (int)(q & 7);
     const size_t qn = (q + stride) >> 3;
@@ -983,19 +1017,20 @@ EMPM_DEV void for_near_nodes_t(const Batch& Bt, const Binning& B, F f) {
   }
 }
 template <typename F>
-EMPM_DEV void for_near_nodes(const Batch& Bt, const Binning& B, F f) {
-  if (Bt.ncand == 1) for_near_nodes_t<true>(Bt, B, f);
-  else for_near_nodes_t<false>(Bt, B, f);
+EMPM_DEV void for_near_nodes(const Batch& Bt, const Binning& B, const ColTable* __restrict__ T, ColTable& Ts, F f) {
+  if (Bt.ncand == 1) for_near_nodes_t<true>(Bt, B, T, Ts, f);
+  else for_near_nodes_t<false>(Bt, B, T, Ts, f);
 }
 
 __global__ __launch_bounds__(kBlock) void k_grid(Batch Bt, const ColTable* __restrict__ T, float dt, Binning B,
                                                  const float4* __restrict__ tiles,
                                                  float4* __restrict__ keep, float4* __restrict__ v_out) {
-  for_near_nodes(Bt, B, [&](const DevModel& M, int ix, int iy, int iz, size_t id) {
+  __shared__ ColTable Ts;
+  for_near_nodes(Bt, B, T, Ts, [&](const DevModel& M, int ix, int iy, int iz, size_t id) {
   const float4 g = gather_node(B, M, tiles, ix, iy, iz);
   if (keep) keep[id] = g;
   float v[3], fac[3];
-  grid_node(M, *T, dt, ix, iy, iz, g, v, fac);
+  grid_node(M, Ts, dt, ix, iy, iz, g, v, fac);
   v_out[id] = make_float4(v[0], v[1], v[2], g.w);
   });
 }
@@ -1316,11 +1351,12 @@ __global__ __launch_bounds__(kBlock) void k_grid_adj(Batch Bt, const ColTable* _
                                                      const float4* __restrict__ tiles,
                                                      const float4* __restrict__ keep, float4* __restrict__ gadj_in,
                                                      int drop_mass) {
-  for_near_nodes(Bt, B, [&](const DevModel& M, int ix, int iy, int iz, size_t id) {
+  __shared__ ColTable Ts;
+  for_near_nodes(Bt, B, T, Ts, [&](const DevModel& M, int ix, int iy, int iz, size_t id) {
   const float4 b = gather_node(B, M, tiles, ix, iy, iz);
   const float4 g = keep[id];
   float v[3], fac[3];
-  grid_node(M, *T, dt, ix, iy, iz, g, v, fac);
+  grid_node(M, Ts, dt, ix, iy, iz, g, v, fac);
   float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
   if (g.w > 1e-15f) {
     const float im = 1.f / g.w;
