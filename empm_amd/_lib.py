@@ -12,7 +12,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libempm.so")
 
-EMPM_ABI_VERSION = 6
+EMPM_ABI_VERSION = 7
 EMPM_MAX_COLLIDERS = 16
 
 EMPM_OK = 0

@@ -266,17 +266,56 @@ __device__ __forceinline__ float4 normalize4(float4 q) {
 // q = normalize(quaternion of R); the blend rotates by q, whose matrix is R
 // to float32 round-off (the conversion's eps = 1e-8 is below it).
 //
-// A bone whose F has rank < 2 (torch.linalg.matrix_rank with its default
-// tolerance: singular values above 3 * eps_f32 * s_max count) sends the
-// reference into a fallback that leaves every bone's rotation at the
-// identity; such bones are counted in *degenerate and k_lbs_points then uses
-// R = I for all bones.  The rank test uses sigma_1 sigma_2 ~ |cof F| and
-// sigma_1^2 ~ |F|^2, in double precision.
+// Ranks (torch.linalg.matrix_rank with its default tolerance: singular
+// values above 3 * eps_f32 * s_max count; dynamic_utils.py:331-338) decide
+// which rotations the reference keeps (:342-405): one per rank-2/3 bone, or,
+// when any bone has rank 1, one per rank-1 bone instead; that set is then
+// assigned to all bones, which works only when it holds every bone or exactly
+// one (a broadcast), else every rotation is the identity.  A rank-1 bone's
+// rotation takes e_x to its F's range direction `axis` (sign: the largest-
+// magnitude component positive; the reference's is torch.svd's) about
+// axis x e_x (:373-397), and is the identity when that is shorter than 1e-6.
+// The rank test uses sigma_0 sigma_1 ~ |cof F| and sigma_0^2 ~ |F|^2 in
+// double precision.  The bones fill the rank state (8 device ints) that
+// k_lbs_points reads:
+//   [0] bones of rank < 2, [1] of rank 1, [2] of rank >= 2 (and bones with a
+//   relation outside [0, n_bones), whose record is NaN), [3] the largest
+//   rank-1 bone index, [4] the largest rank >= 2 bone index, [5] rank-1 bones
+//   whose second singular value exceeds 1e-8 (the reference's allclose
+//   assert fails then, :375, and every rotation is the identity).
+enum { kRankLt2 = 0, kRank1 = 1, kRankFull = 2, kRank1Bone = 3, kRankFullBone = 4, kRank1Loose = 5, kRankState = 8 };
+
+// The rank-1 rotation: x = e_x, perp = axis x x / |axis x x|,
+// R = axis x^T + perp perp^T + (axis x perp)(x x perp)^T.
+__device__ __forceinline__ void rank1_rotation(const float* F, float* R) {
+  double cn[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    cn[c] = (double)F[c] * F[c] + (double)F[3 + c] * F[3 + c] + (double)F[6 + c] * F[6 + c];
+  const int c = cn[0] >= cn[1] ? (cn[0] >= cn[2] ? 0 : 2) : (cn[1] >= cn[2] ? 1 : 2);
+  const double inv = 1.0 / sqrt(cn[c]);
+  double a[3] = {F[c] * inv, F[3 + c] * inv, F[6 + c] * inv};
+  const int big = fabs(a[0]) >= fabs(a[1]) ? (fabs(a[0]) >= fabs(a[2]) ? 0 : 2) : (fabs(a[1]) >= fabs(a[2]) ? 1 : 2);
+  if (a[big] < 0.0) a[0] = -a[0], a[1] = -a[1], a[2] = -a[2];
+  // perp = axis x e_x = (0, a2, -a1)
+  const double pl = sqrt(a[2] * a[2] + a[1] * a[1]);
+  if (pl < 1e-6) {
+    m_eye(R);
+    return;
+  }
+  const double p[3] = {0.0, a[2] / pl, -a[1] / pl};
+  const double th[3] = {0.0, -p[2], p[1]};  // e_x x perp
+  const double ta[3] = {a[1] * p[2] - a[2] * p[1], a[2] * p[0] - a[0] * p[2], a[0] * p[1] - a[1] * p[0]};  // axis x perp
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) R[r * 3 + q] = (float)(a[r] * (q == 0 ? 1.0 : 0.0) + p[r] * p[q] + ta[r] * th[q]);
+}
 template <typename I>
 __global__ __launch_bounds__(kLbsBlock) void k_lbs_bones(int nb, const float* __restrict__ bones,
                                                          const float* __restrict__ motions, int nadj,
                                                          const I* __restrict__ rel, float4* __restrict__ rec,
-                                                         int* __restrict__ degenerate) {
+                                                         int* __restrict__ rank_state) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   const float bx = bones[3 * (size_t)b], by = bones[3 * (size_t)b + 1], bz = bones[3 * (size_t)b + 2];
@@ -314,16 +353,25 @@ __global__ __launch_bounds__(kLbsBlock) void k_lbs_bones(int nb, const float* __
     for (int e = 0; e < 9; ++e) c2 += cf[e] * cf[e];
   }
   const double rtol = 3.0 * 1.1920928955078125e-07;
-  const bool degen = ok && !(c2 > rtol * rtol * f2 * f2);
+  const int rank = !ok ? 3 : (!(f2 > 0.0) ? 0 : (c2 > rtol * rtol * f2 * f2 ? 3 : 1));
   float R[9];
+  if (rank < 2) atomicAdd(&rank_state[kRankLt2], 1);
   if (!ok) {
     // a relation index outside [0, n_bones): the record is NaN
+    atomicAdd(&rank_state[kRankFull], 1);
+    atomicMax(&rank_state[kRankFullBone], b);
 #pragma unroll
     for (int e = 0; e < 9; ++e) R[e] = __builtin_nanf("");
-  } else if (degen) {
-    atomicAdd(degenerate, 1);
+  } else if (rank == 0) {
     m_eye(R);
+  } else if (rank == 1) {
+    atomicAdd(&rank_state[kRank1], 1);
+    atomicMax(&rank_state[kRank1Bone], b);
+    if (c2 > 1e-16 * f2) atomicAdd(&rank_state[kRank1Loose], 1);  // sigma_1 ~ |cof F| / |F| > 1e-8
+    rank1_rotation(F, R);
   } else {
+    atomicAdd(&rank_state[kRankFull], 1);
+    atomicMax(&rank_state[kRankFullBone], b);
     // scale F to unit max entry (R is invariant) so F^T F stays in range
     float mx_abs = 0.f;
 #pragma unroll
@@ -361,8 +409,9 @@ __device__ __forceinline__ void quat_rotate(float4 q, float vx, float vy, float 
 //   x' = sum_j w_j (R_j (x - b_j) + m_j + b_j)
 //   q' = normalize(sum_j w_j q_j) * quat   (when quat is given)
 // with w either given (n x k) or, when null, the inverse-distance weights of
-// calc_weights_vals_from_indices computed from the same bones and x.  With
-// *degenerate != 0 every R_j is the identity and every q_j = (1, 0, 0, 0).
+// calc_weights_vals_from_indices computed from the same bones and x.  The
+// rotations follow the rank state of k_lbs_bones: every bone's own, one
+// bone's for all (a broadcast), or the identity for all (q_j = (1, 0, 0, 0)).
 // One thread per Gaussian; KC = 16 is the reference's K with its index row
 // read as four 16-byte loads, KC = 0 any K <= 32; W: weights given, Q:
 // quaternions given.  Every branch inside the neighbour loop is resolved at
@@ -378,14 +427,20 @@ __global__ __launch_bounds__(kLbsBlock) void k_lbs_points(int n, const float* __
                                                           const I* __restrict__ idx,
                                                           const float* __restrict__ weights, int nbones,
                                                           const float4* __restrict__ rec,
-                                                          const int* __restrict__ degenerate,
+                                                          const int* __restrict__ rank_state,
                                                           float* __restrict__ xyz_out,
                                                           float* __restrict__ quat_out) {
   constexpr int KM = KC ? KC : kMaxK;
   const int k = KC ? KC : k_rt;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const bool ident = *degenerate != 0;
+  // which set of rotations the reference keeps and whether it is
+  // assignable to all bones (dynamic_utils.py:365-405)
+  const int n1 = rank_state[kRank1], nfull = rank_state[kRankFull];
+  const int kept = n1 > 0 ? n1 : nfull;
+  const bool ident = (n1 > 0 && rank_state[kRank1Loose] > 0) || !(kept == nbones || kept == 1);
+  const int bsel = kept == 1 ? rank_state[n1 > 0 ? kRank1Bone : kRankFullBone] : -1;
+  const float4 qsel = (!ident && bsel >= 0) ? rec[4 * (size_t)bsel] : make_float4(1.f, 0.f, 0.f, 0.f);
   const float x = xyz[3 * (size_t)i], y = xyz[3 * (size_t)i + 1], z = xyz[3 * (size_t)i + 2];
   int nb[KM];
   const bool ok = load_row<KC, I>(idx, i, k, nbones, nb);
@@ -419,7 +474,7 @@ __global__ __launch_bounds__(kLbsBlock) void k_lbs_points(int n, const float* __
       const float4* r = rec + 4 * (size_t)nb[j];
       const float4 b = r[1], m = r[2];
       const float4 rq = r[0];
-      const float4 bq = ident ? make_float4(1.f, 0.f, 0.f, 0.f) : rq;
+      const float4 bq = ident ? make_float4(1.f, 0.f, 0.f, 0.f) : (bsel >= 0 ? qsel : rq);
       const float lx = x - b.x, ly = y - b.y, lz = z - b.z;
       float wj;
       if constexpr (W) {
@@ -463,7 +518,11 @@ __global__ __launch_bounds__(kLbsBlock) void k_lbs_points(int n, const float* __
   }
 }
 
-__global__ void k_zero_i32(int* p) { *p = 0; }
+// the rank state before k_lbs_bones: counts 0, bone indices -1
+__global__ void k_init_rank_state(int* p) {
+  const int t = threadIdx.x;
+  p[t] = (t == kRank1Bone || t == kRankFullBone) ? -1 : 0;
+}
 
 bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
@@ -518,36 +577,36 @@ int empm_lbs_weights(int n, const float* xyz, int k, const void* idx, int idx_by
 }
 
 int empm_lbs_bones(int n_bones, const float* bones, const float* motions, int n_adj, const void* relations,
-                   int idx_bytes, float* bone_rec, int* degenerate, void* stream) {
+                   int idx_bytes, float* bone_rec, int* rank_state, void* stream) {
   if (n_bones < 0 || n_adj < 0 || (idx_bytes != 4 && idx_bytes != 8))
     return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_bones: bad arguments (n_bones=%lld, n_adj=%lld)", n_bones, n_adj);
-  if (!degenerate) return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_bones: null degenerate counter");
+  if (!rank_state) return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_bones: null rank state");
   if (n_bones > 0 && (!bones || !motions || !bone_rec || (n_adj > 0 && !relations)))
     return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_bones: null array");
   if (n_bones > 0 && ((uintptr_t)bone_rec & 63u))
     return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_bones: bone_rec not 64-byte aligned");
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, degenerate);
+  hipLaunchKernelGGL(k_init_rank_state, dim3(1), dim3(kRankState), 0, st, rank_state);
   if (n_bones > 0) {
     if (idx_bytes == 4)
       hipLaunchKernelGGL(k_lbs_bones<int32_t>, dim3(lbs_blocks(n_bones)), dim3(kLbsBlock), 0, st, n_bones, bones,
-                         motions, n_adj, (const int32_t*)relations, (float4*)bone_rec, degenerate);
+                         motions, n_adj, (const int32_t*)relations, (float4*)bone_rec, rank_state);
     else
       hipLaunchKernelGGL(k_lbs_bones<int64_t>, dim3(lbs_blocks(n_bones)), dim3(kLbsBlock), 0, st, n_bones, bones,
-                         motions, n_adj, (const int64_t*)relations, (float4*)bone_rec, degenerate);
+                         motions, n_adj, (const int64_t*)relations, (float4*)bone_rec, rank_state);
   }
   LBS_HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }
 
 int empm_lbs_points(int n, const float* xyz, const float* quat, int k, const void* idx, int idx_bytes,
-                    const float* weights, int n_bones, const float* bone_rec, const int* degenerate, float* xyz_out,
+                    const float* weights, int n_bones, const float* bone_rec, const int* rank_state, float* xyz_out,
                     float* quat_out, void* stream) {
   if (n < 0 || k < 1 || k > kMaxK || (idx_bytes != 4 && idx_bytes != 8))
     return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_points: need 1 <= k <= 32 and 4- or 8-byte indices (k=%lld)", k);
   if (n == 0) return EMPM_OK;
   if (n_bones < 1) return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_points: n_bones=%lld", n_bones);
-  if (!xyz || !idx || !bone_rec || !degenerate || !xyz_out || (quat && !quat_out))
+  if (!xyz || !idx || !bone_rec || !rank_state || !xyz_out || (quat && !quat_out))
     return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_points: null array");
   if (xyz_out == xyz || (quat && quat_out == quat))
     return lbs_err(EMPM_E_INVALID_ARG, "empm_lbs_points: outputs alias inputs");
@@ -560,7 +619,7 @@ int empm_lbs_points(int n, const float* xyz, const float* quat, int k, const voi
   const int sel = (v16 ? 8 : 0) | (idx_bytes == 8 ? 4 : 0) | (weights ? 2 : 0) | (quat ? 1 : 0);
 #define EMPM_LBS_POINTS(KC, I, W, Q)                                                                          \
   hipLaunchKernelGGL((k_lbs_points<KC, I, W, Q>), g, blk, 0, st, n, xyz, quat, k, (const I*)idx, weights, n_bones, \
-                     r4, degenerate, xyz_out, quat_out)
+                     r4, rank_state, xyz_out, quat_out)
   switch (sel) {
     case 0: EMPM_LBS_POINTS(0, int32_t, false, false); break;
     case 1: EMPM_LBS_POINTS(0, int32_t, false, true); break;

@@ -22,10 +22,12 @@ Differences a caller can see:
   the blend kernel (the two calls of the loop in one pass);
 - nearest-neighbour searches run on the GPU (knn_weights_sparse computes its
   distance matrix on the CPU in the reference);
-- no diagnostics are printed, and the rank-1 rotation branch of the reference
-  (dynamic_utils.py:365-397) is not reproduced: a bone of rank < 2 makes every
-  rotation the identity, the reference's fallback in all but the cases where
-  one rank-1 bone's rotation is broadcast onto all bones.
+- no diagnostics are printed (and no debugger is entered on a bad
+  determinant);
+- the rank-1 rotation (dynamic_utils.py:373-397) turns e_x to the direction
+  of the bone's F with that direction's largest-magnitude component
+  positive, where the reference takes torch.svd's sign; which rotations are
+  kept and broadcast (:331-405) is the reference's.
 """
 from __future__ import annotations
 
@@ -189,11 +191,12 @@ class LBSInterpolator:
             self._rec = torch.empty((max(n_bones, 1), 16), dtype=torch.float32, device=device)
             if self._rec.data_ptr() % 64:
                 raise RuntimeError("bone records must be 64-byte aligned")
-            self._degen = torch.empty(1, dtype=torch.int32, device=device)
+            self._degen = torch.empty(8, dtype=torch.int32, device=device)  # the rank state
         return self._rec, self._degen
 
     def bones(self, bones, motions, relations):
-        """Bone records of one frame; returns (records, degenerate counter)."""
+        """Bone records of one frame; returns (records, rank state: 8 device
+        ints, [0] the number of bones of rank < 2)."""
         b = _points(bones, "bones")
         m = _points(motions, "motions")
         if m.shape != b.shape:

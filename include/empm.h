@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define EMPM_ABI_VERSION 6
+#define EMPM_ABI_VERSION 7
 #define EMPM_MAX_COLLIDERS 16
 
 enum {
@@ -486,21 +486,34 @@ int empm_lbs_weights(int n, const float *xyz, int k, const void *idx, int idx_by
  * (w, x, y, z).  bone_rec: 16 floats per bone, 64-byte aligned:
  * [q.w q.x q.y q.z  b.x b.y b.z 0  m.x m.y m.z 0  0 0 0 0];
  * a bone with a relation index outside [0, n_bones) gets a NaN rotation.
- * *degenerate (one device int) receives the number of bones whose F has rank
- * below 2; when it is non-zero the reference leaves every rotation at the
- * identity, and empm_lbs_points does the same. */
+ * Ranks (torch.linalg.matrix_rank's tolerance) decide which rotations the
+ * reference keeps (dynamic_utils.py:331-405): one per bone of rank 2 or 3,
+ * or, when any bone has rank 1, one per rank-1 bone (e_x turned to the
+ * direction of F's range about their cross product; that direction's sign,
+ * torch.svd's in the reference, is taken with its largest-magnitude
+ * component positive); the kept set is applied to all bones when it holds
+ * every bone or exactly one (a broadcast), else every rotation is the
+ * identity.  rank_state (device, 8 ints) receives what empm_lbs_points needs
+ * to apply this: [0] the number of bones of rank < 2, [1] of rank 1, [2] of
+ * rank >= 2, [3] and [4] the largest rank-1 and rank >= 2 bone index (-1 if
+ * none), [5] the rank-1 bones whose second singular value exceeds 1e-8
+ * (the reference's allclose check fails and every rotation is the
+ * identity), [6..7] 0. */
 int empm_lbs_bones(int n_bones, const float *bones, const float *motions, int n_adj, const void *relations,
-                   int idx_bytes, float *bone_rec, int *degenerate, void *stream);
+                   int idx_bytes, float *bone_rec, int *rank_state, void *stream);
 
 /* Per-Gaussian blend of interpolate_motions_speedup (dynamic_utils.py:
  * 434-485): xyz_out = sum_j w_j (R_j (xyz - b_j) + m_j + b_j) and, when quat
  * (n x 4, w x y z, 16-byte aligned) is given, quat_out = normalize(sum_j w_j
  * q_j) * quat (Hamilton product).  weights (n x k) may be null: the weights
  * of empm_lbs_weights are then computed from the bones in bone_rec and xyz.
- * bone_rec holds n_bones records; a row holding an index outside
- * [0, n_bones) is NaN.  Outputs must not alias inputs. */
+ * bone_rec holds n_bones records and rank_state (8 device ints) the ranks
+ * empm_lbs_bones found: R_j is bone j's own rotation, one bone's rotation
+ * broadcast to all, or the identity, as the reference assigns them; a row
+ * holding an index outside [0, n_bones) is NaN.  Outputs must not alias
+ * inputs. */
 int empm_lbs_points(int n, const float *xyz, const float *quat, int k, const void *idx, int idx_bytes,
-                    const float *weights, int n_bones, const float *bone_rec, const int *degenerate,
+                    const float *weights, int n_bones, const float *bone_rec, const int *rank_state,
                     float *xyz_out, float *quat_out, void *stream);
 
 #ifdef __cplusplus

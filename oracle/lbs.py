@@ -23,13 +23,19 @@ env_setup/requirements.txt:29, not installed here): Shepperd's method, branch
 trace > 0, else the largest diagonal entry, with eps = 1e-8 under the square
 root.
 
-Bones whose F has rank < 2 (torch.linalg.matrix_rank: singular values above
-3 * eps_float32 * s_max count) send the reference into its except branches,
-which leave every rotation at the identity whenever the surviving subset is
-not assignable to all bones (:390-416).  The oracle returns the identity for
-every bone then.  The rank-1 rotation of :365-397 is reached in the reference
-only when all bones, or exactly one, are rank 1 (a broadcast of that bone's
-rotation onto all); those two cases are not restated (parity unpinned).
+Ranks (torch.linalg.matrix_rank, :331: singular values above
+3 * eps_float32 * s_max count) split the bones (:333-338).  The reference
+computes one rotation per rank-2/3 bone (:342-363) and, when any bone has
+rank 1, replaces that set by one rotation per rank-1 bone (:365-397); the
+set is then assigned to all bones (:399-405), which works only when it holds
+every bone or exactly one (a broadcast of that rotation), and otherwise
+raises into the identity for every bone.  The rank-1 rotation takes the
+x axis to the bone's direction (the left singular vector `axis` of F) about
+axis x e_x (the identity when that is shorter than 1e-6); it is the identity
+for all when a rank-1 F has a second singular value above 1e-8 (the
+allclose assert, :375).  The sign of `axis` is torch.svd's on the
+reference's device; this restatement (and the HIP path) takes the one whose
+largest-magnitude component is positive: parity unpinned for that sign.
 """
 from __future__ import annotations
 
@@ -131,22 +137,62 @@ def bone_covariance(bones, motions, relations):
     return np.einsum("bji,bjk->bik", n, o)
 
 
+def rank1_rotation(F):
+    """The rotation of a rank-1 bone (dynamic_utils.py:373-397) from the
+    direction of its F's range: x = e_x, perp = axis x x normalised,
+    R = axis x^T + perp perp^T + (axis x perp)(x x perp)^T."""
+    cols = np.linalg.norm(F, axis=0)
+    axis = F[:, int(np.argmax(cols))]
+    axis = axis / np.linalg.norm(axis)
+    if axis[int(np.argmax(np.abs(axis)))] < 0:
+        axis = -axis
+    x = np.array([1.0, 0.0, 0.0])
+    perp = np.cross(axis, x)
+    if np.linalg.norm(perp) < 1e-6:
+        return np.eye(3)
+    perp = perp / np.linalg.norm(perp)
+    third = np.cross(x, perp)
+    third_after = np.cross(axis, perp)
+    X = np.stack([x, perp, third], -1)
+    Y = np.stack([axis, perp, third_after], -1)
+    return Y @ X.T
+
+
 def bone_rotations(bones, motions, relations):
-    """(R (n_bones, 3, 3), degenerate count), dynamic_utils.py:321-416."""
+    """(R (n_bones, 3, 3) as the blend uses it, number of bones of rank < 2),
+    dynamic_utils.py:321-416 (see the module notes)."""
     F = bone_covariance(bones, motions, relations)
+    nb = F.shape[0]
     s_all = np.linalg.svd(F, compute_uv=False)
     rank = (s_all > 3 * EPS32 * s_all[:, :1]).sum(axis=1)
-    n_degen = int((rank < 2).sum())
-    if n_degen:
-        return np.broadcast_to(np.eye(3), F.shape).copy(), n_degen
-    U, _, Vh = np.linalg.svd(F)
-    S = np.broadcast_to(np.eye(3), F.shape).copy()
-    S[np.linalg.det(F) < 0, 2, 2] = -1.0
+    n_lt2 = int((rank < 2).sum())
+    eye = np.broadcast_to(np.eye(3), F.shape).copy()
+    one = rank == 1
+    if one.any():
+        if (s_all[one, 1:] > 1e-8).any():      # the allclose assert fails: identity rows
+            return eye, n_lt2
+        R1 = np.stack([rank1_rotation(f) for f in F[one]])
+        if R1.shape[0] == nb:
+            return R1, n_lt2
+        if R1.shape[0] == 1:
+            return np.broadcast_to(R1[0], F.shape).copy(), n_lt2
+        return eye, n_lt2
+    full = rank >= 2
+    if not full.any():
+        return eye, n_lt2
+    Ff = F[full]
+    U, _, Vh = np.linalg.svd(Ff)
+    S = np.broadcast_to(np.eye(3), Ff.shape).copy()
+    S[np.linalg.det(Ff) < 0, 2, 2] = -1.0
     R = U @ S @ Vh
     flip = np.abs(np.linalg.det(R) + 1.0) < 1e-3
     S[flip, 2, 2] *= -1.0
     R = U @ S @ Vh
-    return R, 0
+    if R.shape[0] == nb:
+        return R, n_lt2
+    if R.shape[0] == 1:
+        return np.broadcast_to(R[0], F.shape).copy(), n_lt2
+    return eye, n_lt2
 
 
 def interpolate(bones, motions, relations, xyz, quat, weights, idx):

@@ -181,7 +181,7 @@ def test_bone_records_match_oracle(dev):
     rel, _ = lbs.knn(bones, bones, 16, skip_first=1)
     rec, degen = gs.LBSInterpolator().bones(_t(bones, dev), _t(motions, dev), _t(rel, dev, torch.int32))
     rec = rec.cpu().numpy()
-    assert int(degen.item()) == 0
+    assert int(degen[0].item()) == 0
     R_o, _ = lbs.bone_rotations(bones, motions, rel)
     R = lbs.quat_to_rot(rec[:, 0:4])
     # the closest rotation moves by ~eps / gap under a perturbation of F, with
@@ -211,25 +211,64 @@ def test_bone_records_match_oracle(dev):
         assert np.any(big[clear] == b)
 
 
-def test_degenerate_bones_use_identity(dev):
-    print("[lbs] degenerate", flush=True)
+def _blend_vs_oracle(dev, bones, motions, rel, seed, n=500):
+    rng = np.random.default_rng(seed)
+    xyz = _f32(bones[rng.integers(0, len(bones), n)] + 0.002 * rng.standard_normal((n, 3)))
+    quat = _f32(rng.standard_normal((n, 4)))
+    idx, _ = lbs.knn(xyz, bones, 4)
+    out_o, q_o = lbs.interpolate(bones, motions, rel, xyz, quat, None, idx)
+    interp = gs.LBSInterpolator()
+    out, q = interp(_t(bones, dev), _t(motions, dev), _t(rel, dev), _t(xyz, dev), quat=_t(quat, dev),
+                    weights_indices=_t(idx, dev))
+    np.testing.assert_allclose(out.cpu().numpy(), out_o, atol=POS_TOL)
+    np.testing.assert_allclose(q.cpu().numpy(), q_o, atol=QUAT_TOL * np.abs(quat).max())
+    return interp._degen.cpu().numpy()
+
+
+def test_collinear_bones_use_their_rank1_rotations(dev):
+    # every bone of rank 1: each keeps its own rank-1 rotation (here a turn by
+    # atan(0.1) about z for all)
+    print("[lbs] rank 1, all bones", flush=True)
     nb = 64
     bones = _f32(np.stack([np.linspace(0, 0.2, nb), np.zeros(nb), np.zeros(nb)], 1))
     motions = np.zeros_like(bones)
     motions[:, 1] = 0.1 * bones[:, 0]
     motions = _f32(motions)
     rel, _ = lbs.knn(bones, bones, 4, skip_first=1)
-    rng = np.random.default_rng(15)
-    xyz = _f32(bones[rng.integers(0, nb, 500)] + 0.002 * rng.standard_normal((500, 3)))
-    quat = _f32(rng.standard_normal((500, 4)))
-    idx, _ = lbs.knn(xyz, bones, 4)
-    out_o, q_o = lbs.interpolate(bones, motions, rel, xyz, quat, None, idx)
-    interp = gs.LBSInterpolator()
-    out, q = interp(_t(bones, dev), _t(motions, dev), _t(rel, dev), _t(xyz, dev), quat=_t(quat, dev),
-                    weights_indices=_t(idx, dev))
-    assert int(interp._degen.item()) == nb
-    np.testing.assert_allclose(out.cpu().numpy(), out_o, atol=POS_TOL)
-    np.testing.assert_allclose(q.cpu().numpy(), q_o, atol=QUAT_TOL * np.abs(quat).max())
+    st = _blend_vs_oracle(dev, bones, motions, rel, 15)
+    assert list(st[:6]) == [nb, nb, 0, nb - 1, -1, 0]
+
+
+def _generic(nb, seed):
+    rng = np.random.default_rng(seed)
+    bones = _f32(rng.uniform(0, 0.2, (nb, 3)))
+    rel, _ = lbs.knn(bones, bones, 4, skip_first=1)
+    motions = _f32(0.002 * rng.standard_normal((nb, 3)))
+    return bones, motions, rel.astype(np.int64)
+
+
+def test_one_rank1_bone_is_broadcast_two_give_identity(dev):
+    print("[lbs] rank 1, one and two bones", flush=True)
+    bones, motions, rel = _generic(200, 16)
+    bones[1] = bones[0] + _f32([0.0, 0.01, 0.0])
+    bones[2] = bones[0] + _f32([0.0, -0.006, 0.0])
+    motions[1] = motions[0] + _f32([0.0, 0.01, 0.0008])
+    motions[2] = motions[0] + _f32([0.0, -0.006, -0.00048])
+    rel[0] = [1, 2, 1, 2]
+    st = _blend_vs_oracle(dev, bones, motions, rel, 17)
+    assert list(st[:6]) == [1, 1, 199, 0, 199, 0]
+    bones[4] = bones[3] + _f32([0.008, 0.0, 0.0])
+    rel[3] = [4, 4, 4, 4]
+    st = _blend_vs_oracle(dev, bones, motions, rel, 18)
+    assert list(st[:3]) == [2, 2, 198]
+
+
+def test_rank0_bones_with_one_full_bone_broadcast(dev):
+    print("[lbs] rank 0 with one full bone", flush=True)
+    bones, motions, rel = _generic(100, 19)
+    rel[1:] = np.arange(1, 100)[:, None]
+    st = _blend_vs_oracle(dev, bones, motions, rel, 20)
+    assert list(st[:6]) == [99, 0, 1, -1, 0, 0]
 
 
 def test_reference_frame_loop_matches_oracle(dev):

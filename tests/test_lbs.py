@@ -135,22 +135,98 @@ def test_translation_and_weights_closed_forms():
     np.testing.assert_allclose(out, xyz + m[0], atol=1e-14)
 
 
-def test_degenerate_bones_fall_back_to_identity():
-    # bones on a line: every F has rank 1
+def _rot_z(t):
+    c, s_ = np.cos(t), np.sin(t)
+    return np.array([[c, -s_, 0.0], [s_, c, 0.0], [0.0, 0.0, 1.0]])
+
+
+def _generic_bones(n, seed):
+    rng = np.random.default_rng(seed)
+    bones = rng.uniform(0, 1, (n, 3))
+    rel, _ = lbs.knn(bones, bones, 4, skip_first=1)
+    motions = 0.01 * rng.standard_normal((n, 3))
+    return bones, rel, motions
+
+
+def test_collinear_bones_take_their_rank1_rotation():
+    # bones on a line along x sheared in y: every F = (1, 0.1, 0) (x) (1, 0, 0)
+    # times a sum of squared offsets has rank 1, so every bone keeps its own
+    # rank-1 rotation (dynamic_utils.py:365-397 with all bones rank 1), which
+    # takes e_x to the sheared direction about e_z: a rotation by atan(0.1)
     bones = np.stack([np.linspace(0, 1, 40), np.zeros(40), np.zeros(40)], 1)
     rel, _ = lbs.knn(bones, bones, 4, skip_first=1)
     motions = np.zeros_like(bones)
-    motions[:, 1] = 0.1 * bones[:, 0]         # a shear
-    R, degen = lbs.bone_rotations(bones, motions, rel)
-    assert degen == 40
-    np.testing.assert_array_equal(R, np.broadcast_to(np.eye(3), R.shape))
+    motions[:, 1] = 0.1 * bones[:, 0]
+    R, lt2 = lbs.bone_rotations(bones, motions, rel)
+    assert lt2 == 40
+    np.testing.assert_allclose(R, np.broadcast_to(_rot_z(np.arctan(0.1)), R.shape), atol=1e-12)
+    # and the blend rotates a point's offset from its bones by that rotation
     xyz = np.array([[0.5, 0.01, 0.0]])
     idx, _ = lbs.knn(xyz, bones, 4)
     out, q = lbs.interpolate(bones, motions, rel, xyz, np.array([[1.0, 0, 0, 0]]), None, idx)
     w = lbs.inv_dist_weights(xyz, bones, idx)
-    np.testing.assert_allclose(out[0], xyz[0] + (w[0][:, None] * motions[idx[0]]).sum(0), atol=1e-15)
-    np.testing.assert_allclose(q, [[1.0, 0, 0, 0]], atol=1e-15)
+    want = (w[0][:, None] * ((xyz[0] - bones[idx[0]]) @ _rot_z(np.arctan(0.1)).T + bones[idx[0]]
+                             + motions[idx[0]])).sum(0)
+    np.testing.assert_allclose(out[0], want, atol=1e-14)
+    t = np.arctan(0.1) / 2
+    np.testing.assert_allclose(q, [[np.cos(t), 0, 0, np.sin(t)]], atol=1e-12)
 
+
+def test_one_rank1_bone_is_broadcast_and_two_give_identity():
+    bones, rel, motions = _generic_bones(30, 41)
+    # bone 0's adjacent bones on the line through it along e_y, moved along e_z:
+    # F_0 = e_z-ish (x) e_y, rank 1
+    bones[1] = bones[0] + [0.0, 0.05, 0.0]
+    bones[2] = bones[0] + [0.0, -0.03, 0.0]
+    rel[0] = [1, 2, 1, 2]
+    motions[1] = motions[0] + [0.0, 0.05, 0.004]
+    motions[2] = motions[0] + [0.0, -0.03, -0.0024]
+    R, lt2 = lbs.bone_rotations(bones, motions, rel)
+    assert lt2 == 1
+    F0 = lbs.bone_covariance(bones, motions, rel)[0]
+    R0 = lbs.rank1_rotation(F0)
+    np.testing.assert_allclose(R, np.broadcast_to(R0, R.shape), atol=1e-14)
+    d = F0[:, 1] / np.linalg.norm(F0[:, 1])
+    np.testing.assert_allclose(R0 @ [1.0, 0, 0], d if d[np.argmax(np.abs(d))] > 0 else -d, atol=1e-12)
+    assert abs(np.linalg.det(R0) - 1) < 1e-12
+    # a second rank-1 bone: the set of two is not assignable to all bones
+    bones[4] = bones[3] + [0.04, 0.0, 0.0]
+    rel[3] = [4, 4, 4, 4]
+    R, lt2 = lbs.bone_rotations(bones, motions, rel)
+    assert lt2 == 2
+    np.testing.assert_array_equal(R, np.broadcast_to(np.eye(3), R.shape))
+
+
+def test_rank0_bones_and_one_full_bone_broadcast():
+    # bones whose relations point at themselves have F = 0 (rank 0); with a
+    # single rank-3 bone left its rotation is broadcast to all, with two the
+    # set is not assignable and every rotation is the identity
+    bones, rel, motions = _generic_bones(12, 43)
+    rel[2:] = np.arange(2, 12)[:, None]
+    R, lt2 = lbs.bone_rotations(bones, motions, rel)
+    assert lt2 == 10
+    np.testing.assert_array_equal(R, np.broadcast_to(np.eye(3), R.shape))
+    rel[1] = 1
+    R, lt2 = lbs.bone_rotations(bones, motions, rel)
+    assert lt2 == 11
+    R_all, _ = lbs.bone_rotations(*_generic_bones(12, 43)[:1], motions, _generic_bones(12, 43)[1])
+    np.testing.assert_allclose(R, np.broadcast_to(R_all[0], R.shape), atol=1e-14)
+
+
+def test_rank1_with_a_second_singular_value_above_1e8_gives_identity():
+    # rank 1 for matrix_rank (s1 <= 3 eps_f32 s0) but s1 > 1e-8: the
+    # reference's allclose assert fails and every rotation is the identity
+    k = np.arange(5.0)
+    bones = np.stack([10 * k, 5e-4 * k ** 2, np.zeros(5)], 1)  # a slightly bent line
+    rel, _ = lbs.knn(bones, bones, 2, skip_first=1)
+    motions = np.zeros_like(bones)
+    motions[:, 1] = 0.1 * bones[:, 0]
+    F = lbs.bone_covariance(bones, motions, rel)
+    s = np.linalg.svd(F, compute_uv=False)
+    assert np.all(s[:, 1] <= 3 * lbs.EPS32 * s[:, 0]) and np.any(s[:, 1] > 1e-8)
+    R, lt2 = lbs.bone_rotations(bones, motions, rel)
+    assert lt2 == 5
+    np.testing.assert_array_equal(R, np.broadcast_to(np.eye(3), R.shape))
 
 def test_quat2mat_and_mat2quat():
     R = Rotation.random(300, random_state=21)
