@@ -755,7 +755,7 @@ def bench_loss(args, dev):
     tflops = 9.0 * n * n / (us * 1e-6) / 1e12
     out = {"workload": f"frame loss (Chamfer + tracking) and its gradient: {T} frames of {N} simulated points vs "
                        f"{N} observed points, 80% visible, 90% with valid motion",
-           "value": T / (ms * 1e-3), "unit": "frames/s (loss + gradient)", "ms_per_step": ms, "loss": float(L),
+           "value": T / (ms * 1e-3), "unit": "frames/s (loss + gradient)", "ms_per_step": ms, "loss": float(L.detach()),
            "roofline": {"bound": "valu", "kernel": "k_nn_sqdist", "achieved": tflops, "peak": 157.3,
                         "unit": "TFLOP/s", "frac": tflops / 157.3, "traffic": None,
                         "note": f"{n} x {n} pairs, 9 FP32 operations per pair", "avg_launch_us": us}}

@@ -1948,8 +1948,12 @@ __global__ void k_mass(int n, const float* __restrict__ rho, const float* __rest
 // of `chunk` points: split s writes its best (distance, index) to
 // dist/idx + s * n and k_nn_reduce takes the minimum over the splits, in
 // split order with a strict comparison, which keeps the smallest index.
+// Workgroups of four waves share each 32 KB tile: one-wave workgroups would
+// hold a tile each and fit only five waves on a CU.
 constexpr int kNnTile = 2048;
-__global__ __launch_bounds__(kBlock) void k_nn_sqdist(int n, const float* __restrict__ a, int m,
+constexpr int kNnBlock = 256;
+unsigned nn_blocks(size_t n) { return (unsigned)((n + kNnBlock - 1) / kNnBlock); }
+__global__ __launch_bounds__(kNnBlock) void k_nn_sqdist(int n, const float* __restrict__ a, int m,
                                                       const float* __restrict__ b, int chunk,
                                                       float* __restrict__ dist, int* __restrict__ idx) {
   __shared__ float4 tile[kNnTile];
@@ -2012,7 +2016,7 @@ __global__ __launch_bounds__(kBlock) void k_nn_reduce(int n, int splits, const f
 // with B[b_off[g], b_off[g+1]) (blockIdx.z = g; the frames of the training
 // loss in one launch).  Indices are global into B; a query whose B segment
 // is empty gets +inf and -1.  Blocks past their segment's end exit together.
-__global__ __launch_bounds__(kBlock) void k_nn_sqdist_seg(const int* __restrict__ a_off,
+__global__ __launch_bounds__(kNnBlock) void k_nn_sqdist_seg(const int* __restrict__ a_off,
                                                           const int* __restrict__ b_off,
                                                           const float* __restrict__ a,
                                                           const float* __restrict__ b, int splits, int n_a,
@@ -2072,7 +2076,7 @@ static __device__ __align__(256) float scratch[2 * kNnScratchPairs];
 // Splits of B for n points of A: enough blocks for 8 per CU, at least 512
 // points of B per split.
 int nn_splits(int n, int m, int n_seg = 1) {
-  const int qb = n_seg * ((n + kBlock - 1) / kBlock);
+  const int qb = n_seg * ((n + kNnBlock - 1) / kNnBlock);
   int s = (2048 + qb - 1) / qb;
   s = std::min(s, std::max(1, m / 512));
   if (s < 1) s = 1;
@@ -3381,13 +3385,13 @@ int empm_nn_sqdist_segments(int n_seg, const int* a_off, const int* b_off, int n
   hipStream_t st = (hipStream_t)stream;
   int s = max_b > 0 ? nn_splits(max_a, max_b, n_seg) : 1;
   if (s > 1 && (!workspace || workspace_bytes < (size_t)s * (size_t)n_a * 8 || ((uintptr_t)workspace & 3u))) s = 1;
-  const dim3 grid(nblocks(max_a), s, n_seg);
+  const dim3 grid(nn_blocks(max_a), s, n_seg);
   if (s == 1) {
-    hipLaunchKernelGGL(k_nn_sqdist_seg, grid, dim3(kBlock), 0, st, a_off, b_off, a, b, 1, n_a, dist, idx);
+    hipLaunchKernelGGL(k_nn_sqdist_seg, grid, dim3(kNnBlock), 0, st, a_off, b_off, a, b, 1, n_a, dist, idx);
   } else {
     float* pd = (float*)workspace;
     int* pi = (int*)((char*)workspace + (size_t)s * n_a * 4);
-    hipLaunchKernelGGL(k_nn_sqdist_seg, grid, dim3(kBlock), 0, st, a_off, b_off, a, b, s, n_a, pd, pi);
+    hipLaunchKernelGGL(k_nn_sqdist_seg, grid, dim3(kNnBlock), 0, st, a_off, b_off, a, b, s, n_a, pd, pi);
     hipLaunchKernelGGL(k_nn_reduce, dim3(nblocks(n_a)), dim3(kBlock), 0, st, n_a, s, pd, pi, dist, idx);
   }
   HIP_TRY(hipGetLastError());
@@ -3410,11 +3414,11 @@ int empm_nn_sqdist(int n, const float* a, int m, const float* b, float* dist, in
   if (s > 1 && (!workspace || workspace_bytes < (size_t)s * (size_t)n * 8 || ((uintptr_t)workspace & 3u))) s = 1;
   const int chunk = (m + s - 1) / s;
   if (s == 1) {
-    hipLaunchKernelGGL(k_nn_sqdist, dim3(nblocks(n), 1), dim3(kBlock), 0, st, n, a, m, b, m, dist, idx);
+    hipLaunchKernelGGL(k_nn_sqdist, dim3(nn_blocks(n), 1), dim3(kNnBlock), 0, st, n, a, m, b, m, dist, idx);
   } else {
     float* pd = (float*)workspace;
     int* pi = (int*)((char*)workspace + (size_t)s * n * 4);
-    hipLaunchKernelGGL(k_nn_sqdist, dim3(nblocks(n), s), dim3(kBlock), 0, st, n, a, m, b, chunk, pd, pi);
+    hipLaunchKernelGGL(k_nn_sqdist, dim3(nn_blocks(n), s), dim3(kNnBlock), 0, st, n, a, m, b, chunk, pd, pi);
     hipLaunchKernelGGL(k_nn_reduce, dim3(nblocks(n)), dim3(kBlock), 0, st, n, s, pd, pi, dist, idx);
   }
   HIP_TRY(hipGetLastError());
@@ -3451,7 +3455,7 @@ int empm_nn_sqdist_noalloc(int n, const float* a, int m, const float* b, float* 
   hipStream_t st = (hipStream_t)stream;
   const int s = nn_splits(n, m);
   if (s == 1) {
-    hipLaunchKernelGGL(k_nn_sqdist, dim3(nblocks(n), 1), dim3(kBlock), 0, st, n, a, m, b, m, dist, idx);
+    hipLaunchKernelGGL(k_nn_sqdist, dim3(nn_blocks(n), 1), dim3(kNnBlock), 0, st, n, a, m, b, m, dist, idx);
     HIP_TRY(hipGetLastError());
     return EMPM_OK;
   }
@@ -3468,7 +3472,7 @@ int empm_nn_sqdist_noalloc(int n, const float* a, int m, const float* b, float* 
   float* pd = base;
   int* pi = reinterpret_cast<int*>(base + (size_t)s * n);
   const int chunk = (m + s - 1) / s;
-  hipLaunchKernelGGL(k_nn_sqdist, dim3(nblocks(n), s), dim3(kBlock), 0, st, n, a, m, b, chunk, pd, pi);
+  hipLaunchKernelGGL(k_nn_sqdist, dim3(nn_blocks(n), s), dim3(kNnBlock), 0, st, n, a, m, b, chunk, pd, pi);
   hipLaunchKernelGGL(k_nn_reduce, dim3(nblocks(n)), dim3(kBlock), 0, st, n, s, pd, pi, dist, idx);
   HIP_TRY(hipGetLastError());
   return EMPM_OK;

@@ -12,6 +12,18 @@
 
 namespace empm {
 
+// Hardware transcendentals: v_rcp_f32, v_log_f32 (log2) and v_exp_f32
+// (exp2), about 1 ulp on normal inputs (denormals are flushed in this
+// library).  The guarded reciprocal `c ? 1.f / x : 0.f` would otherwise
+// compile to the correctly rounded division (the divide is hoisted above
+// the select without its accuracy tag), and the library's powf, log1pf and
+// expm1f expand to 100-250 instructions each.
+EMPM_DEV float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+EMPM_DEV float hw_log2(float x) { return __builtin_amdgcn_logf(x); }
+EMPM_DEV float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+// x^y for x > 0 (NaN for x < 0, +inf for x = 0 and y < 0, as powf)
+EMPM_DEV float hw_pow(float x, float y) { return hw_exp2(y * hw_log2(x)); }
+
 EMPM_DEV void m_zero(float* a) {
 #pragma unroll
   for (int i = 0; i < 9; ++i) a[i] = 0.f;
@@ -382,7 +394,7 @@ EMPM_DEV void stress_t(int m, const float* se, float J, float mu, float lam, flo
   } else {  // 6, 7: kirchoff_stress_neoHookean, mpm_utils.py:18-47
     float b[3] = {se[0] * se[0], se[1] * se[1], se[2] * se[2]};
     const float mb = (b[0] + b[1] + b[2]) / 3.f;
-    const float jp = powf(J, -2.f / 3.f);
+    const float jp = hw_pow(J, -2.f / 3.f);
 #pragma unroll
     for (int i = 0; i < 3; ++i) t[i] = mu * jp * (b[i] - mb) + lam / 2.f * (J * J - 1.f);
   }
@@ -459,22 +471,31 @@ EMPM_DEV float safe_den(float d) {
   return a < 1e-20f ? copysignf(1e-20f, d) : d;
 }
 
-// (a^k - b^k) / (a - b) for a, b > 0, through expm1/log1p
+// The divided differences below only enter derivatives.  They are functions
+// of u = a / b through two forms that stay accurate to a few ulp with the
+// hardware log2 / exp2 and need no switch to a series near u = 1:
+// log(u) / (u - 1) (Goldberg; the quotient is insensitive to the rounding of
+// u) and (e^y - 1) / y = (v - 1) / log(v), v = e^y (Kahan; likewise for the
+// rounding of v).
+constexpr float kLn2 = 0.693147180559945f, kLog2e = 1.44269504088896f;
+// log(u) / (u - 1) for u > 0 (1 at u = 1)
+EMPM_DEV float log_over(float u) { return u == 1.f ? 1.f : kLn2 * hw_log2(u) * hw_rcp(u - 1.f); }
+// (e^y - 1) / y (1 at y = 0; -1/y once e^y underflows, +inf once it overflows)
+EMPM_DEV float expm1_over(float y) {
+  const float v = hw_exp2(y * kLog2e);
+  if (v == 1.f) return 1.f;
+  if (v == 0.f) return -hw_rcp(y);
+  if (v == __builtin_inff()) return v;
+  return (v - 1.f) * hw_rcp(kLn2 * hw_log2(v));
+}
+// (a^k - b^k) / (a - b) for a, b > 0: b^(k-1) (u^k - 1) / (u - 1), and with
+// y = k log u,  (u^k - 1) / (u - 1) = k log_over(u) (e^y - 1) / y.
 EMPM_DEV float pow_dd(float a, float b, float k) {
-  const float r = (a - b) / b;
-  float E;
-  if (fabsf(r) < 1e-4f) E = k * (1.f + 0.5f * (k - 1.f) * r);
-  else E = expm1f(k * log1pf(r)) / r;
-  return powf(b, k - 1.f) * E;
+  const float u = a / b;
+  return hw_pow(b, k - 1.f) * k * log_over(u) * expm1_over(k * kLn2 * hw_log2(u));
 }
-// (log a - log b) / (a - b) for a, b > 0
-EMPM_DEV float log_dd(float a, float b) {
-  const float r = (a - b) / b;
-  float E;
-  if (fabsf(r) < 1e-4f) E = 1.f - 0.5f * r;
-  else E = log1pf(r) / r;
-  return E / b;
-}
+// (log a - log b) / (a - b) = log_over(a / b) / b for a, b > 0
+EMPM_DEV float log_dd(float a, float b) { return log_over(a / b) / b; }
 
 EMPM_DEV void iso_adj(const float* U, const float* V, const float* s, const float* g, const float* Dd, const float* Jg,
                       const float* Gbar, float* Fbar_out /* += */) {
@@ -504,7 +525,7 @@ EMPM_DEV void iso_adj(const float* U, const float* V, const float* s, const floa
 // Derivative of the clamped log: d log(max(s, lo)) / ds  (|s| for sand)
 EMPM_DEV float dlog_clamped(float s, float lo, int use_abs) {
   const float a = use_abs ? fabsf(s) : s;
-  return a > lo ? 1.f / s : 0.f;
+  return a > lo ? hw_rcp(s) : 0.f;
 }
 EMPM_DEV float sbar_of(float s, float lo, int use_abs) {
   const float a = use_abs ? fabsf(s) : s;
@@ -587,7 +608,7 @@ EMPM_DEV void stress_adj(const ConstitState& st, int m, float mu, float lam, con
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       e[i] = logf(fmaxf(s[i], 0.01f));
-      de[i] = s[i] > 0.01f ? 1.f / s[i] : 0.f;
+      de[i] = s[i] > 0.01f ? hw_rcp(s[i]) : 0.f;
     }
     const float tr = e[0] + e[1] + e[2];
 #pragma unroll
@@ -635,7 +656,7 @@ EMPM_DEV void stress_adj(const ConstitState& st, int m, float mu, float lam, con
       Dd[pr] = 2.f * mu * dphi - lam * L / (s[i] * s[j]);
     }
   } else {  // NH (6, 7)
-    const float jp = powf(J, -2.f / 3.f);
+    const float jp = hw_pow(J, -2.f / 3.f);
     float b[3] = {s[0] * s[0], s[1] * s[1], s[2] * s[2]};
     const float mb = (b[0] + b[1] + b[2]) / 3.f;
 #pragma unroll
@@ -703,7 +724,7 @@ EMPM_DEV void return_map_adj(const ConstitState& st, const float* Ftr, float mu,
   if (m == 1 || m == 5 || m == 7) {
     // k = ys / (2 mu n), n = |ehat| + 1e-6
     const float n = ehl + 1e-6f;
-    const float inv_ehl = ehl > 0.f ? 1.f / ehl : 0.f;
+    const float inv_ehl = ehl > 0.f ? hw_rcp(ehl) : 0.f;
 #pragma unroll
     for (int q = 0; q < 3; ++q) dk_ds[q] = -ys / (2.f * mu * n * n) * (eh[q] * inv_ehl) * l[q];
     dk_dys = 1.f / (2.f * mu * n);
@@ -727,7 +748,7 @@ EMPM_DEV void return_map_adj(const ConstitState& st, const float* Ftr, float mu,
     const float dk_dsn = -q * c23 * ys / (sn * sn);
     const float dk_dq = -1.f + c23 * ys / sn;
     const float dq_dmuh = q * q * P.plastic_viscosity / (2.f * muh * muh * dt);
-    const float inv_ehl = ehl > 0.f ? 1.f / ehl : 0.f;
+    const float inv_ehl = ehl > 0.f ? hw_rcp(ehl) : 0.f;
 #pragma unroll
     for (int qq = 0; qq < 3; ++qq) {
       const float dsn = 2.f * mu * eh[qq] * inv_ehl * l[qq];
@@ -756,7 +777,7 @@ EMPM_DEV void return_map_adj(const ConstitState& st, const float* Ftr, float mu,
     ysbar += w * dk_dys;
   }
   // divided differences of g_i = A sbar_i^k, A = exp((1-k) tr / 3)
-  const float A = expf((1.f - k) * tr / 3.f);
+  const float A = hw_exp2((1.f - k) * tr * (kLog2e / 3.f));
   float Dd[3];
 #pragma unroll
   for (int pr = 0; pr < 3; ++pr) {
@@ -801,7 +822,7 @@ EMPM_DEV void vm_state_adj(const ConstitState& st, float mu0, float ys0, float m
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     e[i] = logf(fmaxf(st.s[i], 0.01f));
-    l[i] = st.s[i] > 0.01f ? 1.f / st.s[i] : 0.f;
+    l[i] = st.s[i] > 0.01f ? hw_rcp(st.s[i]) : 0.f;
   }
   const float m3 = (e[0] + e[1] + e[2]) / 3.f;
   const float eh[3] = {e[0] - m3, e[1] - m3, e[2] - m3};
@@ -833,7 +854,7 @@ EMPM_DEV void vm_state_adj(const ConstitState& st, float mu0, float ys0, float m
   ysb0 -= dgb / (2.f * mu0);
   mub0 += dgb * ys0 / (2.f * mu0 * mu0);
   // L = |ehat|: dL/ds_m = (ehat_m / L) d log sbar_m / ds_m
-  const float iL = L > 0.f ? 1.f / L : 0.f;
+  const float iL = L > 0.f ? hw_rcp(L) : 0.f;
   const float sb[3] = {Lb * eh[0] * iL * l[0], Lb * eh[1] * iL * l[1], Lb * eh[2] * iL * l[2]};
 #pragma unroll
   for (int r = 0; r < 3; ++r)
