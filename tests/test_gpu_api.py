@@ -168,8 +168,8 @@ def test_trail_gradients_exact_mode_match_fd():
     # Trail returns the mean over particles of per-particle grads
     print(f"[trail exact] loss {loss:.6e}  E.grad {gE:.6e} vs FD/N {fdE / N:.6e}  nu.grad {gnu:.6e} vs FD/N "
           f"{fdnu / N:.6e}", flush=True)
-    assert abs(gE - fdE / N) < 2e-3 * abs(fdE / N)
-    assert abs(gnu - fdnu / N) < 2e-3 * abs(fdnu / N)
+    assert abs(gE - fdE / N) < 1e-3 * abs(fdE / N)
+    assert abs(gnu - fdnu / N) < 1e-3 * abs(fdnu / N)
 
 
 def test_trail_reference_compat_matches_frozen_mass_fd():

@@ -2,7 +2,7 @@
 restatement (oracle/chamfer.py), and its gradient through the simulation:
 per-frame Chamfer + tracking losses seed the reverse sweep at their frames,
 and d loss / d log E matches central differences of FP64 oracle rollouts
-scored by the restated loss (bar 2e-3 relative)."""
+scored by the restated loss (bar 1e-3 relative)."""
 import numpy as np
 import pytest
 import torch
@@ -205,4 +205,4 @@ def test_frame_loss_gradient_through_the_rollout_matches_oracle_fd(dev):
     l0 = oloss(E)
     print(f"[frame loss] GPU {float(L):.8e} vs oracle {l0:.8e}; d/dlogE {gE:.6e} vs FD {fd:.6e}", flush=True)
     assert abs(float(L) - l0) < 1e-4 * abs(l0)
-    assert abs(gE - fd) < 2e-3 * abs(fd)
+    assert abs(gE - fd) < 1e-3 * abs(fd)
