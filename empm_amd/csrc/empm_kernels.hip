@@ -661,7 +661,7 @@ EMPM_DEV void block_scatter(const Batch& Bt, const Binning& B, float4* __restric
   __shared__ float tile[NC][kTileNodes];
   __shared__ int next[2][2];
   const int lane = __lane_id();
-  const int wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave), nwaves = blockDim.x / kWave;
   const int last = B.nblk - 1;
   // List entry w goes to workgroup w (read ahead of the counts, as if
   // heavy).  With up to B.dyn_rounds entries per workgroup the rest are
@@ -697,23 +697,30 @@ EMPM_DEV void block_scatter(const Batch& Bt, const Binning& B, float4* __restric
     tile_origin(blk - M.b0, M.nb, o);
     for (int i = threadIdx.x; i < NC * kTileNodes; i += blockDim.x) (&tile[0][0])[i] = 0.f;
     __syncthreads();
-    int cs[9];
+    // the block's cell ranges and the first chunk of each cell (kc[8] =
+    // chunks in all); wave-uniform, so the chunk -> cell walk below is a
+    // chain of scalar selects
+    int cs[9], kc[9];
 #pragma unroll
     for (int c = 0; c <= 8; ++c) cs[c] = B.offs[blk * 9 + c];
-    int nch = 0;
+    kc[0] = 0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) nch += (cs[c + 1] - cs[c] + kWave - 1) / kWave;
+    for (int c = 0; c < 8; ++c) kc[c + 1] = kc[c] + (cs[c + 1] - cs[c] + kWave - 1) / kWave;
+    const int nch = kc[8];
     for (int ch = wave; ch < nch; ch += nwaves) {
-      // chunk ch -> (cell c, chunk j within the cell)
-      int c = 0, j = ch;
-      while (true) {
-        const int k = (cs[c + 1] - cs[c] + kWave - 1) / kWave;
-        if (j < k) break;
-        j -= k;
-        ++c;
+      // chunk ch -> cell c (the last cell whose first chunk is <= ch), its
+      // range [cb, ce) and the chunk's first particle
+      int c = 0, cb = cs[0], ce = cs[1], c0 = 0;
+#pragma unroll
+      for (int k = 1; k < 8; ++k) {
+        const bool past = ch >= kc[k];
+        c = past ? k : c;
+        cb = past ? cs[k] : cb;
+        ce = past ? cs[k + 1] : ce;
+        c0 = past ? kc[k] : c0;
       }
-      const int i = cs[c] + j * kWave + lane;
-      const int p = i < cs[c + 1] ? B.perm[i] : -1;
+      const int i = cb + (ch - c0) * kWave + lane;
+      const int p = i < ce ? B.perm[i] : -1;
       const bool active = p >= 0 && !(sel && sel[p] != 0);
       // the cell's stencil base and its tile-local offset
       const int t0[3] = {kTileMargin + ((c >> 2) & 1), kTileMargin + ((c >> 1) & 1), kTileMargin + (c & 1)};
