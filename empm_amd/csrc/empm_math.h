@@ -107,12 +107,12 @@ EMPM_DEV void m_udvt(const float* U, const float* d, const float* V, float* o) {
 template <int P, int Q>
 EMPM_DEV void jacobi_rot(float& app, float& aqq, float& apq, float& arp, float& arq, float* W) {
   // t = tan of the angle that zeroes apq: the root of t^2 + 2 theta t - 1 = 0
-  // of smaller magnitude, theta = (aqq - app) / (2 apq) (scale-free).  apq = 0
-  // (or flushed) selects t = 0, the identity, instead of branching around
-  // the update.
+  // of smaller magnitude, theta = (aqq - app) / (2 apq) (scale-free).  A
+  // negligible apq (below 1e-30, so its reciprocal stays finite) selects
+  // t = 0, the identity, instead of branching around the update.
   const float theta = 0.5f * (aqq - app) * hw_rcp(apq);
   const float tt = copysignf(1.f, theta) * hw_rcp(fabsf(theta) + sqrtf(fmaf(theta, theta, 1.f)));
-  const float t = apq == 0.f ? 0.f : tt;
+  const float t = fabsf(apq) < 1e-30f ? 0.f : tt;
   const float c = rsqrtf(fmaf(t, t, 1.f));
   const float s = t * c;
   app -= t * apq;
