@@ -31,7 +31,7 @@ import statistics
 
 import torch
 
-DEFAULT = {"scatter_wg": 256, "scatter_per_cu": 0, "heavy_chunks": 8, "dynamic_rounds": 4, "node_grid_cap": 1024}
+DEFAULT = {"scatter_wg": 256, "scatter_per_cu": 0, "heavy_chunks": 8, "dynamic_rounds": 4, "node_grid_cap": 4096}
 
 # the values searched per parameter (the defaults included)
 SPACE = {
@@ -39,7 +39,7 @@ SPACE = {
     "scatter_per_cu": (0, 1, 2),
     "heavy_chunks": (0, 4, 8, 16),
     "dynamic_rounds": (1, 2, 4, 8),
-    "node_grid_cap": (256, 512, 1024, 2048, 4096),
+    "node_grid_cap": (1024, 2048, 4096, 8192, 16384),
 }
 
 

@@ -996,8 +996,10 @@ __global__ __launch_bounds__(kBlock) void k_restart_F(Batch Bt, const float* __r
 // G2P and the P2G adjoint read only the stencil nodes of particles, which
 // lie in their block's tile.  A capped grid strides over (near block, owned
 // node) pairs of the binning's near list, so the cost follows the occupied
-// part of the grid, not its size.
-constexpr int kNodePassGridCap = 1024;
+// part of the grid, not its size.  The default cap is 4096 one-wave
+// workgroups (a 200k-particle 64^3 rollout's near nodes in one pass; a
+// 32-simulation batch's ~640k in a few).
+constexpr int kNodePassGridCap = 4096;
 // ONE: a batch of one simulation, whose table entry is then read with
 // scalar loads at once instead of per lane after the list entry
 template <bool ONE, typename F>

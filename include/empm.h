@@ -235,7 +235,7 @@ typedef struct {
                           * block counts as heavy, so the list keeps binning order); default 8 */
     int dynamic_rounds;  /* with more active blocks than this per scatter workgroup, blocks are handed out
                           * by a work counter instead of in fixed strides; default 4 */
-    int node_grid_cap;   /* workgroups of a grid node pass (they stride over the near nodes); default 1024 */
+    int node_grid_cap;   /* workgroups of a grid node pass (they stride over the near nodes); default 4096 */
 } empm_tuning;
 
 /* Set (validated; EMPM_E_INVALID_ARG names the bad field) or read the

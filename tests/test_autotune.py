@@ -9,8 +9,8 @@ from empm_amd import autotune as at
 
 def cost(cfg):
     """Separable, with its minimum at scatter_wg 128, scatter_per_cu 2,
-    heavy_chunks 4, dynamic_rounds 8, node_grid_cap 512."""
-    want = {"scatter_wg": 128, "scatter_per_cu": 2, "heavy_chunks": 4, "dynamic_rounds": 8, "node_grid_cap": 512}
+    heavy_chunks 4, dynamic_rounds 8, node_grid_cap 8192."""
+    want = {"scatter_wg": 128, "scatter_per_cu": 2, "heavy_chunks": 4, "dynamic_rounds": 8, "node_grid_cap": 8192}
     return 1.0 + sum(0.1 * (cfg[k] != v) for k, v in want.items())
 
 
@@ -18,7 +18,7 @@ def test_coordinate_search_finds_the_separable_minimum():
     calls = []
     best, t, trace = at.coordinate_search(lambda c: calls.append(c) or cost(c))
     assert best == {"scatter_wg": 128, "scatter_per_cu": 2, "heavy_chunks": 4, "dynamic_rounds": 8,
-                    "node_grid_cap": 512}
+                    "node_grid_cap": 8192}
     assert t == pytest.approx(1.0)
     assert len(trace) == len(calls)
     # the first timing is the default configuration
@@ -31,7 +31,7 @@ def test_coordinate_search_finds_the_separable_minimum():
 
 def test_gains_below_the_tolerance_keep_the_current_value():
     def flat(c):
-        return 1.0 - (0.005 if c["node_grid_cap"] == 4096 else 0.0)
+        return 1.0 - (0.005 if c["node_grid_cap"] == 16384 else 0.0)
     best, _, trace = at.coordinate_search(flat, rel_tol=0.01)
     assert best == at.DEFAULT
     # a pass without a change ends the search: one pass over every other value
@@ -39,14 +39,14 @@ def test_gains_below_the_tolerance_keep_the_current_value():
 
 
 def test_coupled_parameters_need_a_second_pass():
-    # node_grid_cap 256 only pays once scatter_wg is 512 (searched before it)
+    # node_grid_cap 1024 only pays once scatter_wg is 512 (searched before it)
     def coupled(c):
         t = 1.0 - 0.05 * (c["scatter_wg"] == 512)
-        if c["scatter_wg"] == 512 and c["node_grid_cap"] == 256:
+        if c["scatter_wg"] == 512 and c["node_grid_cap"] == 1024:
             t -= 0.05
         return t
     best, t, _ = at.coordinate_search(coupled)
-    assert best["scatter_wg"] == 512 and best["node_grid_cap"] == 256
+    assert best["scatter_wg"] == 512 and best["node_grid_cap"] == 1024
     assert t == pytest.approx(0.9)
 
 
@@ -68,7 +68,7 @@ def test_tune_sets_the_best_and_caches_it(tmp_path, monkeypatch):
     ctx = FakeCtx()
     timer = lambda run, reps: cost(ctx.tuning)
     r = at.tune(ctx, lambda: None, key="k1", timer=timer)
-    assert not r["cached"] and ctx.tuning["scatter_wg"] == 128 and ctx.tuning["node_grid_cap"] == 512
+    assert not r["cached"] and ctx.tuning["scatter_wg"] == 128 and ctx.tuning["node_grid_cap"] == 8192
     assert r["default_seconds"] == pytest.approx(cost(at.DEFAULT))
     saved = json.load(open(tmp_path / "tuning.json"))
     assert saved["k1"]["tuning"] == ctx.tuning
