@@ -1672,7 +1672,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g_adj(Batch Bt, float dt, const fl
 //   k_bin_count   per particle: bin key and rank within the bin (same-bin
 //                 lanes of a wave share one atomic); counts are zero on entry
 //                 (within a rollout the previous substep's G2P does this)
-//   k_bin_alloc   one thread per block, kBinScanWG blocks per workgroup:
+//   k_bin_alloc   one thread per block (four for a large batch), kBinScanWG
+//                 threads per workgroup:
 //                 block totals, their workgroup-local scan and one atomic
 //                 per workgroup for its particle range; per-bin offsets,
 //                 occupancy, active list; re-zeroes the counts
@@ -1844,52 +1845,103 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_finish(Batch Bt, int* __rest
 
 // Offsets of the per-substep binning in one launch (k_bin_scan +
 // k_bin_finish without the grid-wide prefix): each workgroup scans its
-// blocks' totals and takes its particle range with one atomic on ctr[0], so
-// the blocks' ranges are in the order the workgroups got there.  The active
-// list is counted in ctr[1], ctr[2]; k_bin_place copies those counts into
-// act[0], act[1] and re-zeroes ctr[0..2].  Also clears the near-list count
-// and the k_p2g work counter.
+// blocks' particle totals and heavy and light block counts, and takes its
+// particle range with one atomic on ctr[0] and its places in both halves of
+// the active list with one 64-bit atomic on ctr[2..3] (heavy count in the
+// low word, light in the high), so the blocks' ranges are in the order the
+// workgroups got there and no atomic is issued per wave.  k_bin_place
+// copies the list's counts into act[0], act[1] and re-zeroes ctr[0..3].
+// Also clears the near-list count and the k_p2g work counter.  Each thread
+// owns BPT blocks, kBinScanWG apart (four for a large batch's block arena,
+// whose workgroups then fit the device at once).
+template <int BPT>
 __global__ __launch_bounds__(kBinScanWG) void k_bin_alloc(Batch Bt, int* __restrict__ count, int* __restrict__ offs,
                                                           int* __restrict__ act, int* __restrict__ occ,
                                                           int* __restrict__ nlist, int* __restrict__ p2g_work,
                                                           int* __restrict__ ctr,
                                                           int heavy_chunks) {
   __shared__ int smem[kBinScanWG / 64 + 1];
-  __shared__ int wbase;
+  __shared__ int base[3];  // the workgroup's particle range and list places (heavy, light)
   const int nblk = Bt.nblk;
-  const int b = blockIdx.x * kBinScanWG + threadIdx.x;
-  int c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (b < nblk) {
-    const int4* c4 = reinterpret_cast<const int4*>(count + (size_t)b * 8);
-    const int4 lo = c4[0], hi = c4[1];
-    c[0] = lo.x, c[1] = lo.y, c[2] = lo.z, c[3] = lo.w, c[4] = hi.x, c[5] = hi.y, c[6] = hi.z, c[7] = hi.w;
+  const int b0 = blockIdx.x * (kBinScanWG * BPT) + threadIdx.x;
+  int c[BPT][8], tot[BPT];
+  bool heavy[BPT];
+  int sum = 0, nh = 0, nl = 0;
+#pragma unroll
+  for (int j = 0; j < BPT; ++j) {
+    const int b = b0 + j * kBinScanWG;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[j][k] = 0;
+    if (b < nblk) {
+      const int4* c4 = reinterpret_cast<const int4*>(count + (size_t)b * 8);
+      const int4 lo = c4[0], hi = c4[1];
+      c[j][0] = lo.x, c[j][1] = lo.y, c[j][2] = lo.z, c[j][3] = lo.w;
+      c[j][4] = hi.x, c[j][5] = hi.y, c[j][6] = hi.z, c[j][7] = hi.w;
+    }
+    int t = 0, nch = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      t += c[j][k];
+      nch += (c[j][k] + kWave - 1) / kWave;
+    }
+    tot[j] = t;
+    heavy[j] = nch > heavy_chunks;
+    sum += t;
+    nh += (t > 0 && heavy[j]) ? 1 : 0;
+    nl += (t > 0 && !heavy[j]) ? 1 : 0;
   }
-  const int tot = c[0] + c[1] + c[2] + c[3] + c[4] + c[5] + c[6] + c[7];
-  int wtot;
-  const int ex = wg_exclusive_scan(tot, smem, wtot);
-  if (threadIdx.x == 0) wbase = wtot > 0 ? atomicAdd(&ctr[0], wtot) : 0;
-  if (b == 0) {
+  int wtot, htot, ltot;
+  int ex = wg_exclusive_scan(sum, smem, wtot);
+  const int hx = wg_exclusive_scan(nh, smem, htot);
+  const int lx = wg_exclusive_scan(nl, smem, ltot);
+  if (threadIdx.x == 0) {
+    base[0] = wtot > 0 ? atomicAdd(&ctr[0], wtot) : 0;
+    unsigned long long hl = 0;
+    if (htot + ltot > 0)
+      hl = atomicAdd(reinterpret_cast<unsigned long long*>(ctr + 2),
+                     ((unsigned long long)(unsigned)ltot << 32) | (unsigned)htot);
+    base[1] = (int)(unsigned)(hl & 0xffffffffull);
+    base[2] = (int)(unsigned)(hl >> 32);
+  }
+  if (b0 == 0) {
     nlist[0] = 0;
     *p2g_work = 0;  // for the k_p2g launch that follows
   }
   __syncthreads();
-  if (b >= nblk) return;
-  occ[b] = tot > 0 ? 1 : 0;
-  if (tot == 0) return;
-  write_block_offsets(b, wbase + ex, c, offs, count);
-  append_active(b, nblk, c, heavy_chunks, act, &ctr[1], &ctr[2]);
+  const int wb = base[0];
+  int hk = base[1] + hx, lk = base[2] + lx;
+#pragma unroll
+  for (int j = 0; j < BPT; ++j) {
+    const int b = b0 + j * kBinScanWG;
+    if (b >= nblk) break;
+    occ[b] = tot[j] > 0 ? 1 : 0;
+    if (tot[j] > 0) {
+      write_block_offsets(b, wb + ex, c[j], offs, count);
+      // heavy blocks from act[2] upwards, light ones from act[2 + nblk - 1] downwards
+      if (heavy[j]) act[2 + hk++] = b;
+      else act[2 + nblk - 1 - lk++] = b;
+    }
+    ex += tot[j];
+  }
 }
 
-__global__ __launch_bounds__(kBlock) void k_bin_place(Batch Bt, const int2* __restrict__ key_rank,
-                                                      const int* __restrict__ offs, int* __restrict__ perm,
-                                                      const int* __restrict__ occ, int* __restrict__ near,
-                                                      int* __restrict__ nlist, int* __restrict__ act,
-                                                      int* __restrict__ ctr) {
+// WG = 64: one-wave workgroups, one near-list atomic per wave holding near
+// blocks; WG = kBinScanWG (a large batch's block arena, where those
+// atomics would queue on one address): one per workgroup, placed by a scan.
+template <int WG>
+__global__ __launch_bounds__(WG) void k_bin_place(Batch Bt, const int2* __restrict__ key_rank,
+                                                  const int* __restrict__ offs, int* __restrict__ perm,
+                                                  const int* __restrict__ occ, int* __restrict__ near,
+                                                  int* __restrict__ nlist, int* __restrict__ act,
+                                                  int* __restrict__ ctr) {
+  static_assert(WG == kWave || WG == kBinScanWG, "WG");
+  __shared__ int smem[kBinScanWG / 64 + 1];
+  __shared__ int nbase;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (ctr && i == 0) {  // after k_bin_alloc: the active-list counts into the record, counters re-zeroed
-    act[0] = ctr[1];
-    act[1] = ctr[2];
-    ctr[0] = ctr[1] = ctr[2] = 0;
+    act[0] = ctr[2];
+    act[1] = ctr[3];
+    ctr[0] = ctr[1] = ctr[2] = ctr[3] = 0;
   }
   if (i < Bt.N) {
     const int2 kr = key_rank[i];
@@ -1897,7 +1949,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_place(Batch Bt, const int2* __re
   }
   // block i is near when a non-empty block of its simulation has it in its
   // tile (blocks i - {0,1}^3 per axis); near blocks are appended to the
-  // near list, one atomic per wave
+  // near list
   bool is_near = false;
   if (i < Bt.nblk) {
     const DevModel& M = cand_of_block(Bt, i);
@@ -1913,13 +1965,21 @@ __global__ __launch_bounds__(kBlock) void k_bin_place(Batch Bt, const int2* __re
     for (int d = 0; d < 8; ++d) is_near = is_near || f[d] != 0;
     near[i] = is_near ? 1 : 0;
   }
-  const unsigned long long m = __ballot(is_near);
-  if (m == 0) return;
-  const int leader = __ffsll(m) - 1;
-  int base = 0;
-  if (__lane_id() == leader) base = atomicAdd(&nlist[0], __popcll(m));
-  base = __shfl(base, leader);
-  if (is_near) nlist[1 + base + __popcll(m & ((1ull << __lane_id()) - 1ull))] = i;
+  if constexpr (WG == kWave) {
+    const unsigned long long m = __ballot(is_near);
+    if (m == 0) return;
+    const int leader = __ffsll(m) - 1;
+    int base = 0;
+    if (__lane_id() == leader) base = atomicAdd(&nlist[0], __popcll(m));
+    base = __shfl(base, leader);
+    if (is_near) nlist[1 + base + __popcll(m & ((1ull << __lane_id()) - 1ull))] = i;
+  } else {
+    int ntot;
+    const int nx = wg_exclusive_scan(is_near ? 1 : 0, smem, ntot);
+    if (threadIdx.x == 0) nbase = ntot > 0 ? atomicAdd(&nlist[0], ntot) : 0;
+    __syncthreads();
+    if (is_near) nlist[1 + nbase + nx] = i;
+  }
 }
 
 __global__ void k_mu_lam(int n,const float* __restrict__ E, const float* __restrict__ nu, float* __restrict__ mu,
@@ -2131,7 +2191,7 @@ struct empm_ctx {
   int2* key_rank = nullptr;  // N: bin key and rank of each particle
   int* blkoff = nullptr;     // max_blocks: workgroup-local exclusive block offsets
   int* wsum = nullptr;       // block-scan workgroup totals
-  int* bin_ctr = nullptr;    // 3: k_bin_alloc's particle-range and active-list counters, zero between binnings
+  int* bin_ctr = nullptr;    // 4: k_bin_alloc's particle-range counter and (at 2, 64-bit) active-list counts, zero between binnings
   int* work = nullptr;       // 2: work counters of the k_p2g and k_g2p_adj_scatter launches
   empm_tuning tune = {kBlkWG, 0, kHeavyChunks, kDynamicRounds, kNodePassGridCap};
   int cus = 0;               // compute units of the device
@@ -2532,6 +2592,9 @@ int do_bin(empm_ctx* ctx, const HostBatch& hb, const float* x, const BinBuf& out
   ProfScope ps(ctx, EMPM_KERNEL_BIN, st);
   const int nblk = hb.nblk;
   const unsigned wgs = (unsigned)((nblk + kBinScanWG - 1) / kBinScanWG);
+  // a large batch's block arena: more than two k_bin_alloc workgroups per CU
+  // at one block per thread
+  const bool large = wgs > 2u * (unsigned)(ctx->cus > 0 ? ctx->cus : 256);
   if (!(counted && ctx->counts_pending)) {
     if (ctx->counts_pending) {
       const size_t nc = (size_t)ctx->max_blocks * 8;
@@ -2549,12 +2612,24 @@ int do_bin(empm_ctx* ctx, const HostBatch& hb, const float* x, const BinBuf& out
     hipLaunchKernelGGL(k_bin_finish, dim3(wgs), dim3(kBinScanWG), 0, st, hb.dev, ctx->count, ctx->blkoff,
                        ctx->wsum, out.offs, out.act, out.occ, ctx->tune.heavy_chunks);
   } else {
-    hipLaunchKernelGGL(k_bin_alloc, dim3(wgs), dim3(kBinScanWG), 0, st, hb.dev, ctx->count, out.offs, out.act,
-                       out.occ, out.nlist, ctx->work, ctx->bin_ctr, ctx->tune.heavy_chunks);
+    // four blocks per thread for a large batch's block arena
+    if (large) {
+      const unsigned wgs4 = (unsigned)((nblk + 4 * kBinScanWG - 1) / (4 * kBinScanWG));
+      hipLaunchKernelGGL(k_bin_alloc<4>, dim3(wgs4), dim3(kBinScanWG), 0, st, hb.dev, ctx->count, out.offs,
+                         out.act, out.occ, out.nlist, ctx->work, ctx->bin_ctr, ctx->tune.heavy_chunks);
+    } else {
+      hipLaunchKernelGGL(k_bin_alloc<1>, dim3(wgs), dim3(kBinScanWG), 0, st, hb.dev, ctx->count, out.offs,
+                         out.act, out.occ, out.nlist, ctx->work, ctx->bin_ctr, ctx->tune.heavy_chunks);
+    }
   }
-  hipLaunchKernelGGL(k_bin_place, dim3(nblocks(hb.N > nblk ? hb.N : nblk)), dim3(kBlock), 0, st, hb.dev,
-                     ctx->key_rank, out.offs, out.perm, out.occ, out.near, out.nlist, out.act,
-                     ordered ? nullptr : ctx->bin_ctr);
+  const size_t nplace = (size_t)(hb.N > nblk ? hb.N : nblk);
+  if (large)
+    hipLaunchKernelGGL(k_bin_place<kBinScanWG>, dim3((unsigned)((nplace + kBinScanWG - 1) / kBinScanWG)),
+                       dim3(kBinScanWG), 0, st, hb.dev, ctx->key_rank, out.offs, out.perm, out.occ, out.near,
+                       out.nlist, out.act, ordered ? nullptr : ctx->bin_ctr);
+  else
+    hipLaunchKernelGGL(k_bin_place<kWave>, dim3(nblocks(nplace)), dim3(kWave), 0, st, hb.dev, ctx->key_rank,
+                       out.offs, out.perm, out.occ, out.near, out.nlist, out.act, ordered ? nullptr : ctx->bin_ctr);
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }
