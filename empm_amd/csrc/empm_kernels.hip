@@ -1890,10 +1890,13 @@ __global__ __launch_bounds__(kBinScanWG) void k_bin_alloc(Batch Bt, int* __restr
     nh += (t > 0 && heavy[j]) ? 1 : 0;
     nl += (t > 0 && !heavy[j]) ? 1 : 0;
   }
-  int wtot, htot, ltot;
+  // the heavy and light counts share one scan as 16-bit fields (a
+  // workgroup has at most kBinScanWG * BPT <= 4096 blocks of either kind)
+  static_assert(kBinScanWG * BPT < 65536, "BPT");
+  int wtot, hltot;
   int ex = wg_exclusive_scan(sum, smem, wtot);
-  const int hx = wg_exclusive_scan(nh, smem, htot);
-  const int lx = wg_exclusive_scan(nl, smem, ltot);
+  const int hlx = wg_exclusive_scan(nh | (nl << 16), smem, hltot);
+  const int hx = hlx & 0xffff, lx = hlx >> 16, htot = hltot & 0xffff, ltot = hltot >> 16;
   if (threadIdx.x == 0) {
     base[0] = wtot > 0 ? atomicAdd(&ctr[0], wtot) : 0;
     unsigned long long hl = 0;
