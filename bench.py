@@ -309,9 +309,9 @@ def main():
     del gro, gctx
     torch.cuda.empty_cache()
 
-    # the checkpoint-recompute policy SURVEY's B_fb is defined on: a state
-    # every 50 substeps, each segment replayed (recorded) before its reverse
-    # sweep
+    # the checkpoint-recompute policy the per-substep byte count B_fb below is
+    # defined on: a state every 50 substeps, each segment replayed (recorded)
+    # before its reverse sweep
     def kstep():
         final, _, _, tape = ro.forward(s0, cols(), 0.0, dt, T, tape=True, ckpt_every=B_FB_K)
         diff = final.x - target
