@@ -317,8 +317,9 @@ def test_function_variants_shapes_and_consistency():
 @pytest.mark.parametrize("ckpt", [1, 3])
 def test_material_5_forward_rewrites_yield_and_backward_differentiates_it(ckpt):
     """Plasticine rewrites the model's yield stress during the forward
-    (persisting across rollouts, App.-B-8 behaviour of the reference: only
-    mu and lam are recomputed per rollout); the backward differentiates
+    (persisting across rollouts, as the reference's in-place write at
+    mpm_utils.py:133-186 does: only mu and lam are recomputed per rollout);
+    the backward differentiates
     through those updates without touching the model's arrays again, and its
     E gradient matches central differences of FP64 oracle rollouts that start
     from the same softened yield stress."""
