@@ -12,7 +12,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libempm.so")
 
-EMPM_ABI_VERSION = 7
+EMPM_ABI_VERSION = 8
 EMPM_MAX_COLLIDERS = 16
 
 EMPM_OK = 0
@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "empm_set_graphs",
     "empm_graph_stats",
     "empm_set_tuning",
+    "empm_set_deterministic",
     "empm_get_tuning",
     "empm_nn_sqdist_segments",
     "empm_nn_sqdist_segments_workspace",
@@ -233,6 +234,8 @@ def _declare(lib):
     lib.empm_graph_stats.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     lib.empm_set_tuning.restype = C.c_int
     lib.empm_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    lib.empm_set_deterministic.restype = C.c_int
+    lib.empm_set_deterministic.argtypes = [vp, C.c_int]
     lib.empm_get_tuning.restype = C.c_int
     lib.empm_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     lib.empm_read_status.restype = C.c_int

@@ -655,10 +655,16 @@ EMPM_DEV void tile_origin(int blk, int nb, int* o) {
 // the block's cell chunks round robin.  Launched with as many workgroups as
 // fit on the device at once, so no workgroup waits for another to finish
 // before it starts.
-template <int NC, typename Load>
+// DETW > 0 (deterministic mode, DETW = the workgroup's waves): each wave sums
+// into a tile of its own, the lanes of particles scattered lane by lane take
+// turns in lane order, and the wave tiles are added in wave order, so every
+// float sum has an order fixed by the chunks (the binning orders a cell's
+// particles by index in this mode) instead of by the LDS atomics' arrival.
+template <int NC, int DETW, typename Load>
 EMPM_DEV void block_scatter(const Batch& Bt, const Binning& B, float4* __restrict__ tiles, int* __restrict__ work,
                             Load load) {
-  __shared__ float tile[NC][kTileNodes];
+  constexpr int NT = DETW > 0 ? DETW : 1;
+  __shared__ float tile[NT][NC][kTileNodes];
   __shared__ int next[2][2];
   const int lane = __lane_id();
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave), nwaves = blockDim.x / kWave;
@@ -695,8 +701,9 @@ EMPM_DEV void block_scatter(const Batch& Bt, const Binning& B, float4* __restric
     const float inv_dx = M.inv_dx;
     int o[3];
     tile_origin(blk - M.b0, M.nb, o);
-    for (int i = threadIdx.x; i < NC * kTileNodes; i += blockDim.x) (&tile[0][0])[i] = 0.f;
+    for (int i = threadIdx.x; i < NT * NC * kTileNodes; i += blockDim.x) (&tile[0][0][0])[i] = 0.f;
     __syncthreads();
+    float(*wt)[kTileNodes] = tile[DETW > 0 ? wave : 0];
     // the block's cell ranges and the first chunk of each cell (kc[8] =
     // chunks in all); wave-uniform, so the chunk -> cell walk below is a
     // chain of scalar selects
@@ -731,15 +738,29 @@ EMPM_DEV void block_scatter(const Batch& Bt, const Binning& B, float4* __restric
       Stencil s;
       make_stencil(xp, inv_dx, s);
       const bool home = active && s.base[0] == o[0] + t0[0] && s.base[1] == o[1] + t0[1] && s.base[2] == o[2] + t0[2];
-      if (__ballot(home)) group_scatter<NC>(tile, t0, home, P, s, lane);
-      if (active && !home) lane_scatter<NC>(tile, o, P, s);
+      if (__ballot(home)) group_scatter<NC>(wt, t0, home, P, s, lane);
+      if constexpr (DETW > 0) {
+        for (unsigned long long m = __ballot(active && !home); m; m &= m - 1)
+          if (lane == __builtin_ctzll(m)) lane_scatter<NC>(wt, o, P, s);
+      } else if (active && !home) {
+        lane_scatter<NC>(wt, o, P, s);
+      }
     }
     int fetched_blk = 0;
     if (dynamic && threadIdx.x == 0 && fetched < nact) fetched_blk = entry(fetched);
     __syncthreads();
     float4* dst = tiles + (size_t)blk * kTileNodes;
-    for (int k = threadIdx.x; k < kTileNodes; k += blockDim.x)
-      dst[k] = make_float4(tile[0][k], tile[1][k], tile[2][k], NC == 4 ? tile[NC - 1][k] : 0.f);
+    for (int k = threadIdx.x; k < kTileNodes; k += blockDim.x) {
+      float4 t = make_float4(tile[0][0][k], tile[0][1][k], tile[0][2][k], NC == 4 ? tile[0][NC - 1][k] : 0.f);
+#pragma unroll
+      for (int w = 1; w < NT; ++w) {
+        t.x += tile[w][0][k];
+        t.y += tile[w][1][k];
+        t.z += tile[w][2][k];
+        if (NC == 4) t.w += tile[w][NC - 1][k];
+      }
+      dst[k] = t;
+    }
     if (dynamic && threadIdx.x == 0) {
       next[it & 1][0] = fetched;
       next[it & 1][1] = fetched_blk;
@@ -909,11 +930,11 @@ __global__ __launch_bounds__(kBlock) void k_stress(Batch Bt, float dt,
 
 // p2g_apic_with_stress (mpm_utils.py:358-423): APIC momentum, mass and the
 // stress impulse into the block tiles.
-template <int WG>
+template <int WG, bool DET = false>
 __global__ __launch_bounds__(WG) void k_p2g(Batch Bt, const float* __restrict__ x, const float* __restrict__ v,
                                                 const float* __restrict__ C, const float* __restrict__ S, Binning B,
                                                 float4* __restrict__ tiles, int* __restrict__ work) {
-  block_scatter<4>(Bt, B, tiles, work, [&](int p, float* xp, ScatterPayload& P, const DevModel& M) {
+  block_scatter<4, DET ? WG / kWave : 0>(Bt, B, tiles, work, [&](int p, float* xp, ScatterPayload& P, const DevModel& M) {
     const float dx = M.dx;
 #pragma unroll
     for (int d = 0; d < 3; ++d) xp[d] = x[3 * p + d];
@@ -933,11 +954,11 @@ __global__ __launch_bounds__(WG) void k_p2g(Batch Bt, const float* __restrict__ 
 // set_F_C_p2g (mpm_utils.py:843-876) of restart_and_compute_F_C: the mass
 // and the mass-weighted displacement target - x into the block tiles (no
 // APIC or stress term).
-template <int WG>
+template <int WG, bool DET = false>
 __global__ __launch_bounds__(WG) void k_p2g_disp(Batch Bt, const float* __restrict__ x,
                                                  const float* __restrict__ target, Binning B,
                                                  float4* __restrict__ tiles, int* __restrict__ work) {
-  block_scatter<4>(Bt, B, tiles, work, [&](int p, float* xp, ScatterPayload& P, const DevModel& M) {
+  block_scatter<4, DET ? WG / kWave : 0>(Bt, B, tiles, work, [&](int p, float* xp, ScatterPayload& P, const DevModel& M) {
     const float mp = M.mass[p];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -1324,7 +1345,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_adj(Batch Bt, float dt, const fl
 // Per particle: A = dL/dv' (clamp mask applied, from k_g2p_adj), and from
 // the state's cotangents B = 4 inv_dx dL/dC' and S = dt inv_dx dL/dF' Fe^T
 // (F' = (I + dt grad v) Fe).
-template <int WG>
+template <int WG, bool DET = false>
 __global__ __launch_bounds__(WG) void k_g2p_adj_scatter(Batch Bt, float dt, const float* __restrict__ x,
                                                             const float* __restrict__ payload,
                                                             const float* __restrict__ dC1,
@@ -1332,7 +1353,7 @@ __global__ __launch_bounds__(WG) void k_g2p_adj_scatter(Batch Bt, float dt, cons
                                                             const float* __restrict__ Fe, Binning B,
                                                             float4* __restrict__ tiles,
                                                             int* __restrict__ work) {
-  block_scatter<3>(Bt, B, tiles, work, [&](int p, float* xp, ScatterPayload& P, const DevModel& M) {
+  block_scatter<3, DET ? WG / kWave : 0>(Bt, B, tiles, work, [&](int p, float* xp, ScatterPayload& P, const DevModel& M) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) xp[d] = x[3 * p + d];
     const float4 a = reinterpret_cast<const float4*>(payload)[p];
@@ -1985,6 +2006,58 @@ __global__ __launch_bounds__(WG) void k_bin_place(Batch Bt, const int2* __restri
   }
 }
 
+// Deterministic mode, after k_bin_place: each cell's particles in perm in
+// ascending index order (k_bin_place leaves them in the order of the counting
+// atomics).  One workgroup per active block at a time: the block's range is
+// copied (to LDS, or for a block of more than kBinDetLds particles to the
+// rank fields of key_rank, free once k_bin_place has read them and rewritten
+// by the next counting step), then each particle goes to its cell's start +
+// the number of smaller indices in the cell.
+constexpr int kBinDetWG = 256;
+constexpr int kBinDetLds = 4096;
+__global__ __launch_bounds__(kBinDetWG) void k_bin_det(const int* __restrict__ act, int nblk,
+                                                       const int* __restrict__ offs, int* __restrict__ perm,
+                                                       int2* __restrict__ key_rank) {
+  __shared__ int lcopy[kBinDetLds];
+  int* __restrict__ gcopy = reinterpret_cast<int*>(key_rank);  // gcopy[2 i + 1]: perm[i]
+  const int nh = act[0], nact = nh + act[1], last = nblk - 1;
+  for (int e = blockIdx.x; e < nact; e += gridDim.x) {
+    const int blk = e < nh ? act[2 + e] : act[2 + last - (e - nh)];
+    int cs[9];
+#pragma unroll
+    for (int c = 0; c <= 8; ++c) cs[c] = offs[blk * 9 + c];
+    const int b0 = cs[0];
+    if (cs[8] - b0 <= kBinDetLds) {
+      for (int i = b0 + (int)threadIdx.x; i < cs[8]; i += blockDim.x) lcopy[i - b0] = perm[i];
+      __syncthreads();
+      for (int i = b0 + (int)threadIdx.x; i < cs[8]; i += blockDim.x) {
+        int c = 0;
+#pragma unroll
+        for (int k = 1; k < 8; ++k) c = i >= cs[k] ? k : c;
+        const int p = lcopy[i - b0];
+        int r = 0;
+#pragma unroll 8
+        for (int j = cs[c] - b0; j < cs[c + 1] - b0; ++j) r += lcopy[j] < p ? 1 : 0;
+        perm[cs[c] + r] = p;
+      }
+    } else {
+      for (int i = b0 + (int)threadIdx.x; i < cs[8]; i += blockDim.x) gcopy[2 * (size_t)i + 1] = perm[i];
+      __syncthreads();
+      for (int i = b0 + (int)threadIdx.x; i < cs[8]; i += blockDim.x) {
+        int c = 0;
+#pragma unroll
+        for (int k = 1; k < 8; ++k) c = i >= cs[k] ? k : c;
+        const int p = gcopy[2 * (size_t)i + 1];
+        int r = 0;
+#pragma unroll 8
+        for (int j = cs[c]; j < cs[c + 1]; ++j) r += gcopy[2 * (size_t)j + 1] < p ? 1 : 0;
+        perm[cs[c] + r] = p;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void k_mu_lam(int n,const float* __restrict__ E, const float* __restrict__ nu, float* __restrict__ mu,
                          float* __restrict__ lam) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2197,6 +2270,7 @@ struct empm_ctx {
   int* bin_ctr = nullptr;    // 4: k_bin_alloc's particle-range counter and (at 2, 64-bit) active-list counts, zero between binnings
   int* work = nullptr;       // 2: work counters of the k_p2g and k_g2p_adj_scatter launches
   empm_tuning tune = {kBlkWG, 0, kHeavyChunks, kDynamicRounds, kNodePassGridCap};
+  bool deterministic = false;  // empm_set_deterministic: fixed float summation orders
   int cus = 0;               // compute units of the device
   int scatter_res[3][2] = {};  // scatter workgroups resident per CU: [128, 256, 512 threads][k_p2g, k_g2p_adj_scatter]
   DevModel* cand_dev = nullptr;     // device table of the simulations of the current call
@@ -2551,26 +2625,33 @@ int wg_variant(int wg) { return wg == 128 ? 0 : (wg == 512 ? 2 : 1); }
 // Launch width of a block-owned scatter (idx 0: k_p2g, 1: k_g2p_adj_scatter)
 // with the tuned workgroup size: the workgroups resident at once, or the
 // tuning's count per CU when that is lower.
-unsigned scatter_grid(const empm_ctx* ctx, int idx) {
-  int per_cu = ctx->scatter_res[wg_variant(ctx->tune.scatter_wg)][idx];
+unsigned scatter_grid(const empm_ctx* ctx, int idx, int wg) {
+  int per_cu = ctx->scatter_res[wg_variant(wg)][idx];
   if (ctx->tune.scatter_per_cu > 0 && (per_cu <= 0 || ctx->tune.scatter_per_cu < per_cu))
     per_cu = ctx->tune.scatter_per_cu;
   const long long w = (long long)per_cu * ctx->cus;
   return w > 0 ? (unsigned)w : (unsigned)kBlkGridCap;
 }
 
-// KERNEL<tuned workgroup size> of the block-owned scatters
+// KERNEL<tuned workgroup size> of the block-owned scatters; in deterministic
+// mode KERNEL<256, true> (per-wave tiles; the workgroup size is fixed so the
+// summation order does not follow the tuning)
 #define EMPM_LAUNCH_SCATTER(ctx, idx, KERNEL, stream, ...)                                                    \
-  switch ((ctx)->tune.scatter_wg) {                                                                          \
-    case 128:                                                                                                \
-      hipLaunchKernelGGL(KERNEL<128>, dim3(scatter_grid(ctx, idx)), dim3(128), 0, stream, __VA_ARGS__);     \
-      break;                                                                                                 \
-    case 512:                                                                                                \
-      hipLaunchKernelGGL(KERNEL<512>, dim3(scatter_grid(ctx, idx)), dim3(512), 0, stream, __VA_ARGS__);     \
-      break;                                                                                                 \
-    default:                                                                                                 \
-      hipLaunchKernelGGL(KERNEL<256>, dim3(scatter_grid(ctx, idx)), dim3(256), 0, stream, __VA_ARGS__);     \
-      break;                                                                                                 \
+  if ((ctx)->deterministic) {                                                                                \
+    hipLaunchKernelGGL((KERNEL<256, true>), dim3(scatter_grid(ctx, idx, 256)), dim3(256), 0, stream,        \
+                       __VA_ARGS__);                                                                         \
+  } else {                                                                                                   \
+    switch ((ctx)->tune.scatter_wg) {                                                                        \
+      case 128:                                                                                              \
+        hipLaunchKernelGGL(KERNEL<128>, dim3(scatter_grid(ctx, idx, 128)), dim3(128), 0, stream, __VA_ARGS__); \
+        break;                                                                                               \
+      case 512:                                                                                              \
+        hipLaunchKernelGGL(KERNEL<512>, dim3(scatter_grid(ctx, idx, 512)), dim3(512), 0, stream, __VA_ARGS__); \
+        break;                                                                                               \
+      default:                                                                                               \
+        hipLaunchKernelGGL(KERNEL<256>, dim3(scatter_grid(ctx, idx, 256)), dim3(256), 0, stream, __VA_ARGS__); \
+        break;                                                                                               \
+    }                                                                                                        \
   }
 
 
@@ -2633,6 +2714,11 @@ int do_bin(empm_ctx* ctx, const HostBatch& hb, const float* x, const BinBuf& out
   else
     hipLaunchKernelGGL(k_bin_place<kWave>, dim3(nblocks(nplace)), dim3(kWave), 0, st, hb.dev, ctx->key_rank,
                        out.offs, out.perm, out.occ, out.near, out.nlist, out.act, ordered ? nullptr : ctx->bin_ctr);
+  if (ctx->deterministic && hb.N > 0) {
+    const unsigned cap = 8u * (unsigned)(ctx->cus > 0 ? ctx->cus : 256);
+    hipLaunchKernelGGL(k_bin_det, dim3((unsigned)nblk < cap ? (unsigned)(nblk > 0 ? nblk : 1) : cap),
+                       dim3(kBinDetWG), 0, st, out.act, nblk, out.offs, out.perm, ctx->key_rank);
+  }
   HIP_TRY(hipGetLastError());
   return EMPM_OK;
 }
@@ -3273,6 +3359,19 @@ int empm_set_tuning(empm_ctx* ctx, const empm_tuning* t) {
 int empm_get_tuning(empm_ctx* ctx, empm_tuning* t) {
   if (!ctx || !t) return set_err(EMPM_E_INVALID_ARG, "empm_get_tuning: null argument");
   *t = ctx->tune;
+  return EMPM_OK;
+}
+
+int empm_set_deterministic(empm_ctx* ctx, int enable) {
+  if (!ctx) return set_err(EMPM_E_INVALID_ARG, "empm_set_deterministic: null context");
+  if ((enable != 0) != ctx->deterministic && !ctx->gcache.empty()) {
+    // the cached graphs bake the other mode's kernels
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->gstream));
+    for (auto& g : ctx->gcache) (void)hipGraphExecDestroy(g.exec);
+    ctx->gcache.clear();
+  }
+  ctx->deterministic = enable != 0;
   return EMPM_OK;
 }
 

@@ -79,6 +79,13 @@ class Context:
         check(self.lib.empm_get_tuning(self._h, C.byref(t)))
         return t.as_dict()
 
+    def set_deterministic(self, enable: bool = True):
+        """Bitwise-reproducible results: float sums in an order fixed by the
+        particles' indices instead of by atomic arrival (slower; for tests
+        and debugging)."""
+        check(self.lib.empm_set_deterministic(self._h, int(bool(enable))))
+        return self
+
     def graph_stats(self):
         """(captures, replays) of this context's graph cache."""
         a, b = C.c_longlong(), C.c_longlong()

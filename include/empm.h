@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define EMPM_ABI_VERSION 7
+#define EMPM_ABI_VERSION 8
 #define EMPM_MAX_COLLIDERS 16
 
 enum {
@@ -243,6 +243,18 @@ typedef struct {
  * launch shapes it changes. */
 int empm_set_tuning(empm_ctx *ctx, const empm_tuning *tuning);
 int empm_get_tuning(empm_ctx *ctx, empm_tuning *tuning);
+
+/* Deterministic mode (enable != 0): a call's results no longer depend on the
+ * order in which float atomic sums arrive, so the same inputs give the same
+ * bits run after run.  The binning then orders each cell's particles by
+ * index (not by the order of its counting atomics), and the block-owned
+ * scatters run 256-thread workgroups (whatever the tuning says) whose waves
+ * each sum into an LDS tile of their own, added in wave order.  Slower (one
+ * more binning pass, four tiles per scatter workgroup); for tests and
+ * debugging.  Drops the context's cached graphs when the mode changes.  No
+ * counterpart in the reference, whose P2G float atomic_add
+ * (mpm_utils.py:420-423) sums in arrival order. */
+int empm_set_deterministic(empm_ctx *ctx, int enable);
 
 /* Failure flags of the context's simulations, set on the device by every
  * forward substep since the last read (a per-rollout check that costs one
