@@ -98,6 +98,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def emit(args, result):
+    """Print the result line; with --metrics-jsonl also append it there."""
+    print(json.dumps(result), flush=True)
+    if args.metrics_jsonl:
+        from empm_amd import metrics
+
+        metrics.append_record(args.metrics_jsonl, result, argv=sys.argv[1:])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +129,8 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (3 times) and a read-only sum of it "
                          "(3 times): the known-size kernels that calibrate FETCH_SIZE / WRITE_SIZE readings")
+    ap.add_argument("--metrics-jsonl", default=None,
+                    help="also append the result line to this JSON-lines file (time- and host-stamped)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -149,22 +160,22 @@ def main():
         log(f"[bench] read probe: {args.copy_probe_mib} MiB read per launch (sum)")
         del src, dst
     if args.only == "population":
-        print(json.dumps({"population": bench_population(args, dev)}), flush=True)
+        emit(args, {"population": bench_population(args, dev)})
         return
     if args.only == "serving":
-        print(json.dumps({"serving": bench_serving(args, dev)}), flush=True)
+        emit(args, {"serving": bench_serving(args, dev)})
         return
     if args.only == "long_horizon":
-        print(json.dumps({"long_horizon": bench_long_horizon(args, dev)}), flush=True)
+        emit(args, {"long_horizon": bench_long_horizon(args, dev)})
         return
     if args.only == "loss":
-        print(json.dumps({"loss": bench_loss(args, dev)}), flush=True)
+        emit(args, {"loss": bench_loss(args, dev)})
         return
     if args.only == "lbs":
-        print(json.dumps({"lbs": bench_lbs(args, dev)}), flush=True)
+        emit(args, {"lbs": bench_lbs(args, dev)})
         return
     if args.only == "c1":
-        print(json.dumps({"c1": bench_c1(args, dev)}), flush=True)
+        emit(args, {"c1": bench_c1(args, dev)})
         return
     sc = scenes.ellipsoid_c2(n_particles=args.particles, n_grid=args.grid, n_steps=args.substeps, param_seed=3)
     tprm = scenes.sample_params_c2(2)
@@ -413,7 +424,7 @@ def main():
         "lbs": lbs,
         "c1": c1,
     }
-    print(json.dumps(out), flush=True)
+    emit(args, out)
 
 
 def bench_population(args, dev):
